@@ -1,0 +1,414 @@
+#include "gen.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <sys/stat.h>
+#include <unordered_set>
+#include <vector>
+
+namespace csm {
+namespace {
+
+// xoshiro256** seeded through splitmix64 (public-domain generators by Blackman and Vigna).
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        for (int i = 0; i < 4; i++) {
+            seed += 0x9E3779B97F4A7C15ull;
+            uint64_t z = seed;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45);
+        return r;
+    }
+    uint64_t below(uint64_t n) { return n ? next() % n : 0; }  // modulo bias is irrelevant here
+    double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        double u1 = uniform(), u2 = uniform();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+struct Alias {  // Walker's alias method over non-negative weights
+    std::vector<double> prob;
+    std::vector<uint32_t> alias;
+    explicit Alias(const std::vector<double>& w) {
+        const size_t n = w.size();
+        prob.resize(n);
+        alias.resize(n);
+        double sum = 0;
+        for (double x : w) sum += x;
+        std::vector<double> p(n);
+        std::vector<uint32_t> small, large;
+        for (size_t i = 0; i < n; i++) {
+            p[i] = w[i] * n / sum;
+            (p[i] < 1.0 ? small : large).push_back(static_cast<uint32_t>(i));
+        }
+        while (!small.empty() && !large.empty()) {
+            uint32_t s = small.back(), l = large.back();
+            small.pop_back();
+            prob[s] = p[s];
+            alias[s] = l;
+            p[l] = (p[l] + p[s]) - 1.0;
+            if (p[l] < 1.0) { large.pop_back(); small.push_back(l); }
+        }
+        for (uint32_t i : large) { prob[i] = 1.0; alias[i] = i; }
+        for (uint32_t i : small) { prob[i] = 1.0; alias[i] = i; }
+    }
+    uint32_t sample(Rng& r) const {
+        const uint32_t i = static_cast<uint32_t>(r.below(prob.size()));
+        return r.uniform() < prob[i] ? i : alias[i];
+    }
+};
+
+struct Edge { uint32_t a, b, l; };
+
+// Chung-Lu: expected degree of vertex i proportional to (i + 1)^(-1/(gamma-1)), capped at dmax,
+// rescaled so that expected degrees sum to 2m; endpoints are drawn independently by weight and
+// self-loops / duplicates are discarded.  Vertex ids are shuffled so hubs are spread out.
+std::vector<uint64_t> chung_lu(uint32_t n, uint64_t m, double gamma, double dmax, Rng& rng) {
+    std::vector<double> w(n);
+    const double ex = 1.0 / (gamma - 1.0);
+    for (uint32_t i = 0; i < n; i++) w[i] = std::pow(static_cast<double>(i) + 1.0, -ex);
+    for (int it = 0; it < 60; it++) {
+        double sum = 0;
+        for (double x : w) sum += x;
+        const double scale = (2.0 * m) / sum;
+        bool capped = false;
+        for (auto& x : w) {
+            x *= scale;
+            if (x > dmax) { x = dmax; capped = true; }
+        }
+        if (!capped) break;
+    }
+    std::vector<uint32_t> perm(n);
+    for (uint32_t i = 0; i < n; i++) perm[i] = i;
+    for (uint32_t i = n; i > 1; i--) std::swap(perm[i - 1], perm[rng.below(i)]);
+    Alias al(w);
+    std::vector<uint64_t> keys;
+    keys.reserve(m + m / 8);
+    while (keys.size() < m) {
+        const uint64_t need = (m - keys.size()) + (m - keys.size()) / 16 + 16;
+        for (uint64_t k = 0; k < need; k++) {
+            uint32_t a = perm[al.sample(rng)], b = perm[al.sample(rng)];
+            if (a == b) continue;
+            if (a > b) std::swap(a, b);
+            keys.push_back((static_cast<uint64_t>(a) << 32) | b);
+        }
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    }
+    // Random subset of exactly m edges, in random order.
+    for (size_t i = keys.size(); i > 1; i--) std::swap(keys[i - 1], keys[rng.below(i)]);
+    keys.resize(m);
+    return keys;
+}
+
+std::vector<uint32_t> labels(uint32_t n, uint32_t k, bool zipf, Rng& rng) {
+    std::vector<uint32_t> lab(n);
+    if (!zipf) {
+        for (auto& x : lab) x = static_cast<uint32_t>(rng.below(k));
+        return lab;
+    }
+    std::vector<double> cdf(k);
+    double s = 0;
+    for (uint32_t i = 0; i < k; i++) { s += 1.0 / (i + 1.0); cdf[i] = s; }
+    for (auto& x : lab) {
+        const double u = rng.uniform() * s;
+        x = static_cast<uint32_t>(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        if (x >= k) x = k - 1;
+    }
+    return lab;
+}
+
+struct Csr {  // undirected adjacency of the final graph, sorted lists (used for query extraction)
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> nbr;
+    std::vector<uint32_t> lab;  // edge label aligned with nbr
+    void build(uint32_t n, const std::vector<Edge>& es) {
+        off.assign(static_cast<size_t>(n) + 1, 0);
+        for (const auto& e : es) { off[e.a + 1]++; off[e.b + 1]++; }
+        for (uint32_t i = 0; i < n; i++) off[i + 1] += off[i];
+        nbr.resize(off[n]);
+        lab.resize(off[n]);
+        std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+        for (const auto& e : es) {
+            nbr[pos[e.a]] = e.b; lab[pos[e.a]++] = e.l;
+            nbr[pos[e.b]] = e.a; lab[pos[e.b]++] = e.l;
+        }
+        for (uint32_t v = 0; v < n; v++) {
+            std::vector<std::pair<uint32_t, uint32_t>> t;
+            for (uint64_t k = off[v]; k < off[v + 1]; k++) t.emplace_back(nbr[k], lab[k]);
+            std::sort(t.begin(), t.end());
+            for (uint64_t k = off[v]; k < off[v + 1]; k++) { nbr[k] = t[k - off[v]].first; lab[k] = t[k - off[v]].second; }
+        }
+    }
+    uint64_t deg(uint32_t v) const { return off[v + 1] - off[v]; }
+    int64_t find(uint32_t a, uint32_t b) const {
+        auto lo = nbr.begin() + off[a], hi = nbr.begin() + off[a + 1];
+        auto it = std::lower_bound(lo, hi, b);
+        return (it != hi && *it == b) ? (it - nbr.begin()) : -1;
+    }
+};
+
+enum Shape { kTree, kSparse, kDense, kCycles, kC1 };
+
+struct QueryOut {
+    std::vector<uint32_t> verts;  // data vertices, query vertex id = position
+    std::vector<Edge> edges;      // query vertex ids
+};
+
+// Random walk with restarts from visited vertices (so trees branch); returns the vertices and the
+// first-visit (tree) edges, or false.
+bool walk(const Csr& g, uint32_t n_target, const std::vector<uint64_t>& edge_keys, Rng& rng,
+          std::vector<uint32_t>& vs, std::vector<std::pair<uint32_t, uint32_t>>& tree) {
+    const uint64_t key = edge_keys[rng.below(edge_keys.size())];
+    uint32_t cur = static_cast<uint32_t>(key >> 32);
+    vs.assign(1, cur);
+    tree.clear();
+    for (uint32_t step = 0; step < 64 * n_target && vs.size() < n_target; step++) {
+        if (g.deg(cur) == 0) return false;
+        const uint32_t nx = g.nbr[g.off[cur] + rng.below(g.deg(cur))];
+        auto it = std::find(vs.begin(), vs.end(), nx);
+        if (it == vs.end()) {
+            const uint32_t pi = static_cast<uint32_t>(std::find(vs.begin(), vs.end(), cur) - vs.begin());
+            vs.push_back(nx);
+            tree.emplace_back(pi, static_cast<uint32_t>(vs.size() - 1));
+            cur = nx;
+        } else {
+            cur = nx;
+        }
+        if (rng.uniform() < 0.35) cur = vs[rng.below(vs.size())];
+    }
+    return vs.size() == n_target;
+}
+
+bool extract(const Csr& g, const std::vector<uint32_t>& vlab, uint32_t nq, Shape shape,
+             const std::vector<uint64_t>& edge_keys, Rng& rng, QueryOut& q) {
+    for (int attempt = 0; attempt < 20000; attempt++) {
+        std::vector<uint32_t> vs;
+        std::vector<std::pair<uint32_t, uint32_t>> tree;
+        if (shape == kC1) {
+            // 4-cycle a-b-c-d with chord a-c, plus a fifth vertex hanging off the cycle.
+            const uint64_t key = edge_keys[rng.below(edge_keys.size())];
+            const uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key);
+            std::vector<uint32_t> common;
+            for (uint64_t k = g.off[a]; k < g.off[a + 1]; k++)
+                if (g.nbr[k] != c && g.find(c, g.nbr[k]) >= 0) common.push_back(g.nbr[k]);
+            if (common.size() < 2) continue;
+            const uint32_t b = common[rng.below(common.size())];
+            uint32_t d = b;
+            for (int t = 0; t < 16 && d == b; t++) d = common[rng.below(common.size())];
+            if (d == b) continue;
+            vs = {a, b, c, d};
+            const uint32_t hang = static_cast<uint32_t>(rng.below(4));
+            const uint32_t h = vs[hang];
+            if (g.deg(h) < 2) continue;
+            uint32_t e = vs[0];
+            for (int t = 0; t < 32; t++) {
+                const uint32_t x = g.nbr[g.off[h] + rng.below(g.deg(h))];
+                if (std::find(vs.begin(), vs.end(), x) == vs.end()) { e = x; break; }
+            }
+            if (std::find(vs.begin(), vs.end(), e) != vs.end()) continue;
+            vs.push_back(e);
+            q.verts = vs;
+            q.edges.clear();
+            const uint32_t pairs[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 2}, {hang, 4}};
+            for (auto& p : pairs) {
+                const int64_t pos = g.find(vs[p[0]], vs[p[1]]);
+                q.edges.push_back(Edge{p[0], p[1], g.lab[pos]});
+            }
+            return true;
+        }
+        if (!walk(g, nq, edge_keys, rng, vs, tree)) continue;
+        std::vector<std::pair<uint32_t, uint32_t>> extra;
+        for (uint32_t i = 0; i < nq; i++)
+            for (uint32_t j = i + 1; j < nq; j++) {
+                if (g.find(vs[i], vs[j]) < 0) continue;
+                bool in_tree = false;
+                for (auto& t : tree)
+                    if ((t.first == i && t.second == j) || (t.first == j && t.second == i)) in_tree = true;
+                if (!in_tree) extra.emplace_back(i, j);
+            }
+        for (size_t i = extra.size(); i > 1; i--) std::swap(extra[i - 1], extra[rng.below(i)]);
+        size_t take = 0;
+        const size_t half3 = (3 * nq + 1) / 2;  // |E| >= 1.5 n  <=>  average degree >= 3
+        if (shape == kTree) take = 0;
+        else if (shape == kSparse) {
+            const size_t max_extra = half3 - 1 - (nq - 1);  // keep |E| < 1.5 n
+            if (extra.empty() || max_extra == 0) continue;
+            take = 1 + rng.below(std::min(extra.size(), max_extra));
+        } else if (shape == kDense) {
+            if (nq - 1 + extra.size() < half3) continue;
+            take = extra.size();
+        } else if (shape == kCycles) {
+            const size_t want = 1 + rng.below(2);
+            if (extra.size() < want) continue;
+            take = want;
+        }
+        q.verts = vs;
+        q.edges.clear();
+        for (auto& t : tree) q.edges.push_back(Edge{t.first, t.second, g.lab[g.find(vs[t.first], vs[t.second])]});
+        for (size_t k = 0; k < take; k++)
+            q.edges.push_back(Edge{extra[k].first, extra[k].second, g.lab[g.find(vs[extra[k].first], vs[extra[k].second])]});
+        (void)vlab;
+        return true;
+    }
+    return false;
+}
+
+bool write_query(const std::string& path, const QueryOut& q, const std::vector<uint32_t>& vlab) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "t # 0\n");
+    for (size_t i = 0; i < q.verts.size(); i++) std::fprintf(f, "v %zu %u\n", i, vlab[q.verts[i]]);
+    for (const auto& e : q.edges) std::fprintf(f, "e %u %u %u\n", e.a, e.b, e.l);
+    std::fclose(f);
+    return true;
+}
+
+struct Spec {
+    const char* name;
+    uint32_t n;
+    uint64_t m;
+    double gamma;
+    double dmax;
+    uint32_t nlabels;
+    bool zipf;
+    double holdout;     // fraction of edges held out as insertions
+    double delete_frac; // fraction of the stream that are deletions
+    uint32_t batch;
+    std::vector<std::pair<Shape, uint32_t>> queries;  // (shape, vertices)
+};
+
+Spec spec_of(const std::string& c) {
+    Spec s{};
+    if (c == "tiny") {
+        s = Spec{"tiny", 300, 1500, 2.3, 60, 3, false, 0.2, 0.1, 100, {}};
+        s.queries = {{kTree, 4}, {kSparse, 5}, {kDense, 5}};
+    } else if (c == "c1") {
+        s = Spec{"c1", 10000, 50000, 2.3, 1000, 8, false, 0.10, 0.0, 0xFFFFFFFFu, {}};
+        s.queries = {{kC1, 5}};
+    } else if (c == "c2" || c == "c4") {
+        s = Spec{c == "c2" ? "c2" : "c4", 400000, 3400000, 2.1, 3000, 32, true, 0.10, 0.10, 65536, {}};
+        if (c == "c2") {
+            s.queries = {{kSparse, 6}, {kSparse, 7}, {kSparse, 8}, {kDense, 6}, {kDense, 7}, {kDense, 8}};
+        } else {
+            for (int i = 0; i < 16; i++) s.queries.push_back({kTree, 4 + (i % 9)});
+            for (int i = 0; i < 24; i++) s.queries.push_back({kSparse, 4 + (i % 9)});
+            for (int i = 0; i < 24; i++) s.queries.push_back({kDense, 4 + (i % 9)});
+        }
+    } else if (c == "c3") {
+        s = Spec{"c3", 4000000, 35000000, 2.0, 20000, 16, false, 0.05, 0.05, 1048576, {}};
+        s.queries = {{kCycles, 8}, {kCycles, 9}, {kCycles, 10}, {kCycles, 8}, {kCycles, 9}, {kCycles, 10}};
+    }
+    return s;
+}
+
+}  // namespace
+
+int gen_main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: gen <tiny|c1|c2|c3|c4> <out_dir> [--seed S] [--scale F]\n");
+        return 2;
+    }
+    const std::string cfg = argv[1];
+    const std::string out = argv[2];
+    uint64_t seed = 1;
+    double scale = 1.0;
+    for (int i = 3; i + 1 < argc; i += 2) {
+        if (!std::strcmp(argv[i], "--seed")) seed = std::strtoull(argv[i + 1], nullptr, 10);
+        else if (!std::strcmp(argv[i], "--scale")) scale = std::atof(argv[i + 1]);
+    }
+    Spec sp = spec_of(cfg);
+    if (!sp.name) { std::fprintf(stderr, "unknown config %s\n", cfg.c_str()); return 2; }
+    if (scale != 1.0) {
+        sp.n = std::max<uint32_t>(64, static_cast<uint32_t>(sp.n * scale));
+        sp.m = std::max<uint64_t>(256, static_cast<uint64_t>(sp.m * scale));
+        sp.dmax = std::max(16.0, sp.dmax * std::sqrt(scale));
+        if (sp.batch != 0xFFFFFFFFu) sp.batch = std::max<uint32_t>(64, static_cast<uint32_t>(sp.batch * scale));
+    }
+    mkdir(out.c_str(), 0755);
+    Rng rng(seed * 0x100000001B3ull + 7);
+
+    std::vector<uint64_t> keys = chung_lu(sp.n, sp.m, sp.gamma, sp.dmax, rng);
+    std::vector<uint32_t> vlab = labels(sp.n, sp.nlabels, sp.zipf, rng);
+    const size_t n_hold = static_cast<size_t>(keys.size() * sp.holdout);
+    const size_t n_init = keys.size() - n_hold;
+    // Stream: all held-out edges are inserted; deletions are distinct initial edges, so every
+    // deletion removes an existing edge and nothing is reinserted.
+    const size_t n_del = sp.delete_frac > 0 ? static_cast<size_t>(n_hold * sp.delete_frac / (1.0 - sp.delete_frac)) : 0;
+    std::vector<std::pair<uint64_t, uint8_t>> stream;
+    stream.reserve(n_hold + n_del);
+    for (size_t k = n_init; k < keys.size(); k++) stream.emplace_back(keys[k], 0);
+    {
+        std::vector<size_t> pick(n_init);
+        for (size_t k = 0; k < n_init; k++) pick[k] = k;
+        for (size_t k = 0; k < n_del && k < n_init; k++) {
+            const size_t j = k + rng.below(n_init - k);
+            std::swap(pick[k], pick[j]);
+            stream.emplace_back(keys[pick[k]], 1);
+        }
+    }
+    for (size_t i = stream.size(); i > 1; i--) std::swap(stream[i - 1], stream[rng.below(i)]);
+
+    // data graph file
+    {
+        FILE* f = std::fopen((out + "/data.graph").c_str(), "w");
+        if (!f) { std::fprintf(stderr, "cannot write %s\n", out.c_str()); return 1; }
+        std::fprintf(f, "t # 0\n");
+        for (uint32_t v = 0; v < sp.n; v++) std::fprintf(f, "v %u %u\n", v, vlab[v]);
+        for (size_t k = 0; k < n_init; k++)
+            std::fprintf(f, "e %u %u 0\n", static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]));
+        std::fclose(f);
+    }
+    {
+        FILE* f = std::fopen((out + "/update.stream").c_str(), "w");
+        for (const auto& e : stream)
+            std::fprintf(f, "%se %u %u 0\n", e.second ? "-" : "", static_cast<uint32_t>(e.first >> 32), static_cast<uint32_t>(e.first));
+        std::fclose(f);
+    }
+    // queries are extracted from the final graph (initial minus deletions plus insertions)
+    std::vector<Edge> fin;
+    {
+        std::unordered_set<uint64_t> dels;
+        for (const auto& e : stream) if (e.second) dels.insert(e.first);
+        fin.reserve(keys.size());
+        for (size_t k = 0; k < keys.size(); k++)
+            if (k >= n_init || !dels.count(keys[k])) fin.push_back(Edge{static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), 0});
+    }
+    Csr g;
+    g.build(sp.n, fin);
+    std::vector<uint64_t> fin_keys;
+    fin_keys.reserve(fin.size());
+    for (const auto& e : fin) fin_keys.push_back((static_cast<uint64_t>(e.a) << 32) | e.b);
+    uint32_t qk = 0;
+    for (const auto& qs : sp.queries) {
+        QueryOut q;
+        if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, q)) {
+            std::fprintf(stderr, "could not extract query %u\n", qk);
+            return 1;
+        }
+        write_query(out + "/q" + std::to_string(qk) + ".graph", q, vlab);
+        qk++;
+    }
+    FILE* f = std::fopen((out + "/meta.txt").c_str(), "w");
+    std::fprintf(f, "config %s\nseed %llu\nn %u\nm_initial %zu\nstream %zu\ninserts %zu\ndeletes %zu\nbatch_size %u\nqueries %u\n",
+                 sp.name, (unsigned long long)seed, sp.n, n_init, stream.size(), n_hold, n_del, sp.batch, qk);
+    std::fclose(f);
+    return 0;
+}
+
+}  // namespace csm

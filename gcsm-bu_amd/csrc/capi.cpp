@@ -1,0 +1,251 @@
+// C ABI over the engine (include/csm.h).
+#include "../../include/csm.h"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+using namespace csm;
+
+struct csm_session {
+    Engine engine;
+    DataGraph graph;
+    bool have_graph = false;
+    std::string error;
+    EngineOptions opt;
+};
+
+namespace {
+
+csm_status fail(csm_session* s, csm_status st, const std::string& msg) {
+    if (s) s->error = msg;
+    return st;
+}
+
+csm_status classify(csm_session* s, const std::string& err) {
+    if (err.empty()) return CSM_OK;
+    csm_status st = CSM_ERR_DEVICE;
+    if (err.rfind("Failed to open", 0) == 0) st = CSM_ERR_IO;
+    else if (err.find("malformed") != std::string::npos || err.find("vertex update") != std::string::npos ||
+             err.find("undeclared") != std::string::npos || err.find("declared twice") != std::string::npos ||
+             err.find("duplicate") != std::string::npos || err.find("self-loop") != std::string::npos ||
+             err.find("contiguous") != std::string::npos || err.find("connected") != std::string::npos ||
+             err.find("less than 2") != std::string::npos)
+        st = CSM_ERR_FORMAT;
+    else if (err.find("at most 16") != std::string::npos || err.find("32-bit") != std::string::npos ||
+             err.find("2^32") != std::string::npos || err.find("out of range") != std::string::npos)
+        st = CSM_ERR_LIMIT;
+    else if (err.find("pool") != std::string::npos || err.find("arena") != std::string::npos || err.find("ring") != std::string::npos)
+        st = CSM_ERR_MEMORY;
+    else if (err.find("precondition") != std::string::npos)
+        st = CSM_ERR_PRECONDITION;
+    else if (err.find("before") != std::string::npos || err.find("already") != std::string::npos ||
+             err.find("not built") != std::string::npos || err.find("must be") != std::string::npos)
+        st = CSM_ERR_STATE;
+    return fail(s, st, err);
+}
+
+}  // namespace
+
+extern "C" {
+
+csm_status csm_session_create(const csm_options* options, csm_session** out) {
+    if (!out) return CSM_ERR_ARGUMENT;
+    *out = nullptr;
+    auto* s = new csm_session();
+    EngineOptions o;
+    if (options) {
+        o.device = options->device;
+        o.directed = options->directed != 0;
+        o.pool_bytes = options->pool_bytes;
+        if (options->ring_bytes) o.ring_bytes = options->ring_bytes;
+        if (options->cp_list_bytes) o.cp_list_bytes = options->cp_list_bytes;
+        o.max_batch_edges = options->max_batch_edges;
+        if (options->semi_bfs_threshold) o.semi_bfs_threshold = options->semi_bfs_threshold;
+    }
+    s->opt = o;
+    std::string err = s->engine.init(o);
+    if (!err.empty()) {
+        // keep the session so the caller can read the message
+        s->error = err;
+        *out = s;
+        return CSM_ERR_DEVICE;
+    }
+    *out = s;
+    return CSM_OK;
+}
+
+void csm_session_destroy(csm_session* s) { delete s; }
+
+const char* csm_last_error(const csm_session* s) { return s ? s->error.c_str() : "no session"; }
+
+csm_status csm_load_graph_file(csm_session* s, const char* path) {
+    if (!s || !path) return CSM_ERR_ARGUMENT;
+    std::string err = load_data_graph(path, s->graph);
+    if (!err.empty()) return classify(s, err);
+    s->have_graph = true;
+    return classify(s, s->engine.set_graph(s->graph));
+}
+
+csm_status csm_load_query_file(csm_session* s, const char* path, uint32_t* query_id) {
+    if (!s || !path) return CSM_ERR_ARGUMENT;
+    Query q;
+    std::string err = parse_query(path, q);
+    if (!err.empty()) return classify(s, err);
+    uint32_t id = 0;
+    err = s->engine.add_query(q, id);
+    if (!err.empty()) return classify(s, err);
+    if (query_id) *query_id = id;
+    return CSM_OK;
+}
+
+csm_status csm_open_update_file(csm_session* s, const char* path, uint32_t batch_size, uint64_t* num_batches) {
+    if (!s || !path) return CSM_ERR_ARGUMENT;
+    if (!s->have_graph) return fail(s, CSM_ERR_STATE, "load the data graph before the update file");
+    UpdateStream us;
+    std::string err = load_update_stream(path, s->graph, us);
+    if (!err.empty()) return classify(s, err);
+    err = s->engine.set_stream(us, batch_size);
+    if (!err.empty()) return classify(s, err);
+    if (num_batches) *num_batches = s->engine.num_batches();
+    return CSM_OK;
+}
+
+csm_status csm_build_index(csm_session* s) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.build());
+}
+
+csm_status csm_plan(csm_session* s, uint32_t query_id, const csm_technique* t) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    Technique tech;
+    if (t) {
+        if (t->matching_order < 0 || t->matching_order > 3 || t->search_strategy < 0 || t->search_strategy > 3)
+            return fail(s, CSM_ERR_ARGUMENT, "matching order and search strategy must be 0..3");
+        tech.order = static_cast<MatchingOrder>(t->matching_order);
+        tech.strategy = static_cast<SearchStrategy>(t->search_strategy);
+        tech.local_index = t->local_index != 0;
+        tech.cartesian_product = t->cartesian_product != 0;
+        tech.heavy_vertex_avoidance = t->heavy_vertex_avoidance != 0;
+        tech.relation_switch = t->relation_switch != 0;
+    }
+    return classify(s, s->engine.plan(query_id, tech));
+}
+
+csm_status csm_get_plan_text(csm_session* s, uint32_t query_id, char* buf, size_t cap, size_t* needed) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (query_id >= s->engine.num_queries() || !s->engine.planned(query_id)) return fail(s, CSM_ERR_STATE, "query not planned");
+    const std::string t = print_orders(s->engine.query(query_id), s->engine.plan_of(query_id));
+    if (needed) *needed = t.size() + 1;
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, t.size());
+        std::memcpy(buf, t.data(), n);
+        buf[n] = 0;
+    }
+    return CSM_OK;
+}
+
+csm_status csm_get_summary(csm_session* s, uint32_t query_id, uint32_t* card, float* avg, uint32_t cap, uint32_t* num_relations) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (query_id >= s->engine.num_queries()) return fail(s, CSM_ERR_ARGUMENT, "query id out of range");
+    const auto& c = s->engine.cardinalities(query_id);
+    const auto& a = s->engine.avg_degrees(query_id);
+    if (c.empty()) return fail(s, CSM_ERR_STATE, "index not built");
+    if (num_relations) *num_relations = static_cast<uint32_t>(c.size());
+    for (uint32_t r = 0; r < c.size() && r < cap; r++) {
+        if (card) card[r] = c[r];
+        if (avg) avg[r] = a[r];
+    }
+    return CSM_OK;
+}
+
+csm_status csm_apply_batch(csm_session* s, uint64_t batch_index, csm_query_result* results, uint32_t num_queries, csm_batch_info* info) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    const uint64_t bs = s->engine.batch_size();
+    if (batch_index >= s->engine.num_batches()) return fail(s, CSM_ERR_ARGUMENT, "batch index out of range");
+    const uint64_t lo = batch_index * bs;
+    const uint64_t n = std::min<uint64_t>(bs, s->engine.stream().size() - lo);
+    std::vector<QueryResult> out;
+    BatchInfo bi;
+    std::string err = s->engine.run_batch(lo, n, out, bi);
+    if (!err.empty()) return classify(s, err);
+    for (uint32_t q = 0; q < num_queries && q < out.size(); q++) {
+        results[q].positive = out[q].pos;
+        results[q].negative = out[q].neg;
+        results[q].positive_hash = out[q].pos_hash;
+        results[q].negative_hash = out[q].neg_hash;
+    }
+    if (info) {
+        info->edges = bi.edges;
+        info->inserts = bi.inserts;
+        info->deletes = bi.deletes;
+        info->ms_total = bi.ms_total;
+        info->ms_index = bi.ms_index;
+        info->ms_match = bi.ms_match;
+    }
+    return CSM_OK;
+}
+
+csm_status csm_get_relation_edge_counts(csm_session* s, uint32_t query_id, uint64_t batch_index, uint64_t* counts, uint32_t cap,
+                                        uint32_t* num_relations) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (query_id >= s->engine.num_queries()) return fail(s, CSM_ERR_ARGUMENT, "query id out of range");
+    const Query& q = s->engine.query(query_id);
+    std::vector<uint64_t> c;
+    if (batch_index == UINT64_MAX) {
+        relation_edge_counts(q, s->graph, s->graph.src.data(), s->graph.dst.data(), s->graph.elabel.data(), s->graph.src.size(), c);
+    } else {
+        const UpdateStream& us = s->engine.stream();
+        const uint64_t bs = s->engine.batch_size();
+        if (batch_index >= s->engine.num_batches()) return fail(s, CSM_ERR_ARGUMENT, "batch index out of range");
+        const uint64_t lo = batch_index * bs, n = std::min<uint64_t>(bs, us.size() - lo);
+        relation_edge_counts(q, s->graph, us.src.data() + lo, us.dst.data() + lo, us.elabel.data() + lo, n, c);
+    }
+    if (num_relations) *num_relations = static_cast<uint32_t>(c.size());
+    for (uint32_t r = 0; r < c.size() && r < cap; r++) counts[r] = c[r];
+    return CSM_OK;
+}
+
+csm_status csm_get_counts(csm_session* s, uint32_t* nv, uint32_t* nq) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (nv) *nv = s->graph.dv;
+    if (nq) *nq = s->engine.num_queries();
+    return CSM_OK;
+}
+
+csm_status csm_pool_usage(csm_session* s, uint64_t* used, uint64_t* capacity) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (used) *used = s->engine.pool_used();
+    if (capacity) *capacity = s->engine.pool_capacity();
+    return CSM_OK;
+}
+
+csm_status csm_read_candidates(csm_session* s, uint32_t query_id, uint32_t u, uint32_t* words, uint32_t cap) {
+    if (!s || !words) return CSM_ERR_ARGUMENT;
+    std::vector<uint32_t> bm;
+    std::string err = s->engine.read_candidates(query_id, u, bm);
+    if (!err.empty()) return classify(s, err);
+    if (cap < bm.size()) return fail(s, CSM_ERR_ARGUMENT, "buffer too small");
+    std::memcpy(words, bm.data(), bm.size() * 4);
+    return CSM_OK;
+}
+
+csm_status csm_read_adjacency(csm_session* s, uint32_t v, uint32_t dir, uint32_t* ids, uint32_t cap, uint32_t* degree) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    std::vector<uint32_t> idv, lab;
+    std::string err = s->engine.read_adjacency(v, dir, idv, lab);
+    if (!err.empty()) return classify(s, err);
+    if (degree) *degree = static_cast<uint32_t>(idv.size());
+    if (ids)
+        for (uint32_t k = 0; k < idv.size() && k < cap; k++) ids[k] = idv[k];
+    return CSM_OK;
+}
+
+csm_status csm_synchronize(csm_session* s) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.synchronize());
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// Device-side layout and wave64 helpers for the matching path on gfx950.
+//
+// Layout in HBM (one labelled data graph shared by every standing query):
+//   adj[v]   = {offset, degree}: vertex v's neighbour keys live at arena[offset .. offset+degree),
+//              sorted ascending; capacity cap[v] >= degree (slack for merge-in-place growth).
+//   key      = (label_slot << id_bits) | neighbour id, label_slot = edge label rank * #vertex labels
+//              + neighbour label rank, so each (edge label, neighbour label) run is contiguous and
+//              sorted by id: the neighbours of v along one query relation are one sorted run.
+//   delta    = the current batch, per direction: seg[v] = {ins_off, ins_len, del_off, del_len}
+//              into ins_keys / del_keys (same key format, sorted per vertex); zero when untouched.
+// Directed graphs keep a second adjacency (in-arcs) and a second delta; undirected graphs alias
+// both directions to the same arrays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../common/csm_common.h"
+
+namespace csm {
+
+enum ErrBit : uint32_t {
+    kErrArenaFull = 1u,      // adjacency arena exhausted
+    kErrRingFull = 2u,       // partial-result ring too small (handled by falling back to DFS)
+    kErrPrecondition = 4u,   // inserted edge already present / deleted edge absent
+    kErrSegmentTooLong = 8u, // one vertex received more batch arcs than the segment sorter holds
+    kErrProductOverflow = 16u,
+    kErrScratchFull = 32u,
+    kErrLabelOverflow = 64u,
+};
+
+struct DevAdj {
+    uint2* adj;      // [dv] {offset into arena, degree}
+    uint32_t* cap;   // [dv]
+};
+
+struct DevDelta {
+    uint4* seg;       // [dv] {ins_off, ins_len, del_off, del_len}
+    uint32_t* ins;    // insertion keys
+    uint32_t* del;    // deletion keys
+};
+
+struct DevGraph {
+    uint32_t dv;
+    uint32_t id_bits;
+    uint32_t id_mask;
+    uint32_t nvl;         // vertex-label ranks
+    const uint32_t* vrank;  // [dv] label rank of each vertex (kNoVertex if undeclared)
+    uint32_t* arena;
+    unsigned long long* arena_top;  // bump pointer (words) for relocated adjacency lists
+    unsigned long long arena_words;
+    DevAdj dir[2];        // 0 = out (or undirected), 1 = in
+    DevDelta delta[2];
+    uint32_t* err;        // error bits
+};
+
+// One backward neighbour of the vertex matched at some depth: the partner is at position `pos`
+// of the matching order, its list along the query relation is the run `slot` of its adjacency in
+// direction `dir`, seen through `view` (relation views of the current batch step).
+struct BackEntry {
+    uint32_t slot;
+    uint8_t pos;
+    uint8_t view;
+    uint8_t dir;
+    uint8_t pad;
+};
+
+// Everything one (query, batch step, phase) needs on the device; passed by value.
+struct StepDesc {
+    uint32_t n;                       // query vertices
+    uint32_t phase;                   // 0 insertion (delta = ins), 1 deletion (delta = del)
+    uint32_t order[kMaxQV];           // query vertex matched at each depth
+    uint32_t vrank[kMaxQV];           // data label rank required at each depth
+    uint8_t boff[kMaxQV + 1];         // backward entries of depth d: be[boff[d] .. boff[d+1])
+    uint8_t pivot[kMaxQV];            // entry whose list generates the candidates of depth d
+    uint8_t pad0[3];
+    BackEntry be[kMaxQE];
+    const uint32_t* filter[kMaxQV];   // candidate bitmap of the vertex at depth d (local or global)
+};
+
+// ---- wave64 helpers --------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// First index in a[lo, hi) whose value is >= key (hi if none): search.cuh:7-25 semantics.
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t key) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ bool contains_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t key) {
+    const uint32_t p = lower_bound_u32(a, lo, hi, key);
+    return p < hi && a[p] == key;
+}
+
+// The run of `slot` inside the sorted list a[lo, hi).
+__device__ __forceinline__ void slot_range(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t slot, uint32_t id_bits,
+                                           uint32_t& rlo, uint32_t& rhi) {
+    const uint64_t k0 = static_cast<uint64_t>(slot) << id_bits;
+    const uint64_t k1 = static_cast<uint64_t>(slot + 1) << id_bits;
+    rlo = lower_bound_u32(a, lo, hi, static_cast<uint32_t>(k0));
+    rhi = k1 > 0xFFFFFFFFull ? hi : lower_bound_u32(a, rlo, hi, static_cast<uint32_t>(k1));
+}
+
+__device__ __forceinline__ bool bit_test(const uint32_t* bm, uint32_t v) { return (bm[v >> 5] >> (v & 31)) & 1u; }
+
+}  // namespace csm

@@ -1,0 +1,424 @@
+// Edge-anchored incremental enumeration on wave64.
+//
+//   init_from_pairs / init_from_set   initial partial results of the first matching-order edge
+//                                      (write_initial_partial_results, enumeration.cu:24-79)
+//   extend                             one wave per partial result, depth-first down to end_depth
+//                                      with a per-wave LDS stack of 64-candidate chunks; counts and
+//                                      hashes complete embeddings or writes partial results of
+//                                      end_depth (a BFS level when end_depth = d0 + 1).  Replaces
+//                                      extendBFSDFSRegTwo (enumeration.cu:407-781) and gamma_enumerate
+//                                      (gamma_enumeration.cu:359-880).
+//   cp_prepare / cp_enumerate          Cartesian-product flattening of trailing tree leaves
+//                                      (GetNumTree + enumerateCartesianProductTree,
+//                                      cartesian_product.cu:193-284).
+// Candidates of depth d come from the pivot backward neighbour's view run; each lane tests one
+// candidate: candidate bitmap, injectivity against the matched prefix, and membership in every other
+// backward neighbour's view run by binary search (search.cuh:7-25 semantics).
+#include "kernels.h"
+#include "view.h"
+
+namespace csm {
+namespace {
+
+constexpr uint32_t kWaves = 4;  // waves per block
+
+struct WaveState {
+    uint32_t f[kMaxQV];
+    unsigned long long ph[kMaxQV + 1];  // ph[d] = sum_{j<d} pair_hash(order[j], f[j])
+    RunView run[kMaxQE];
+    uint32_t cur[kMaxQV], tot[kMaxQV], nb[kMaxQV], bcnt[kMaxQV], bpos[kMaxQV];
+    uint32_t buf[kMaxQV][64];
+};
+
+__device__ __forceinline__ const uint32_t* entry_delta(const DevGraph& g, const StepDesc& d, const BackEntry& e) {
+    return phase_delta(g, e.dir, d.phase);
+}
+
+// Computes the view runs of every backward entry of `depth` and resets its frame.
+__device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d, WaveState& W, uint32_t depth, uint32_t lane) {
+    const uint32_t e0 = d.boff[depth], e1 = d.boff[depth + 1];
+    if (lane < e1 - e0) {
+        const BackEntry e = d.be[e0 + lane];
+        W.run[e0 + lane] = make_run(g, d.phase, e.view, e.dir, W.f[e.pos], e.slot);
+    }
+    wave_sync();
+    if (lane == 0) {
+        const uint32_t p = d.pivot[depth];
+        const RunView r = W.run[p];
+        W.nb[depth] = run_base_len(r, d.be[p].view);
+        W.tot[depth] = run_seq_len(r, d.be[p].view);
+        W.cur[depth] = 0;
+        W.bcnt[depth] = 0;
+        W.bpos[depth] = 0;
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ bool test_candidate(const DevGraph& g, const StepDesc& d, const WaveState& W, uint32_t depth, uint32_t key) {
+    const uint32_t c = key & g.id_mask;
+    if (!bit_test(d.filter[depth], c)) return false;
+    for (uint32_t j = 0; j < depth; j++)
+        if (W.f[j] == c) return false;
+    const uint32_t p = d.pivot[depth];
+    const BackEntry pe = d.be[p];
+    if (!run_elem_live(entry_delta(g, d, pe), W.run[p], pe.view, key)) return false;
+    for (uint32_t e = d.boff[depth]; e < d.boff[depth + 1]; e++) {
+        if (e == p) continue;
+        const BackEntry be = d.be[e];
+        const uint32_t k2 = (be.slot << g.id_bits) | c;
+        if (!run_contains(g, entry_delta(g, d, be), W.run[e], be.view, k2)) return false;
+    }
+    return true;
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, uint32_t d0, uint32_t end_depth, Ring r, MatchOut out,
+                                                         uint32_t* work, int schedule) {
+    __shared__ WaveState ws[kWaves];
+    WaveState& W = ws[threadIdx.x >> 6];
+    const uint32_t lane = lane_id();
+    const uint32_t nrec = min(*r.in_count, r.in_max);
+    const uint32_t total_waves = gridDim.x * kWaves;
+    const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    unsigned long long my_cnt = 0, my_hash = 0;
+
+    uint32_t item;
+    if (schedule == 0) item = gw;
+    else {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(work, 1u);
+        item = uniform(__shfl(t, 0, 64));
+    }
+    while (item < nrec) {
+        if (kMode == kWriteAtEnd && uniform(__hip_atomic_load(r.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
+        if (lane < d0) W.f[lane] = r.in[static_cast<uint64_t>(item) * d0 + lane];
+        wave_sync();
+        if (lane == 0) {
+            unsigned long long h = 0;
+            W.ph[0] = 0;
+            for (uint32_t j = 0; j < d0; j++) { h += pair_hash(d.order[j], W.f[j]); W.ph[j + 1] = h; }
+        }
+        wave_sync();
+        uint32_t depth = d0;
+        setup_frame(g, d, W, depth, lane);
+        while (true) {
+            if (W.bpos[depth] < W.bcnt[depth]) {
+                const uint32_t c = W.buf[depth][W.bpos[depth]];
+                wave_sync();
+                if (lane == 0) {
+                    W.bpos[depth] += 1;
+                    W.f[depth] = c;
+                    W.ph[depth + 1] = W.ph[depth] + pair_hash(d.order[depth], c);
+                }
+                wave_sync();
+                depth += 1;
+                setup_frame(g, d, W, depth, lane);
+                continue;
+            }
+            if (W.cur[depth] >= W.tot[depth]) {
+                if (depth == d0) break;
+                depth -= 1;
+                continue;
+            }
+            const uint32_t t = W.cur[depth] + lane;
+            const bool valid = t < W.tot[depth];
+            const uint32_t p = d.pivot[depth];
+            const uint32_t key = valid ? run_elem(g, entry_delta(g, d, d.be[p]), W.run[p], W.nb[depth], t) : 0u;
+            const bool ok = valid && test_candidate(g, d, W, depth, key);
+            const uint32_t c = key & g.id_mask;
+            const uint64_t m = __ballot(ok);
+            wave_sync();
+            if (lane == 0) W.cur[depth] += 64;
+            if (depth + 1 == end_depth) {
+                if (kMode == kCountAtEnd) {
+                    if (ok) {
+                        my_cnt += 1;
+                        my_hash += embedding_hash(W.ph[depth] + pair_hash(d.order[depth], c));
+                    }
+                } else if (m) {
+                    const uint32_t np = static_cast<uint32_t>(__popcll(m));
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(r.out_count, np);
+                    base = __shfl(base, 0, 64);
+                    if (static_cast<uint64_t>(base) + np > r.out_cap) {
+                        if (lane == 0) atomicOr(r.overflow, 1u);
+                    } else if (ok) {
+                        uint32_t* o = r.out + static_cast<uint64_t>(base + lanes_below(m)) * end_depth;
+                        for (uint32_t j = 0; j < depth; j++) o[j] = W.f[j];
+                        o[depth] = c;
+                    }
+                }
+            } else {
+                if (ok) W.buf[depth][lanes_below(m)] = c;
+                if (lane == 0) { W.bcnt[depth] = static_cast<uint32_t>(__popcll(m)); W.bpos[depth] = 0; }
+            }
+            wave_sync();
+        }
+        if (schedule == 0) item += total_waves;
+        else {
+            uint32_t t2 = 0;
+            if (lane == 0) t2 = atomicAdd(work, 1u);
+            item = uniform(__shfl(t2, 0, 64));
+        }
+    }
+    if (kMode == kCountAtEnd) {
+        my_cnt = wave_sum_u64(my_cnt);
+        my_hash = wave_sum_u64(my_hash);
+        if (lane == 0 && my_cnt) { atomicAdd(out.count, my_cnt); atomicAdd(out.hash, my_hash); }
+    }
+}
+
+__global__ void k_init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t max_pairs, int swap, uint32_t* out,
+                                  uint32_t* out_count) {
+    const uint32_t n = min(*n_pairs, max_pairs);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t a = pairs[2 * i], b = pairs[2 * i + 1];
+        out[2 * i] = swap ? b : a;
+        out[2 * i + 1] = swap ? a : b;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = n;
+}
+
+// One wave per start vertex: pairs (v, x) with x in the view run of v along a->b and x in filter(b).
+__global__ void __launch_bounds__(256) k_init_from_set(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                                                       const uint32_t* list, const uint32_t* len, uint32_t max_len,
+                                                       const uint32_t* filter_b, Ring r) {
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t n = min(*len, max_len);
+    const uint32_t* dl = phase_delta(g, dir, phase);
+    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb) {
+        const uint32_t v = list[i];
+        const RunView rv = make_run(g, phase, view, dir, v, slot);
+        const uint32_t nb = run_base_len(rv, view), tot = run_seq_len(rv, view);
+        for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            const bool valid = t < tot;
+            const uint32_t key = valid ? run_elem(g, dl, rv, nb, t) : 0u;
+            const uint32_t x = key & g.id_mask;
+            const bool ok = valid && x != v && bit_test(filter_b, x) && run_elem_live(dl, rv, view, key);
+            const uint64_t m = __ballot(ok);
+            if (!m) continue;
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(r.out_count, static_cast<uint32_t>(__popcll(m)));
+            base = __shfl(base, 0, 64);
+            if (static_cast<uint64_t>(base) + __popcll(m) > r.out_cap) {
+                if (lane == 0) atomicOr(r.overflow, 1u);
+            } else if (ok) {
+                const uint32_t o = base + lanes_below(m);
+                r.out[2 * static_cast<uint64_t>(o)] = v;
+                r.out[2 * static_cast<uint64_t>(o) + 1] = x;
+            }
+        }
+    }
+}
+
+// ---- Cartesian product -------------------------------------------------------------------------
+// Per record (wave): filtered candidate list of every trailing leaf (candidate bitmap, not in the
+// matched prefix, alive in its view), their product, and the record's prefix hash.
+__global__ void __launch_bounds__(256) k_cp_prepare(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
+                                                    uint32_t max_rec, CpScratch c) {
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t n = min(*n_rec, max_rec);
+    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb) {
+        uint32_t f[kMaxQV];
+        for (uint32_t j = 0; j < d0; j++) f[j] = rec[static_cast<uint64_t>(i) * d0 + j];
+        uint32_t total = 0;
+        unsigned long long prod = 1;
+        bool overflow = false;
+        // pass 1: sizes
+        for (uint32_t k = d0; k < d.n; k++) {
+            const uint32_t e = d.boff[k];
+            const BackEntry be = d.be[e];
+            const uint32_t* dl = phase_delta(g, be.dir, d.phase);
+            const RunView rv = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
+            const uint32_t nb = run_base_len(rv, be.view), tot = run_seq_len(rv, be.view);
+            uint32_t s = 0;
+            for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
+                const uint32_t t = t0 + lane;
+                bool ok = t < tot;
+                const uint32_t key = ok ? run_elem(g, dl, rv, nb, t) : 0u;
+                const uint32_t x = key & g.id_mask;
+                if (ok) ok = bit_test(d.filter[k], x) && run_elem_live(dl, rv, be.view, key);
+                for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != x;
+                s += static_cast<uint32_t>(__popcll(__ballot(ok)));
+            }
+            if (lane == 0) c.leaf_len[static_cast<uint64_t>(i) * kMaxQV + (k - d0)] = s;
+            total += s;
+            if (s == 0) prod = 0;
+            else if (prod > (~0ull >> 1) / s) overflow = true;
+            else prod *= s;
+        }
+        unsigned long long h = 0;
+        for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
+        if (overflow) {
+            if (lane == 0) atomicOr(g.err, kErrProductOverflow);
+            prod = 0;
+        }
+        unsigned long long base = 0;
+        if (prod && lane == 0) {
+            base = atomicAdd(c.lists_top, static_cast<unsigned long long>(total));
+            if (base + total > c.lists_cap) { atomicOr(g.err, kErrScratchFull); base = ~0ull; }
+        }
+        base = __shfl(base, 0, 64);
+        if (base == ~0ull) prod = 0;
+        if (lane == 0) {
+            c.prod[i] = prod;
+            c.rec_hash[i] = h;
+            c.list_base[i] = prod ? static_cast<uint32_t>(base) : 0u;
+            if (prod) atomicMax(c.max_prod, prod);
+        }
+        if (!prod) continue;
+        // pass 2: write the lists
+        uint32_t woff = static_cast<uint32_t>(base);
+        for (uint32_t k = d0; k < d.n; k++) {
+            const uint32_t e = d.boff[k];
+            const BackEntry be = d.be[e];
+            const uint32_t* dl = phase_delta(g, be.dir, d.phase);
+            const RunView rv = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
+            const uint32_t nb = run_base_len(rv, be.view), tot = run_seq_len(rv, be.view);
+            for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
+                const uint32_t t = t0 + lane;
+                bool ok = t < tot;
+                const uint32_t key = ok ? run_elem(g, dl, rv, nb, t) : 0u;
+                const uint32_t x = key & g.id_mask;
+                if (ok) ok = bit_test(d.filter[k], x) && run_elem_live(dl, rv, be.view, key);
+                for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != x;
+                const uint64_t m = __ballot(ok);
+                if (ok) c.lists[woff + lanes_below(m)] = x;
+                woff += static_cast<uint32_t>(__popcll(m));
+            }
+        }
+    }
+}
+
+// Thread per flattened combination t: record r owns [off[r], off[r+1]); digits of t - off[r] in the
+// mixed radix of the leaf list sizes select one candidate per leaf; leaves must be distinct.
+__global__ void __launch_bounds__(256) k_cp_enumerate(StepDesc d, uint32_t d0, const uint32_t* n_rec, uint32_t max_rec, CpScratch c,
+                                                      MatchOut out) {
+    __shared__ unsigned long long sc[256], sh[256];
+    const uint32_t n = min(*n_rec, max_rec);
+    const unsigned long long T = c.prod[n];
+    const uint32_t nl = d.n - d0;
+    unsigned long long cnt = 0, hsum = 0;
+    for (unsigned long long t = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; t < T;
+         t += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+        // last record with off[r] <= t
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (c.prod[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint32_t r = lo;
+        unsigned long long local = t - c.prod[r];
+        const uint32_t* ll = c.leaf_len + static_cast<uint64_t>(r) * kMaxQV;
+        uint32_t x[kMaxQV];
+        uint32_t loff = c.list_base[r];
+        unsigned long long h = c.rec_hash[r];
+        bool ok = true;
+        for (uint32_t k = 0; k < nl; k++) {
+            const uint32_t s = ll[k];
+            const uint32_t idx = static_cast<uint32_t>(local % s);
+            local /= s;
+            x[k] = c.lists[loff + idx];
+            loff += s;
+            for (uint32_t j = 0; j < k; j++) ok = ok && x[j] != x[k];
+            h += pair_hash(d.order[d0 + k], x[k]);
+        }
+        if (ok) { cnt += 1; hsum += embedding_hash(h); }
+    }
+    sc[threadIdx.x] = cnt;
+    sh[threadIdx.x] = hsum;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { sc[threadIdx.x] += sc[threadIdx.x + o]; sh[threadIdx.x] += sh[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && sc[0]) { atomicAdd(out.count, sc[0]); atomicAdd(out.hash, sh[0]); }
+}
+
+__global__ void __launch_bounds__(256) k_count_records(StepDesc d, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
+                                                       MatchOut out) {
+    __shared__ unsigned long long sc[256], sh[256];
+    const uint32_t n = min(*n_rec, max_rec);
+    unsigned long long cnt = 0, hsum = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned long long h = 0;
+        for (uint32_t j = 0; j < d.n; j++) h += pair_hash(d.order[j], rec[static_cast<uint64_t>(i) * d.n + j]);
+        cnt += 1;
+        hsum += embedding_hash(h);
+    }
+    sc[threadIdx.x] = cnt;
+    sh[threadIdx.x] = hsum;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { sc[threadIdx.x] += sc[threadIdx.x + o]; sh[threadIdx.x] += sh[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && sc[0]) { atomicAdd(out.count, sc[0]); atomicAdd(out.hash, sh[0]); }
+}
+
+__global__ void __launch_bounds__(256) k_bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_t* len) {
+    const uint32_t words = (dv + 31) / 32;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x) {
+        uint32_t b = bitmap[w];
+        if (!b) continue;
+        const uint32_t p = atomicAdd(len, static_cast<uint32_t>(__popc(b)));
+        uint32_t k = 0;
+        while (b) {
+            const uint32_t bit = static_cast<uint32_t>(__ffs(b)) - 1;
+            list[p + k++] = w * 32 + bit;
+            b &= b - 1;
+        }
+    }
+}
+
+int grid_for(uint64_t n, int per_block, int cap) {
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(cap), (n + per_block - 1) / per_block)));
+}
+
+}  // namespace
+
+void init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t max_pairs, bool swap, uint32_t* out,
+                     uint32_t* out_count, hipStream_t s) {
+    k_init_from_pairs<<<grid_for(max_pairs, 256, 4096), 256, 0, s>>>(pairs, n_pairs, max_pairs, swap ? 1 : 0, out, out_count);
+}
+
+void init_from_set(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* list,
+                   const uint32_t* len, uint32_t max_len, const uint32_t* filter_b, Ring r, hipStream_t s) {
+    k_init_from_set<<<grid_for(max_len, 4, 8192), 256, 0, s>>>(g, phase, view, dir, slot, list, len, max_len, filter_b, r);
+}
+
+void count_records(const StepDesc& d, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec, MatchOut out, hipStream_t s) {
+    k_count_records<<<grid_for(max_rec, 256, 4096), 256, 0, s>>>(d, rec, n_rec, max_rec, out);
+}
+
+void bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_t* len, hipStream_t s) {
+    k_bitmap_to_list<<<grid_for((dv + 31) / 32, 256, 4096), 256, 0, s>>>(bitmap, dv, list, len);
+}
+
+void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r, MatchOut out,
+            uint32_t* work_counter, int schedule, hipStream_t s) {
+    // Persistent grid: enough waves to fill 256 CUs several times over, never more than records.
+    const int grid = grid_for(r.in_max, kWaves, 2048);
+    if (mode == kCountAtEnd)
+        k_extend<kCountAtEnd><<<grid, kWaves * 64, 0, s>>>(g, d, d0, end_depth, r, out, work_counter, schedule);
+    else
+        k_extend<kWriteAtEnd><<<grid, kWaves * 64, 0, s>>>(g, d, d0, end_depth, r, out, work_counter, schedule);
+}
+
+void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
+                CpScratch c, hipStream_t s) {
+    k_cp_prepare<<<grid_for(max_rec, 4, 8192), 256, 0, s>>>(g, d, d0, rec, n_rec, max_rec, c);
+    scan_exclusive<unsigned long long>(c.prod, c.prod, n_rec, max_rec, true, c.scan_tmp, s);
+}
+
+void cp_enumerate(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
+                  CpScratch c, MatchOut out, hipStream_t s) {
+    (void)g;
+    (void)rec;
+    k_cp_enumerate<<<4096, 256, 0, s>>>(d, d0, n_rec, max_rec, c, out);
+}
+
+}  // namespace csm

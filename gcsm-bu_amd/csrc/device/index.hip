@@ -1,0 +1,243 @@
+// Global candidate bitmaps (NLF), relation summary and the per-step local index.
+//
+// Global index.  The reference keeps, per query relation, a materialised copy of the data arcs whose
+// endpoints are NLF candidates (UpdateGlobalIndex, match_gpu.cu:611-834).  Here the data graph is
+// shared by all queries and the global index is only the candidate bitmap of every query vertex:
+// a relation's global-index arcs are the relation run of the adjacency filtered by the two bitmaps.
+// The NLF test is the reference's (indexing.cu:303-338): for every neighbour group (edge label,
+// neighbour label) of u, v needs at least as many neighbours in the group as u has.
+//
+// Local index.  For batch step i the reference rebuilds candidate bitmaps along the indexing order
+// (BuildLocalIndex_v2, match_gpu.cu:1098-1343, kernels indexing.cu:645-1436): the anchors give
+// L(u0), L(u1); a later vertex u keeps the candidates that have a neighbour in L(b) for every
+// earlier neighbour b.  Here it is computed sparsely: expand from one earlier neighbour's list,
+// then semi-join the survivors against the other earlier neighbours' bitmaps.
+#include "kernels.h"
+#include "view.h"
+
+namespace csm {
+namespace {
+
+__device__ __forceinline__ bool nlf_pass(const DevGraph& g, const NlfEntry& e, const NlfGroup* groups, uint32_t v, int mode) {
+    for (uint32_t k = e.g0; k < e.g1; k++) {
+        const NlfGroup gr = groups[k];
+        const uint2 a = g.dir[gr.dir].adj[v];
+        uint32_t lo, hi;
+        slot_range(g.arena, a.x, a.x + a.y, gr.slot, g.id_bits, lo, hi);
+        uint32_t deg = hi - lo;
+        if (mode == 0) {
+            const uint4 s = g.delta[gr.dir].seg[v];
+            if (s.y) {
+                uint32_t dlo, dhi;
+                slot_range(g.delta[gr.dir].ins, s.x, s.x + s.y, gr.slot, g.id_bits, dlo, dhi);
+                deg += dhi - dlo;
+            }
+        }
+        if (deg < gr.count) return false;
+    }
+    return true;
+}
+
+// One wave per 64 consecutive vertices: every bitmap word is rewritten.
+__global__ void __launch_bounds__(256) k_nlf_full(DevGraph g, const NlfEntry* entries, uint32_t n_entries, const NlfGroup* groups) {
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t nw = (g.dv + 63) / 64;
+    const uint32_t words = (g.dv + 31) / 32;
+    for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < nw; w += gridDim.x * wpb) {
+        const uint32_t v = w * 64 + lane;
+        const uint32_t vr = v < g.dv ? g.vrank[v] : kNoVertex;
+        for (uint32_t e = 0; e < n_entries; e++) {
+            const NlfEntry en = entries[e];
+            const bool pass = v < g.dv && vr == en.vrank && en.vrank != kNoVertex && nlf_pass(g, en, groups, v, 1);
+            const uint64_t m = __ballot(pass);
+            if (lane == 0) en.bitmap[2 * w] = static_cast<uint32_t>(m);
+            if (lane == 1 && 2 * w + 1 < words) en.bitmap[2 * w + 1] = static_cast<uint32_t>(m >> 32);
+        }
+    }
+}
+
+__global__ void k_nlf_touched(DevGraph g, const NlfEntry* entries, uint32_t n_entries, const NlfGroup* groups,
+                              const uint32_t* touched, const uint32_t* n_touched, int mode) {
+    const uint32_t T = *n_touched;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        const uint32_t v = touched[i];
+        const uint32_t vr = g.vrank[v];
+        for (uint32_t e = 0; e < n_entries; e++) {
+            const NlfEntry en = entries[e];
+            if (en.vrank != vr || vr == kNoVertex) continue;
+            const bool pass = nlf_pass(g, en, groups, v, mode);
+            const uint32_t bit = 1u << (v & 31);
+            if (pass) atomicOr(&en.bitmap[v >> 5], bit);
+            else if (mode == 1) atomicAnd(&en.bitmap[v >> 5], ~bit);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_relation_summary(DevGraph g, uint32_t dir, uint32_t slot, uint32_t vrank_u,
+                                                          const uint32_t* cand_u, const uint32_t* cand_w, unsigned long long* out) {
+    __shared__ unsigned long long sc[256], ss[256];
+    unsigned long long card = 0, srcs = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < g.dv; v += gridDim.x * blockDim.x) {
+        if (g.vrank[v] != vrank_u || !bit_test(cand_u, v)) continue;
+        const uint2 a = g.dir[dir].adj[v];
+        uint32_t lo, hi;
+        slot_range(g.arena, a.x, a.x + a.y, slot, g.id_bits, lo, hi);
+        uint32_t c = 0;
+        for (uint32_t k = lo; k < hi; k++) c += bit_test(cand_w, g.arena[k] & g.id_mask) ? 1u : 0u;
+        card += c;
+        srcs += c ? 1u : 0u;
+    }
+    sc[threadIdx.x] = card;
+    ss[threadIdx.x] = srcs;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { sc[threadIdx.x] += sc[threadIdx.x + o]; ss[threadIdx.x] += ss[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { atomicAdd(&out[0], sc[0]); atomicAdd(&out[1], ss[0]); }
+}
+
+__device__ __forceinline__ void vset_add(VSet s, uint32_t v) {
+    const uint32_t bit = 1u << (v & 31);
+    const uint32_t old = atomicOr(&s.bitmap[v >> 5], bit);
+    if (!(old & bit)) {
+        const uint32_t p = atomicAdd(s.len, 1u);
+        if (p < s.cap) s.list[p] = v;
+    }
+}
+
+__global__ void k_local_anchors(DevGraph g, uint32_t phase, uint32_t dir, uint32_t slot, uint32_t vr0, const uint32_t* cand0,
+                                const uint32_t* cand1, const uint32_t* touched, const uint32_t* n_touched, uint32_t* pairs,
+                                uint32_t* n_pairs, uint32_t max_pairs, VSet L0, VSet L1) {
+    const uint32_t T = *n_touched;
+    const uint32_t* dl = phase_delta(g, dir, phase);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        const uint32_t v = touched[i];
+        if (g.vrank[v] != vr0 || !bit_test(cand0, v)) continue;
+        const uint4 s = g.delta[dir].seg[v];
+        const uint32_t off = phase ? s.z : s.x, len = phase ? s.w : s.y;
+        if (!len) continue;
+        uint32_t lo, hi;
+        slot_range(dl, off, off + len, slot, g.id_bits, lo, hi);
+        bool any = false;
+        for (uint32_t k = lo; k < hi; k++) {
+            const uint32_t x = dl[k] & g.id_mask;
+            if (!bit_test(cand1, x)) continue;
+            const uint32_t p = atomicAdd(n_pairs, 1u);
+            if (p < max_pairs) { pairs[2 * p] = v; pairs[2 * p + 1] = x; }
+            vset_add(L1, x);
+            any = true;
+        }
+        if (any) vset_add(L0, v);
+    }
+}
+
+// One wave per frontier vertex, lanes over its view run.
+__global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                                                      const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
+                                                      const uint32_t* cand_u, VSet out) {
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t F = min(*frontier_len, max_frontier);
+    const uint32_t* dl = phase_delta(g, dir, phase);
+    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < F; i += gridDim.x * wpb) {
+        const uint32_t x = frontier[i];
+        const RunView r = make_run(g, phase, view, dir, x, slot);
+        const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
+        for (uint32_t t = lane; t < tot; t += 64) {
+            const uint32_t key = run_elem(g, dl, r, nb, t);
+            const uint32_t y = key & g.id_mask;
+            if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) vset_add(out, y);
+        }
+    }
+}
+
+__global__ void k_local_filter(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* in,
+                               const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b, uint32_t* out, uint32_t* out_len) {
+    const uint32_t n = min(*in_len, max_in);
+    const uint32_t* dl = phase_delta(g, dir, phase);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t y = in[i];
+        const RunView r = make_run(g, phase, view, dir, y, slot);
+        const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
+        bool hit = false;
+        for (uint32_t t = 0; t < tot && !hit; t++) {
+            const uint32_t key = run_elem(g, dl, r, nb, t);
+            hit = bit_test(bitmap_b, key & g.id_mask) && run_elem_live(dl, r, view, key);
+        }
+        if (hit) out[atomicAdd(out_len, 1u)] = y;
+    }
+}
+
+__global__ void k_set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap) {
+    const uint32_t n = min(*len, max_len);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicOr(&bitmap[list[i] >> 5], 1u << (list[i] & 31));
+}
+__global__ void k_clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap) {
+    const uint32_t n = min(*len, max_len);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) bitmap[list[i] >> 5] = 0u;
+}
+__global__ void k_copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len) {
+    const uint32_t n = min(*in_len, max_len);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_len = n;
+}
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+int grid_for(uint64_t n, int threads, int cap) {
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(cap), (n + threads - 1) / threads)));
+}
+
+}  // namespace
+
+void nlf_full(const DevGraph& g, const NlfEntry* entries, uint32_t n_entries, const NlfGroup* groups, hipStream_t s) {
+    const uint32_t nw = (g.dv + 63) / 64;
+    k_nlf_full<<<grid_for(nw, 4, 16384), 256, 0, s>>>(g, entries, n_entries, groups);
+}
+
+void nlf_touched(const DevGraph& g, const NlfEntry* entries, uint32_t n_entries, const NlfGroup* groups, const uint32_t* touched,
+                 const uint32_t* n_touched, uint32_t max_touched, int mode, hipStream_t s) {
+    k_nlf_touched<<<grid_for(max_touched, 256, 4096), 256, 0, s>>>(g, entries, n_entries, groups, touched, n_touched, mode);
+}
+
+void relation_summary(const DevGraph& g, uint32_t dir, uint32_t slot, uint32_t vrank_u, const uint32_t* cand_u,
+                      const uint32_t* cand_w, unsigned long long* out, hipStream_t s) {
+    k_relation_summary<<<grid_for(g.dv, 256, 2048), 256, 0, s>>>(g, dir, slot, vrank_u, cand_u, cand_w, out);
+}
+
+void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slot, uint32_t vr0, const uint32_t* cand0,
+                   const uint32_t* cand1, const uint32_t* touched, const uint32_t* n_touched, uint32_t max_touched, uint32_t* pairs,
+                   uint32_t* n_pairs, uint32_t max_pairs, VSet L0, VSet L1, hipStream_t s) {
+    k_local_anchors<<<grid_for(max_touched, 256, 4096), 256, 0, s>>>(g, phase, dir, slot, vr0, cand0, cand1, touched, n_touched,
+                                                                      pairs, n_pairs, max_pairs, L0, L1);
+}
+
+void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
+                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, hipStream_t s) {
+    k_local_expand<<<grid_for(max_frontier, 4, 8192), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier,
+                                                                     cand_u, out);
+}
+
+void local_filter(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* in,
+                  const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b, uint32_t* out, uint32_t* out_len, hipStream_t s) {
+    k_local_filter<<<grid_for(max_in, 256, 8192), 256, 0, s>>>(g, phase, view, dir, slot, in, in_len, max_in, bitmap_b, out, out_len);
+}
+
+void set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s) {
+    k_set_bits<<<grid_for(max_len, 256, 4096), 256, 0, s>>>(list, len, max_len, bitmap);
+}
+void clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s) {
+    k_clear_bits<<<grid_for(max_len, 256, 4096), 256, 0, s>>>(list, len, max_len, bitmap);
+}
+void copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len, hipStream_t s) {
+    k_copy_list<<<grid_for(max_len, 256, 4096), 256, 0, s>>>(in, in_len, max_len, out, out_len);
+}
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+    k_fill_u32<<<grid_for(n, 256, 8192), 256, 0, s>>>(p, v, n);
+}
+
+}  // namespace csm

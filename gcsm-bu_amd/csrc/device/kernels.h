@@ -1,0 +1,159 @@
+// Launch interface of the HIP kernels (graph.hip, index.hip, enum.hip).  Host code passes plain
+// device pointers; sizes that only the device knows stay in device memory and every grid is sized
+// for the host-known maximum, so no launch waits for a device-to-host copy.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "dev_common.h"
+
+namespace csm {
+
+constexpr uint32_t kMaxBigSegments = 4096;
+
+// The whole update stream, resident in HBM: edge k is src[k] -- dst[k] with precomputed keys.
+struct BatchInput {
+    const uint32_t* src;
+    const uint32_t* dst;
+    const uint32_t* key_fwd;  // key of dst in src's list
+    const uint32_t* key_bwd;  // key of src in dst's list
+    const uint8_t* op;        // 0 insert, 1 delete
+};
+
+struct BatchScratch {
+    uint32_t* cnt_ins;   // [dv]
+    uint32_t* cnt_del;   // [dv]
+    uint32_t* tflag;     // [dv]
+    uint32_t* touched;   // [max_arcs]
+    uint32_t* n_touched; // [1]
+    uint32_t* tmp_ins;   // [max_arcs + 1]
+    uint32_t* tmp_del;
+    uint32_t* off_ins;
+    uint32_t* off_del;
+    uint32_t* ins_pos;   // [max_arcs]
+    uint32_t* scan_tmp;  // scan_scratch_elems(max_arcs)
+    uint32_t* n_big;     // [1]
+    uint32_t* big_ptr;   // [kMaxBigSegments]
+    uint32_t* big_len;
+};
+
+template <typename T>
+void scan_exclusive(const T* in, T* out, const uint32_t* n_dev, uint32_t n_max, bool write_total, T* scratch, hipStream_t s);
+uint64_t scan_scratch_elems(uint32_t n_max);
+
+void batch_build(const DevGraph& g, int dir, int dirsel, const BatchInput& b, uint32_t lo, uint32_t n,
+                 const BatchScratch& w, hipStream_t s);
+void batch_apply_deletions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
+void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
+void batch_cleanup(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
+
+// ---- global candidates (NLF), summary ---------------------------------------------------------
+// Per query vertex: required label rank and the neighbour-label-frequency groups it must satisfy
+// (graph.cpp:295-330 NLF; indexing.cu:303-338 test).  One entry per (query, query vertex).
+struct NlfGroup {
+    uint32_t slot;
+    uint32_t count;
+    uint32_t dir;
+};
+struct NlfEntry {
+    uint32_t vrank;     // kNoVertex: label absent from the data graph -> no candidates
+    uint32_t g0, g1;    // groups [g0, g1)
+    uint32_t* bitmap;   // candidate bitmap of this query vertex
+};
+
+void nlf_full(const DevGraph& g, const NlfEntry* entries, uint32_t n_entries, const NlfGroup* groups, hipStream_t s);
+// mode 0: upper bound with the pending insertions (only sets bits); mode 1: exact on the base
+// adjacency (sets and clears bits).  Runs over the touched vertices of direction `dir`.
+void nlf_touched(const DevGraph& g, const NlfEntry* entries, uint32_t n_entries, const NlfGroup* groups,
+                 const uint32_t* touched, const uint32_t* n_touched, uint32_t max_touched, int mode, hipStream_t s);
+// Cardinality and number of non-empty sources of the relation u->w restricted to candidates
+// (statisticIndex, indexing.cu:265-301).  out[0] += card, out[1] += #sources.
+void relation_summary(const DevGraph& g, uint32_t dir, uint32_t slot, uint32_t vrank_u, const uint32_t* cand_u,
+                      const uint32_t* cand_w, unsigned long long* out, hipStream_t s);
+
+// ---- local index ------------------------------------------------------------------------------
+// A vertex set built for one batch step: bitmap + compact list + device length.
+struct VSet {
+    uint32_t* bitmap;
+    uint32_t* list;
+    uint32_t* len;
+    uint32_t cap;
+};
+
+// Anchors of the step's query edge u0->u1: batch arcs (v, x) of the phase with v in cand(u0) of
+// label rank vr0 and x in cand(u1).  Writes pairs (v, x), and fills L(u0) = {v}, L(u1) = {x}.
+void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slot, uint32_t vr0,
+                   const uint32_t* cand0, const uint32_t* cand1, const uint32_t* touched, const uint32_t* n_touched,
+                   uint32_t max_touched, uint32_t* pairs, uint32_t* n_pairs, uint32_t max_pairs, VSet L0, VSet L1,
+                   hipStream_t s);
+// L(u) seed: every vertex reached from the frontier through the relation view (b -> u) that is a
+// candidate of u; the output set's bitmap (cleared beforehand) deduplicates the list.
+void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                  const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
+                  const uint32_t* cand_u, VSet out, hipStream_t s);
+// Keeps the vertices y of `in` with a neighbour in L(b) along the view (u -> b).
+void local_filter(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                  const uint32_t* in, const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b,
+                  uint32_t* out, uint32_t* out_len, hipStream_t s);
+void set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s);
+void clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s);
+void copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len, hipStream_t s);
+
+// ---- enumeration ------------------------------------------------------------------------------
+struct MatchOut {
+    unsigned long long* count;  // [1]
+    unsigned long long* hash;   // [1]
+};
+
+struct Ring {
+    uint32_t* in;               // records of in_depth words
+    const uint32_t* in_count;   // device count
+    uint32_t in_max;            // host bound on the count
+    uint32_t* out;              // output records (write modes)
+    uint32_t* out_count;
+    uint32_t out_cap;           // records
+    uint32_t* overflow;         // set when the output does not fit
+};
+
+// Initial partial results of depth 2 from the anchor pairs (optionally swapped).
+void init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t max_pairs, bool swap,
+                     uint32_t* out, uint32_t* out_count, hipStream_t s);
+// Initial partial results of depth 2 from a start vertex set: (v, x) with x in the view a->b of v.
+void init_from_set(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                   const uint32_t* list, const uint32_t* len, uint32_t max_len, const uint32_t* filter_b,
+                   Ring r, hipStream_t s);
+// Counts and hashes records that are already complete embeddings (two-vertex queries).
+void count_records(const StepDesc& d, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec, MatchOut out, hipStream_t s);
+// Compacts the set bits of a bitmap over [0, dv) into a list.
+void bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_t* len, hipStream_t s);
+
+enum ExtendMode : int { kCountAtEnd = 0, kWriteAtEnd = 1 };
+// DFS from the records at depth d0 down to end_depth: counts/hashes complete embeddings when
+// end_depth == n, or writes the partial results of depth end_depth to the ring.
+// schedule: 0 static (record r by wave r mod W), 1 dynamic (global work counter), with root
+// splitting of long candidate lists when split > 0.
+void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r,
+            MatchOut out, uint32_t* work_counter, int schedule, hipStream_t s);
+
+// Cartesian-product flattening of trailing tree leaves (cartesian_product.cu:193-284).
+struct CpScratch {
+    unsigned long long* prod;      // [max_records + 1] products, scanned in place into offsets
+    unsigned long long* rec_hash;  // [max_records] pair-hash sum of the record's prefix
+    unsigned long long* scan_tmp;
+    uint32_t* list_base;           // [max_records] start of the record's leaf lists in `lists`
+    uint32_t* leaf_len;            // [max_records * kMaxQV]
+    uint32_t* lists;               // leaf candidates
+    unsigned long long* lists_top; // [1]
+    unsigned long long lists_cap;
+    unsigned long long* max_prod;  // [1]
+};
+void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
+                uint32_t max_rec, CpScratch c, hipStream_t s);
+void cp_enumerate(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
+                  uint32_t max_rec, CpScratch c, MatchOut out, hipStream_t s);
+
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+}  // namespace csm

@@ -1,0 +1,55 @@
+// Relation views of one vertex's neighbour run during a batch step (device side).
+#pragma once
+#include "dev_common.h"
+
+namespace csm {
+
+// The neighbours of v along one query relation (label slot, direction) as the current step sees
+// them: a base run a[lo, hi) and a delta run dl[dlo, dhi) of the phase's batch arcs.
+struct RunView {
+    uint32_t lo, hi, dlo, dhi;
+};
+
+__device__ __forceinline__ const uint32_t* phase_delta(const DevGraph& g, uint32_t dir, uint32_t phase) {
+    return phase ? g.delta[dir].del : g.delta[dir].ins;
+}
+
+__device__ __forceinline__ RunView make_run(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t v, uint32_t slot) {
+    RunView r{0, 0, 0, 0};
+    if (view != kViewDelta) {
+        const uint2 a = g.dir[dir].adj[v];
+        slot_range(g.arena, a.x, a.x + a.y, slot, g.id_bits, r.lo, r.hi);
+    }
+    if (view != kViewBase) {
+        const uint4 s = g.delta[dir].seg[v];
+        const uint32_t off = phase ? s.z : s.x;
+        const uint32_t len = phase ? s.w : s.y;
+        if (len) slot_range(phase_delta(g, dir, phase), off, off + len, slot, g.id_bits, r.dlo, r.dhi);
+    }
+    return r;
+}
+
+// Number of elements in the iteration sequence of a view (BaseMinus iterates the base run and
+// skips deleted keys).
+__device__ __forceinline__ uint32_t run_base_len(const RunView& r, uint32_t view) { return view == kViewDelta ? 0u : r.hi - r.lo; }
+__device__ __forceinline__ uint32_t run_seq_len(const RunView& r, uint32_t view) {
+    return run_base_len(r, view) + ((view == kViewBasePlusDelta || view == kViewDelta) ? r.dhi - r.dlo : 0u);
+}
+__device__ __forceinline__ uint32_t run_elem(const DevGraph& g, const uint32_t* dl, const RunView& r, uint32_t nb, uint32_t t) {
+    return t < nb ? g.arena[r.lo + t] : dl[r.dlo + (t - nb)];
+}
+// Whether element key (from the iteration sequence) is really in the view: only BaseMinus can
+// reject, when the base key is one of the batch's deletions.
+__device__ __forceinline__ bool run_elem_live(const uint32_t* dl, const RunView& r, uint32_t view, uint32_t key) {
+    return view != kViewBaseMinusDelta || !contains_u32(dl, r.dlo, r.dhi, key);
+}
+__device__ __forceinline__ bool run_contains(const DevGraph& g, const uint32_t* dl, const RunView& r, uint32_t view, uint32_t key) {
+    switch (view) {
+        case kViewBase: return contains_u32(g.arena, r.lo, r.hi, key);
+        case kViewBasePlusDelta: return contains_u32(g.arena, r.lo, r.hi, key) || contains_u32(dl, r.dlo, r.dhi, key);
+        case kViewDelta: return contains_u32(dl, r.dlo, r.dhi, key);
+        default: return contains_u32(g.arena, r.lo, r.hi, key) && !contains_u32(dl, r.dlo, r.dhi, key);
+    }
+}
+
+}  // namespace csm

@@ -1,0 +1,815 @@
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <thread>
+
+namespace csm {
+
+#define CSM_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call; \
+    } while (0)
+
+// ---- pool -------------------------------------------------------------------------------------
+DevicePool::~DevicePool() {
+    if (base_) (void)hipFree(base_);
+}
+
+std::string DevicePool::reserve(uint64_t bytes) {
+    if (base_) return "device pool already reserved";
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&base_), bytes);
+    if (e != hipSuccess) {
+        base_ = nullptr;
+        return std::string("device pool allocation of ") + std::to_string(bytes >> 20) + " MiB failed: " + hipGetErrorString(e);
+    }
+    cap_ = bytes;
+    used_ = 0;
+    return "";
+}
+
+void* DevicePool::take_bytes(uint64_t bytes) {
+    const uint64_t aligned = (used_ + 255) & ~255ull;
+    if (aligned + bytes > cap_) {
+        failed_ = true;
+        return nullptr;
+    }
+    used_ = aligned + bytes;
+    return base_ + aligned;
+}
+
+// ---- engine -----------------------------------------------------------------------------------
+Engine::~Engine() {
+    if (s_) (void)hipStreamSynchronize(s_);
+    if (h_pinned_) (void)hipHostFree(h_pinned_);
+    if (s_) (void)hipStreamDestroy(s_);
+}
+
+std::string Engine::init(const EngineOptions& o) {
+    opt_ = o;
+    int n = 0;
+    CSM_HIP(hipGetDeviceCount(&n));
+    if (n == 0) return "no HIP device available";
+    if (o.device < 0 || o.device >= n) return "device id out of range";
+    CSM_HIP(hipSetDevice(o.device));
+    CSM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096, hipHostMallocDefault));
+    return "";
+}
+
+std::string Engine::set_graph(const DataGraph& g) {
+    if (built_) return "graph already uploaded";
+    host_g_ = g;
+    return "";
+}
+
+std::string Engine::set_stream(const UpdateStream& s, uint32_t batch_size) {
+    if (built_) return "stream must be set before the index is built";
+    if (batch_size == 0) return "batch size must be positive";
+    stream_ = s;
+    batch_size_ = batch_size;
+    return "";
+}
+
+std::string Engine::add_query(const Query& q, uint32_t& qid) {
+    if (built_) return "queries must be added before the index is built";
+    auto st = std::make_unique<QState>();
+    st->q = q;
+    qid = static_cast<uint32_t>(qs_.size());
+    qs_.push_back(std::move(st));
+    return "";
+}
+
+uint64_t Engine::num_batches() const {
+    if (!batch_size_) return 0;
+    return (stream_.size() + batch_size_ - 1) / batch_size_;
+}
+
+namespace {
+uint32_t rank_of(const std::vector<std::pair<uint32_t, uint32_t>>& t, uint32_t raw) {
+    auto it = std::lower_bound(t.begin(), t.end(), std::make_pair(raw, 0u));
+    return (it != t.end() && it->first == raw) ? it->second : kNoVertex;
+}
+std::vector<std::pair<uint32_t, uint32_t>> make_ranks(std::vector<uint32_t> v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    std::vector<std::pair<uint32_t, uint32_t>> t;
+    for (uint32_t i = 0; i < v.size(); i++) t.emplace_back(v[i], i);
+    return t;
+}
+uint32_t cap_for(uint32_t deg) {  // max(8, 2^(ceil(log2 deg) + 1)), deg 0 -> 8 (indexing.cu:46-59)
+    uint32_t p = 1;
+    while (p < deg) p <<= 1;
+    return std::max<uint32_t>(kMinNbrSize, deg == 0 ? 0u : 2u * p);
+}
+}  // namespace
+
+uint32_t Engine::key_of_edge(uint32_t nbr, uint32_t elabel_raw) const {
+    const uint32_t er = rank_of(elabel_rank_, elabel_raw);
+    const uint32_t slot = er * nvl_ + rank_of(vlabel_rank_, host_g_.vlabel[nbr]);
+    return (slot << id_bits_) | nbr;
+}
+
+uint32_t Engine::read_u32(const uint32_t* p) {
+    (void)hipMemcpyAsync(h_pinned_, p, 4, hipMemcpyDeviceToHost, s_);
+    (void)hipStreamSynchronize(s_);
+    return h_pinned_[0];
+}
+
+std::string Engine::build() {
+    if (built_) return "index already built";
+    if (host_g_.dv == 0) return "no data graph loaded";
+    if (qs_.empty()) return "no query loaded";
+    host_g_.directed = opt_.directed;
+    ndirs_ = opt_.directed ? 2 : 1;
+    dv_ = host_g_.dv;
+    words_ = (dv_ + 31) / 32;
+
+    // label ranks over the graph, the stream and the queries
+    std::vector<uint32_t> vl, el;
+    for (uint32_t v = 0; v < dv_; v++) if (host_g_.vlabel[v] != kNoVertex) vl.push_back(host_g_.vlabel[v]);
+    el = host_g_.elabel;
+    el.insert(el.end(), stream_.elabel.begin(), stream_.elabel.end());
+    for (auto& qs : qs_) {
+        for (uint32_t x : qs->q.vlabel_raw) vl.push_back(x);
+        for (uint32_t x : qs->q.elabel_raw) el.push_back(x);
+    }
+    vlabel_rank_ = make_ranks(vl);
+    elabel_rank_ = make_ranks(el);
+    nvl_ = std::max<uint32_t>(1, static_cast<uint32_t>(vlabel_rank_.size()));
+    nel_ = std::max<uint32_t>(1, static_cast<uint32_t>(elabel_rank_.size()));
+    id_bits_ = 1;
+    while ((1ull << id_bits_) < dv_) id_bits_++;
+    const uint64_t nslots = static_cast<uint64_t>(nvl_) * nel_;
+    if ((nslots << id_bits_) > (1ull << 32))
+        return "label ranks times vertex ids exceed 32-bit adjacency keys (too many labels for this vertex count)";
+
+    max_batch_ = opt_.max_batch_edges ? opt_.max_batch_edges : std::max<uint32_t>(1, std::min<uint64_t>(batch_size_, stream_.size()));
+    if (batch_size_ > 0 && batch_size_ < 0xFFFFFFFFu) max_batch_ = std::max<uint32_t>(max_batch_, std::min<uint64_t>(batch_size_, stream_.size()));
+    max_batch_ = std::max<uint32_t>(max_batch_, 1);
+
+    // ---- host CSR of the initial graph ----
+    std::vector<std::vector<uint32_t>> deg(ndirs_, std::vector<uint32_t>(dv_, 0));
+    const size_t ne = host_g_.src.size();
+    for (size_t k = 0; k < ne; k++) {
+        deg[0][host_g_.src[k]]++;
+        deg[opt_.directed ? 1 : 0][host_g_.dst[k]]++;
+    }
+    std::vector<std::vector<uint64_t>> off(ndirs_, std::vector<uint64_t>(dv_ + 1, 0));
+    uint64_t total_cap = 0;
+    for (int d = 0; d < ndirs_; d++)
+        for (uint32_t v = 0; v < dv_; v++) {
+            off[d][v] = total_cap;
+            total_cap += cap_for(deg[d][v]);
+        }
+    uint64_t stream_arcs = 2ull * stream_.size();
+    const uint64_t arena_words = 3 * total_cap + 8 * stream_arcs + (1u << 20);
+    if (arena_words >= (1ull << 32)) return "adjacency arena exceeds 2^32 words";
+
+    // ---- sizes of everything else, then one reservation ----
+    const uint64_t max_arcs = 2ull * max_batch_;
+    ring_words_ = std::max<uint64_t>(opt_.ring_bytes / 4, 1u << 20) & ~1ull;
+    const uint64_t cp_rec = std::max<uint32_t>(1024, opt_.cp_max_records);
+    const uint64_t cp_list_words = std::max<uint64_t>(opt_.cp_list_bytes / 4, 1u << 20);
+    uint64_t need = 0;
+    auto add = [&](uint64_t bytes) { need += (bytes + 255) & ~255ull; };
+    add(arena_words * 4);
+    add(dv_ * 4ull);                                            // vrank
+    for (int d = 0; d < ndirs_; d++) {
+        add(dv_ * 8ull); add(dv_ * 4ull);                       // adj, cap
+        add(dv_ * 16ull); add(max_arcs * 4); add(max_arcs * 4); // delta seg, ins, del
+        add(dv_ * 12ull);                                       // cnt_ins, cnt_del, tflag
+        add(max_arcs * 4); add(16); add(4 * (max_arcs + 1) * 4); add(max_arcs * 4);
+        add(scan_scratch_elems(static_cast<uint32_t>(max_arcs)) * 4); add(16); add(2 * kMaxBigSegments * 4);
+    }
+    add(stream_.size() * 17ull + max_batch_ * 17ull);          // stream + live batch
+    add(kMaxQV * (words_ * 4ull + dv_ * 4ull) + 128);            // local sets
+    add(dv_ * 4ull + 16);                                        // tmp list
+    add(max_arcs * 8 + 16);                                      // anchor pairs
+    add(ring_words_ * 4); add(256);
+    add((cp_rec + 1) * 8); add(cp_rec * 8); add(scan_scratch_elems(static_cast<uint32_t>(cp_rec)) * 8);
+    add(cp_rec * 4); add(cp_rec * kMaxQV * 4); add(cp_list_words * 4); add(64);
+    for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
+    add(qs_.size() * 32 + 64);
+    add(kMaxQV * qs_.size() * sizeof(NlfEntry) + kMaxRel * qs_.size() * sizeof(NlfGroup) + 512);
+    need += 1 << 20;
+    if (opt_.pool_bytes && need > opt_.pool_bytes)
+        return "device pool budget too small: need " + std::to_string(need >> 20) + " MiB, budget " + std::to_string(opt_.pool_bytes >> 20) + " MiB";
+    std::string err = pool_.reserve(need);
+    if (!err.empty()) return err;
+
+    // ---- graph ----
+    g_.dv = dv_;
+    g_.id_bits = id_bits_;
+    g_.id_mask = id_bits_ >= 32 ? 0xFFFFFFFFu : ((1u << id_bits_) - 1u);
+    g_.nvl = nvl_;
+    uint32_t* d_vrank = pool_.take<uint32_t>(dv_);
+    g_.vrank = d_vrank;
+    g_.arena = pool_.take<uint32_t>(arena_words);
+    g_.arena_words = arena_words;
+    g_.err = nullptr;
+    {
+        std::vector<uint32_t> vr(dv_);
+        for (uint32_t v = 0; v < dv_; v++) vr[v] = host_g_.vlabel[v] == kNoVertex ? kNoVertex : rank_of(vlabel_rank_, host_g_.vlabel[v]);
+        CSM_HIP(hipMemcpyAsync(d_vrank, vr.data(), dv_ * 4ull, hipMemcpyHostToDevice, s_));
+        std::vector<uint32_t> arena(total_cap, 0u);
+        std::vector<std::vector<uint64_t>> pos = off;
+        auto key = [&](uint32_t nbr, uint32_t el) {
+            return ((rank_of(elabel_rank_, el) * nvl_ + vr[nbr]) << id_bits_) | nbr;
+        };
+        for (size_t k = 0; k < ne; k++) {
+            const uint32_t a = host_g_.src[k], b = host_g_.dst[k], l = host_g_.elabel[k];
+            arena[pos[0][a]++] = key(b, l);
+            arena[pos[opt_.directed ? 1 : 0][b]++] = key(a, l);
+        }
+        const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; t++)
+            th.emplace_back([&, t]() {
+                for (int d = 0; d < ndirs_; d++)
+                    for (uint32_t v = t; v < dv_; v += nth) {
+                        auto* b = arena.data() + off[d][v];
+                        std::sort(b, b + deg[d][v]);
+                    }
+            });
+        for (auto& x : th) x.join();
+        for (int d = 0; d < ndirs_; d++)
+            for (uint32_t v = 0; v < dv_; v++)
+                if (deg[d][v] > 1)
+                    for (uint64_t i = off[d][v] + 1; i < off[d][v] + deg[d][v]; i++)
+                        if (arena[i] == arena[i - 1]) return "duplicate edge in the data graph";
+        CSM_HIP(hipMemcpyAsync(g_.arena, arena.data(), total_cap * 4, hipMemcpyHostToDevice, s_));
+        for (int d = 0; d < ndirs_; d++) {
+            g_.dir[d].adj = pool_.take<uint2>(dv_);
+            g_.dir[d].cap = pool_.take<uint32_t>(dv_);
+            std::vector<uint2> adj(dv_);
+            std::vector<uint32_t> cap(dv_);
+            for (uint32_t v = 0; v < dv_; v++) {
+                adj[v] = make_uint2(static_cast<uint32_t>(off[d][v]), deg[d][v]);
+                cap[v] = cap_for(deg[d][v]);
+            }
+            CSM_HIP(hipMemcpyAsync(g_.dir[d].adj, adj.data(), dv_ * 8ull, hipMemcpyHostToDevice, s_));
+            CSM_HIP(hipMemcpyAsync(g_.dir[d].cap, cap.data(), dv_ * 4ull, hipMemcpyHostToDevice, s_));
+            g_.delta[d].seg = pool_.take<uint4>(dv_);
+            g_.delta[d].ins = pool_.take<uint32_t>(max_arcs);
+            g_.delta[d].del = pool_.take<uint32_t>(max_arcs);
+            CSM_HIP(hipMemsetAsync(g_.delta[d].seg, 0, dv_ * 16ull, s_));
+            BatchScratch& w = bs_[d];
+            w.cnt_ins = pool_.take<uint32_t>(dv_);
+            w.cnt_del = pool_.take<uint32_t>(dv_);
+            w.tflag = pool_.take<uint32_t>(dv_);
+            CSM_HIP(hipMemsetAsync(w.cnt_ins, 0, dv_ * 4ull, s_));
+            CSM_HIP(hipMemsetAsync(w.cnt_del, 0, dv_ * 4ull, s_));
+            CSM_HIP(hipMemsetAsync(w.tflag, 0, dv_ * 4ull, s_));
+            w.touched = pool_.take<uint32_t>(max_arcs);
+            w.n_touched = pool_.take<uint32_t>(4);
+            w.n_big = w.n_touched + 1;
+            CSM_HIP(hipMemsetAsync(w.n_touched, 0, 16, s_));
+            w.tmp_ins = pool_.take<uint32_t>(max_arcs + 1);
+            w.tmp_del = pool_.take<uint32_t>(max_arcs + 1);
+            w.off_ins = pool_.take<uint32_t>(max_arcs + 1);
+            w.off_del = pool_.take<uint32_t>(max_arcs + 1);
+            w.ins_pos = pool_.take<uint32_t>(max_arcs);
+            w.scan_tmp = pool_.take<uint32_t>(scan_scratch_elems(static_cast<uint32_t>(max_arcs)));
+            w.big_ptr = pool_.take<uint32_t>(kMaxBigSegments);
+            w.big_len = pool_.take<uint32_t>(kMaxBigSegments);
+        }
+        if (ndirs_ == 1) { g_.dir[1] = g_.dir[0]; g_.delta[1] = g_.delta[0]; bs_[1] = bs_[0]; }
+        unsigned long long* top = pool_.take<unsigned long long>(4);
+        g_.arena_top = top;
+        uint32_t* errp = reinterpret_cast<uint32_t*>(top + 2);
+        g_.err = errp;
+        h_pinned_[0] = static_cast<uint32_t>(total_cap);
+        h_pinned_[1] = static_cast<uint32_t>(total_cap >> 32);
+        h_pinned_[2] = 0; h_pinned_[3] = 0; h_pinned_[4] = 0; h_pinned_[5] = 0;
+        CSM_HIP(hipMemcpyAsync(top, h_pinned_, 32, hipMemcpyHostToDevice, s_));
+        CSM_HIP(hipStreamSynchronize(s_));
+    }
+
+    // ---- stream ----
+    {
+        stream_cap_ = stream_.size();
+        uint32_t* sb = pool_.take<uint32_t>(4 * (stream_cap_ + max_batch_));
+        uint8_t* ob = pool_.take<uint8_t>(stream_cap_ + max_batch_);
+        if (!sb || !ob) return "device pool exhausted (stream)";
+        const uint64_t S = stream_cap_ + max_batch_;
+        bin_.src = sb;
+        bin_.dst = sb + S;
+        bin_.key_fwd = sb + 2 * S;
+        bin_.key_bwd = sb + 3 * S;
+        bin_.op = ob;
+        if (stream_cap_) {
+            std::vector<uint32_t> kf(stream_cap_), kb(stream_cap_);
+            for (uint64_t k = 0; k < stream_cap_; k++) {
+                const uint32_t a = stream_.src[k], b = stream_.dst[k], l = stream_.elabel[k];
+                kf[k] = ((rank_of(elabel_rank_, l) * nvl_ + rank_of(vlabel_rank_, host_g_.vlabel[b])) << id_bits_) | b;
+                kb[k] = ((rank_of(elabel_rank_, l) * nvl_ + rank_of(vlabel_rank_, host_g_.vlabel[a])) << id_bits_) | a;
+            }
+            CSM_HIP(hipMemcpyAsync(sb, stream_.src.data(), stream_cap_ * 4, hipMemcpyHostToDevice, s_));
+            CSM_HIP(hipMemcpyAsync(sb + S, stream_.dst.data(), stream_cap_ * 4, hipMemcpyHostToDevice, s_));
+            CSM_HIP(hipMemcpyAsync(sb + 2 * S, kf.data(), stream_cap_ * 4, hipMemcpyHostToDevice, s_));
+            CSM_HIP(hipMemcpyAsync(sb + 3 * S, kb.data(), stream_cap_ * 4, hipMemcpyHostToDevice, s_));
+            CSM_HIP(hipMemcpyAsync(ob, stream_.op.data(), stream_cap_, hipMemcpyHostToDevice, s_));
+            CSM_HIP(hipStreamSynchronize(s_));
+        }
+    }
+
+    // ---- runtime buffers ----
+    for (uint32_t u = 0; u < kMaxQV; u++) {
+        L_[u].bitmap = pool_.take<uint32_t>(words_);
+        L_[u].list = pool_.take<uint32_t>(dv_);
+        L_[u].cap = dv_;
+        CSM_HIP(hipMemsetAsync(L_[u].bitmap, 0, words_ * 4ull, s_));
+    }
+    uint32_t* lens = pool_.take<uint32_t>(32);
+    for (uint32_t u = 0; u < kMaxQV; u++) L_[u].len = lens + u;
+    tmp_len_ = lens + kMaxQV;
+    n_pairs_ = lens + kMaxQV + 1;
+    tmp_list_ = pool_.take<uint32_t>(dv_);
+    pairs_ = pool_.take<uint32_t>(2 * max_arcs);
+    ring_ = pool_.take<uint32_t>(ring_words_);
+    counters_ = pool_.take<uint32_t>(16);
+    cp_.prod = pool_.take<unsigned long long>(cp_rec + 1);
+    cp_.rec_hash = pool_.take<unsigned long long>(cp_rec);
+    cp_.scan_tmp = pool_.take<unsigned long long>(scan_scratch_elems(static_cast<uint32_t>(cp_rec)));
+    cp_.list_base = pool_.take<uint32_t>(cp_rec);
+    cp_.leaf_len = pool_.take<uint32_t>(cp_rec * kMaxQV);
+    cp_.lists = pool_.take<uint32_t>(cp_list_words);
+    cp_.lists_cap = cp_list_words;
+    unsigned long long* cpc = pool_.take<unsigned long long>(4);
+    cp_.lists_top = cpc;
+    cp_.max_prod = cpc + 1;
+    opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
+    results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);
+    summary_ = pool_.take<unsigned long long>(4);
+
+    // ---- queries: relation slots, NLF programs, candidate bitmaps ----
+    for (auto& qsp : qs_) {
+        QState& qs = *qsp;
+        const Query& q = qs.q;
+        qs.vrank.assign(q.n, kNoVertex);
+        for (uint32_t u = 0; u < q.n; u++) {
+            qs.vrank[u] = rank_of(vlabel_rank_, q.vlabel_raw[u]);
+            if (qs.vrank[u] == kNoVertex) qs.matchable = false;
+        }
+        qs.rel.assign(2 * q.m, RelInfo{});
+        for (uint32_t r = 0; r < 2 * q.m; r++) {
+            const auto ep = q.rel_endpoints(r);
+            uint32_t el_local = 0;
+            for (const auto& e : q.nbrs[ep.first]) if (e.first == ep.second) el_local = e.second;
+            const uint32_t er = rank_of(elabel_rank_, q.elabel_raw[el_local]);
+            RelInfo& ri = qs.rel[r];
+            ri.step = q.rel_step(r);
+            ri.dir = opt_.directed ? (q.rel_out[r] ? 0u : 1u) : 0u;
+            ri.valid = er != kNoVertex && qs.vrank[ep.second] != kNoVertex;
+            ri.slot = ri.valid ? er * nvl_ + qs.vrank[ep.second] : 0u;
+            if (!ri.valid) qs.matchable = false;
+        }
+        qs.cand = pool_.take<uint32_t>(static_cast<uint64_t>(q.n) * words_);
+        if (!qs.cand) return "device pool exhausted (candidates)";
+        CSM_HIP(hipMemsetAsync(qs.cand, 0, static_cast<uint64_t>(q.n) * words_ * 4, s_));
+        qs.entry0 = static_cast<uint32_t>(h_entries_.size());
+        for (uint32_t u = 0; u < q.n; u++) {
+            std::map<std::pair<uint32_t, uint32_t>, uint32_t> grp;  // (dir, slot) -> count
+            for (uint32_t r = q.qv_offs[u]; r < q.qv_offs[u + 1]; r++)
+                if (qs.rel[r].valid) grp[{qs.rel[r].dir, qs.rel[r].slot}]++;
+            NlfEntry e{};
+            e.vrank = qs.matchable ? qs.vrank[u] : kNoVertex;
+            e.g0 = static_cast<uint32_t>(h_groups_.size());
+            for (const auto& kv : grp) h_groups_.push_back(NlfGroup{kv.first.second, kv.second, kv.first.first});
+            e.g1 = static_cast<uint32_t>(h_groups_.size());
+            e.bitmap = qs.cand + static_cast<uint64_t>(u) * words_;
+            h_entries_.push_back(e);
+        }
+        qs.entry1 = static_cast<uint32_t>(h_entries_.size());
+    }
+    n_entries_ = static_cast<uint32_t>(h_entries_.size());
+    d_entries_ = pool_.take<NlfEntry>(std::max<size_t>(1, h_entries_.size()));
+    d_groups_ = pool_.take<NlfGroup>(std::max<size_t>(1, h_groups_.size()));
+    if (pool_.failed()) return "device pool exhausted";
+    CSM_HIP(hipMemcpyAsync(d_entries_, h_entries_.data(), h_entries_.size() * sizeof(NlfEntry), hipMemcpyHostToDevice, s_));
+    if (!h_groups_.empty())
+        CSM_HIP(hipMemcpyAsync(d_groups_, h_groups_.data(), h_groups_.size() * sizeof(NlfGroup), hipMemcpyHostToDevice, s_));
+
+    // ---- offline index: candidate bitmaps on G_0, then per-relation summaries ----
+    nlf_full(g_, d_entries_, n_entries_, d_groups_, s_);
+    for (auto& qsp : qs_) {
+        QState& qs = *qsp;
+        const Query& q = qs.q;
+        qs.card.assign(2 * q.m, 0);
+        qs.avg.assign(2 * q.m, 0.0f);
+        for (uint32_t r = 0; r < 2 * q.m; r++) {
+            const auto ep = q.rel_endpoints(r);
+            uint64_t c = 0, ns = 0;
+            if (qs.rel[r].valid && qs.matchable) {
+                CSM_HIP(hipMemsetAsync(summary_, 0, 16, s_));
+                relation_summary(g_, qs.rel[r].dir, qs.rel[r].slot, qs.vrank[ep.first], qs.cand + static_cast<uint64_t>(ep.first) * words_,
+                                 qs.cand + static_cast<uint64_t>(ep.second) * words_, summary_, s_);
+                CSM_HIP(hipMemcpyAsync(h_pinned_, summary_, 16, hipMemcpyDeviceToHost, s_));
+                CSM_HIP(hipStreamSynchronize(s_));
+                unsigned long long v[2];
+                std::memcpy(v, h_pinned_, 16);
+                c = v[0];
+                ns = v[1];
+            }
+            // match_gpu.cu:606-607: 32-bit sums, float division (0/0 gives NaN)
+            const uint32_t c32 = static_cast<uint32_t>(c), n32 = static_cast<uint32_t>(ns);
+            qs.card[r] = c32;
+            qs.avg[r] = static_cast<float>(c32) / static_cast<float>(n32);
+        }
+    }
+    CSM_HIP(hipStreamSynchronize(s_));
+    built_ = true;
+    return check_err("build");
+}
+
+std::string Engine::plan(uint32_t qid, const Technique& t) {
+    if (!built_) return "build the index before planning";
+    if (qid >= qs_.size()) return "query id out of range";
+    QState& qs = *qs_[qid];
+    qs.tech = t;
+    std::string err = make_plan(qs.q, t.order, qs.card.data(), qs.avg.data(), qs.plan);
+    if (!err.empty()) return err;
+    qs.planned = true;
+    return "";
+}
+
+std::string Engine::check_err(const char* where) {
+    uint32_t e = read_u32(g_.err);
+    last_err_ = e;
+    if (!e) return "";
+    (void)hipMemsetAsync(g_.err, 0, 4, s_);
+    std::string msg = std::string(where) + ": device error";
+    if (e & kErrArenaFull) msg += " [adjacency arena full]";
+    if (e & kErrPrecondition) msg += " [batch precondition violated: inserted edge exists or deleted edge missing]";
+    if (e & kErrSegmentTooLong) msg += " [too many batch arcs on one vertex]";
+    if (e & kErrProductOverflow) msg += " [Cartesian product exceeds 2^63]";
+    if (e & kErrScratchFull) msg += " [Cartesian-product scratch full]";
+    return msg;
+}
+
+uint32_t Engine::view_of(const QState& qs, uint32_t rel, uint32_t i, uint32_t ph) const {
+    const uint32_t s = qs.rel[rel].step;
+    if (s < i) return ph ? kViewBaseMinusDelta : kViewBasePlusDelta;
+    if (s == i) return kViewDelta;
+    return kViewBase;
+}
+
+std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
+    const Query& q = qs.q;
+    const Plan& p = qs.plan;
+    const uint32_t u0 = q.qe_list[i].first, u1 = q.qe_list[i].second;
+    const uint32_t r01 = q.rel(u0, u1);
+    const uint32_t max_arcs = 2 * max_batch_;
+    auto cand = [&](uint32_t u) { return qs.cand + static_cast<uint64_t>(u) * words_; };
+
+    // clear the local sets of the previous step
+    for (uint32_t u = 0; u < kMaxQV; u++)
+        if (L_used_ >> u & 1u) clear_bits(L_[u].list, L_[u].len, dv_, L_[u].bitmap, s_);
+    L_used_ = 0;
+    fill_u32(L_[0].len, 0u, kMaxQV + 2, s_);
+
+    const RelInfo& R = qs.rel[r01];
+    local_anchors(g_, ph, R.dir, R.slot, qs.vrank[u0], cand(u0), cand(u1), bs_[R.dir].touched, bs_[R.dir].n_touched, max_arcs, pairs_,
+                  n_pairs_, max_arcs, L_[u0], L_[u1], s_);
+    L_used_ |= static_cast<uint16_t>((1u << u0) | (1u << u1));
+    const uint32_t np = read_u32(n_pairs_);
+    if (np == 0) return "empty";
+    uint32_t built = (1u << u0) | (1u << u1);
+
+    if (qs.tech.local_index) {
+        const OrderPerEdge& io = p.indexing[i];
+        for (uint32_t pos = 2; pos < q.n; pos++) {
+            const uint32_t u = io.vs[pos];
+            if (!(p.rebuild_B[i] >> u & 1u)) continue;
+            std::vector<uint32_t> bsv;
+            for (uint32_t k = io.bni_offs[pos]; k < io.bni_offs[pos + 1]; k++) {
+                const uint32_t b = io.vs[io.bni[k]];
+                if (built >> b & 1u) bsv.push_back(b);
+            }
+            if (bsv.empty()) continue;
+            size_t first = 0;
+            if (qs.tech.heavy_vertex_avoidance) {
+                for (size_t k = 1; k < bsv.size(); k++)
+                    if (qs.avg[q.rel(bsv[k], u)] < qs.avg[q.rel(bsv[first], u)]) first = k;
+            }
+            const uint32_t b0 = bsv[first];
+            const uint32_t rb = q.rel(b0, u);
+            local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, L_[b0].list, L_[b0].len, dv_, cand(u), L_[u], s_);
+            L_used_ |= static_cast<uint16_t>(1u << u);
+            for (size_t k = 0; k < bsv.size(); k++) {
+                if (k == first) continue;
+                const uint32_t b = bsv[k];
+                const uint32_t rub = q.rel(u, b);
+                fill_u32(tmp_len_, 0u, 1, s_);
+                local_filter(g_, ph, view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, L_[u].list, L_[u].len, dv_, L_[b].bitmap,
+                             tmp_list_, tmp_len_, s_);
+                clear_bits(L_[u].list, L_[u].len, dv_, L_[u].bitmap, s_);
+                copy_list(tmp_list_, tmp_len_, dv_, L_[u].list, L_[u].len, s_);
+                set_bits(L_[u].list, L_[u].len, dv_, L_[u].bitmap, s_);
+            }
+            built |= 1u << u;
+        }
+        CSM_HIP(hipMemcpyAsync(h_pinned_, L_[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, s_));
+        CSM_HIP(hipStreamSynchronize(s_));
+        for (uint32_t u = 0; u < q.n; u++)
+            if ((built >> u & 1u) && h_pinned_[u] == 0) return "empty";
+    }
+
+    // ---- step descriptor ----
+    const OrderPerEdge& o = p.orders[i];
+    StepDesc d{};
+    d.n = q.n;
+    d.phase = ph;
+    uint32_t ne = 0;
+    for (uint32_t k = 0; k < q.n; k++) {
+        const uint32_t u = o.vs[k];
+        d.order[k] = u;
+        d.vrank[k] = qs.vrank[u];
+        d.boff[k] = static_cast<uint8_t>(ne);
+        d.pivot[k] = static_cast<uint8_t>(ne);
+        d.filter[k] = (built >> u & 1u) ? L_[u].bitmap : cand(u);
+        if (k == 0) continue;
+        bool have_delta = false;
+        for (uint32_t kk = o.bni_offs[k]; kk < o.bni_offs[k + 1]; kk++) {
+            const uint32_t pos = o.bni[kk];
+            const uint32_t r = q.rel(o.vs[pos], u);
+            BackEntry e{};
+            e.slot = qs.rel[r].slot;
+            e.pos = static_cast<uint8_t>(pos);
+            e.view = static_cast<uint8_t>(view_of(qs, r, i, ph));
+            e.dir = static_cast<uint8_t>(qs.rel[r].dir);
+            if (e.view == kViewDelta && !have_delta) { d.pivot[k] = static_cast<uint8_t>(ne); have_delta = true; }
+            d.be[ne++] = e;
+        }
+    }
+    d.boff[q.n] = static_cast<uint8_t>(ne);
+
+    // ---- initial partial results (depth 2) into ring half A ----
+    const uint64_t half = ring_words_ / 2;
+    uint32_t* A = ring_;
+    uint32_t* B = ring_ + half;
+    uint32_t* cA = counters_;
+    uint32_t* cB = counters_ + 1;
+    uint32_t* ovf = counters_ + 2;
+    uint32_t* work = counters_ + 3;
+    fill_u32(counters_, 0u, 8, s_);
+    const uint32_t a = o.vs[0], b = o.vs[1];
+    uint32_t count;
+    if ((a == u0 && b == u1) || (a == u1 && b == u0)) {
+        if (2ull * np > half) return "partial-result ring too small for the batch's anchors";
+        init_from_pairs(pairs_, n_pairs_, max_arcs, a == u1, A, cA, s_);
+        count = np;
+    } else {
+        const uint32_t rab = q.rel(a, b);
+        const uint32_t* list;
+        const uint32_t* len;
+        if (built >> a & 1u) { list = L_[a].list; len = L_[a].len; }
+        else {
+            fill_u32(tmp_len_, 0u, 1, s_);
+            bitmap_to_list(cand(a), dv_, tmp_list_, tmp_len_, s_);
+            list = tmp_list_;
+            len = tmp_len_;
+        }
+        Ring r{nullptr, nullptr, 0, A, cA, static_cast<uint32_t>(std::min<uint64_t>(half / 2, 0xFFFFFFFFull)), ovf};
+        init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, s_);
+        CSM_HIP(hipMemcpyAsync(h_pinned_, counters_, 12, hipMemcpyDeviceToHost, s_));
+        CSM_HIP(hipStreamSynchronize(s_));
+        if (h_pinned_[2]) return "partial-result ring too small for the initial partial results";
+        count = h_pinned_[0];
+    }
+    if (count == 0) return "";
+    const uint32_t res_slot = static_cast<uint32_t>(std::find_if(qs_.begin(), qs_.end(), [&](const std::unique_ptr<QState>& x) { return x.get() == &qs; }) - qs_.begin());
+    if (q.n == 2) {
+        count_records(d, A, cA, count, MatchOut{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1}, s_);
+        return "";
+    }
+
+    const int sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
+    uint32_t depth = 2;
+    uint32_t* cur = A;
+    uint32_t* cur_cnt = cA;
+    uint32_t* oth = B;
+    uint32_t* oth_cnt = cB;
+    // One level (or a DFS down to `end`) written to the other half; returns false on overflow.
+    auto write_to = [&](uint32_t end, bool& ok) -> std::string {
+        fill_u32(oth_cnt, 0u, 1, s_);
+        fill_u32(ovf, 0u, 2, s_);
+        Ring r{cur, cur_cnt, count, oth, oth_cnt, static_cast<uint32_t>(std::min<uint64_t>(half / end, 0xFFFFFFFFull)), ovf};
+        extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr}, work, sched, s_);
+        CSM_HIP(hipMemcpyAsync(h_pinned_, counters_, 12, hipMemcpyDeviceToHost, s_));
+        CSM_HIP(hipStreamSynchronize(s_));
+        ok = h_pinned_[2] == 0;
+        if (ok) {
+            count = oth == B ? h_pinned_[1] : h_pinned_[0];
+            std::swap(cur, oth);
+            std::swap(cur_cnt, oth_cnt);
+            depth = end;
+        }
+        return "";
+    };
+    const uint8_t* cpi = p.cp_info[i];
+    auto tail_from = [&](uint32_t from) {
+        uint32_t nd = from;
+        while (nd < q.n && cpi[nd] != kCpTreeProduct && cpi[nd] != kCpTreeSingle) nd++;
+        return nd;
+    };
+
+    std::string err;
+    switch (qs.tech.strategy) {
+        case SearchStrategy::kDynamicBfsDfs: {  // match_gpu.cu:1916-2127
+            bool oom = false;
+            while (count < opt_.semi_bfs_threshold && q.n - depth > 1) {
+                bool ok;
+                err = write_to(depth + 1, ok);
+                if (!err.empty()) return err;
+                if (!ok) { oom = true; break; }
+                if (count == 0) return "";
+            }
+            if (!oom && cpi[depth] != kCpTreeProduct && q.n - depth > 2) {
+                const uint32_t nd = tail_from(depth);
+                if (nd < q.n) {
+                    bool ok;
+                    err = write_to(nd, ok);
+                    if (!err.empty()) return err;
+                    if (ok && count == 0) return "";
+                }
+            }
+            break;
+        }
+        case SearchStrategy::kBfs: {  // match_gpu.cu:2129-2294
+            const uint32_t target = qs.tech.cartesian_product ? std::min(tail_from(2), q.n - 1) : q.n - 1;
+            while (depth < target) {
+                bool ok;
+                err = write_to(depth + 1, ok);
+                if (!err.empty()) return err;
+                if (!ok) break;
+                if (count == 0) return "";
+            }
+            break;
+        }
+        default: {  // DFS, with or without work stealing (match_gpu.cu:1721-1914)
+            const uint32_t t = tail_from(depth);
+            if (qs.tech.cartesian_product && t < q.n && t > depth && cpi[t] == kCpTreeProduct) {
+                bool ok;
+                err = write_to(t, ok);
+                if (!err.empty()) return err;
+                if (ok && count == 0) return "";
+            }
+            break;
+        }
+    }
+    return finish_records(qs, d, i, depth, count, cur, cur_cnt, res_slot);
+}
+
+std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, uint32_t* rec,
+                                   const uint32_t* rec_count, uint32_t res_slot) {
+    const uint32_t ph = d.phase;
+    MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1};
+    const int sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
+    bool use_cp = qs.tech.cartesian_product && depth < d.n && qs.plan.cp_info[i][depth] == kCpTreeProduct && count <= opt_.cp_max_records;
+    if (use_cp) {
+        // match_gpu.cu:2046-2085: flatten unless the products are uniformly large
+        CSM_HIP(hipMemsetAsync(cp_.lists_top, 0, 16, s_));
+        cp_prepare(g_, d, depth, rec, rec_count, count, cp_, s_);
+        unsigned long long v[2];
+        CSM_HIP(hipMemcpyAsync(h_pinned_, cp_.max_prod, 8, hipMemcpyDeviceToHost, s_));
+        CSM_HIP(hipMemcpyAsync(h_pinned_ + 2, cp_.prod + count, 8, hipMemcpyDeviceToHost, s_));
+        CSM_HIP(hipStreamSynchronize(s_));
+        std::memcpy(&v[0], h_pinned_, 8);
+        std::memcpy(&v[1], h_pinned_ + 2, 8);
+        uint32_t e = read_u32(g_.err);
+        if (e & (kErrScratchFull | kErrProductOverflow)) {
+            CSM_HIP(hipMemsetAsync(g_.err, 0, 4, s_));
+            use_cp = false;  // fall back to counting by DFS: same embeddings
+        } else {
+            const float avg = static_cast<float>(v[1]) / static_cast<float>(count);
+            const float ratio = static_cast<float>(v[0]) / avg;
+            if (ratio < 50.f && avg >= static_cast<float>(kSemiBfsThreshold)) use_cp = false;
+        }
+        if (use_cp) {
+            cp_enumerate(g_, d, depth, rec, rec_count, count, cp_, out, s_);
+            return "";
+        }
+    }
+    fill_u32(counters_ + 3, 0u, 1, s_);
+    Ring r{rec, rec_count, count, nullptr, nullptr, 0, nullptr};
+    extend(g_, d, depth, d.n, kCountAtEnd, r, out, counters_ + 3, sched, s_);
+    return "";
+}
+
+std::string Engine::run_phase(uint32_t ph) {
+    for (auto& qsp : qs_) {
+        QState& qs = *qsp;
+        if (!qs.planned || !qs.matchable) continue;
+        for (uint32_t i = 0; i < qs.q.m; i++) {
+            std::string e = run_step(qs, i, ph);
+            if (e == "empty") continue;
+            if (!e.empty()) return e;
+        }
+    }
+    return "";
+}
+
+std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info) {
+    if (!built_) return "index not built";
+    if (lo + n > stream_.size()) return "batch range beyond the update stream";
+    if (n > max_batch_) return "batch larger than the scratch was sized for";
+    for (auto& qs : qs_) if (!qs->planned) return "every query must be planned before matching";
+    info = BatchInfo{};
+    info.edges = n;
+    for (uint64_t k = lo; k < lo + n; k++) (stream_.op[k] ? info.deletes : info.inserts)++;
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t max_arcs = static_cast<uint32_t>(2 * n);
+    if (n == 0) { out.assign(qs_.size(), QueryResult{}); return ""; }
+    for (int d = 0; d < ndirs_; d++) batch_build(g_, d, ndirs_ == 1 ? 0 : d + 1, bin_, static_cast<uint32_t>(lo), static_cast<uint32_t>(n), bs_[d], s_);
+    for (int d = 0; d < ndirs_; d++)
+        nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 0, s_);
+    fill_u32(reinterpret_cast<uint32_t*>(results_), 0u, qs_.size() * 8, s_);
+    CSM_HIP(hipStreamSynchronize(s_));
+    std::string err = check_err("batch build");
+    if (!err.empty()) return err;
+    const auto t1 = std::chrono::steady_clock::now();
+    double match_ms = 0;
+    if (info.deletes) {
+        const auto a = std::chrono::steady_clock::now();
+        err = run_phase(1);
+        if (!err.empty()) return err;
+        CSM_HIP(hipStreamSynchronize(s_));
+        match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+        for (int d = 0; d < ndirs_; d++) batch_apply_deletions(g_, d, max_arcs, bs_[d], s_);
+    }
+    if (info.inserts) {
+        CSM_HIP(hipStreamSynchronize(s_));
+        const auto a = std::chrono::steady_clock::now();
+        err = run_phase(0);
+        if (!err.empty()) return err;
+        CSM_HIP(hipStreamSynchronize(s_));
+        match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+        for (int d = 0; d < ndirs_; d++) batch_apply_insertions(g_, d, max_arcs, bs_[d], s_);
+    }
+    for (int d = 0; d < ndirs_; d++)
+        nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 1, s_);
+    for (int d = 0; d < ndirs_; d++) batch_cleanup(g_, d, max_arcs, bs_[d], s_);
+    std::vector<unsigned long long> res(qs_.size() * 4);
+    CSM_HIP(hipMemcpyAsync(res.data(), results_, res.size() * 8, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    err = check_err("batch apply");
+    if (!err.empty()) return err;
+    out.assign(qs_.size(), QueryResult{});
+    for (size_t q = 0; q < qs_.size(); q++) {
+        out[q].pos = res[4 * q];
+        out[q].pos_hash = res[4 * q + 1];
+        out[q].neg = res[4 * q + 2];
+        out[q].neg_hash = res[4 * q + 3];
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    info.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    info.ms_match = match_ms;
+    info.ms_index = info.ms_total - match_ms;
+    (void)t1;
+    return "";
+}
+
+std::string Engine::run_edges(const uint32_t*, const uint32_t*, const uint32_t*, const uint8_t*, uint64_t, std::vector<QueryResult>&,
+                              BatchInfo&) {
+    return "in-memory batches are not supported yet";
+}
+
+std::string Engine::synchronize() {
+    CSM_HIP(hipStreamSynchronize(s_));
+    return "";
+}
+
+std::string Engine::read_candidates(uint32_t qid, uint32_t u, std::vector<uint32_t>& bitmap) {
+    if (!built_ || qid >= qs_.size() || u >= qs_[qid]->q.n) return "bad arguments";
+    bitmap.resize(words_);
+    CSM_HIP(hipMemcpyAsync(bitmap.data(), qs_[qid]->cand + static_cast<uint64_t>(u) * words_, words_ * 4ull, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    return "";
+}
+
+std::string Engine::read_adjacency(uint32_t v, uint32_t dir, std::vector<uint32_t>& ids, std::vector<uint32_t>& labels) {
+    if (!built_ || v >= dv_ || dir >= static_cast<uint32_t>(ndirs_)) return "bad arguments";
+    uint2 a;
+    CSM_HIP(hipMemcpyAsync(&a, g_.dir[dir].adj + v, 8, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    std::vector<uint32_t> keys(a.y);
+    if (a.y) CSM_HIP(hipMemcpyAsync(keys.data(), g_.arena + a.x, a.y * 4ull, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    ids.resize(a.y);
+    labels.resize(a.y);
+    for (uint32_t k = 0; k < a.y; k++) {
+        ids[k] = keys[k] & g_.id_mask;
+        const uint32_t slot = keys[k] >> id_bits_;
+        labels[k] = elabel_rank_.empty() ? 0 : elabel_rank_[slot / nvl_].first;  // edge label value
+    }
+    return "";
+}
+
+}  // namespace csm

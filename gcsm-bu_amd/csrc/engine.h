@@ -1,0 +1,184 @@
+// Host orchestration of the matching path on one GPU.
+//
+// Mirrors the reference driver loop (TechniquePermutationProcess, technique_permutation.h:29-225) and
+// the MatchGPU class (graph/match_gpu.h:80-162), redesigned around one query-independent device
+// graph: load -> offline index (candidate bitmaps) -> summary -> plan -> per batch: build the batch
+// delta, deletion steps, apply deletions, insertion steps, apply insertions.  Each step is one query
+// edge of one query: local index, then one of the four search strategies.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device/kernels.h"
+#include "host/graph_io.h"
+#include "host/plan.h"
+#include "host/query.h"
+
+namespace csm {
+
+struct Technique {
+    MatchingOrder order = MatchingOrder::kGcsm;
+    SearchStrategy strategy = SearchStrategy::kDynamicBfsDfs;
+    bool local_index = true;
+    bool cartesian_product = true;
+    bool heavy_vertex_avoidance = true;
+    bool relation_switch = true;
+};
+
+struct EngineOptions {
+    int device = 0;
+    uint64_t pool_bytes = 0;          // HBM budget for the device pool (0: no cap)
+    uint64_t ring_bytes = 1ull << 30; // partial-result ring
+    uint64_t cp_list_bytes = 1ull << 30;
+    uint32_t cp_max_records = 1u << 24;
+    uint32_t max_batch_edges = 0;     // 0: the stream's batch size
+    uint32_t semi_bfs_threshold = kSemiBfsThreshold;
+    bool directed = false;
+};
+
+struct QueryResult {
+    uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;
+};
+
+struct BatchInfo {
+    uint64_t edges = 0, inserts = 0, deletes = 0;
+    double ms_total = 0, ms_index = 0, ms_match = 0;
+    uint32_t empty_steps = 0, steps = 0;
+};
+
+// Bump allocator over one device allocation (the reference's MemPool, utils/mem_pool.h:8-37, sized
+// at build time from the workload instead of fixed compile-time constants).
+class DevicePool {
+  public:
+    ~DevicePool();
+    std::string reserve(uint64_t bytes);
+    template <typename T>
+    T* take(uint64_t count) { return static_cast<T*>(take_bytes(count * sizeof(T))); }
+    void* take_bytes(uint64_t bytes);
+    uint64_t used() const { return used_; }
+    uint64_t capacity() const { return cap_; }
+    bool failed() const { return failed_; }
+
+  private:
+    char* base_ = nullptr;
+    uint64_t cap_ = 0, used_ = 0;
+    bool failed_ = false;
+};
+
+class Engine {
+  public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    std::string init(const EngineOptions& o);
+    std::string set_graph(const DataGraph& g);
+    std::string set_stream(const UpdateStream& s, uint32_t batch_size);
+    std::string add_query(const Query& q, uint32_t& qid);
+    std::string build();                                  // upload + offline index + summaries
+    std::string plan(uint32_t qid, const Technique& t);   // plans need the summaries
+    std::string run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info);
+    // In-memory batch (serving): edges are appended to the resident stream.
+    std::string run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
+                          std::vector<QueryResult>& out, BatchInfo& info);
+    std::string synchronize();
+
+    uint32_t num_queries() const { return static_cast<uint32_t>(qs_.size()); }
+    uint64_t num_batches() const;
+    uint32_t batch_size() const { return batch_size_; }
+    const Query& query(uint32_t q) const { return qs_[q]->q; }
+    const Plan& plan_of(uint32_t q) const { return qs_[q]->plan; }
+    bool planned(uint32_t q) const { return qs_[q]->planned; }
+    const std::vector<uint32_t>& cardinalities(uint32_t q) const { return qs_[q]->card; }
+    const std::vector<float>& avg_degrees(uint32_t q) const { return qs_[q]->avg; }
+    uint64_t pool_used() const { return pool_.used(); }
+    uint64_t pool_capacity() const { return pool_.capacity(); }
+    const DataGraph& graph() const { return host_g_; }
+    const UpdateStream& stream() const { return stream_; }
+    uint32_t dv() const { return dv_; }
+    // Read-back helpers for parity tests.
+    std::string read_candidates(uint32_t qid, uint32_t u, std::vector<uint32_t>& bitmap);
+    std::string read_adjacency(uint32_t v, uint32_t dir, std::vector<uint32_t>& ids, std::vector<uint32_t>& labels);
+    uint32_t id_bits() const { return id_bits_; }
+    uint32_t last_error_bits() const { return last_err_; }
+
+  private:
+    struct RelInfo {
+        uint32_t slot = 0, dir = 0, step = 0;
+        bool valid = false;
+    };
+    struct QState {
+        Query q;
+        Plan plan;
+        Technique tech;
+        bool planned = false;
+        bool matchable = true;  // every label and edge label of the query occurs in the data
+        std::vector<RelInfo> rel;           // per relation id
+        std::vector<uint32_t> vrank;        // per query vertex
+        uint32_t* cand = nullptr;           // n bitmaps of `words` words
+        std::vector<uint32_t> card;
+        std::vector<float> avg;
+        uint32_t entry0 = 0, entry1 = 0;    // NLF entries [entry0, entry1)
+    };
+
+    std::string upload_graph();
+    std::string upload_stream();
+    std::string alloc_runtime();
+    std::string run_phase(uint32_t ph);
+    std::string run_step(QState& qs, uint32_t i, uint32_t ph);
+    std::string finish_records(QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, uint32_t* rec,
+                               const uint32_t* rec_count, uint32_t res_slot);
+    uint32_t view_of(const QState& qs, uint32_t rel, uint32_t i, uint32_t ph) const;
+    uint32_t key_of_edge(uint32_t nbr, uint32_t elabel_raw) const;
+    std::string check_err(const char* where);
+    uint32_t read_u32(const uint32_t* p);
+
+    EngineOptions opt_;
+    hipStream_t s_ = nullptr;
+    DevicePool pool_;
+    bool built_ = false;
+
+    // host copies
+    DataGraph host_g_;
+    UpdateStream stream_;
+    uint32_t batch_size_ = 0;
+    std::vector<std::unique_ptr<QState>> qs_;
+
+    // label ranks
+    std::vector<std::pair<uint32_t, uint32_t>> vlabel_rank_, elabel_rank_;  // sorted (raw, rank)
+    uint32_t nvl_ = 0, nel_ = 0, id_bits_ = 1, dv_ = 0, words_ = 0;
+    int ndirs_ = 1;
+
+    // device state
+    DevGraph g_{};
+    BatchInput bin_{};
+    uint64_t stream_cap_ = 0;       // edges the device stream buffers hold
+    uint32_t max_batch_ = 0;        // edges per batch the scratch is sized for
+    BatchScratch bs_[2]{};
+    NlfEntry* d_entries_ = nullptr;
+    NlfGroup* d_groups_ = nullptr;
+    uint32_t n_entries_ = 0;
+    std::vector<NlfEntry> h_entries_;
+    std::vector<NlfGroup> h_groups_;
+    VSet L_[kMaxQV]{};
+    uint32_t* tmp_list_ = nullptr;
+    uint32_t* tmp_len_ = nullptr;
+    uint32_t* pairs_ = nullptr;
+    uint32_t* n_pairs_ = nullptr;
+    uint32_t* ring_ = nullptr;
+    uint64_t ring_words_ = 0;
+    uint32_t* counters_ = nullptr;  // [0]: ring A count, [1]: ring B count, [2]: overflow, [3]: work, [4]: list len
+    CpScratch cp_{};
+    unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash
+    unsigned long long* summary_ = nullptr;   // [2]
+    uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer
+    uint16_t L_used_ = 0;                     // query vertices whose local set holds data
+    uint32_t last_err_ = 0;
+};
+
+}  // namespace csm

@@ -1,6 +1,7 @@
 // csm_tool: host-side utilities around the matching path.
 //   csm_tool plan <query> <summary|-> <matching_order>   query metadata + plan dump + PrintOrders
 //   csm_tool load <query> <data> <update> <batch_size>    per-relation edge grouping dump
+//   csm_tool meta <query>                                  relation-id lines as SetQueryMeta prints them
 //   csm_tool gen  <config> <out_dir> [--seed S] [...]      seeded synthetic workload (see gen.h)
 #include <cstdio>
 #include <cstring>
@@ -37,6 +38,15 @@ static int cmd_plan(int argc, char** argv) {
     return 0;
 }
 
+static int cmd_meta(int argc, char** argv) {
+    if (argc != 3) return 2;
+    Query q;
+    std::string err = parse_query(argv[2], q);
+    if (!err.empty()) { std::fprintf(stderr, "%s\n", err.c_str()); return 1; }
+    std::cout << query_meta_lines(q);
+    return 0;
+}
+
 static int cmd_load(int argc, char** argv) {
     if (argc != 6) return 2;
     Query q;
@@ -61,6 +71,7 @@ int main(int argc, char** argv) {
     int rc = 2;
     if (c == "plan") rc = cmd_plan(argc, argv);
     else if (c == "load") rc = cmd_load(argc, argv);
+    else if (c == "meta") rc = cmd_meta(argc, argv);
     else if (c == "gen") rc = gen_main(argc - 1, argv + 1);
     if (rc == 2) std::fprintf(stderr, "bad arguments\n");
     return rc;

@@ -298,6 +298,9 @@ Spec spec_of(const std::string& c) {
     if (c == "tiny") {
         s = Spec{"tiny", 300, 1500, 2.3, 60, 3, false, 0.2, 0.1, 100, {}};
         s.queries = {{kTree, 4}, {kSparse, 5}, {kDense, 5}};
+    } else if (c == "tinyins") {
+        s = Spec{"tinyins", 200, 900, 2.3, 40, 3, false, 0.2, 0.0, 100, {}};
+        s.queries = {{kTree, 4}, {kSparse, 5}, {kDense, 5}};
     } else if (c == "c1") {
         s = Spec{"c1", 10000, 50000, 2.3, 1000, 8, false, 0.10, 0.0, 0xFFFFFFFFu, {}};
         s.queries = {{kC1, 5}};

@@ -34,6 +34,7 @@ std::string set_query_meta(Query& q) {
         for (const auto& e : q.nbrs[u]) {
             q.eidx[u * q.n + e.first] = static_cast<uint8_t>(pos);
             q.qv_nbrs[pos] = static_cast<uint8_t>(e.first);
+            q.rel_out[pos] = static_cast<uint8_t>((q.out_mask[u] >> e.first) & 1u);
             pos++;
         }
     }
@@ -135,6 +136,7 @@ std::string parse_stream(std::istream& is, Query& q) {
             const uint32_t l = it->second;
             for (const auto& e : q.nbrs[a])
                 if (e.first == b) return "duplicate query edge";
+            q.out_mask[a] |= static_cast<uint16_t>(1u << b);
             auto p = std::make_pair(b, l);
             q.nbrs[a].insert(std::lower_bound(q.nbrs[a].begin(), q.nbrs[a].end(), p), p);
             p = std::make_pair(a, l);
@@ -163,6 +165,15 @@ std::string parse_query(const std::string& path, Query& q) {
 std::string parse_query_text(const std::string& text, Query& q) {
     std::istringstream is(text);
     return parse_stream(is, q);
+}
+
+std::string query_meta_lines(const Query& q) {
+    std::ostringstream os;
+    for (uint32_t u = 0; u < q.n; u++) {
+        for (const auto& e : q.nbrs[u]) os << "[" << u << "," << e.first << "]: " << (uint32_t)q.eidx[u * q.n + e.first] << ' ';
+        os << '\n';
+    }
+    return os.str();
 }
 
 std::string dump_query_meta(const Query& q) {

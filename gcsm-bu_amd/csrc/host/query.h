@@ -34,6 +34,10 @@ struct Query {
     std::array<uint8_t, kMaxRel> nlf{};          // NLF count at the group's first relation
     std::array<uint8_t, kMaxRel> first_nl{};
     std::array<uint8_t, kMaxRel> last_nl{};
+    // rel_out[r] = 1 when the file lists relation r's edge as source -> destination (u -> w);
+    // only directed data graphs look at it.
+    std::array<uint8_t, kMaxRel> rel_out{};
+    std::array<uint16_t, kMaxQV> out_mask{};   // out_mask[a] bit b: the file has "e a b".
     // Step order of the undirected query edges: u ascending, then neighbour ascending, u <= w.
     std::array<std::pair<uint32_t, uint32_t>, kMaxQE> qe_list{};
     std::array<std::pair<uint32_t, uint32_t>, kMaxQE> qe_eidx{};  // (eidx[u][w], eidx[w][u])
@@ -58,5 +62,8 @@ std::string parse_query_text(const std::string& text, Query& q);
 
 // Writes the metadata block in the same layout as the oracle's reference driver dump.
 std::string dump_query_meta(const Query& q);
+// The relation-id lines SetQueryMeta prints ("[u,w]: r " per neighbour, one line per vertex;
+// graph.cpp:349-356).
+std::string query_meta_lines(const Query& q);
 
 }  // namespace csm

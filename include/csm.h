@@ -1,0 +1,111 @@
+/* C ABI of the MI355X continuous-subgraph-matching library (libcsm.so).
+ *
+ * The reference exposes this path only as the C++ class API of graph/graph.{h,cpp},
+ * graph/plan.{h,cpp} and graph/match_gpu.{h,cu}, driven by TechniquePermutationProcess
+ * (technique_permutation.h:29-225) and the csmg command line (main.cpp:162-189).  Each entry point
+ * below names the reference interface it replaces.  Conventions: opaque session handle, plain
+ * pointers and sizes, status codes instead of exit(), one GPU per session, calls on one session
+ * are serialised by the caller.  csm_last_error() returns the message of the last failure.
+ */
+#ifndef CSM_H_
+#define CSM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct csm_session csm_session;
+
+typedef enum {
+    CSM_OK = 0,
+    CSM_ERR_ARGUMENT = 1,
+    CSM_ERR_IO = 2,          /* missing or unreadable file (reference: exit, graph.cpp:198-202) */
+    CSM_ERR_FORMAT = 3,      /* malformed file, vertex line in an update file (graph.cpp:502-506) */
+    CSM_ERR_LIMIT = 4,       /* query over 16 vertices / 42 edges (graph.cpp:275-280), key width */
+    CSM_ERR_DEVICE = 5,      /* HIP runtime failure (reference: cudaErrorCheck -> exit) */
+    CSM_ERR_MEMORY = 6,      /* device pool budget or adjacency arena exhausted (match_gpu.cu:772-775) */
+    CSM_ERR_STATE = 7,       /* call out of order (e.g. plan before build) */
+    CSM_ERR_PRECONDITION = 8 /* batch inserts an existing edge or deletes a missing one */
+} csm_status;
+
+typedef struct {
+    int32_t device;          /* main.cpp:168 --device */
+    int32_t directed;        /* extension: arcs src -> dst only (the reference is undirected) */
+    uint64_t pool_bytes;     /* HBM budget for the device pool, 0 = no cap */
+    uint64_t ring_bytes;     /* partial-result ring (config.h:39 RES_SPACE), 0 = 1 GiB */
+    uint64_t cp_list_bytes;  /* Cartesian-product leaf lists (config.h:43 SIZE_SPACE), 0 = 1 GiB */
+    uint32_t max_batch_edges;/* largest batch the scratch is sized for, 0 = the stream's batch */
+    uint32_t semi_bfs_threshold; /* config.h:47 MIN_NUM_RESULTS_TO_GPU, 0 = 65536 */
+} csm_options;
+
+/* TechniqueOption (utils/technique_option.h:37-56); values as the csmg flags. */
+typedef struct {
+    int32_t matching_order;          /* 0 indexing, 1 Gamma, 2 GCSM (default), 3 RI */
+    int32_t search_strategy;         /* 0 DFS w/ stealing, 1 DFS w/o, 2 dynamic semi-BFS, 3 BFS */
+    int32_t local_index;             /* 1 / 0 */
+    int32_t cartesian_product;
+    int32_t heavy_vertex_avoidance;
+    int32_t relation_switch;
+} csm_technique;
+
+/* Per query per batch: the reference prints only the positive count, summed over batches
+ * (technique_permutation.h:224); deletions and hashes are extensions. */
+typedef struct {
+    uint64_t positive;
+    uint64_t negative;
+    uint64_t positive_hash;  /* sum over new embeddings of H(f), see DESIGN */
+    uint64_t negative_hash;
+} csm_query_result;
+
+typedef struct {
+    uint64_t edges, inserts, deletes;
+    double ms_total, ms_index, ms_match;
+} csm_batch_info;
+
+/* MatchGPU constructor + cudaSetDevice (match_gpu.cu:84-128, main.cpp:179). */
+csm_status csm_session_create(const csm_options* options, csm_session** out);
+void csm_session_destroy(csm_session* s);
+const char* csm_last_error(const csm_session* s);
+
+/* GraphFileIoManager::LoadInitial (graph.cpp:359-468). */
+csm_status csm_load_graph_file(csm_session* s, const char* path);
+/* GraphFileIoManager(query_path) + SetQueryMeta (graph.cpp:191-357).  Several queries may stand. */
+csm_status csm_load_query_file(csm_session* s, const char* path, uint32_t* query_id);
+/* GraphFileIoManager::LoadUpdate (graph.cpp:482-563); batch_size as --batch_size (main.cpp:172). */
+csm_status csm_open_update_file(csm_session* s, const char* path, uint32_t batch_size, uint64_t* num_batches);
+/* BuildTries(initial) + AllocRelations + UpdateGlobalIndex offline + GetSummary
+ * (technique_permutation.h:62-97, match_gpu.cu:336-377, 589-834). */
+csm_status csm_build_index(csm_session* s);
+/* PlanManager::GenerateIndexingOrders + GenerateMatchingOrders (plan.cpp:185-193, 2117-2146). */
+csm_status csm_plan(csm_session* s, uint32_t query_id, const csm_technique* technique);
+/* PlanManager::PrintOrders text (plan.cpp:1200-1291).  *needed receives the size incl. NUL. */
+csm_status csm_get_plan_text(csm_session* s, uint32_t query_id, char* buf, size_t cap, size_t* needed);
+/* MatchGPU::GetSummary output: cardinalities and average degrees per relation (match_gpu.cu:589-609). */
+csm_status csm_get_summary(csm_session* s, uint32_t query_id, uint32_t* cardinalities, float* avg_degrees, uint32_t cap,
+                           uint32_t* num_relations);
+/* One batch of the update file: the per-batch body of technique_permutation.h:127-215
+ * (BuildTries, UpdateGlobalIndex, BuildLocalIndex_v2 / GammaBuildLocalIndex, Match per query edge). */
+csm_status csm_apply_batch(csm_session* s, uint64_t batch_index, csm_query_result* results, uint32_t num_queries,
+                           csm_batch_info* info);
+/* Per-relation edge counts of the reference's grouping (GraphFileIoManager::PrintNumDataEdges,
+ * graph.cpp:470-480): batch_index = UINT64_MAX for the initial graph. */
+csm_status csm_get_relation_edge_counts(csm_session* s, uint32_t query_id, uint64_t batch_index, uint64_t* counts, uint32_t cap,
+                                        uint32_t* num_relations);
+/* Number of data vertices (DV_COUNT, graph.cpp:464) and of standing queries. */
+csm_status csm_get_counts(csm_session* s, uint32_t* num_vertices, uint32_t* num_queries);
+/* Device pool usage (HBM budget accounting). */
+csm_status csm_pool_usage(csm_session* s, uint64_t* used, uint64_t* capacity);
+/* Read-back for parity tests: candidate bitmap of query vertex u (global index), ceil(dv/32) words. */
+csm_status csm_read_candidates(csm_session* s, uint32_t query_id, uint32_t u, uint32_t* words, uint32_t cap);
+/* Read-back for parity tests: current neighbour ids of v (sorted by label run, then id). */
+csm_status csm_read_adjacency(csm_session* s, uint32_t v, uint32_t direction, uint32_t* ids, uint32_t cap, uint32_t* degree);
+csm_status csm_synchronize(csm_session* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSM_H_ */

@@ -1,0 +1,175 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle.
+
+Every batch's positive/negative counts and embedding-set hashes must be bit-identical to the
+oracle's anchored exactly-once counts (technique_permutation.h:127-215 semantics, deletions
+mirrored), for every search strategy, matching order and ablation switch; candidate bitmaps and the
+offline summary must equal the oracle's closed form, and the resident adjacency after the stream
+must equal the final graph.
+"""
+import itertools
+import os
+import struct
+import sys
+
+import pytest
+
+import util
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    util.ensure_built()
+    return util.ffi()
+
+
+def run_library(ffi, data, update, batch, queries, tech, **opts):
+    with ffi.Session(device=0, **opts) as s:
+        s.load_graph(data)
+        qids = [s.add_query(q) for q in queries]
+        nb = s.open_updates(update, batch)
+        s.build()
+        for q in qids:
+            try:
+                s.plan(q, tech)
+            except ffi.CsmError as e:
+                # the reference planner exits on this query/order (plan.cpp:1697-1698)
+                if "order generation" in str(e):
+                    return None
+                raise
+        out = {}
+        for b in range(nb):
+            res, _ = s.apply_batch(b)
+            for k, r in enumerate(res):
+                out[(b, k)] = r
+        return out
+
+
+TECHS = [
+    # (matching order, search strategy, local index, cartesian product)
+    (2, 2, True, True),    # GCSM-BU defaults (algorithm 1)
+    (1, 0, False, True),   # QO-GAMMA switches (algorithm 2)
+    (0, 2, False, True),   # algorithm 4
+    (1, 2, False, True),   # algorithm 5
+    (2, 0, True, True),
+    (2, 1, True, True),
+    (2, 3, True, True),
+    (2, 2, True, False),
+    (2, 3, True, False),
+    (0, 1, True, True),
+    (3, 2, True, True),
+    (3, 1, False, False),
+]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("batch", [100, 37])
+def test_tiny_stream_all_techniques(lib, tmp_path, seed, batch):
+    meta, queries = util.gen("tiny", tmp_path, seed=seed)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    want, _ = util.oracle_stream(data, upd, batch, queries)
+    for (mo, ss, li, cp) in TECHS:
+        say(f"case tiny seed={seed} batch={batch} order={mo} strategy={ss} local={li} cp={cp}")
+        tech = lib.technique(mo, ss, li, cp)
+        got = run_library(lib, str(data), str(upd), batch, queries, tech)
+        if got is None:
+            continue
+        assert got == want, f"order={mo} strategy={ss} local_index={li} cp={cp}"
+
+
+def test_c1_single_batch(lib, tmp_path):
+    meta, queries = util.gen("c1", tmp_path, seed=1)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = 0xFFFFFFFF
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=8)
+    for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 3, True, True), (0, 1, True, False)]:
+        say(f"case c1 order={mo} strategy={ss} local={li} cp={cp}")
+        got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp))
+        assert got == want
+    assert want[(0, 0)][0] > 0
+
+
+def test_c1_batch_size_invariance(lib, tmp_path):
+    """Summed positives telescope to |Emb(G_final)| - |Emb(G_0)| whatever the batch size."""
+    meta, queries = util.gen("c1", tmp_path, seed=2)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    final = tmp_path / "final.graph"
+    util.write_final_graph(data, upd, final)
+    c0, h0 = util.oracle_count(data, queries[0])
+    c1, h1 = util.oracle_count(final, queries[0])
+    for batch in [5000, 1000, 333]:
+        say(f"case c1 telescoping batch={batch}")
+        got = run_library(lib, str(data), str(upd), batch, queries, lib.technique())
+        pos = sum(v[0] - v[1] for v in got.values())
+        hsh = sum(v[2] - v[3] for v in got.values()) % (1 << 64)
+        assert pos == c1 - c0
+        assert hsh == (h1 - h0) % (1 << 64)
+
+
+def _oracle_summary(data, query):
+    out = util.run([util.ORACLE, "summary", str(data), str(query)])
+    card, avg = [], []
+    for line in out.splitlines():
+        r, c, bits = line.split()
+        card.append(int(c))
+        avg.append(struct.unpack("<f", struct.pack("<I", int(bits, 16)))[0])
+    return card, avg
+
+
+def test_summary_and_candidates(lib, tmp_path):
+    meta, queries = util.gen("tiny", tmp_path, seed=5)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    with lib.Session(device=0) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        s.open_updates(str(upd), 100)
+        s.build()
+        for q, path in zip(qids, queries):
+            say(f"case summary query={q}")
+            card, avg = s.summary(q)
+            wc, wa = _oracle_summary(data, path)
+            assert card == wc
+            assert [struct.pack("<f", x) for x in avg] == [struct.pack("<f", x) for x in wa]
+            out = util.run([util.ORACLE, "nlf", str(data), str(path)])
+            for line in out.splitlines():
+                _, u, _, cnt, _, h = line.split()
+                words = s.candidates(q, int(u))
+                ids = [w * 32 + b for w, x in enumerate(words) for b in range(32) if x >> b & 1]
+                hh = 1469598103934665603
+                for v in ids:
+                    hh ^= v
+                    hh = (hh * 1099511628211) % (1 << 64)
+                assert len(ids) == int(cnt)
+                assert hh == int(h, 16)
+
+
+def test_adjacency_after_stream(lib, tmp_path):
+    meta, queries = util.gen("tiny", tmp_path, seed=4)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    final = tmp_path / "final.graph"
+    util.write_final_graph(data, upd, final)
+    nbrs = {}
+    for line in open(final):
+        t = line.split()
+        if t and t[0] == "e":
+            a, b = int(t[1]), int(t[2])
+            nbrs.setdefault(a, set()).add(b)
+            nbrs.setdefault(b, set()).add(a)
+    with lib.Session(device=0) as s:
+        s.load_graph(str(data))
+        q = s.add_query(queries[0])
+        nb = s.open_updates(str(upd), 50)
+        s.build()
+        s.plan(q)
+        for b in range(nb):
+            s.apply_batch(b)
+        nv, _ = s.counts()
+        for v in range(nv):
+            got = s.adjacency(v)
+            assert sorted(got) == sorted(nbrs.get(v, ())), f"vertex {v}"
+            assert len(set(got)) == len(got)

@@ -1,0 +1,49 @@
+"""The C ABI library loads and exports exactly what include/csm.h declares (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import util
+
+
+def header_functions():
+    text = open(util.HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"\b(csm_[a-z_]+)\s*\(", text)
+
+
+def test_header_and_binding_agree():
+    ffi = util.ffi()
+    assert header_functions() == ffi.EXPORTED
+
+
+def test_library_exports_every_declared_symbol():
+    util.ensure_built()
+    lib = ctypes.CDLL(os.path.join(util.PKG, "lib", "libcsm.so"))
+    for name in header_functions():
+        assert hasattr(lib, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", os.path.join(util.PKG, "lib", "libcsm.so")],
+                        capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in nm.splitlines() if " T " in l and l.split()[-1].startswith("csm_")}
+    assert exported == set(header_functions())
+
+
+def test_session_without_gpu_fails_cleanly():
+    import pytest
+    ffi = util.ffi()
+    try:
+        s = ffi.Session(device=0)
+    except ffi.CsmError as e:
+        assert e.status == 5  # CSM_ERR_DEVICE
+        return
+    s.close()
+    pytest.skip("a GPU is present")
+
+
+def test_csmg_reports_errors_cleanly(tmp_path):
+    util.ensure_built()
+    r = subprocess.run([util.CSMG, "--query", str(tmp_path / "missing"), "--data", "x", "--update", "y"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "error (" in r.stdout

@@ -1,0 +1,131 @@
+"""The oracle, pinned before it is trusted.
+
+count:   brute force vs closed forms computed independently with numpy (edges, wedges, stars,
+         triangles = trace(A^3), 4-cycles from trace(A^4)), and the labelled edge count.
+stream:  per-batch anchored counts telescope to brute-force differences, with and without
+         deletions, and the sum does not depend on the batch size.
+summary: closed form == literal replay of the reference's offline UpdateGlobalIndex.
+A planted mistake (a dropped query edge) is caught by the closed forms.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import util
+
+
+def write_graph(path, n, edges, labels=None, elabels=None):
+    with open(path, "w") as f:
+        for v in range(n):
+            f.write(f"v {v} {labels[v] if labels else 0}\n")
+        for k, (a, b) in enumerate(edges):
+            f.write(f"e {a} {b} {elabels[k] if elabels else 0}\n")
+
+
+def write_query(path, n, edges, labels=None):
+    write_graph(path, n, edges, labels)
+
+
+def random_graph(rng, n, p):
+    edges = [(a, b) for a in range(n) for b in range(a + 1, n) if rng.random() < p]
+    A = np.zeros((n, n), dtype=np.int64)
+    for a, b in edges:
+        A[a, b] = A[b, a] = 1
+    return edges, A
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_closed_forms(tmp_path, seed):
+    util.ensure_built()
+    rng = np.random.default_rng(seed)
+    n = 14
+    edges, A = random_graph(rng, n, 0.35)
+    g = tmp_path / "g.graph"
+    write_graph(g, n, edges)
+    d = A.sum(1)
+    cases = {
+        "edge": (2, [(0, 1)], 2 * len(edges)),
+        "wedge": (3, [(0, 1), (0, 2)], int((d * (d - 1)).sum())),
+        "star3": (4, [(0, 1), (0, 2), (0, 3)], int((d * (d - 1) * (d - 2)).sum())),
+        "triangle": (3, [(0, 1), (1, 2), (0, 2)], int(np.trace(A @ A @ A))),
+        "4cycle": (4, [(0, 1), (1, 2), (2, 3), (0, 3)],
+                   int(np.trace(A @ A @ A @ A) - 2 * (d * d).sum() + 2 * len(edges))),
+    }
+    for name, (qn, qe, want) in cases.items():
+        q = tmp_path / f"{name}.graph"
+        write_query(q, qn, qe)
+        got, _ = util.oracle_count(g, q)
+        assert got == want, name
+
+
+def test_labelled_edge_count(tmp_path):
+    util.ensure_built()
+    rng = np.random.default_rng(5)
+    n = 30
+    edges, _ = random_graph(rng, n, 0.2)
+    labels = [int(x) for x in rng.integers(0, 3, n)]
+    g = tmp_path / "g.graph"
+    write_graph(g, n, edges, labels)
+    q = tmp_path / "q.graph"
+    write_query(q, 2, [(0, 1)], [0, 1])
+    want = sum(1 for a, b in edges if {labels[a], labels[b]} == {0, 1})
+    assert util.oracle_count(g, q)[0] == want
+
+
+def test_planted_mistake_is_caught(tmp_path):
+    """Counting a 4-cycle with one query edge dropped (a 4-path) must disagree with the closed form."""
+    util.ensure_built()
+    rng = np.random.default_rng(1)
+    edges, A = random_graph(rng, 12, 0.4)
+    g = tmp_path / "g.graph"
+    write_graph(g, 12, edges)
+    d = A.sum(1)
+    c4 = int(np.trace(A @ A @ A @ A) - 2 * (d * d).sum() + 2 * len(edges))
+    q = tmp_path / "q.graph"
+    write_query(q, 4, [(0, 1), (1, 2), (2, 3)])
+    assert util.oracle_count(g, q)[0] != c4
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_stream_telescopes(tmp_path, seed):
+    util.ensure_built()
+    meta, queries = util.gen("tiny", tmp_path, seed=seed)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    final = tmp_path / "final.graph"
+    util.write_final_graph(data, upd, final)
+    sums = {}
+    for batch in [100, 23, 10 ** 9]:
+        res, _ = util.oracle_stream(data, upd, batch, queries)
+        for k, q in enumerate(queries):
+            pos = sum(v[0] - v[1] for (b, kk), v in res.items() if kk == k)
+            hsh = sum(v[2] - v[3] for (b, kk), v in res.items() if kk == k) % (1 << 64)
+            sums.setdefault(k, set()).add((pos, hsh))
+    for k, q in enumerate(queries):
+        c0, h0 = util.oracle_count(data, q)
+        c1, h1 = util.oracle_count(final, q)
+        assert sums[k] == {(c1 - c0, (h1 - h0) % (1 << 64))}
+
+
+def test_summary_closed_form_equals_literal_replay(tmp_path):
+    util.ensure_built()
+    meta, queries = util.gen("tiny", tmp_path, seed=3)
+    for q in queries:
+        a = util.run([util.ORACLE, "summary", str(tmp_path / "data.graph"), q])
+        b = util.run([util.ORACLE, "summary", str(tmp_path / "data.graph"), q, "--literal"])
+        assert a == b
+
+
+def test_directed_telescopes(tmp_path):
+    """Directed extension: per-batch anchored counts telescope on a directed reading of the files."""
+    util.ensure_built()
+    meta, queries = util.gen("tiny", tmp_path, seed=6)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    final = tmp_path / "final.graph"
+    util.write_final_graph(data, upd, final)
+    res, _ = util.oracle_stream(data, upd, 50, queries, directed=True)
+    for k, q in enumerate(queries):
+        pos = sum(v[0] - v[1] for (b, kk), v in res.items() if kk == k)
+        c0, _ = util.oracle_count(data, q, directed=True)
+        c1, _ = util.oracle_count(final, q, directed=True)
+        assert pos == c1 - c0

@@ -16,7 +16,7 @@ EXPORTED = [
     "csm_load_query_file", "csm_open_update_file", "csm_build_index", "csm_plan",
     "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_get_relation_edge_counts",
     "csm_get_counts", "csm_pool_usage", "csm_read_candidates", "csm_read_adjacency",
-    "csm_synchronize",
+    "csm_synchronize", "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
 ]
 
 STATUS = {0: "OK", 1: "ARGUMENT", 2: "IO", 3: "FORMAT", 4: "LIMIT", 5: "DEVICE", 6: "MEMORY", 7: "STATE",
@@ -81,6 +81,9 @@ def load():
         lib.csm_read_candidates.argtypes = [vp, u32, u32, P(u32), u32]
         lib.csm_read_adjacency.argtypes = [vp, u32, u32, P(u32), u32, P(u32)]
         lib.csm_synchronize.argtypes = [vp]
+        lib.csm_profile_enable.argtypes = [vp, u32]
+        lib.csm_profile_read.argtypes = [vp, u32, P(u64), P(ctypes.c_double), P(u64)]
+        lib.csm_calibration_copy.argtypes = [vp, u64, P(ctypes.c_double)]
         for name in EXPORTED:
             f = getattr(lib, name)
             if f.restype is ctypes.c_int:
@@ -189,6 +192,22 @@ class Session:
         buf = (ctypes.c_uint32 * max(1, deg.value))()
         self._check(self.lib.csm_read_adjacency(self.h, v, direction, buf, deg.value, ctypes.byref(deg)))
         return list(buf[:deg.value])
+
+    def synchronize(self):
+        self._check(self.lib.csm_synchronize(self.h))
+
+    def profile_enable(self, mask):
+        self._check(self.lib.csm_profile_enable(self.h, mask))
+
+    def profile_read(self, family):
+        n, ms, b = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
+        self._check(self.lib.csm_profile_read(self.h, family, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
+        return n.value, ms.value, b.value
+
+    def calibration_copy(self, words):
+        ms = ctypes.c_double()
+        self._check(self.lib.csm_calibration_copy(self.h, words, ctypes.byref(ms)))
+        return ms.value
 
     def pool_usage(self):
         u, c = ctypes.c_uint64(), ctypes.c_uint64()

@@ -243,6 +243,31 @@ csm_status csm_read_adjacency(csm_session* s, uint32_t v, uint32_t dir, uint32_t
     return CSM_OK;
 }
 
+csm_status csm_profile_enable(csm_session* s, uint32_t mask) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.profile_enable(mask));
+}
+
+csm_status csm_profile_read(csm_session* s, uint32_t family, uint64_t* launches, double* ms, uint64_t* bytes) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    Engine::KernelStat st;
+    std::string err = s->engine.profile_read(family, st);
+    if (!err.empty()) return classify(s, err);
+    if (launches) *launches = st.launches;
+    if (ms) *ms = st.ms;
+    if (bytes) *bytes = st.bytes;
+    return CSM_OK;
+}
+
+csm_status csm_calibration_copy(csm_session* s, uint64_t words, double* ms) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    double m = 0;
+    std::string err = s->engine.calibration_copy(words, m);
+    if (!err.empty()) return classify(s, err);
+    if (ms) *ms = m;
+    return CSM_OK;
+}
+
 csm_status csm_synchronize(csm_session* s) {
     if (!s) return CSM_ERR_ARGUMENT;
     return classify(s, s->engine.synchronize());

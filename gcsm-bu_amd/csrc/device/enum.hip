@@ -35,7 +35,9 @@ __device__ __forceinline__ const uint32_t* entry_delta(const DevGraph& g, const 
 }
 
 // Computes the view runs of every backward entry of `depth` and resets its frame.
-__device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d, WaveState& W, uint32_t depth, uint32_t lane) {
+// Algorithmic bytes of a frame: every view run of the depth's backward lists read once.
+__device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d, WaveState& W, uint32_t depth, uint32_t lane,
+                                            unsigned long long& bytes) {
     const uint32_t e0 = d.boff[depth], e1 = d.boff[depth + 1];
     if (lane < e1 - e0) {
         const BackEntry e = d.be[e0 + lane];
@@ -43,6 +45,7 @@ __device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d
     }
     wave_sync();
     if (lane == 0) {
+        for (uint32_t e = e0; e < e1; e++) bytes += 4ull * ((W.run[e].hi - W.run[e].lo) + (W.run[e].dhi - W.run[e].dlo));
         const uint32_t p = d.pivot[depth];
         const RunView r = W.run[p];
         W.nb[depth] = run_base_len(r, d.be[p].view);
@@ -80,7 +83,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
     const uint32_t nrec = min(*r.in_count, r.in_max);
     const uint32_t total_waves = gridDim.x * kWaves;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
-    unsigned long long my_cnt = 0, my_hash = 0;
+    unsigned long long my_cnt = 0, my_hash = 0, my_bytes = 0;
 
     uint32_t item;
     if (schedule == 0) item = gw;
@@ -100,7 +103,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
         }
         wave_sync();
         uint32_t depth = d0;
-        setup_frame(g, d, W, depth, lane);
+        setup_frame(g, d, W, depth, lane, my_bytes);
         while (true) {
             if (W.bpos[depth] < W.bcnt[depth]) {
                 const uint32_t c = W.buf[depth][W.bpos[depth]];
@@ -112,7 +115,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
                 }
                 wave_sync();
                 depth += 1;
-                setup_frame(g, d, W, depth, lane);
+                setup_frame(g, d, W, depth, lane, my_bytes);
                 continue;
             }
             if (W.cur[depth] >= W.tot[depth]) {
@@ -128,7 +131,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
             const uint32_t c = key & g.id_mask;
             const uint64_t m = __ballot(ok);
             wave_sync();
-            if (lane == 0) W.cur[depth] += 64;
+            if (lane == 0) { W.cur[depth] += 64; my_bytes += 4ull * static_cast<uint32_t>(__popcll(m)); }
             if (depth + 1 == end_depth) {
                 if (kMode == kCountAtEnd) {
                     if (ok) {
@@ -166,6 +169,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
         my_hash = wave_sum_u64(my_hash);
         if (lane == 0 && my_cnt) { atomicAdd(out.count, my_cnt); atomicAdd(out.hash, my_hash); }
     }
+    if (out.bytes && lane == 0 && my_bytes) atomicAdd(out.bytes, my_bytes);
 }
 
 __global__ void k_init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t max_pairs, int swap, uint32_t* out,

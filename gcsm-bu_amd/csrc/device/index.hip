@@ -188,6 +188,11 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) p[i] = v;
 }
 
+__global__ void k_copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 int grid_for(uint64_t n, int threads, int cap) {
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(cap), (n + threads - 1) / threads)));
 }
@@ -236,6 +241,10 @@ void clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uin
 void copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len, hipStream_t s) {
     k_copy_list<<<grid_for(max_len, 256, 4096), 256, 0, s>>>(in, in_len, max_len, out, out_len);
 }
+void copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n, hipStream_t s) {
+    k_copy_u32<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, dst, n);
+}
+
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
     k_fill_u32<<<grid_for(n, 256, 8192), 256, 0, s>>>(p, v, n);
 }

@@ -105,6 +105,7 @@ void copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uin
 struct MatchOut {
     unsigned long long* count;  // [1]
     unsigned long long* hash;   // [1]
+    unsigned long long* bytes;  // [1] algorithmic bytes (lists read + candidates written), or null
 };
 
 struct Ring {
@@ -155,5 +156,7 @@ void cp_enumerate(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint3
                   uint32_t max_rec, CpScratch c, MatchOut out, hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// Plain copy of n words (calibration of the HBM traffic counters).
+void copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n, hipStream_t s);
 
 }  // namespace csm

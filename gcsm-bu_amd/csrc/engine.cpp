@@ -46,6 +46,7 @@ void* DevicePool::take_bytes(uint64_t bytes) {
 // ---- engine -----------------------------------------------------------------------------------
 Engine::~Engine() {
     if (s_) (void)hipStreamSynchronize(s_);
+    for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
     if (s_) (void)hipStreamDestroy(s_);
 }
@@ -196,6 +197,7 @@ std::string Engine::build() {
     add(cp_rec * 4); add(cp_rec * kMaxQV * 4); add(cp_list_words * 4); add(64);
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
     add(qs_.size() * 32 + 64);
+    add(kKFamilies * 8);
     add(kMaxQV * qs_.size() * sizeof(NlfEntry) + kMaxRel * qs_.size() * sizeof(NlfGroup) + 512);
     need += 1 << 20;
     if (opt_.pool_bytes && need > opt_.pool_bytes)
@@ -346,6 +348,8 @@ std::string Engine::build() {
     cp_.max_prod = cpc + 1;
     opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
     results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);
+    prof_bytes_ = pool_.take<unsigned long long>(kKFamilies);
+    CSM_HIP(hipMemsetAsync(prof_bytes_, 0, kKFamilies * 8, s_));
     summary_ = pool_.take<unsigned long long>(4);
 
     // ---- queries: relation slots, NLF programs, candidate bitmaps ----
@@ -475,8 +479,10 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
     fill_u32(L_[0].len, 0u, kMaxQV + 2, s_);
 
     const RelInfo& R = qs.rel[r01];
+    prof_begin(kKAnchors);
     local_anchors(g_, ph, R.dir, R.slot, qs.vrank[u0], cand(u0), cand(u1), bs_[R.dir].touched, bs_[R.dir].n_touched, max_arcs, pairs_,
                   n_pairs_, max_arcs, L_[u0], L_[u1], s_);
+    prof_end(kKAnchors);
     L_used_ |= static_cast<uint16_t>((1u << u0) | (1u << u1));
     const uint32_t np = read_u32(n_pairs_);
     if (np == 0) return "empty";
@@ -500,6 +506,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
             }
             const uint32_t b0 = bsv[first];
             const uint32_t rb = q.rel(b0, u);
+            prof_begin(kKLocalIndex);
             local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, L_[b0].list, L_[b0].len, dv_, cand(u), L_[u], s_);
             L_used_ |= static_cast<uint16_t>(1u << u);
             for (size_t k = 0; k < bsv.size(); k++) {
@@ -513,6 +520,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
                 copy_list(tmp_list_, tmp_len_, dv_, L_[u].list, L_[u].len, s_);
                 set_bits(L_[u].list, L_[u].len, dv_, L_[u].bitmap, s_);
             }
+            prof_end(kKLocalIndex);
             built |= 1u << u;
         }
         CSM_HIP(hipMemcpyAsync(h_pinned_, L_[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, s_));
@@ -586,7 +594,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
     if (count == 0) return "";
     const uint32_t res_slot = static_cast<uint32_t>(std::find_if(qs_.begin(), qs_.end(), [&](const std::unique_ptr<QState>& x) { return x.get() == &qs; }) - qs_.begin());
     if (q.n == 2) {
-        count_records(d, A, cA, count, MatchOut{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1}, s_);
+        count_records(d, A, cA, count, MatchOut{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr}, s_);
         return "";
     }
 
@@ -601,7 +609,9 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
         fill_u32(oth_cnt, 0u, 1, s_);
         fill_u32(ovf, 0u, 2, s_);
         Ring r{cur, cur_cnt, count, oth, oth_cnt, static_cast<uint32_t>(std::min<uint64_t>(half / end, 0xFFFFFFFFull)), ovf};
-        extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr}, work, sched, s_);
+        prof_begin(kKExtendWrite);
+        extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, work, sched, s_);
+        prof_end(kKExtendWrite);
         CSM_HIP(hipMemcpyAsync(h_pinned_, counters_, 12, hipMemcpyDeviceToHost, s_));
         CSM_HIP(hipStreamSynchronize(s_));
         ok = h_pinned_[2] == 0;
@@ -670,13 +680,15 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
 std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, uint32_t* rec,
                                    const uint32_t* rec_count, uint32_t res_slot) {
     const uint32_t ph = d.phase;
-    MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1};
+    MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr};
     const int sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
     bool use_cp = qs.tech.cartesian_product && depth < d.n && qs.plan.cp_info[i][depth] == kCpTreeProduct && count <= opt_.cp_max_records;
     if (use_cp) {
         // match_gpu.cu:2046-2085: flatten unless the products are uniformly large
         CSM_HIP(hipMemsetAsync(cp_.lists_top, 0, 16, s_));
+        prof_begin(kKCpPrepare);
         cp_prepare(g_, d, depth, rec, rec_count, count, cp_, s_);
+        prof_end(kKCpPrepare);
         unsigned long long v[2];
         CSM_HIP(hipMemcpyAsync(h_pinned_, cp_.max_prod, 8, hipMemcpyDeviceToHost, s_));
         CSM_HIP(hipMemcpyAsync(h_pinned_ + 2, cp_.prod + count, 8, hipMemcpyDeviceToHost, s_));
@@ -693,13 +705,18 @@ std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, ui
             if (ratio < 50.f && avg >= static_cast<float>(kSemiBfsThreshold)) use_cp = false;
         }
         if (use_cp) {
+            prof_begin(kKCpEnumerate);
             cp_enumerate(g_, d, depth, rec, rec_count, count, cp_, out, s_);
+            prof_end(kKCpEnumerate);
             return "";
         }
     }
     fill_u32(counters_ + 3, 0u, 1, s_);
     Ring r{rec, rec_count, count, nullptr, nullptr, 0, nullptr};
+    out.bytes = bytes_ptr(kKExtendCount);
+    prof_begin(kKExtendCount);
     extend(g_, d, depth, d.n, kCountAtEnd, r, out, counters_ + 3, sched, s_);
+    prof_end(kKExtendCount);
     return "";
 }
 
@@ -727,7 +744,9 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t max_arcs = static_cast<uint32_t>(2 * n);
     if (n == 0) { out.assign(qs_.size(), QueryResult{}); return ""; }
+    prof_begin(kKBatchBuild);
     for (int d = 0; d < ndirs_; d++) batch_build(g_, d, ndirs_ == 1 ? 0 : d + 1, bin_, static_cast<uint32_t>(lo), static_cast<uint32_t>(n), bs_[d], s_);
+    prof_end(kKBatchBuild);
     for (int d = 0; d < ndirs_; d++)
         nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 0, s_);
     fill_u32(reinterpret_cast<uint32_t*>(results_), 0u, qs_.size() * 8, s_);
@@ -742,7 +761,9 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
         if (!err.empty()) return err;
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+        prof_begin(kKApply);
         for (int d = 0; d < ndirs_; d++) batch_apply_deletions(g_, d, max_arcs, bs_[d], s_);
+        prof_end(kKApply);
     }
     if (info.inserts) {
         CSM_HIP(hipStreamSynchronize(s_));
@@ -751,7 +772,9 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
         if (!err.empty()) return err;
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+        prof_begin(kKApply);
         for (int d = 0; d < ndirs_; d++) batch_apply_insertions(g_, d, max_arcs, bs_[d], s_);
+        prof_end(kKApply);
     }
     for (int d = 0; d < ndirs_; d++)
         nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 1, s_);
@@ -760,6 +783,8 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
     CSM_HIP(hipMemcpyAsync(res.data(), results_, res.size() * 8, hipMemcpyDeviceToHost, s_));
     CSM_HIP(hipStreamSynchronize(s_));
     err = check_err("batch apply");
+    if (!err.empty()) return err;
+    err = prof_collect();
     if (!err.empty()) return err;
     out.assign(qs_.size(), QueryResult{});
     for (size_t q = 0; q < qs_.size(); q++) {
@@ -779,6 +804,87 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
 std::string Engine::run_edges(const uint32_t*, const uint32_t*, const uint32_t*, const uint8_t*, uint64_t, std::vector<QueryResult>&,
                               BatchInfo&) {
     return "in-memory batches are not supported yet";
+}
+
+void Engine::prof_begin(uint32_t family) {
+    if (!(prof_mask_ >> family & 1u)) return;
+    if (ev_used_ + 2 > ev_pool_.size()) {
+        for (int k = 0; k < 64; k++) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            ev_pool_.push_back(e);
+        }
+    }
+    hipEvent_t a = ev_pool_[ev_used_++];
+    (void)hipEventRecord(a, s_);
+    ev_open_[family] = a;
+}
+
+void Engine::prof_end(uint32_t family) {
+    if (!(prof_mask_ >> family & 1u) || !ev_open_[family]) return;
+    hipEvent_t b = ev_pool_[ev_used_++];
+    (void)hipEventRecord(b, s_);
+    ev_pending_.push_back({family, {ev_open_[family], b}});
+    ev_open_[family] = nullptr;
+}
+
+std::string Engine::prof_collect() {
+    if (!prof_mask_) return "";
+    CSM_HIP(hipStreamSynchronize(s_));
+    for (const auto& p : ev_pending_) {
+        float ms = 0;
+        CSM_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        kstat_[p.first].launches += 1;
+        kstat_[p.first].ms += ms;
+    }
+    ev_pending_.clear();
+    ev_used_ = 0;
+    return "";
+}
+
+std::string Engine::profile_enable(uint32_t mask) {
+    if (!built_) return "build the index before profiling";
+    std::string err = prof_collect();
+    if (!err.empty()) return err;
+    prof_mask_ = mask;
+    for (auto& k : kstat_) k = KernelStat{};
+    CSM_HIP(hipMemsetAsync(prof_bytes_, 0, kKFamilies * 8, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    return "";
+}
+
+std::string Engine::profile_read(uint32_t family, KernelStat& st) {
+    if (family >= kKFamilies) return "kernel family out of range";
+    std::string err = prof_collect();
+    if (!err.empty()) return err;
+    unsigned long long b[kKFamilies];
+    CSM_HIP(hipMemcpyAsync(b, prof_bytes_, sizeof b, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    st = kstat_[family];
+    st.bytes = b[family];
+    return "";
+}
+
+std::string Engine::calibration_copy(uint64_t words, double& ms) {
+    uint32_t *a = nullptr, *b = nullptr;
+    CSM_HIP(hipMalloc(reinterpret_cast<void**>(&a), words * 4));
+    CSM_HIP(hipMalloc(reinterpret_cast<void**>(&b), words * 4));
+    hipEvent_t e0, e1;
+    CSM_HIP(hipEventCreate(&e0));
+    CSM_HIP(hipEventCreate(&e1));
+    fill_u32(a, 1u, words, s_);
+    CSM_HIP(hipEventRecord(e0, s_));
+    copy_u32(a, b, words, s_);
+    CSM_HIP(hipEventRecord(e1, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    float f = 0;
+    CSM_HIP(hipEventElapsedTime(&f, e0, e1));
+    ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return "";
 }
 
 std::string Engine::synchronize() {

@@ -34,7 +34,7 @@ struct EngineOptions {
     uint64_t pool_bytes = 0;          // HBM budget for the device pool (0: no cap)
     uint64_t ring_bytes = 1ull << 30; // partial-result ring
     uint64_t cp_list_bytes = 1ull << 30;
-    uint32_t cp_max_records = 1u << 24;
+    uint32_t cp_max_records = 1u << 22;
     uint32_t max_batch_edges = 0;     // 0: the stream's batch size
     uint32_t semi_bfs_threshold = kSemiBfsThreshold;
     bool directed = false;
@@ -106,6 +106,18 @@ class Engine {
     std::string read_adjacency(uint32_t v, uint32_t dir, std::vector<uint32_t>& ids, std::vector<uint32_t>& labels);
     uint32_t id_bits() const { return id_bits_; }
     uint32_t last_error_bits() const { return last_err_; }
+
+    // Kernel families timed with HIP events on the engine's stream when profiling is enabled.
+    enum KernelFamily : uint32_t { kKExtendCount = 0, kKExtendWrite, kKCpEnumerate, kKLocalIndex, kKBatchBuild, kKApply, kKCpPrepare,
+                                   kKAnchors, kKFamilies };
+    struct KernelStat {
+        uint64_t launches = 0;
+        double ms = 0;
+        uint64_t bytes = 0;
+    };
+    std::string profile_enable(uint32_t mask);
+    std::string profile_read(uint32_t family, KernelStat& st);
+    std::string calibration_copy(uint64_t words, double& ms);
 
   private:
     struct RelInfo {
@@ -179,6 +191,19 @@ class Engine {
     uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer
     uint16_t L_used_ = 0;                     // query vertices whose local set holds data
     uint32_t last_err_ = 0;
+
+    // profiling
+    void prof_begin(uint32_t family);
+    void prof_end(uint32_t family);
+    std::string prof_collect();
+    uint32_t prof_mask_ = 0;
+    KernelStat kstat_[kKFamilies];
+    std::vector<hipEvent_t> ev_pool_;
+    size_t ev_used_ = 0;
+    std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> ev_pending_;
+    hipEvent_t ev_open_[kKFamilies]{};
+    unsigned long long* prof_bytes_ = nullptr;  // device counters per family
+    unsigned long long* bytes_ptr(uint32_t family) { return (prof_mask_ >> family & 1u) ? prof_bytes_ + family : nullptr; }
 };
 
 }  // namespace csm

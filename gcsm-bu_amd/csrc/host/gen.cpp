@@ -8,6 +8,7 @@
 #include <cstring>
 #include <string>
 #include <sys/stat.h>
+#include <functional>
 #include <unordered_set>
 #include <vector>
 
@@ -172,10 +173,16 @@ struct QueryOut {
 
 // Random walk with restarts from visited vertices (so trees branch); returns the vertices and the
 // first-visit (tree) edges, or false.
-bool walk(const Csr& g, uint32_t n_target, const std::vector<uint64_t>& edge_keys, Rng& rng,
+bool walk(const Csr& g, uint32_t n_target, const std::vector<uint64_t>& edge_keys, Rng& rng, bool uniform_start,
           std::vector<uint32_t>& vs, std::vector<std::pair<uint32_t, uint32_t>>& tree) {
     const uint64_t key = edge_keys[rng.below(edge_keys.size())];
     uint32_t cur = static_cast<uint32_t>(key >> 32);
+    if (uniform_start) {  // a uniformly chosen vertex keeps walks out of the hub core
+        for (int t = 0; t < 64; t++) {
+            const uint32_t v = static_cast<uint32_t>(rng.below(g.off.size() - 1));
+            if (g.deg(v) >= 1) { cur = v; break; }
+        }
+    }
     vs.assign(1, cur);
     tree.clear();
     for (uint32_t step = 0; step < 64 * n_target && vs.size() < n_target; step++) {
@@ -232,7 +239,7 @@ bool extract(const Csr& g, const std::vector<uint32_t>& vlab, uint32_t nq, Shape
             }
             return true;
         }
-        if (!walk(g, nq, edge_keys, rng, vs, tree)) continue;
+        if (!walk(g, nq, edge_keys, rng, shape != kDense, vs, tree)) continue;
         std::vector<std::pair<uint32_t, uint32_t>> extra;
         for (uint32_t i = 0; i < nq; i++)
             for (uint32_t j = i + 1; j < nq; j++) {
@@ -269,6 +276,105 @@ bool extract(const Csr& g, const std::vector<uint32_t>& vlab, uint32_t nq, Shape
     return false;
 }
 
+// Estimated number of embeddings of q in g (injective, label and edge-label preserving): exact
+// completion counts from `samples` uniformly drawn data arcs standing for query edge 0, scaled by the
+// number of arcs.  Returns false when one sample needs more than node_budget search nodes.
+bool estimate_embeddings(const Csr& g, const std::vector<uint32_t>& vlab, const QueryOut& q, Rng& rng, uint32_t samples,
+                         uint64_t node_budget, double& est) {
+    const uint32_t n = static_cast<uint32_t>(q.verts.size());
+    std::vector<uint32_t> qlab(n);
+    for (uint32_t i = 0; i < n; i++) qlab[i] = vlab[q.verts[i]];
+    // BFS order from query edge 0; back[k] = earlier neighbours (position, edge label)
+    std::vector<uint32_t> order{q.edges[0].a, q.edges[0].b}, pos(n, UINT32_MAX);
+    pos[order[0]] = 0;
+    pos[order[1]] = 1;
+    for (size_t k = 0; k < order.size(); k++)
+        for (const auto& e : q.edges) {
+            uint32_t x = UINT32_MAX;
+            if (e.a == order[k]) x = e.b; else if (e.b == order[k]) x = e.a;
+            if (x != UINT32_MAX && pos[x] == UINT32_MAX) { pos[x] = static_cast<uint32_t>(order.size()); order.push_back(x); }
+        }
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> back(n);
+    for (uint32_t k = 1; k < n; k++)
+        for (const auto& e : q.edges) {
+            const uint32_t u = order[k];
+            if (e.a == u && pos[e.b] < k) back[k].emplace_back(pos[e.b], e.l);
+            else if (e.b == u && pos[e.a] < k) back[k].emplace_back(pos[e.a], e.l);
+        }
+    const uint64_t arcs = g.nbr.size();
+    std::vector<uint32_t> f(n);
+    uint64_t nodes = 0;
+    bool over = false;
+    double total = 0;
+    std::function<double(uint32_t)> rec = [&](uint32_t k) -> double {
+        if (k == n) return 1.0;
+        if (++nodes > node_budget) { over = true; return 0.0; }
+        const auto& b0 = back[k][0];
+        const uint32_t p = f[b0.first];
+        double c = 0;
+        for (uint64_t i = g.off[p]; i < g.off[p + 1] && !over; i++) {
+            const uint32_t x = g.nbr[i];
+            if (g.lab[i] != b0.second || vlab[x] != qlab[order[k]]) continue;
+            bool ok = true;
+            for (uint32_t j = 0; j < k && ok; j++) ok = f[j] != x;
+            for (size_t t = 1; t < back[k].size() && ok; t++) {
+                const int64_t e = g.find(f[back[k][t].first], x);
+                ok = e >= 0 && g.lab[e] == back[k][t].second;
+            }
+            if (!ok) continue;
+            f[k] = x;
+            c += rec(k + 1);
+        }
+        return c;
+    };
+    uint64_t spent = 0;
+    const uint64_t total_budget = 25 * node_budget;
+    auto completions = [&](uint64_t a, double& c) {
+        const uint32_t x = g.nbr[a];
+        const uint32_t v = static_cast<uint32_t>(std::upper_bound(g.off.begin(), g.off.end(), a) - g.off.begin() - 1);
+        c = 0;
+        if (vlab[v] != qlab[order[0]] || vlab[x] != qlab[order[1]] || g.lab[a] != q.edges[0].l) return true;
+        f[0] = v;
+        f[1] = x;
+        nodes = 0;
+        c = rec(2);
+        spent += nodes;
+        return !over && spent <= total_budget;
+    };
+    // Stratified by log2 of the source degree: every stratum gets the same number of samples, so
+    // the hub-anchored arcs that dominate heavy queries are well represented; each stratum's mean is
+    // weighted by its number of arcs.
+    const uint32_t nv = static_cast<uint32_t>(g.off.size() - 1);
+    std::vector<std::vector<uint32_t>> strata(40);
+    for (uint32_t v = 0; v < nv; v++) {
+        const uint64_t d = g.deg(v);
+        if (!d || vlab[v] != qlab[order[0]]) continue;
+        uint32_t b = 0;
+        while ((2ull << b) <= d) b++;
+        strata[b].push_back(v);
+    }
+    for (const auto& st : strata) {
+        if (st.empty()) continue;
+        uint64_t st_arcs = 0;
+        for (uint32_t v : st) st_arcs += g.deg(v);
+        const uint32_t per = std::max<uint32_t>(8, samples / 12);
+        double sum = 0;
+        for (uint32_t k = 0; k < per; k++) {
+            // uniform arc of the stratum: vertex by degree via rejection against the stratum's max
+            uint32_t v = st[rng.below(st.size())];
+            const uint64_t dmax = 2ull * g.deg(st[0]) + 2;
+            for (int t = 0; t < 64 && rng.below(dmax) >= g.deg(v); t++) v = st[rng.below(st.size())];
+            double c;
+            if (!completions(g.off[v] + rng.below(g.deg(v)), c)) return false;
+            sum += c;
+        }
+        total += sum / per * static_cast<double>(st_arcs);
+    }
+    (void)arcs;
+    est = total;
+    return true;
+}
+
 bool write_query(const std::string& path, const QueryOut& q, const std::vector<uint32_t>& vlab) {
     FILE* f = std::fopen(path.c_str(), "w");
     if (!f) return false;
@@ -291,6 +397,7 @@ struct Spec {
     double delete_frac; // fraction of the stream that are deletions
     uint32_t batch;
     std::vector<std::pair<Shape, uint32_t>> queries;  // (shape, vertices)
+    double emb_cap = 0;  // reject queries with more estimated embeddings in the final graph (0: none)
 };
 
 Spec spec_of(const std::string& c) {
@@ -306,6 +413,7 @@ Spec spec_of(const std::string& c) {
         s.queries = {{kC1, 5}};
     } else if (c == "c2" || c == "c4") {
         s = Spec{c == "c2" ? "c2" : "c4", 400000, 3400000, 2.1, 3000, 32, true, 0.10, 0.10, 65536, {}};
+        s.emb_cap = 2e8;
         if (c == "c2") {
             s.queries = {{kSparse, 6}, {kSparse, 7}, {kSparse, 8}, {kDense, 6}, {kDense, 7}, {kDense, 8}};
         } else {
@@ -316,6 +424,7 @@ Spec spec_of(const std::string& c) {
     } else if (c == "c3") {
         s = Spec{"c3", 4000000, 35000000, 2.0, 20000, 16, false, 0.05, 0.05, 1048576, {}};
         s.queries = {{kCycles, 8}, {kCycles, 9}, {kCycles, 10}, {kCycles, 8}, {kCycles, 9}, {kCycles, 10}};
+        s.emb_cap = 1e9;
     }
     return s;
 }
@@ -400,7 +509,14 @@ int gen_main(int argc, char** argv) {
     uint32_t qk = 0;
     for (const auto& qs : sp.queries) {
         QueryOut q;
-        if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, q)) {
+        bool found = false;
+        for (int attempt = 0; attempt < 4000 && !found; attempt++) {
+            if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, q)) break;
+            if (sp.emb_cap <= 0) { found = true; break; }
+            double est = 0;
+            found = estimate_embeddings(g, vlab, q, rng, 480, 200000, est) && est <= sp.emb_cap;
+        }
+        if (!found) {
             std::fprintf(stderr, "could not extract query %u\n", qk);
             return 1;
         }

@@ -104,6 +104,17 @@ csm_status csm_read_candidates(csm_session* s, uint32_t query_id, uint32_t u, ui
 csm_status csm_read_adjacency(csm_session* s, uint32_t v, uint32_t direction, uint32_t* ids, uint32_t cap, uint32_t* degree);
 csm_status csm_synchronize(csm_session* s);
 
+/* Measurement (no counterpart in the reference, whose timing is the TIME_* macros of
+ * utils/cuda_helpers.h:78-102).  Kernel families: 0 enumeration (count), 1 enumeration (write a
+ * level), 2 Cartesian product, 3 local index, 4 batch build, 5 apply batch, 6 Cartesian-product
+ * preparation, 7 anchors.  Bit f of mask times family f with HIP events on the session's stream and,
+ * for the enumeration kernels, counts algorithmic bytes (every view run read once plus candidates
+ * written). */
+csm_status csm_profile_enable(csm_session* s, uint32_t mask);
+csm_status csm_profile_read(csm_session* s, uint32_t family, uint64_t* launches, double* ms, uint64_t* bytes);
+/* Times a plain device copy of `words` 32-bit words (calibration of HBM traffic counters). */
+csm_status csm_calibration_copy(csm_session* s, uint64_t words, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

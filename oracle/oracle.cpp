@@ -9,14 +9,15 @@
 //       |Emb(Q, G)|: injective, vertex-label and edge-label preserving, non-induced maps; every
 //       automorphic image counts (reference: no automorphism handling, match_gpu.cu:279-286).
 //   oracle stream  <data> <update> <batch> <query>... [--threads T] [--directed] [--max-batches K]
-//                  [--anchor-stride S]
+//                  [--anchor-stride S] [--time-limit-ms T]
 //       Per batch t and query: N_t (deletion-anchored) and P_t (insertion-anchored) with the
 //       embedding-set hashes, following the exactly-once interleaving of
 //       technique_permutation.h:127-215: for the step of query edge e_i (qe_list order,
 //       graph.cpp:332-347), f(e_i) is a batch edge, edges e_j (j < i) see the batch applied,
 //       edges e_j (j > i) do not.  Deletions mirror it on G_t: f(e_j) in G_t \ D_t for j < i and in
 //       G_t for j > i; they are processed before insertions.  --anchor-stride S processes only
-//       every S-th anchor (a bounded CPU-baseline sample; counts are then partial).
+//       every S-th anchor (a bounded CPU-baseline sample; counts are then partial); --time-limit-ms
+//       stops taking anchors after T ms per batch (partial counts, reported with the anchors done).
 //   oracle summary <data> <query> [--literal]
 //       Offline global-index summary (match_gpu.cu:589-609): per relation r the cardinality and the
 //       float average degree card/#sources.  Default: closed form (candidates = vertices passing
@@ -152,7 +153,7 @@ bool load_updates(const char* path, std::vector<Update>& u) {
         if (c == 't') { char x; uint32_t y; L.ch(x); L.num(y); continue; }
         bool del = false;
         if (c == '-') { char e; L.ch(e); del = true; }
-        uint32_t a, b, l;
+        uint32_t a = 0, b = 0, l = 0;
         L.num(a); L.num(b); L.num(l);
         u.push_back({a, b, l, del});
     }
@@ -311,6 +312,9 @@ struct Anchored {
 struct StepResult { uint64_t count = 0, hash = 0; };
 
 // Anchored counts for one kind of batch arcs (deletions or insertions) over all steps.
+std::chrono::steady_clock::time_point g_deadline = std::chrono::steady_clock::time_point::max();
+std::atomic<uint64_t> g_anchors_done{0}, g_anchors_total{0};
+
 StepResult run_phase(const Graph& g, const Query& q, const std::vector<Update>& batch, bool deletion,
                      int threads, uint32_t stride) {
     DeltaSet d;
@@ -337,13 +341,17 @@ StepResult run_phase(const Graph& g, const Query& q, const std::vector<Update>& 
         while (true) {
             const size_t k0 = next.fetch_add(64);
             if (k0 >= work.size()) break;
+            if (std::chrono::steady_clock::now() > g_deadline) break;
             for (size_t k = k0; k < std::min(work.size(), k0 + 64); k++) {
                 if (stride > 1 && (k % stride) != 0) continue;
+                if (std::chrono::steady_clock::now() > g_deadline) break;
                 ctx[work[k].first]->anchor(work[k].second.first, work[k].second.second);
+                g_anchors_done++;
             }
         }
         for (auto* c : ctx) { part[t].count += c->count; part[t].hash += c->hash; delete c; }
     };
+    for (size_t k = 0; k < work.size(); k++) if (stride <= 1 || k % stride == 0) g_anchors_total++;
     std::vector<std::thread> ts;
     for (int t = 0; t < threads; t++) ts.emplace_back(body, t);
     for (auto& t : ts) t.join();
@@ -502,6 +510,7 @@ int main(int argc, char** argv) {
     int threads = 1;
     uint32_t max_batches = UINT32_MAX, stride = 1;
     bool literal = false;
+    double time_limit_ms = 0;
     std::vector<const char*> pos;
     for (int i = 2; i < argc; i++) {
         if (!std::strcmp(argv[i], "--directed")) directed = true;
@@ -509,6 +518,7 @@ int main(int argc, char** argv) {
         else if (!std::strcmp(argv[i], "--threads") && i + 1 < argc) threads = std::max(1, std::atoi(argv[++i]));
         else if (!std::strcmp(argv[i], "--max-batches") && i + 1 < argc) max_batches = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
         else if (!std::strcmp(argv[i], "--anchor-stride") && i + 1 < argc) stride = std::max(1u, (uint32_t)std::strtoul(argv[++i], nullptr, 10));
+        else if (!std::strcmp(argv[i], "--time-limit-ms") && i + 1 < argc) time_limit_ms = std::atof(argv[++i]);
         else pos.push_back(argv[i]);
     }
     if (cmd == "count" && pos.size() == 2) {
@@ -554,6 +564,9 @@ int main(int argc, char** argv) {
         for (size_t b = 0; b < nb && b < max_batches; b++) {
             std::vector<Update> batch(ups.begin() + b * bs, ups.begin() + std::min(ups.size(), (b + 1) * (size_t)bs));
             auto t0 = std::chrono::steady_clock::now();
+            if (time_limit_ms > 0) g_deadline = t0 + std::chrono::microseconds((long long)(time_limit_ms * 1000));
+            g_anchors_done = 0;
+            g_anchors_total = 0;
             std::vector<StepResult> neg(qs.size()), posr(qs.size());
             for (size_t k = 0; k < qs.size(); k++) neg[k] = run_phase(g, qs[k], batch, true, threads, stride);
             for (auto& e : batch) if (e.del) g.del_edge(e.a, e.b, e.l);
@@ -565,6 +578,7 @@ int main(int argc, char** argv) {
                             (unsigned long long)neg[k].count, (unsigned long long)neg[k].hash,
                             (unsigned long long)posr[k].count, (unsigned long long)posr[k].hash);
             std::printf("batch %zu edges %zu ms %.3f\n", b, batch.size(), ms);
+            std::printf("batch %zu anchors %llu of %llu\n", b, (unsigned long long)g_anchors_done.load(), (unsigned long long)g_anchors_total.load());
             std::fflush(stdout);
         }
         return 0;

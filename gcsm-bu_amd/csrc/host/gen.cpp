@@ -8,7 +8,9 @@
 #include <cstring>
 #include <string>
 #include <sys/stat.h>
+#include <atomic>
 #include <functional>
+#include <thread>
 #include <unordered_set>
 #include <vector>
 
@@ -510,11 +512,36 @@ int gen_main(int argc, char** argv) {
     for (const auto& qs : sp.queries) {
         QueryOut q;
         bool found = false;
-        for (int attempt = 0; attempt < 4000 && !found; attempt++) {
-            if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, q)) break;
-            if (sp.emb_cap <= 0) { found = true; break; }
-            double est = 0;
-            found = estimate_embeddings(g, vlab, q, rng, 480, 200000, est) && est <= sp.emb_cap;
+        // Candidates are drawn in fixed groups of 16 from the main generator; their estimates run
+        // in parallel with per-candidate seeds, and the first accepted candidate in group order is
+        // kept, so the outcome does not depend on the number of threads.
+        constexpr int kGroup = 16;
+        for (int group = 0; group < 256 && !found; group++) {
+            std::vector<QueryOut> cands;
+            std::vector<uint64_t> seeds;
+            for (int k = 0; k < kGroup; k++) {
+                QueryOut c;
+                if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, c)) break;
+                cands.push_back(c);
+                seeds.push_back(rng.next());
+            }
+            if (cands.empty()) break;
+            if (sp.emb_cap <= 0) { q = cands[0]; found = true; break; }
+            std::vector<char> ok(cands.size(), 0);
+            std::vector<std::thread> th;
+            const unsigned nth = std::max(1u, std::min<unsigned>(kGroup, std::thread::hardware_concurrency()));
+            std::atomic<size_t> next{0};
+            for (unsigned t = 0; t < nth; t++)
+                th.emplace_back([&]() {
+                    for (size_t i = next++; i < cands.size(); i = next++) {
+                        Rng r(seeds[i]);
+                        double est = 0;
+                        ok[i] = estimate_embeddings(g, vlab, cands[i], r, 480, 200000, est) && est <= sp.emb_cap;
+                    }
+                });
+            for (auto& x : th) x.join();
+            for (size_t i = 0; i < cands.size(); i++)
+                if (ok[i]) { q = cands[i]; found = true; break; }
         }
         if (!found) {
             std::fprintf(stderr, "could not extract query %u\n", qk);

@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-mask", type=int, default=0xFF)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a plain copy kernel of 2^26 words (HBM traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU: the matching path is single-GPU "
@@ -165,6 +167,12 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                     "avg_launch_us": round(ms * 1e3 / launches, 3) if launches else None, "launches": launches,
                     "algorithmic_bytes": nbytes}
+    calib = None
+    if args.calibrate:
+        words = 1 << 26
+        ms = s.calibration_copy(words)
+        calib = {"kernel": "k_copy_u32", "bytes_read": 4 * words, "bytes_written": 4 * words, "ms": ms,
+                 "GB/s": round(8 * words / (ms / 1e3) / 1e9, 1)}
     cpu = None if args.no_cpu_baseline else cpu_baseline(data_dir, queries, batch, args.cpu_seconds)
     s.close()
     lat_ms = sorted(x * 1e3 for x in lat)
@@ -181,6 +189,8 @@ def main():
         "kernel_ms": {k: round(v[1], 3) for k, v in stats.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if calib:
+        out["calibration_copy"] = calib
     print(json.dumps(out))
     return 0
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -47,6 +48,7 @@ void* DevicePool::take_bytes(uint64_t bytes) {
 Engine::~Engine() {
     if (s_) (void)hipStreamSynchronize(s_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+    if (trace_) std::fclose(trace_);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
     if (s_) (void)hipStreamDestroy(s_);
 }
@@ -60,6 +62,7 @@ std::string Engine::init(const EngineOptions& o) {
     CSM_HIP(hipSetDevice(o.device));
     CSM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096, hipHostMallocDefault));
+    if (const char* t = std::getenv("CSM_TRACE")) trace_ = std::fopen(t, "a");
     return "";
 }
 
@@ -591,6 +594,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
         if (h_pinned_[2]) return "partial-result ring too small for the initial partial results";
         count = h_pinned_[0];
     }
+    if (trace_) trace_line_ += ", \"anchors\": " + std::to_string(np) + ", \"initial\": " + std::to_string(count);
     if (count == 0) return "";
     const uint32_t res_slot = static_cast<uint32_t>(std::find_if(qs_.begin(), qs_.end(), [&](const std::unique_ptr<QState>& x) { return x.get() == &qs; }) - qs_.begin());
     if (q.n == 2) {
@@ -615,6 +619,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
         CSM_HIP(hipMemcpyAsync(h_pinned_, counters_, 12, hipMemcpyDeviceToHost, s_));
         CSM_HIP(hipStreamSynchronize(s_));
         ok = h_pinned_[2] == 0;
+        if (trace_) trace_line_ += ", \"level_" + std::to_string(end) + "\": " + (ok ? std::to_string(oth == B ? h_pinned_[1] : h_pinned_[0]) : std::string("\"overflow\""));
         if (ok) {
             count = oth == B ? h_pinned_[1] : h_pinned_[0];
             std::swap(cur, oth);
@@ -704,6 +709,7 @@ std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, ui
             const float ratio = static_cast<float>(v[0]) / avg;
             if (ratio < 50.f && avg >= static_cast<float>(kSemiBfsThreshold)) use_cp = false;
         }
+        if (trace_) trace_line_ += ", \"cp_total\": " + std::to_string(v[1]) + ", \"cp_max\": " + std::to_string(v[0]) + ", \"cp_used\": " + std::to_string(use_cp ? 1 : 0);
         if (use_cp) {
             prof_begin(kKCpEnumerate);
             cp_enumerate(g_, d, depth, rec, rec_count, count, cp_, out, s_);
@@ -713,6 +719,7 @@ std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, ui
     }
     fill_u32(counters_ + 3, 0u, 1, s_);
     Ring r{rec, rec_count, count, nullptr, nullptr, 0, nullptr};
+    if (trace_) trace_line_ += ", \"count_from\": " + std::to_string(depth) + ", \"records\": " + std::to_string(count);
     out.bytes = bytes_ptr(kKExtendCount);
     prof_begin(kKExtendCount);
     extend(g_, d, depth, d.n, kCountAtEnd, r, out, counters_ + 3, sched, s_);
@@ -721,15 +728,24 @@ std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, ui
 }
 
 std::string Engine::run_phase(uint32_t ph) {
-    for (auto& qsp : qs_) {
-        QState& qs = *qsp;
+    for (size_t qi = 0; qi < qs_.size(); qi++) {
+        QState& qs = *qs_[qi];
         if (!qs.planned || !qs.matchable) continue;
         for (uint32_t i = 0; i < qs.q.m; i++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            trace_line_.clear();
             std::string e = run_step(qs, i, ph);
+            if (trace_) {
+                (void)hipStreamSynchronize(s_);
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                std::fprintf(trace_, "{\"query\": %zu, \"step\": %u, \"phase\": %u, \"ms\": %.3f, \"empty\": %d%s}\n", qi, i, ph, ms,
+                             e == "empty" ? 1 : 0, trace_line_.c_str());
+            }
             if (e == "empty") continue;
             if (!e.empty()) return e;
         }
     }
+    if (trace_) std::fflush(trace_);
     return "";
 }
 

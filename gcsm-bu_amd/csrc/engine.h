@@ -204,6 +204,9 @@ class Engine {
     hipEvent_t ev_open_[kKFamilies]{};
     unsigned long long* prof_bytes_ = nullptr;  // device counters per family
     unsigned long long* bytes_ptr(uint32_t family) { return (prof_mask_ >> family & 1u) ? prof_bytes_ + family : nullptr; }
+    // Per-step trace (environment variable CSM_TRACE = output path): one JSON line per step.
+    FILE* trace_ = nullptr;
+    std::string trace_line_;
 };
 
 }  // namespace csm

@@ -23,7 +23,11 @@ ORACLE = os.path.join(ROOT, "oracle", "bin", "oracle")
 METRIC = "incremental matches/s & update edges/s on 1 GPU; p50/p99 batch latency"
 HBM_PEAK_GBS = 8000.0
 FAMILIES = ["enumerate_count", "enumerate_write", "cartesian_product", "local_index", "batch_build", "apply_batch",
-            "cartesian_prepare", "anchors"]
+            "cartesian_prepare", "anchors", "level_count", "level_write"]
+# families whose kernels count algorithmic bytes on the device (the roofline candidates)
+BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_write"]
+KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level<0>",
+             "level_write": "k_level<1>"}
 WORKLOADS = {
     "c2": "plain method on one GPU: Chung-Lu 400K vertices / ~3.4M edges, gamma 2.1, max degree ~3000, "
           "Zipf(1.0) labels over 32, 10% held out, 90/10 insert/delete stream, batches of 65,536 edges, "
@@ -104,7 +108,7 @@ def main():
     ap.add_argument("--data-dir", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-mask", type=int, default=0xFF)
+    ap.add_argument("--profile-mask", type=int, default=0x3FF)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of 2^26 words (HBM traffic counter calibration)")
     args = ap.parse_args()
@@ -154,16 +158,14 @@ def main():
         log(f"step {k}: {dt * 1e3:.1f} ms, {sum(r[0] + r[1] for r in res)} matches, {info.edges} edges")
     total = sum(lat)
     stats = {FAMILIES[f]: s.profile_read(f) for f in range(len(FAMILIES)) if args.profile_mask >> f & 1}
-    dom = max(stats, key=lambda k: stats[k][1]) if stats else None
+    # dominant kernel: the byte-counting family with the most kernel time
+    cands = [k for k in BYTE_FAMILIES if k in stats and stats[k][1] > 0]
+    dom = max(cands, key=lambda k: stats[k][1]) if cands else None
     roofline = None
     if dom:
         launches, ms, nbytes = stats[dom]
-        if nbytes == 0 and dom != "enumerate_count":
-            # only the enumeration kernel counts algorithmic bytes; report it
-            launches, ms, nbytes = stats.get("enumerate_count", (launches, ms, nbytes))
-            dom = "enumerate_count"
         achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        roofline = {"bound": "hbm", "kernel": KERNEL_OF[dom], "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                     "avg_launch_us": round(ms * 1e3 / launches, 3) if launches else None, "launches": launches,
                     "algorithmic_bytes": nbytes}

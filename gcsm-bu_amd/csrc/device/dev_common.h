@@ -51,6 +51,11 @@ struct DevGraph {
     DevAdj dir[2];        // 0 = out (or undirected), 1 = in
     DevDelta delta[2];
     uint32_t* err;        // error bits
+    // Label-slot directory (optional, nullptr when disabled): row v of sdir[d] holds nslots + 1
+    // absolute arena offsets, row[s] = first position of v's direction-d list whose key has slot
+    // >= s, so the run of slot s is [row[s], row[s + 1]).
+    uint32_t nslots;
+    uint32_t* sdir[2];
 };
 
 // One backward neighbour of the vertex matched at some depth: the partner is at position `pos`

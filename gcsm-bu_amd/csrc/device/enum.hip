@@ -22,13 +22,18 @@ namespace {
 
 constexpr uint32_t kWaves = 4;  // waves per block
 
-struct WaveState {
-    uint32_t f[kMaxQV];
+// Per-wave DFS state in LDS.  Behind it (dynamic LDS, per wave): `rows` candidate buffers of 64
+// words, one per depth d0 .. end_depth-2.
+struct WaveHdr {
     unsigned long long ph[kMaxQV + 1];  // ph[d] = sum_{j<d} pair_hash(order[j], f[j])
     RunView run[kMaxQE];
+    uint32_t f[kMaxQV];
     uint32_t cur[kMaxQV], tot[kMaxQV], nb[kMaxQV], bcnt[kMaxQV], bpos[kMaxQV];
-    uint32_t buf[kMaxQV][64];
 };
+
+__host__ __device__ constexpr uint32_t wave_lds_bytes(uint32_t rows) {
+    return static_cast<uint32_t>((sizeof(WaveHdr) + rows * 256u + 15u) & ~15u);
+}
 
 __device__ __forceinline__ const uint32_t* entry_delta(const DevGraph& g, const StepDesc& d, const BackEntry& e) {
     return phase_delta(g, e.dir, d.phase);
@@ -36,17 +41,18 @@ __device__ __forceinline__ const uint32_t* entry_delta(const DevGraph& g, const 
 
 // Computes the view runs of every backward entry of `depth` and resets its frame.
 // Algorithmic bytes of a frame: every view run of the depth's backward lists read once.
-__device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d, WaveState& W, uint32_t depth, uint32_t lane,
-                                            unsigned long long& bytes) {
+__device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d, WaveHdr& W, uint32_t depth, uint32_t lane,
+                                            unsigned long long& bytes, bool count_bytes) {
     const uint32_t e0 = d.boff[depth], e1 = d.boff[depth + 1];
     if (lane < e1 - e0) {
         const BackEntry e = d.be[e0 + lane];
         W.run[e0 + lane] = make_run(g, d.phase, e.view, e.dir, W.f[e.pos], e.slot);
     }
     wave_sync();
+    const uint32_t p = d.pivot[depth];
     if (lane == 0) {
-        for (uint32_t e = e0; e < e1; e++) bytes += 4ull * ((W.run[e].hi - W.run[e].lo) + (W.run[e].dhi - W.run[e].dlo));
-        const uint32_t p = d.pivot[depth];
+        if (count_bytes)
+            for (uint32_t e = e0; e < e1; e++) bytes += 4ull * ((W.run[e].hi - W.run[e].lo) + (W.run[e].dhi - W.run[e].dlo));
         const RunView r = W.run[p];
         W.nb[depth] = run_base_len(r, d.be[p].view);
         W.tot[depth] = run_seq_len(r, d.be[p].view);
@@ -57,7 +63,7 @@ __device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d
     wave_sync();
 }
 
-__device__ __forceinline__ bool test_candidate(const DevGraph& g, const StepDesc& d, const WaveState& W, uint32_t depth, uint32_t key) {
+__device__ __forceinline__ bool test_candidate(const DevGraph& g, const StepDesc& d, const WaveHdr& W, uint32_t depth, uint32_t key) {
     const uint32_t c = key & g.id_mask;
     if (!bit_test(d.filter[depth], c)) return false;
     for (uint32_t j = 0; j < depth; j++)
@@ -74,27 +80,50 @@ __device__ __forceinline__ bool test_candidate(const DevGraph& g, const StepDesc
     return true;
 }
 
+// Last record r with off[r] <= item, given off[0] <= item < off[n]: a 64-ary search in which every
+// lane probes one of 64 evenly spaced offsets per round.
+__device__ __forceinline__ uint32_t find_record(const unsigned long long* off, uint32_t n, unsigned long long item, uint32_t lane) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t p = lo + lane * step;
+        const uint64_t m = __ballot(p < hi && off[p] <= item);
+        lo += (63u - static_cast<uint32_t>(__clzll(m))) * step;
+        hi = min(hi, lo + step);
+    }
+    return uniform(lo);
+}
+
 template <int kMode>
 __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, uint32_t d0, uint32_t end_depth, Ring r, MatchOut out,
-                                                         uint32_t* work, int schedule) {
-    __shared__ WaveState ws[kWaves];
-    WaveState& W = ws[threadIdx.x >> 6];
+                                                         const unsigned long long* split, unsigned long long* work, int schedule) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const uint32_t rows = end_depth > d0 + 1 ? end_depth - 1 - d0 : 0u;
+    unsigned char* wave_base = lds_raw + (threadIdx.x >> 6) * wave_lds_bytes(rows);
+    WaveHdr& W = *reinterpret_cast<WaveHdr*>(wave_base);
+    uint32_t* const buf = reinterpret_cast<uint32_t*>(wave_base + sizeof(WaveHdr));  // row (depth - d0)
     const uint32_t lane = lane_id();
     const uint32_t nrec = min(*r.in_count, r.in_max);
+    const unsigned long long nitems = split ? split[nrec] : nrec;
     const uint32_t total_waves = gridDim.x * kWaves;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
     unsigned long long my_cnt = 0, my_hash = 0, my_bytes = 0;
 
-    uint32_t item;
+    unsigned long long item;
     if (schedule == 0) item = gw;
     else {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(work, 1u);
-        item = uniform(__shfl(t, 0, 64));
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(work, 1ull);
+        item = __shfl(t, 0, 64);
     }
-    while (item < nrec) {
+    while (item < nitems) {
         if (kMode == kWriteAtEnd && uniform(__hip_atomic_load(r.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
-        if (lane < d0) W.f[lane] = r.in[static_cast<uint64_t>(item) * d0 + lane];
+        uint32_t rec_idx = static_cast<uint32_t>(item), chunk = 0;
+        if (split) {
+            rec_idx = find_record(split, nrec, item, lane);
+            chunk = static_cast<uint32_t>(item - split[rec_idx]);
+        }
+        if (lane < d0) W.f[lane] = r.in[static_cast<uint64_t>(rec_idx) * d0 + lane];
         wave_sync();
         if (lane == 0) {
             unsigned long long h = 0;
@@ -103,10 +132,17 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
         }
         wave_sync();
         uint32_t depth = d0;
-        setup_frame(g, d, W, depth, lane, my_bytes);
+        setup_frame(g, d, W, depth, lane, my_bytes, chunk == 0);
+        if (split) {
+            if (lane == 0) {
+                W.cur[d0] = chunk * 64u;
+                W.tot[d0] = min(W.tot[d0], chunk * 64u + 64u);
+            }
+            wave_sync();
+        }
         while (true) {
             if (W.bpos[depth] < W.bcnt[depth]) {
-                const uint32_t c = W.buf[depth][W.bpos[depth]];
+                const uint32_t c = buf[(depth - d0) * 64u + W.bpos[depth]];
                 wave_sync();
                 if (lane == 0) {
                     W.bpos[depth] += 1;
@@ -115,7 +151,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
                 }
                 wave_sync();
                 depth += 1;
-                setup_frame(g, d, W, depth, lane, my_bytes);
+                setup_frame(g, d, W, depth, lane, my_bytes, true);
                 continue;
             }
             if (W.cur[depth] >= W.tot[depth]) {
@@ -152,16 +188,16 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
                     }
                 }
             } else {
-                if (ok) W.buf[depth][lanes_below(m)] = c;
+                if (ok) buf[(depth - d0) * 64u + lanes_below(m)] = c;
                 if (lane == 0) { W.bcnt[depth] = static_cast<uint32_t>(__popcll(m)); W.bpos[depth] = 0; }
             }
             wave_sync();
         }
         if (schedule == 0) item += total_waves;
         else {
-            uint32_t t2 = 0;
-            if (lane == 0) t2 = atomicAdd(work, 1u);
-            item = uniform(__shfl(t2, 0, 64));
+            unsigned long long t2 = 0;
+            if (lane == 0) t2 = atomicAdd(work, 1ull);
+            item = __shfl(t2, 0, 64);
         }
     }
     if (kMode == kCountAtEnd) {
@@ -170,6 +206,125 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
         if (lane == 0 && my_cnt) { atomicAdd(out.count, my_cnt); atomicAdd(out.hash, my_hash); }
     }
     if (out.bytes && lane == 0 && my_bytes) atomicAdd(out.bytes, my_bytes);
+}
+
+// ---- One level, flattened over the whole level ------------------------------------------------
+// k_level_len: per record, the length of its pivot view run (and, when counting, its prefix hash);
+// after an exclusive scan, k_level_flat gives one thread per candidate of the whole level: thread t
+// finds its record by binary search over the offsets, so skewed records spread over the grid.
+// Algorithmic bytes (as the DFS kernel's frames): every view run of the depth's backward lists
+// read once per record, plus 4 per passing candidate (counted by k_level_flat).
+__global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                   unsigned long long* len, unsigned long long* ph, unsigned long long* bytes_out) {
+    const uint32_t p = d.pivot[d0];
+    const BackEntry pe = d.be[p];
+    unsigned long long bytes = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t* f = rec + static_cast<uint64_t>(i) * d0;
+        const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
+        len[i] = run_seq_len(rv, pe.view);
+        if (ph) {
+            unsigned long long h = 0;
+            for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
+            ph[i] = h;
+        }
+        if (bytes_out) {
+            for (uint32_t e = d.boff[d0]; e < d.boff[d0 + 1]; e++) {
+                const BackEntry be = d.be[e];
+                const RunView ov = e == p ? rv : make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
+                bytes += 4ull * ((ov.hi - ov.lo) + (ov.dhi - ov.dlo));
+            }
+        }
+    }
+    if (bytes_out) {
+        bytes = wave_sum_u64(bytes);
+        if (lane_id() == 0 && bytes) atomicAdd(bytes_out, bytes);
+    }
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                    const unsigned long long* off, const unsigned long long* ph, Ring r, MatchOut out) {
+    const uint32_t lane = lane_id();
+    const uint32_t p = d.pivot[d0];
+    const BackEntry pe = d.be[p];
+    const uint32_t* pdl = phase_delta(g, pe.dir, d.phase);
+    const uint32_t e0 = d.boff[d0], e1 = d.boff[d0 + 1];
+    const uint32_t* filter = d.filter[d0];
+    const uint32_t qv = d.order[d0];
+    const unsigned long long total = off[n];
+    const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    unsigned long long cnt = 0, hsum = 0, bytes = 0;
+    // every lane of a wave runs the same number of iterations (ballots below)
+    for (unsigned long long t0 = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); t0 < total; t0 += stride) {
+        const unsigned long long t = t0 + lane;
+        bool ok = false;
+        uint32_t c = 0, k = 0;
+        if (t < total) {
+            uint32_t lo = 0, hi = n;  // last k with off[k] <= t
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (off[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            k = lo;
+            const uint32_t* f = rec + static_cast<uint64_t>(k) * d0;
+            const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
+            const uint32_t key = run_elem(g, pdl, rv, run_base_len(rv, pe.view), static_cast<uint32_t>(t - off[k]));
+            c = key & g.id_mask;
+            ok = bit_test(filter, c) && run_elem_live(pdl, rv, pe.view, key);
+            for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != c;
+            for (uint32_t e = e0; e < e1 && ok; e++) {
+                if (e == p) continue;
+                const BackEntry be = d.be[e];
+                const RunView ov = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
+                ok = run_contains(g, phase_delta(g, be.dir, d.phase), ov, be.view, (be.slot << g.id_bits) | c);
+            }
+            if (out.bytes && ok) bytes += 4;
+        }
+        if (kMode == kCountAtEnd) {
+            if (ok) {
+                cnt += 1;
+                hsum += embedding_hash(ph[k] + pair_hash(qv, c));
+            }
+        } else {
+            const uint64_t m = __ballot(ok);
+            if (m) {
+                const uint32_t np = static_cast<uint32_t>(__popcll(m));
+                uint32_t wb = 0;
+                if (lane == 0) wb = atomicAdd(r.out_count, np);
+                wb = __shfl(wb, 0, 64);
+                if (static_cast<uint64_t>(wb) + np > r.out_cap) {
+                    if (lane == 0) atomicOr(r.overflow, 1u);
+                } else if (ok) {
+                    uint32_t* o = r.out + static_cast<uint64_t>(wb + lanes_below(m)) * (d0 + 1);
+                    const uint32_t* f = rec + static_cast<uint64_t>(k) * d0;
+                    for (uint32_t j = 0; j < d0; j++) o[j] = f[j];
+                    o[d0] = c;
+                }
+            }
+        }
+    }
+    if (kMode == kCountAtEnd) {
+        cnt = wave_sum_u64(cnt);
+        hsum = wave_sum_u64(hsum);
+        if (lane == 0 && cnt) { atomicAdd(out.count, cnt); atomicAdd(out.hash, hsum); }
+    }
+    if (out.bytes) {
+        bytes = wave_sum_u64(bytes);
+        if (lane == 0 && bytes) atomicAdd(out.bytes, bytes);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_root_chunks(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
+                                                     uint32_t max_rec, unsigned long long* chunks) {
+    const uint32_t n = min(*n_rec, max_rec);
+    const BackEntry be = d.be[d.pivot[d0]];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t v = rec[static_cast<uint64_t>(i) * d0 + be.pos];
+        const RunView rv = make_run(g, d.phase, be.view, be.dir, v, be.slot);
+        chunks[i] = (run_seq_len(rv, be.view) + 63u) / 64u;
+    }
 }
 
 __global__ void k_init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t max_pairs, int swap, uint32_t* out,
@@ -403,13 +558,37 @@ void bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_
 }
 
 void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r, MatchOut out,
-            uint32_t* work_counter, int schedule, hipStream_t s) {
-    // Persistent grid: enough waves to fill 256 CUs several times over, never more than records.
-    const int grid = grid_for(r.in_max, kWaves, 2048);
+            const unsigned long long* split, unsigned long long* work_counter, int schedule, hipStream_t s) {
+    // Persistent grid: enough waves to fill 256 CUs several times over, never more than records
+    // (a split launch has at least as many items as records with candidates).
+    const int grid = split ? 2048 : grid_for(r.in_max, kWaves, 2048);
+    const uint32_t rows = end_depth > d0 + 1 ? end_depth - 1 - d0 : 0u;
+    const size_t lds = static_cast<size_t>(kWaves) * wave_lds_bytes(rows);
     if (mode == kCountAtEnd)
-        k_extend<kCountAtEnd><<<grid, kWaves * 64, 0, s>>>(g, d, d0, end_depth, r, out, work_counter, schedule);
+        k_extend<kCountAtEnd><<<grid, kWaves * 64, lds, s>>>(g, d, d0, end_depth, r, out, split, work_counter, schedule);
     else
-        k_extend<kWriteAtEnd><<<grid, kWaves * 64, 0, s>>>(g, d, d0, end_depth, r, out, work_counter, schedule);
+        k_extend<kWriteAtEnd><<<grid, kWaves * 64, lds, s>>>(g, d, d0, end_depth, r, out, split, work_counter, schedule);
+}
+
+void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, unsigned long long* off,
+                   unsigned long long* ph, unsigned long long* scan_tmp, unsigned long long* bytes, hipStream_t s) {
+    k_level_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, off, ph, bytes);
+    scan_exclusive<unsigned long long>(off, off, nullptr, n, true, scan_tmp, s);
+}
+
+void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode,
+                const unsigned long long* off, const unsigned long long* ph, Ring r, MatchOut out, hipStream_t s) {
+    constexpr int kGrid = 4096;  // grid-stride over the level's candidates (total known on the device only)
+    if (mode == kCountAtEnd)
+        k_level_flat<kCountAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, off, ph, r, out);
+    else
+        k_level_flat<kWriteAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, off, ph, r, out);
+}
+
+void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
+                 unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s) {
+    k_root_chunks<<<grid_for(max_rec, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n_rec, max_rec, chunks);
+    scan_exclusive<unsigned long long>(chunks, chunks, n_rec, max_rec, true, scan_tmp, s);
 }
 
 void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,

@@ -382,6 +382,23 @@ __global__ void k_batch_cleanup(BatchScratch w, DevDelta d) {
     }
 }
 
+// Thread per (vertex, slot): row[s] = lower bound of key (s << id_bits) in the vertex's list;
+// row[nslots] = end of the list.
+__global__ void __launch_bounds__(256) k_slot_dir(DevGraph g, int dir, const uint32_t* list, const uint32_t* n_list, uint32_t max_n) {
+    const uint32_t n = list ? min(*n_list, max_n) : g.dv;
+    const uint32_t w = g.nslots + 1;
+    const uint64_t total = static_cast<uint64_t>(n) * w;
+    for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < total;
+         t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t i = static_cast<uint32_t>(t / w), sl = static_cast<uint32_t>(t % w);
+        const uint32_t v = list ? list[i] : i;
+        const uint2 a = g.dir[dir].adj[v];
+        const uint32_t key0 = sl == 0 ? 0u : (sl << g.id_bits);  // sl < nslots: fits the key width
+        const uint32_t r = sl == g.nslots ? a.x + a.y : lower_bound_u32(g.arena, a.x, a.x + a.y, key0);
+        g.sdir[dir][static_cast<uint64_t>(v) * w + sl] = r;
+    }
+}
+
 __global__ void k_reset_counters(BatchScratch w) {
     if (threadIdx.x == 0 && blockIdx.x == 0) { *w.n_touched = 0; *w.n_big = 0; }
 }
@@ -404,6 +421,13 @@ void batch_build(const DevGraph& g, int dir, int dirsel, const BatchInput& b, ui
     const int sgrid = static_cast<int>(std::min<uint32_t>(2048, (max_arcs + kSortWaves - 1) / kSortWaves + 1));
     k_batch_segsort<<<sgrid, kSortWaves * 64, 0, s>>>(w, g.delta[dir]);
     k_batch_segsort_big<<<64, 1024, 0, s>>>(w, g.delta[dir], g.err);
+}
+
+void slot_dir_update(const DevGraph& g, int dir, const uint32_t* list, const uint32_t* n_list, uint32_t max_n, hipStream_t s) {
+    if (!g.sdir[dir]) return;
+    const uint64_t total = static_cast<uint64_t>(list ? max_n : g.dv) * (g.nslots + 1);
+    const int grid = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(8192, (total + 255) / 256)));
+    k_slot_dir<<<grid, 256, 0, s>>>(g, dir, list, n_list, max_n);
 }
 
 void batch_apply_deletions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s) {

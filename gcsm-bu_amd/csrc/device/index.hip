@@ -21,9 +21,8 @@ namespace {
 __device__ __forceinline__ bool nlf_pass(const DevGraph& g, const NlfEntry& e, const NlfGroup* groups, uint32_t v, int mode) {
     for (uint32_t k = e.g0; k < e.g1; k++) {
         const NlfGroup gr = groups[k];
-        const uint2 a = g.dir[gr.dir].adj[v];
         uint32_t lo, hi;
-        slot_range(g.arena, a.x, a.x + a.y, gr.slot, g.id_bits, lo, hi);
+        base_range(g, gr.dir, v, gr.slot, lo, hi);
         uint32_t deg = hi - lo;
         if (mode == 0) {
             const uint4 s = g.delta[gr.dir].seg[v];
@@ -80,9 +79,8 @@ __global__ void __launch_bounds__(256) k_relation_summary(DevGraph g, uint32_t d
     unsigned long long card = 0, srcs = 0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < g.dv; v += gridDim.x * blockDim.x) {
         if (g.vrank[v] != vrank_u || !bit_test(cand_u, v)) continue;
-        const uint2 a = g.dir[dir].adj[v];
         uint32_t lo, hi;
-        slot_range(g.arena, a.x, a.x + a.y, slot, g.id_bits, lo, hi);
+        base_range(g, dir, v, slot, lo, hi);
         uint32_t c = 0;
         for (uint32_t k = lo; k < hi; k++) c += bit_test(cand_w, g.arena[k] & g.id_mask) ? 1u : 0u;
         card += c;

@@ -45,6 +45,9 @@ uint64_t scan_scratch_elems(uint32_t n_max);
 
 void batch_build(const DevGraph& g, int dir, int dirsel, const BatchInput& b, uint32_t lo, uint32_t n,
                  const BatchScratch& w, hipStream_t s);
+// Recomputes label-slot directory rows of direction dir: every vertex when list is null, else
+// the vertices list[0, min(*n_list, max_n)).
+void slot_dir_update(const DevGraph& g, int dir, const uint32_t* list, const uint32_t* n_list, uint32_t max_n, hipStream_t s);
 void batch_apply_deletions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
 void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
 void batch_cleanup(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
@@ -133,10 +136,24 @@ void bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_
 enum ExtendMode : int { kCountAtEnd = 0, kWriteAtEnd = 1 };
 // DFS from the records at depth d0 down to end_depth: counts/hashes complete embeddings when
 // end_depth == n, or writes the partial results of depth end_depth to the ring.
-// schedule: 0 static (record r by wave r mod W), 1 dynamic (global work counter), with root
-// splitting of long candidate lists when split > 0.
+// schedule: 0 static (item t by wave t mod W), 1 dynamic (global work counter, zeroed by the
+// caller).  Without `split` an item is one record.  With `split` (the exclusive scan of
+// root_chunks, total at [n]) an item is one 64-candidate chunk of a record's depth-d0 candidates,
+// so heavy records spread over many waves.
 void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r,
-            MatchOut out, uint32_t* work_counter, int schedule, hipStream_t s);
+            MatchOut out, const unsigned long long* split, unsigned long long* work_counter, int schedule, hipStream_t s);
+// One level from the n records of depth d0 at rec (n known on the host), flattened over all its
+// candidates: level_prepare writes the scanned pivot-run lengths of the records to off[0..n]
+// (off[n] = total candidates) and, when ph is set, each record's prefix hash; level_flat then runs
+// one thread per candidate and counts/hashes complete embeddings (d0 + 1 == n) or writes records
+// of depth d0 + 1.
+void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, unsigned long long* off,
+                   unsigned long long* ph, unsigned long long* scan_tmp, unsigned long long* bytes, hipStream_t s);
+void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode,
+                const unsigned long long* off, const unsigned long long* ph, Ring r, MatchOut out, hipStream_t s);
+// chunks[i] = ceil(|pivot view run of record i at depth d0| / 64), then scanned in place (total at [n]).
+void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
+                 uint32_t max_rec, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s);
 
 // Cartesian-product flattening of trailing tree leaves (cartesian_product.cu:193-284).
 struct CpScratch {

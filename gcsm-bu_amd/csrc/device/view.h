@@ -14,12 +14,22 @@ __device__ __forceinline__ const uint32_t* phase_delta(const DevGraph& g, uint32
     return phase ? g.delta[dir].del : g.delta[dir].ins;
 }
 
+// The arena run of slot `slot` in v's direction-dir list: two adjacent directory words when the
+// slot directory is on, else two binary searches over the whole list.
+__device__ __forceinline__ void base_range(const DevGraph& g, uint32_t dir, uint32_t v, uint32_t slot, uint32_t& lo, uint32_t& hi) {
+    if (g.sdir[dir]) {
+        const uint32_t* row = g.sdir[dir] + static_cast<uint64_t>(v) * (g.nslots + 1);
+        lo = row[slot];
+        hi = row[slot + 1];
+    } else {
+        const uint2 a = g.dir[dir].adj[v];
+        slot_range(g.arena, a.x, a.x + a.y, slot, g.id_bits, lo, hi);
+    }
+}
+
 __device__ __forceinline__ RunView make_run(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t v, uint32_t slot) {
     RunView r{0, 0, 0, 0};
-    if (view != kViewDelta) {
-        const uint2 a = g.dir[dir].adj[v];
-        slot_range(g.arena, a.x, a.x + a.y, slot, g.id_bits, r.lo, r.hi);
-    }
+    if (view != kViewDelta) base_range(g, dir, v, slot, r.lo, r.hi);
     if (view != kViewBase) {
         const uint4 s = g.delta[dir].seg[v];
         const uint32_t off = phase ? s.z : s.x;

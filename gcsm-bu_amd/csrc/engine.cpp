@@ -63,6 +63,11 @@ std::string Engine::init(const EngineOptions& o) {
     CSM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096, hipHostMallocDefault));
     if (const char* t = std::getenv("CSM_TRACE")) trace_ = std::fopen(t, "a");
+    // A/B switches of the scheduling and layout aids (all on by default)
+    if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) opt_.split_roots = std::atoi(t) != 0;
+    if (const char* t = std::getenv("CSM_SLOT_DIRECTORY")) opt_.slot_directory = std::atoi(t) != 0;
+    if (const char* t = std::getenv("CSM_LEVEL_KERNEL")) opt_.level_kernel = std::atoi(t) != 0;
+    if (const char* t = std::getenv("CSM_CP_MAX_RECORDS")) opt_.cp_max_records = static_cast<uint32_t>(std::max(1024L, std::atol(t)));
     return "";
 }
 
@@ -202,6 +207,10 @@ std::string Engine::build() {
     add(qs_.size() * 32 + 64);
     add(kKFamilies * 8);
     add(kMaxQV * qs_.size() * sizeof(NlfEntry) + kMaxRel * qs_.size() * sizeof(NlfGroup) + 512);
+    const uint64_t sdir_bytes = (nslots + 1) * dv_ * 4ull;
+    const bool use_sdir = opt_.slot_directory && sdir_bytes * ndirs_ <= kMaxSlotDirectoryBytes;
+    if (use_sdir)
+        for (int d = 0; d < ndirs_; d++) add(sdir_bytes);
     need += 1 << 20;
     if (opt_.pool_bytes && need > opt_.pool_bytes)
         return "device pool budget too small: need " + std::to_string(need >> 20) + " MiB, budget " + std::to_string(opt_.pool_bytes >> 20) + " MiB";
@@ -213,6 +222,8 @@ std::string Engine::build() {
     g_.id_bits = id_bits_;
     g_.id_mask = id_bits_ >= 32 ? 0xFFFFFFFFu : ((1u << id_bits_) - 1u);
     g_.nvl = nvl_;
+    g_.nslots = static_cast<uint32_t>(nslots);
+    g_.sdir[0] = g_.sdir[1] = nullptr;
     uint32_t* d_vrank = pool_.take<uint32_t>(dv_);
     g_.vrank = d_vrank;
     g_.arena = pool_.take<uint32_t>(arena_words);
@@ -284,7 +295,12 @@ std::string Engine::build() {
             w.big_ptr = pool_.take<uint32_t>(kMaxBigSegments);
             w.big_len = pool_.take<uint32_t>(kMaxBigSegments);
         }
-        if (ndirs_ == 1) { g_.dir[1] = g_.dir[0]; g_.delta[1] = g_.delta[0]; bs_[1] = bs_[0]; }
+        if (use_sdir)
+            for (int d = 0; d < ndirs_; d++) {
+                g_.sdir[d] = pool_.take<uint32_t>((nslots + 1) * dv_);
+                slot_dir_update(g_, d, nullptr, nullptr, 0, s_);
+            }
+        if (ndirs_ == 1) { g_.dir[1] = g_.dir[0]; g_.delta[1] = g_.delta[0]; bs_[1] = bs_[0]; g_.sdir[1] = g_.sdir[0]; }
         unsigned long long* top = pool_.take<unsigned long long>(4);
         g_.arena_top = top;
         uint32_t* errp = reinterpret_cast<uint32_t*>(top + 2);
@@ -568,8 +584,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
     uint32_t* cA = counters_;
     uint32_t* cB = counters_ + 1;
     uint32_t* ovf = counters_ + 2;
-    uint32_t* work = counters_ + 3;
-    fill_u32(counters_, 0u, 8, s_);
+    fill_u32(counters_, 0u, 10, s_);
     const uint32_t a = o.vs[0], b = o.vs[1];
     uint32_t count;
     if ((a == u0 && b == u1) || (a == u1 && b == u0)) {
@@ -608,14 +623,44 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
     uint32_t* cur_cnt = cA;
     uint32_t* oth = B;
     uint32_t* oth_cnt = cB;
-    // One level (or a DFS down to `end`) written to the other half; returns false on overflow.
-    auto write_to = [&](uint32_t end, bool& ok) -> std::string {
+    // One level (or a DFS down to `end`) written to the other half.  ok = false when the ring
+    // half cannot hold it (the records at `depth` stay intact).  With `check_fit`, a one-level
+    // write first compares the level's candidate count (an upper bound of its records) with the
+    // ring half and skips the write when it cannot fit.
+    auto write_to = [&](uint32_t end, bool check_fit, bool& ok) -> std::string {
+        const uint64_t cap = std::min<uint64_t>(half / end, 0xFFFFFFFFull);
         fill_u32(oth_cnt, 0u, 1, s_);
-        fill_u32(ovf, 0u, 2, s_);
-        Ring r{cur, cur_cnt, count, oth, oth_cnt, static_cast<uint32_t>(std::min<uint64_t>(half / end, 0xFFFFFFFFull)), ovf};
-        prof_begin(kKExtendWrite);
-        extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, work, sched, s_);
-        prof_end(kKExtendWrite);
+        fill_u32(ovf, 0u, 1, s_);
+        fill_u32(counters_ + 8, 0u, 2, s_);
+        Ring r{cur, cur_cnt, count, oth, oth_cnt, static_cast<uint32_t>(cap), ovf};
+        if (opt_.level_kernel && end == depth + 1) {
+            uint64_t cands = 0;
+            for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
+                const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
+                const uint32_t* rec = cur + static_cast<uint64_t>(s0) * depth;
+                prof_begin(kKLevelWrite);
+                level_prepare(g_, d, depth, rec, nc, cp_.prod, nullptr, cp_.scan_tmp, bytes_ptr(kKLevelWrite), s_);
+                if (check_fit) {
+                    CSM_HIP(hipMemcpyAsync(h_pinned_ + 8, cp_.prod + nc, 8, hipMemcpyDeviceToHost, s_));
+                    CSM_HIP(hipStreamSynchronize(s_));
+                    uint64_t t;
+                    std::memcpy(&t, h_pinned_ + 8, 8);
+                    cands += t;
+                    if (cands > cap) {
+                        prof_end(kKLevelWrite);
+                        if (trace_) trace_line_ += ", \"level_" + std::to_string(end) + "\": \"skipped\"";
+                        ok = false;
+                        return "";
+                    }
+                }
+                level_flat(g_, d, depth, rec, nc, kWriteAtEnd, cp_.prod, nullptr, r, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, s_);
+                prof_end(kKLevelWrite);
+            }
+        } else {
+            prof_begin(kKExtendWrite);
+            extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, nullptr, work_counter(), sched, s_);
+            prof_end(kKExtendWrite);
+        }
         CSM_HIP(hipMemcpyAsync(h_pinned_, counters_, 12, hipMemcpyDeviceToHost, s_));
         CSM_HIP(hipStreamSynchronize(s_));
         ok = h_pinned_[2] == 0;
@@ -638,10 +683,16 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
     std::string err;
     switch (qs.tech.strategy) {
         case SearchStrategy::kDynamicBfsDfs: {  // match_gpu.cu:1916-2127
+            // Breadth-first while the level is small (the reference's rule).  With one-level
+            // flattened kernels, also past the threshold while the next level fits the ring half,
+            // stopping where the Cartesian product takes over; depth-first from there otherwise.
             bool oom = false;
-            while (count < opt_.semi_bfs_threshold && q.n - depth > 1) {
+            while (q.n - depth > 1) {
+                const bool small = count < opt_.semi_bfs_threshold;
+                if (!small && !opt_.level_kernel) break;
+                if (!small && qs.tech.cartesian_product && cpi[depth] == kCpTreeProduct) break;
                 bool ok;
-                err = write_to(depth + 1, ok);
+                err = write_to(depth + 1, !small, ok);
                 if (!err.empty()) return err;
                 if (!ok) { oom = true; break; }
                 if (count == 0) return "";
@@ -650,7 +701,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
                 const uint32_t nd = tail_from(depth);
                 if (nd < q.n) {
                     bool ok;
-                    err = write_to(nd, ok);
+                    err = write_to(nd, false, ok);
                     if (!err.empty()) return err;
                     if (ok && count == 0) return "";
                 }
@@ -661,7 +712,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
             const uint32_t target = qs.tech.cartesian_product ? std::min(tail_from(2), q.n - 1) : q.n - 1;
             while (depth < target) {
                 bool ok;
-                err = write_to(depth + 1, ok);
+                err = write_to(depth + 1, false, ok);
                 if (!err.empty()) return err;
                 if (!ok) break;
                 if (count == 0) return "";
@@ -672,7 +723,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
             const uint32_t t = tail_from(depth);
             if (qs.tech.cartesian_product && t < q.n && t > depth && cpi[t] == kCpTreeProduct) {
                 bool ok;
-                err = write_to(t, ok);
+                err = write_to(t, false, ok);
                 if (!err.empty()) return err;
                 if (ok && count == 0) return "";
             }
@@ -717,12 +768,28 @@ std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, ui
             return "";
         }
     }
-    fill_u32(counters_ + 3, 0u, 1, s_);
+    fill_u32(counters_ + 8, 0u, 2, s_);
     Ring r{rec, rec_count, count, nullptr, nullptr, 0, nullptr};
     if (trace_) trace_line_ += ", \"count_from\": " + std::to_string(depth) + ", \"records\": " + std::to_string(count);
+    if (opt_.level_kernel && depth + 1 == d.n) {
+        out.bytes = bytes_ptr(kKLevelCount);
+        for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
+            const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
+            const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
+            prof_begin(kKLevelCount);
+            level_prepare(g_, d, depth, rc, nc, cp_.prod, cp_.rec_hash, cp_.scan_tmp, out.bytes, s_);
+            level_flat(g_, d, depth, rc, nc, kCountAtEnd, cp_.prod, cp_.rec_hash, r, out, s_);
+            prof_end(kKLevelCount);
+        }
+        return "";
+    }
     out.bytes = bytes_ptr(kKExtendCount);
+    // Root splitting (dynamic schedules, few records: a handful of heavy records would otherwise
+    // serialise on single waves).  The CP offsets buffer is free here.
+    const bool split = opt_.split_roots && sched == 1 && count <= kSplitMaxRecords;
     prof_begin(kKExtendCount);
-    extend(g_, d, depth, d.n, kCountAtEnd, r, out, counters_ + 3, sched, s_);
+    if (split) root_chunks(g_, d, depth, rec, rec_count, count, cp_.prod, cp_.scan_tmp, s_);
+    extend(g_, d, depth, d.n, kCountAtEnd, r, out, split ? cp_.prod : nullptr, work_counter(), sched, s_);
     prof_end(kKExtendCount);
     return "";
 }
@@ -778,7 +845,10 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         prof_begin(kKApply);
-        for (int d = 0; d < ndirs_; d++) batch_apply_deletions(g_, d, max_arcs, bs_[d], s_);
+        for (int d = 0; d < ndirs_; d++) {
+            batch_apply_deletions(g_, d, max_arcs, bs_[d], s_);
+            slot_dir_update(g_, d, bs_[d].touched, bs_[d].n_touched, max_arcs, s_);
+        }
         prof_end(kKApply);
     }
     if (info.inserts) {
@@ -789,7 +859,10 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         prof_begin(kKApply);
-        for (int d = 0; d < ndirs_; d++) batch_apply_insertions(g_, d, max_arcs, bs_[d], s_);
+        for (int d = 0; d < ndirs_; d++) {
+            batch_apply_insertions(g_, d, max_arcs, bs_[d], s_);
+            slot_dir_update(g_, d, bs_[d].touched, bs_[d].n_touched, max_arcs, s_);
+        }
         prof_end(kKApply);
     }
     for (int d = 0; d < ndirs_; d++)

@@ -32,13 +32,19 @@ struct Technique {
 struct EngineOptions {
     int device = 0;
     uint64_t pool_bytes = 0;          // HBM budget for the device pool (0: no cap)
-    uint64_t ring_bytes = 1ull << 30; // partial-result ring
+    uint64_t ring_bytes = 4ull << 30; // partial-result ring (two halves)
     uint64_t cp_list_bytes = 1ull << 30;
     uint32_t cp_max_records = 1u << 22;
     uint32_t max_batch_edges = 0;     // 0: the stream's batch size
     uint32_t semi_bfs_threshold = kSemiBfsThreshold;
     bool directed = false;
+    bool split_roots = true;          // spread heavy records of the counting DFS over many waves
+    bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
+    bool level_kernel = true;         // single-level extensions load-balanced over candidates
 };
+
+constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
+constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
 
 struct QueryResult {
     uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;
@@ -109,7 +115,7 @@ class Engine {
 
     // Kernel families timed with HIP events on the engine's stream when profiling is enabled.
     enum KernelFamily : uint32_t { kKExtendCount = 0, kKExtendWrite, kKCpEnumerate, kKLocalIndex, kKBatchBuild, kKApply, kKCpPrepare,
-                                   kKAnchors, kKFamilies };
+                                   kKAnchors, kKLevelCount, kKLevelWrite, kKFamilies };
     struct KernelStat {
         uint64_t launches = 0;
         double ms = 0;
@@ -149,6 +155,7 @@ class Engine {
     uint32_t key_of_edge(uint32_t nbr, uint32_t elabel_raw) const;
     std::string check_err(const char* where);
     uint32_t read_u32(const uint32_t* p);
+    unsigned long long* work_counter() { return reinterpret_cast<unsigned long long*>(counters_ + 8); }
 
     EngineOptions opt_;
     hipStream_t s_ = nullptr;
@@ -184,7 +191,7 @@ class Engine {
     uint32_t* n_pairs_ = nullptr;
     uint32_t* ring_ = nullptr;
     uint64_t ring_words_ = 0;
-    uint32_t* counters_ = nullptr;  // [0]: ring A count, [1]: ring B count, [2]: overflow, [3]: work, [4]: list len
+    uint32_t* counters_ = nullptr;  // [0]: ring A count, [1]: ring B count, [2]: overflow, [8..9]: u64 work counter
     CpScratch cp_{};
     unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash
     unsigned long long* summary_ = nullptr;   // [2]

@@ -35,7 +35,7 @@ typedef struct {
     int32_t device;          /* main.cpp:168 --device */
     int32_t directed;        /* extension: arcs src -> dst only (the reference is undirected) */
     uint64_t pool_bytes;     /* HBM budget for the device pool, 0 = no cap */
-    uint64_t ring_bytes;     /* partial-result ring (config.h:39 RES_SPACE), 0 = 1 GiB */
+    uint64_t ring_bytes;     /* partial-result ring (config.h:39 RES_SPACE), 0 = 4 GiB */
     uint64_t cp_list_bytes;  /* Cartesian-product leaf lists (config.h:43 SIZE_SPACE), 0 = 1 GiB */
     uint32_t max_batch_edges;/* largest batch the scratch is sized for, 0 = the stream's batch */
     uint32_t semi_bfs_threshold; /* config.h:47 MIN_NUM_RESULTS_TO_GPU, 0 = 65536 */
@@ -105,9 +105,10 @@ csm_status csm_read_adjacency(csm_session* s, uint32_t v, uint32_t direction, ui
 csm_status csm_synchronize(csm_session* s);
 
 /* Measurement (no counterpart in the reference, whose timing is the TIME_* macros of
- * utils/cuda_helpers.h:78-102).  Kernel families: 0 enumeration (count), 1 enumeration (write a
- * level), 2 Cartesian product, 3 local index, 4 batch build, 5 apply batch, 6 Cartesian-product
- * preparation, 7 anchors.  Bit f of mask times family f with HIP events on the session's stream and,
+ * utils/cuda_helpers.h:78-102).  Kernel families: 0 depth-first enumeration (count), 1 depth-first
+ * enumeration (write partial results), 2 Cartesian product, 3 local index, 4 batch build, 5 apply
+ * batch, 6 Cartesian-product preparation, 7 anchors, 8 one-level extension (count), 9 one-level
+ * extension (write a level).  Bit f of mask times family f with HIP events on the session's stream and,
  * for the enumeration kernels, counts algorithmic bytes (every view run read once plus candidates
  * written). */
 csm_status csm_profile_enable(csm_session* s, uint32_t mask);

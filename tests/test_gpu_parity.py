@@ -173,3 +173,69 @@ def test_adjacency_after_stream(lib, tmp_path):
             got = s.adjacency(v)
             assert sorted(got) == sorted(nbrs.get(v, ())), f"vertex {v}"
             assert len(set(got)) == len(got)
+
+
+@pytest.mark.parametrize("split,sdir,level", [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)])
+def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, level):
+    """Root splitting, the label-slot directory and the one-level kernel change scheduling and
+    lookup, never results."""
+    monkeypatch.setenv("CSM_SPLIT_ROOTS", str(split))
+    monkeypatch.setenv("CSM_SLOT_DIRECTORY", str(sdir))
+    monkeypatch.setenv("CSM_LEVEL_KERNEL", str(level))
+    meta, queries = util.gen("tinyins", tmp_path, seed=7)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    want, _ = util.oracle_stream(data, upd, 64, queries)
+    for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 2, True, False), (0, 1, True, True)]:
+        say(f"case split={split} sdir={sdir} level={level} order={mo} strategy={ss} local={li} cp={cp}")
+        got = run_library(lib, str(data), str(upd), 64, queries, lib.technique(mo, ss, li, cp))
+        if got is not None:
+            assert got == want
+    (tmp_path / "c1").mkdir()
+    meta, queries = util.gen("c1", tmp_path / "c1", seed=3)
+    data, upd = tmp_path / "c1" / "data.graph", tmp_path / "c1" / "update.stream"
+    want, _ = util.oracle_stream(data, upd, 2000, queries, threads=8)
+    say(f"case split={split} sdir={sdir} level={level} c1 batch=2000")
+    assert run_library(lib, str(data), str(upd), 2000, queries, lib.technique()) == want
+
+
+def test_c2_shape_scaled_stream(lib, tmp_path):
+    """The benchmark's workload family (Chung-Lu, Zipf labels, 90/10 insert/delete stream, six
+    sparse and dense queries) at 5% scale: every batch bit-exact for the default method and the
+    QO-GAMMA switches."""
+    meta, queries = util.gen("c2", tmp_path, seed=5, scale=0.05)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = int(meta["batch_size"])
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=16)
+    assert sum(v[0] + v[1] for v in want.values()) > 10 ** 6
+    assert any(v[1] for v in want.values())  # deletions destroy matches too
+    for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True)]:
+        say(f"case c2 scale=0.05 seed=5 order={mo} strategy={ss} local={li} cp={cp}")
+        assert run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp)) == want
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_directed_stream(lib, tmp_path, seed):
+    """Directed extension: the same files read as arcs src -> dst, both sides."""
+    meta, queries = util.gen("tiny", tmp_path, seed=seed)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    want, _ = util.oracle_stream(data, upd, 50, queries, directed=True)
+    for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (0, 3, True, False)]:
+        say(f"case directed tiny seed={seed} order={mo} strategy={ss} local={li} cp={cp}")
+        got = run_library(lib, str(data), str(upd), 50, queries, lib.technique(mo, ss, li, cp), directed=True)
+        if got is not None:
+            assert got == want
+
+
+def test_small_ring_and_record_chunks(lib, tmp_path, monkeypatch):
+    """Levels that do not fit the ring are skipped or abandoned (depth-first from the level before),
+    and levels of more records than the per-launch scratch are processed in chunks: same results."""
+    monkeypatch.setenv("CSM_CP_MAX_RECORDS", "1024")
+    meta, queries = util.gen("c2", tmp_path, seed=6, scale=0.02)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = int(meta["batch_size"])
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=16)
+    for ring in [0, 4 << 20]:
+        for (mo, ss, li, cp) in [(2, 2, True, True), (2, 3, True, False)]:
+            say(f"case c2 scale=0.02 seed=6 ring={ring} cp_max_records=1024 order={mo} strategy={ss} cp={cp}")
+            got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp), ring_bytes=ring)
+            assert got == want

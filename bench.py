@@ -23,11 +23,11 @@ ORACLE = os.path.join(ROOT, "oracle", "bin", "oracle")
 METRIC = "incremental matches/s & update edges/s on 1 GPU; p50/p99 batch latency"
 HBM_PEAK_GBS = 8000.0
 FAMILIES = ["enumerate_count", "enumerate_write", "cartesian_product", "local_index", "batch_build", "apply_batch",
-            "cartesian_prepare", "anchors", "level_count", "level_write"]
+            "cartesian_prepare", "anchors", "level_count", "level_write", "level_prepare"]
 # families whose kernels count algorithmic bytes on the device (the roofline candidates)
 BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_write"]
-KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level<0>",
-             "level_write": "k_level<1>"}
+KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat<0>",
+             "level_write": "k_level_flat<1>"}
 WORKLOADS = {
     "c2": "plain method on one GPU: Chung-Lu 400K vertices / ~3.4M edges, gamma 2.1, max degree ~3000, "
           "Zipf(1.0) labels over 32, 10% held out, 90/10 insert/delete stream, batches of 65,536 edges, "
@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--data-dir", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-mask", type=int, default=0x3FF)
+    ap.add_argument("--profile-mask", type=int, default=0x7FF)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of 2^26 words (HBM traffic counter calibration)")
     args = ap.parse_args()

@@ -242,10 +242,16 @@ __global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint3
     }
 }
 
+// Each wave walks one contiguous range of the level's candidates (equal shares, multiples of 64).
+// It finds the record of its first candidate with a 64-ary search; per 64 candidates it then loads
+// the next 65 record offsets into LDS with one coalesced load and each lane finds its record there
+// (a global search only for lanes past the window, when records have under one candidate each).
 template <int kMode>
 __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
                                                     const unsigned long long* off, const unsigned long long* ph, Ring r, MatchOut out) {
+    __shared__ unsigned long long s_win[4][65];
     const uint32_t lane = lane_id();
+    unsigned long long* win = s_win[threadIdx.x >> 6];
     const uint32_t p = d.pivot[d0];
     const BackEntry pe = d.be[p];
     const uint32_t* pdl = phase_delta(g, pe.dir, d.phase);
@@ -253,21 +259,42 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
     const uint32_t* filter = d.filter[d0];
     const uint32_t qv = d.order[d0];
     const unsigned long long total = off[n];
-    const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * (blockDim.x >> 6);
+    const unsigned long long gw = static_cast<unsigned long long>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const unsigned long long per = ((total + waves - 1) / waves + 63) & ~63ull;
+    const unsigned long long w0 = gw * per, w1 = min(total, w0 + per);
     unsigned long long cnt = 0, hsum = 0, bytes = 0;
-    // every lane of a wave runs the same number of iterations (ballots below)
-    for (unsigned long long t0 = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); t0 < total; t0 += stride) {
+    uint32_t rb = w0 < total ? find_record(off, n, w0, lane) : 0u;  // off[rb] <= w0
+    for (unsigned long long t0 = w0; t0 < w1; t0 += 64) {
+        // a level that overflows is redone by the host with fewer records: stop early (polled every
+        // 64 iterations; a load per iteration would sit on every iteration's latency chain)
+        if (kMode == kWriteAtEnd && ((t0 - w0) & (64ull * 64 - 1)) == 0 && t0 != w0 &&
+            uniform(__hip_atomic_load(r.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+            break;
+        win[lane] = off[min(rb + lane, n)];
+        if (lane == 0) win[64] = off[min(rb + 64, n)];
+        wave_sync();
         const unsigned long long t = t0 + lane;
         bool ok = false;
-        uint32_t c = 0, k = 0;
-        if (t < total) {
-            uint32_t lo = 0, hi = n;  // last k with off[k] <= t
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (off[mid] <= t) lo = mid;
-                else hi = mid;
+        uint32_t c = 0, k = rb;
+        if (t < w1) {
+            if (win[64] <= t) {  // past the window: last k in [rb + 64, n) with off[k] <= t
+                uint32_t lo = rb + 64, hi = n;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (off[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                k = lo;
+            } else {  // last j < 64 with win[j] <= t (win[0] <= t0 <= t)
+                uint32_t lo = 0, hi = 64;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (win[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                k = rb + lo;
             }
-            k = lo;
             const uint32_t* f = rec + static_cast<uint64_t>(k) * d0;
             const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
             const uint32_t key = run_elem(g, pdl, rv, run_base_len(rv, pe.view), static_cast<uint32_t>(t - off[k]));
@@ -304,6 +331,8 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
                 }
             }
         }
+        rb = uniform(__shfl(k, 63, 64));  // lane 63 holds the highest record of this window (all lanes valid unless last)
+        wave_sync();                     // the window is rewritten next iteration
     }
     if (kMode == kCountAtEnd) {
         cnt = wave_sum_u64(cnt);
@@ -412,13 +441,13 @@ __global__ void __launch_bounds__(256) k_cp_prepare(DevGraph g, StepDesc d, uint
         unsigned long long h = 0;
         for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
         if (overflow) {
-            if (lane == 0) atomicOr(g.err, kErrProductOverflow);
+            if (lane == 0) atomicOr(c.err, kErrProductOverflow);
             prod = 0;
         }
         unsigned long long base = 0;
         if (prod && lane == 0) {
             base = atomicAdd(c.lists_top, static_cast<unsigned long long>(total));
-            if (base + total > c.lists_cap) { atomicOr(g.err, kErrScratchFull); base = ~0ull; }
+            if (base + total > c.lists_cap) { atomicOr(c.err, kErrScratchFull); base = ~0ull; }
         }
         base = __shfl(base, 0, 64);
         if (base == ~0ull) prod = 0;

@@ -112,7 +112,7 @@ struct MatchOut {
 };
 
 struct Ring {
-    uint32_t* in;               // records of in_depth words
+    const uint32_t* in;         // records of in_depth words
     const uint32_t* in_count;   // device count
     uint32_t in_max;            // host bound on the count
     uint32_t* out;              // output records (write modes)
@@ -166,6 +166,7 @@ struct CpScratch {
     unsigned long long* lists_top; // [1]
     unsigned long long lists_cap;
     unsigned long long* max_prod;  // [1]
+    uint32_t* err;                 // kErrProductOverflow / kErrScratchFull of this scratch's last preparation
 };
 void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
                 uint32_t max_rec, CpScratch c, hipStream_t s);

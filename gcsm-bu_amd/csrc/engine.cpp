@@ -5,7 +5,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -47,7 +49,16 @@ void* DevicePool::take_bytes(uint64_t bytes) {
 // ---- engine -----------------------------------------------------------------------------------
 Engine::~Engine() {
     if (s_) (void)hipStreamSynchronize(s_);
-    for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+    for (Lane& w : lanes_)
+        if (w.s) (void)hipStreamSynchronize(w.s);
+    for (hipEvent_t e : prof_main_.pool) (void)hipEventDestroy(e);
+    for (Lane& w : lanes_) {
+        for (hipEvent_t e : w.prof.pool) (void)hipEventDestroy(e);
+        if (w.done) (void)hipEventDestroy(w.done);
+        if (w.h_pinned) (void)hipHostFree(w.h_pinned);
+        if (w.s) (void)hipStreamDestroy(w.s);
+    }
+    if (phase_start_) (void)hipEventDestroy(phase_start_);
     if (trace_) std::fclose(trace_);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
     if (s_) (void)hipStreamDestroy(s_);
@@ -62,11 +73,12 @@ std::string Engine::init(const EngineOptions& o) {
     CSM_HIP(hipSetDevice(o.device));
     CSM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096, hipHostMallocDefault));
+    CSM_HIP(hipEventCreateWithFlags(&phase_start_, hipEventDisableTiming));
     if (const char* t = std::getenv("CSM_TRACE")) trace_ = std::fopen(t, "a");
+    if (const char* t = std::getenv("CSM_LANES")) opt_.lanes = static_cast<uint32_t>(std::max(1, std::atoi(t)));
     // A/B switches of the scheduling and layout aids (all on by default)
     if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) opt_.split_roots = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_SLOT_DIRECTORY")) opt_.slot_directory = std::atoi(t) != 0;
-    if (const char* t = std::getenv("CSM_LEVEL_KERNEL")) opt_.level_kernel = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_CP_MAX_RECORDS")) opt_.cp_max_records = static_cast<uint32_t>(std::max(1024L, std::atol(t)));
     return "";
 }
@@ -124,10 +136,10 @@ uint32_t Engine::key_of_edge(uint32_t nbr, uint32_t elabel_raw) const {
     return (slot << id_bits_) | nbr;
 }
 
-uint32_t Engine::read_u32(const uint32_t* p) {
-    (void)hipMemcpyAsync(h_pinned_, p, 4, hipMemcpyDeviceToHost, s_);
-    (void)hipStreamSynchronize(s_);
-    return h_pinned_[0];
+uint32_t Engine::read_u32(hipStream_t st, uint32_t* pinned, const uint32_t* p) {
+    (void)hipMemcpyAsync(pinned, p, 4, hipMemcpyDeviceToHost, st);
+    (void)hipStreamSynchronize(st);
+    return pinned[0];
 }
 
 std::string Engine::build() {
@@ -197,12 +209,16 @@ std::string Engine::build() {
         add(scan_scratch_elems(static_cast<uint32_t>(max_arcs)) * 4); add(16); add(2 * kMaxBigSegments * 4);
     }
     add(stream_.size() * 17ull + max_batch_ * 17ull);          // stream + live batch
-    add(kMaxQV * (words_ * 4ull + dv_ * 4ull) + 128);            // local sets
-    add(dv_ * 4ull + 16);                                        // tmp list
-    add(max_arcs * 8 + 16);                                      // anchor pairs
-    add(ring_words_ * 4); add(256);
-    add((cp_rec + 1) * 8); add(cp_rec * 8); add(scan_scratch_elems(static_cast<uint32_t>(cp_rec)) * 8);
-    add(cp_rec * 4); add(cp_rec * kMaxQV * 4); add(cp_list_words * 4); add(64);
+    const uint32_t n_lanes = std::max<uint32_t>(1, std::min<uint32_t>(opt_.lanes, kMaxLanes));
+    for (uint32_t l = 0; l < n_lanes; l++) {  // per lane: mirrors the takes of alloc below
+        add(kMaxQV * (words_ * 4ull + dv_ * 4ull + 512) + 128);  // local sets
+        add(dv_ * 4ull + 16);                                      // tmp list
+        add(max_arcs * 8 + 16);                                    // anchor pairs
+        add(ring_words_ * 4); add(256);
+        add((cp_rec + 1) * 8); add(cp_rec * 8); add(scan_scratch_elems(static_cast<uint32_t>(cp_rec)) * 8);
+        add(cp_rec * 4); add(cp_rec * kMaxQV * 4); add(cp_list_words * 4); add(64);
+        add(64 << 10);                                             // alignment slack of the takes
+    }
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
     add(qs_.size() * 32 + 64);
     add(kKFamilies * 8);
@@ -340,31 +356,42 @@ std::string Engine::build() {
         }
     }
 
-    // ---- runtime buffers ----
-    for (uint32_t u = 0; u < kMaxQV; u++) {
-        L_[u].bitmap = pool_.take<uint32_t>(words_);
-        L_[u].list = pool_.take<uint32_t>(dv_);
-        L_[u].cap = dv_;
-        CSM_HIP(hipMemsetAsync(L_[u].bitmap, 0, words_ * 4ull, s_));
+    // ---- runtime buffers: one set per lane ----
+    lanes_.assign(n_lanes, Lane{});
+    for (Lane& w : lanes_) {
+        CSM_HIP(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking));
+        CSM_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+        CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&w.h_pinned), 4096, hipHostMallocDefault));
+        for (uint32_t u = 0; u < kMaxQV; u++) {
+            w.L[u].bitmap = pool_.take<uint32_t>(words_);
+            w.L[u].list = pool_.take<uint32_t>(dv_);
+            w.L[u].cap = dv_;
+            if (!w.L[u].bitmap || !w.L[u].list) return "device pool exhausted (local sets)";
+            CSM_HIP(hipMemsetAsync(w.L[u].bitmap, 0, words_ * 4ull, s_));
+        }
+        uint32_t* lens = pool_.take<uint32_t>(32);
+        if (!lens) return "device pool exhausted (local sets)";
+        for (uint32_t u = 0; u < kMaxQV; u++) w.L[u].len = lens + u;
+        w.tmp_len = lens + kMaxQV;
+        w.n_pairs = lens + kMaxQV + 1;
+        w.tmp_list = pool_.take<uint32_t>(dv_);
+        w.pairs = pool_.take<uint32_t>(2 * max_arcs);
+        w.ring = pool_.take<uint32_t>(ring_words_);
+        w.ring_words = ring_words_;
+        w.counters = pool_.take<uint32_t>(16);
+        w.cp.prod = pool_.take<unsigned long long>(cp_rec + 1);
+        w.cp.rec_hash = pool_.take<unsigned long long>(cp_rec);
+        w.cp.scan_tmp = pool_.take<unsigned long long>(scan_scratch_elems(static_cast<uint32_t>(cp_rec)));
+        w.cp.list_base = pool_.take<uint32_t>(cp_rec);
+        w.cp.leaf_len = pool_.take<uint32_t>(cp_rec * kMaxQV);
+        w.cp.lists = pool_.take<uint32_t>(cp_list_words);
+        w.cp.lists_cap = cp_list_words;
+        unsigned long long* cpc = pool_.take<unsigned long long>(4);
+        if (!cpc) return "device pool exhausted (lane scratch)";
+        w.cp.lists_top = cpc;
+        w.cp.max_prod = cpc + 1;
+        w.cp.err = reinterpret_cast<uint32_t*>(cpc + 2);
     }
-    uint32_t* lens = pool_.take<uint32_t>(32);
-    for (uint32_t u = 0; u < kMaxQV; u++) L_[u].len = lens + u;
-    tmp_len_ = lens + kMaxQV;
-    n_pairs_ = lens + kMaxQV + 1;
-    tmp_list_ = pool_.take<uint32_t>(dv_);
-    pairs_ = pool_.take<uint32_t>(2 * max_arcs);
-    ring_ = pool_.take<uint32_t>(ring_words_);
-    counters_ = pool_.take<uint32_t>(16);
-    cp_.prod = pool_.take<unsigned long long>(cp_rec + 1);
-    cp_.rec_hash = pool_.take<unsigned long long>(cp_rec);
-    cp_.scan_tmp = pool_.take<unsigned long long>(scan_scratch_elems(static_cast<uint32_t>(cp_rec)));
-    cp_.list_base = pool_.take<uint32_t>(cp_rec);
-    cp_.leaf_len = pool_.take<uint32_t>(cp_rec * kMaxQV);
-    cp_.lists = pool_.take<uint32_t>(cp_list_words);
-    cp_.lists_cap = cp_list_words;
-    unsigned long long* cpc = pool_.take<unsigned long long>(4);
-    cp_.lists_top = cpc;
-    cp_.max_prod = cpc + 1;
     opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
     results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);
     prof_bytes_ = pool_.take<unsigned long long>(kKFamilies);
@@ -463,7 +490,7 @@ std::string Engine::plan(uint32_t qid, const Technique& t) {
 }
 
 std::string Engine::check_err(const char* where) {
-    uint32_t e = read_u32(g_.err);
+    uint32_t e = read_u32(s_, h_pinned_, g_.err);
     last_err_ = e;
     if (!e) return "";
     (void)hipMemsetAsync(g_.err, 0, 4, s_);
@@ -483,8 +510,9 @@ uint32_t Engine::view_of(const QState& qs, uint32_t rel, uint32_t i, uint32_t ph
     return kViewBase;
 }
 
-std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
+std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
     const Query& q = qs.q;
+    w.sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
     const Plan& p = qs.plan;
     const uint32_t u0 = q.qe_list[i].first, u1 = q.qe_list[i].second;
     const uint32_t r01 = q.rel(u0, u1);
@@ -493,17 +521,17 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
 
     // clear the local sets of the previous step
     for (uint32_t u = 0; u < kMaxQV; u++)
-        if (L_used_ >> u & 1u) clear_bits(L_[u].list, L_[u].len, dv_, L_[u].bitmap, s_);
-    L_used_ = 0;
-    fill_u32(L_[0].len, 0u, kMaxQV + 2, s_);
+        if (w.L_used >> u & 1u) clear_bits(w.L[u].list, w.L[u].len, dv_, w.L[u].bitmap, w.s);
+    w.L_used = 0;
+    fill_u32(w.L[0].len, 0u, kMaxQV + 2, w.s);
 
     const RelInfo& R = qs.rel[r01];
-    prof_begin(kKAnchors);
-    local_anchors(g_, ph, R.dir, R.slot, qs.vrank[u0], cand(u0), cand(u1), bs_[R.dir].touched, bs_[R.dir].n_touched, max_arcs, pairs_,
-                  n_pairs_, max_arcs, L_[u0], L_[u1], s_);
-    prof_end(kKAnchors);
-    L_used_ |= static_cast<uint16_t>((1u << u0) | (1u << u1));
-    const uint32_t np = read_u32(n_pairs_);
+    prof_begin(w.prof, w.s, kKAnchors);
+    local_anchors(g_, ph, R.dir, R.slot, qs.vrank[u0], cand(u0), cand(u1), bs_[R.dir].touched, bs_[R.dir].n_touched, max_arcs, w.pairs,
+                  w.n_pairs, max_arcs, w.L[u0], w.L[u1], w.s);
+    prof_end(w.prof, w.s, kKAnchors);
+    w.L_used |= static_cast<uint16_t>((1u << u0) | (1u << u1));
+    const uint32_t np = read_u32(w.s, w.h_pinned, w.n_pairs);
     if (np == 0) return "empty";
     uint32_t built = (1u << u0) | (1u << u1);
 
@@ -525,27 +553,27 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
             }
             const uint32_t b0 = bsv[first];
             const uint32_t rb = q.rel(b0, u);
-            prof_begin(kKLocalIndex);
-            local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, L_[b0].list, L_[b0].len, dv_, cand(u), L_[u], s_);
-            L_used_ |= static_cast<uint16_t>(1u << u);
+            prof_begin(w.prof, w.s, kKLocalIndex);
+            local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u], w.s);
+            w.L_used |= static_cast<uint16_t>(1u << u);
             for (size_t k = 0; k < bsv.size(); k++) {
                 if (k == first) continue;
                 const uint32_t b = bsv[k];
                 const uint32_t rub = q.rel(u, b);
-                fill_u32(tmp_len_, 0u, 1, s_);
-                local_filter(g_, ph, view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, L_[u].list, L_[u].len, dv_, L_[b].bitmap,
-                             tmp_list_, tmp_len_, s_);
-                clear_bits(L_[u].list, L_[u].len, dv_, L_[u].bitmap, s_);
-                copy_list(tmp_list_, tmp_len_, dv_, L_[u].list, L_[u].len, s_);
-                set_bits(L_[u].list, L_[u].len, dv_, L_[u].bitmap, s_);
+                fill_u32(w.tmp_len, 0u, 1, w.s);
+                local_filter(g_, ph, view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[u].list, w.L[u].len, dv_, w.L[b].bitmap,
+                             w.tmp_list, w.tmp_len, w.s);
+                clear_bits(w.L[u].list, w.L[u].len, dv_, w.L[u].bitmap, w.s);
+                copy_list(w.tmp_list, w.tmp_len, dv_, w.L[u].list, w.L[u].len, w.s);
+                set_bits(w.L[u].list, w.L[u].len, dv_, w.L[u].bitmap, w.s);
             }
-            prof_end(kKLocalIndex);
+            prof_end(w.prof, w.s, kKLocalIndex);
             built |= 1u << u;
         }
-        CSM_HIP(hipMemcpyAsync(h_pinned_, L_[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, s_));
-        CSM_HIP(hipStreamSynchronize(s_));
+        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.L[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, w.s));
+        CSM_HIP(hipStreamSynchronize(w.s));
         for (uint32_t u = 0; u < q.n; u++)
-            if ((built >> u & 1u) && h_pinned_[u] == 0) return "empty";
+            if ((built >> u & 1u) && w.h_pinned[u] == 0) return "empty";
     }
 
     // ---- step descriptor ----
@@ -560,7 +588,7 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
         d.vrank[k] = qs.vrank[u];
         d.boff[k] = static_cast<uint8_t>(ne);
         d.pivot[k] = static_cast<uint8_t>(ne);
-        d.filter[k] = (built >> u & 1u) ? L_[u].bitmap : cand(u);
+        d.filter[k] = (built >> u & 1u) ? w.L[u].bitmap : cand(u);
         if (k == 0) continue;
         bool have_delta = false;
         for (uint32_t kk = o.bni_offs[k]; kk < o.bni_offs[k + 1]; kk++) {
@@ -577,242 +605,256 @@ std::string Engine::run_step(QState& qs, uint32_t i, uint32_t ph) {
     }
     d.boff[q.n] = static_cast<uint8_t>(ne);
 
-    // ---- initial partial results (depth 2) into ring half A ----
-    const uint64_t half = ring_words_ / 2;
-    uint32_t* A = ring_;
-    uint32_t* B = ring_ + half;
-    uint32_t* cA = counters_;
-    uint32_t* cB = counters_ + 1;
-    uint32_t* ovf = counters_ + 2;
-    fill_u32(counters_, 0u, 10, s_);
+    // ---- initial partial results (depth 2) at the bottom of the lane's ring ----
+    uint32_t* ring = w.ring;
+    fill_u32(w.counters, 0u, 10, w.s);
     const uint32_t a = o.vs[0], b = o.vs[1];
     uint32_t count;
     if ((a == u0 && b == u1) || (a == u1 && b == u0)) {
-        if (2ull * np > half) return "partial-result ring too small for the batch's anchors";
-        init_from_pairs(pairs_, n_pairs_, max_arcs, a == u1, A, cA, s_);
+        if (2ull * np > w.ring_words) return "partial-result ring too small for the batch's anchors";
+        init_from_pairs(w.pairs, w.n_pairs, max_arcs, a == u1, ring, w.counters, w.s);
         count = np;
     } else {
         const uint32_t rab = q.rel(a, b);
         const uint32_t* list;
         const uint32_t* len;
-        if (built >> a & 1u) { list = L_[a].list; len = L_[a].len; }
+        if (built >> a & 1u) { list = w.L[a].list; len = w.L[a].len; }
         else {
-            fill_u32(tmp_len_, 0u, 1, s_);
-            bitmap_to_list(cand(a), dv_, tmp_list_, tmp_len_, s_);
-            list = tmp_list_;
-            len = tmp_len_;
+            fill_u32(w.tmp_len, 0u, 1, w.s);
+            bitmap_to_list(cand(a), dv_, w.tmp_list, w.tmp_len, w.s);
+            list = w.tmp_list;
+            len = w.tmp_len;
         }
-        Ring r{nullptr, nullptr, 0, A, cA, static_cast<uint32_t>(std::min<uint64_t>(half / 2, 0xFFFFFFFFull)), ovf};
-        init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, s_);
-        CSM_HIP(hipMemcpyAsync(h_pinned_, counters_, 12, hipMemcpyDeviceToHost, s_));
-        CSM_HIP(hipStreamSynchronize(s_));
-        if (h_pinned_[2]) return "partial-result ring too small for the initial partial results";
-        count = h_pinned_[0];
+        Ring r{nullptr, nullptr, 0, ring, w.counters, static_cast<uint32_t>(std::min<uint64_t>(w.ring_words / 2, 0xFFFFFFFFull)),
+               w.counters + 2};
+        init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, w.s);
+        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
+        CSM_HIP(hipStreamSynchronize(w.s));
+        if (w.h_pinned[2]) return "partial-result ring too small for the initial partial results";
+        count = w.h_pinned[0];
     }
-    if (trace_) trace_line_ += ", \"anchors\": " + std::to_string(np) + ", \"initial\": " + std::to_string(count);
+    if (trace_) w.trace_line += ", \"anchors\": " + std::to_string(np) + ", \"initial\": " + std::to_string(count);
     if (count == 0) return "";
     const uint32_t res_slot = static_cast<uint32_t>(std::find_if(qs_.begin(), qs_.end(), [&](const std::unique_ptr<QState>& x) { return x.get() == &qs; }) - qs_.begin());
     if (q.n == 2) {
-        count_records(d, A, cA, count, MatchOut{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr}, s_);
+        count_records(d, ring, w.counters, count, MatchOut{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr}, w.s);
         return "";
     }
 
-    const int sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
-    uint32_t depth = 2;
-    uint32_t* cur = A;
-    uint32_t* cur_cnt = cA;
-    uint32_t* oth = B;
-    uint32_t* oth_cnt = cB;
-    // One level (or a DFS down to `end`) written to the other half.  ok = false when the ring
-    // half cannot hold it (the records at `depth` stay intact).  With `check_fit`, a one-level
-    // write first compares the level's candidate count (an upper bound of its records) with the
-    // ring half and skips the write when it cannot fit.
-    auto write_to = [&](uint32_t end, bool check_fit, bool& ok) -> std::string {
-        const uint64_t cap = std::min<uint64_t>(half / end, 0xFFFFFFFFull);
-        fill_u32(oth_cnt, 0u, 1, s_);
-        fill_u32(ovf, 0u, 1, s_);
-        fill_u32(counters_ + 8, 0u, 2, s_);
-        Ring r{cur, cur_cnt, count, oth, oth_cnt, static_cast<uint32_t>(cap), ovf};
-        if (opt_.level_kernel && end == depth + 1) {
-            uint64_t cands = 0;
-            for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
-                const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
-                const uint32_t* rec = cur + static_cast<uint64_t>(s0) * depth;
-                prof_begin(kKLevelWrite);
-                level_prepare(g_, d, depth, rec, nc, cp_.prod, nullptr, cp_.scan_tmp, bytes_ptr(kKLevelWrite), s_);
-                if (check_fit) {
-                    CSM_HIP(hipMemcpyAsync(h_pinned_ + 8, cp_.prod + nc, 8, hipMemcpyDeviceToHost, s_));
-                    CSM_HIP(hipStreamSynchronize(s_));
-                    uint64_t t;
-                    std::memcpy(&t, h_pinned_ + 8, 8);
-                    cands += t;
-                    if (cands > cap) {
-                        prof_end(kKLevelWrite);
-                        if (trace_) trace_line_ += ", \"level_" + std::to_string(end) + "\": \"skipped\"";
-                        ok = false;
-                        return "";
-                    }
-                }
-                level_flat(g_, d, depth, rec, nc, kWriteAtEnd, cp_.prod, nullptr, r, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, s_);
-                prof_end(kKLevelWrite);
-            }
-        } else {
-            prof_begin(kKExtendWrite);
-            extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, nullptr, work_counter(), sched, s_);
-            prof_end(kKExtendWrite);
-        }
-        CSM_HIP(hipMemcpyAsync(h_pinned_, counters_, 12, hipMemcpyDeviceToHost, s_));
-        CSM_HIP(hipStreamSynchronize(s_));
-        ok = h_pinned_[2] == 0;
-        if (trace_) trace_line_ += ", \"level_" + std::to_string(end) + "\": " + (ok ? std::to_string(oth == B ? h_pinned_[1] : h_pinned_[0]) : std::string("\"overflow\""));
-        if (ok) {
-            count = oth == B ? h_pinned_[1] : h_pinned_[0];
-            std::swap(cur, oth);
-            std::swap(cur_cnt, oth_cnt);
-            depth = end;
-        }
-        return "";
-    };
     const uint8_t* cpi = p.cp_info[i];
     auto tail_from = [&](uint32_t from) {
         uint32_t nd = from;
         while (nd < q.n && cpi[nd] != kCpTreeProduct && cpi[nd] != kCpTreeSingle) nd++;
         return nd;
     };
-
-    std::string err;
+    const uint64_t top = 2ull * count;  // first free word of the ring
     switch (qs.tech.strategy) {
-        case SearchStrategy::kDynamicBfsDfs: {  // match_gpu.cu:1916-2127
-            // Breadth-first while the level is small (the reference's rule).  With one-level
-            // flattened kernels, also past the threshold while the next level fits the ring half,
-            // stopping where the Cartesian product takes over; depth-first from there otherwise.
-            bool oom = false;
-            while (q.n - depth > 1) {
-                const bool small = count < opt_.semi_bfs_threshold;
-                if (!small && !opt_.level_kernel) break;
-                if (!small && qs.tech.cartesian_product && cpi[depth] == kCpTreeProduct) break;
-                bool ok;
-                err = write_to(depth + 1, !small, ok);
-                if (!err.empty()) return err;
-                if (!ok) { oom = true; break; }
-                if (count == 0) return "";
-            }
-            if (!oom && cpi[depth] != kCpTreeProduct && q.n - depth > 2) {
-                const uint32_t nd = tail_from(depth);
-                if (nd < q.n) {
-                    bool ok;
-                    err = write_to(nd, false, ok);
-                    if (!err.empty()) return err;
-                    if (ok && count == 0) return "";
-                }
-            }
-            break;
-        }
-        case SearchStrategy::kBfs: {  // match_gpu.cu:2129-2294
+        case SearchStrategy::kDynamicBfsDfs:  // match_gpu.cu:1916-2127
+            return search(w, qs, d, i, ring, count, 2, top, q.n, res_slot);
+        case SearchStrategy::kBfs: {  // match_gpu.cu:2129-2294: breadth-first to the Cartesian-product tail
             const uint32_t target = qs.tech.cartesian_product ? std::min(tail_from(2), q.n - 1) : q.n - 1;
-            while (depth < target) {
-                bool ok;
-                err = write_to(depth + 1, false, ok);
-                if (!err.empty()) return err;
-                if (!ok) break;
-                if (count == 0) return "";
-            }
-            break;
+            return search(w, qs, d, i, ring, count, 2, top, target, res_slot);
         }
         default: {  // DFS, with or without work stealing (match_gpu.cu:1721-1914)
-            const uint32_t t = tail_from(depth);
-            if (qs.tech.cartesian_product && t < q.n && t > depth && cpi[t] == kCpTreeProduct) {
+            const uint32_t t = tail_from(2);
+            if (qs.tech.cartesian_product && t < q.n && t > 2 && cpi[t] == kCpTreeProduct) {
+                // depth-first down to the tail, written for the Cartesian product
                 bool ok;
-                err = write_to(t, false, ok);
+                uint32_t n_out;
+                uint32_t* out = ring + top;
+                std::string err = extend_to(w, d, ring, count, 2, t, out, (w.ring_words - top) / t, ok, n_out);
                 if (!err.empty()) return err;
-                if (ok && count == 0) return "";
+                if (ok) return n_out ? finish_records(w, qs, d, i, t, n_out, out, res_slot) : "";
             }
-            break;
+            return finish_records(w, qs, d, i, 2, count, ring, res_slot);
         }
     }
-    return finish_records(qs, d, i, depth, count, cur, cur_cnt, res_slot);
 }
 
-std::string Engine::finish_records(QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, uint32_t* rec,
-                                   const uint32_t* rec_count, uint32_t res_slot) {
+// Writes the records of depth `end` that extend the `count` records of depth `depth` at `rec` to
+// `out` (room for `cap` records): one level with the flattened kernels, several with the
+// depth-first kernel.  ok = false when they do not fit.
+std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, uint32_t count, uint32_t depth, uint32_t end, uint32_t* out,
+                              uint64_t cap, bool& ok, uint32_t& n_out) {
+    cap = std::min<uint64_t>(cap, 0xFFFFFFFFull);
+    fill_u32(w.counters, count, 1, w.s);   // [0]: input count
+    fill_u32(w.counters + 1, 0u, 2, w.s);  // [1]: output count, [2]: overflow
+    fill_u32(w.counters + 8, 0u, 2, w.s);  // work counter
+    Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
+    if (end == depth + 1) {
+        for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
+            const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
+            const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
+            prof_begin(w.prof, w.s, kKLevelPrepare);
+            level_prepare(g_, d, depth, rc, nc, w.cp.prod, nullptr, w.cp.scan_tmp, bytes_ptr(kKLevelWrite), w.s);
+            prof_end(w.prof, w.s, kKLevelPrepare);
+            prof_begin(w.prof, w.s, kKLevelWrite);
+            level_flat(g_, d, depth, rc, nc, kWriteAtEnd, w.cp.prod, nullptr, r, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
+            prof_end(w.prof, w.s, kKLevelWrite);
+        }
+    } else {
+        prof_begin(w.prof, w.s, kKExtendWrite);
+        extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, nullptr, w.work_counter(), w.sched,
+               w.s);
+        prof_end(w.prof, w.s, kKExtendWrite);
+    }
+    CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
+    CSM_HIP(hipStreamSynchronize(w.s));
+    ok = w.h_pinned[2] == 0;
+    n_out = w.h_pinned[1];
+    if (trace_) w.trace_line += ", \"level_" + std::to_string(end) + "\": " + (ok ? std::to_string(n_out) : std::string("\"overflow\""));
+    return "";
+}
+
+// Breadth-first levels kept as a stack in the lane's ring: the `count` records of depth `depth` at
+// `rec` are extended one level into the free words above `top`, and so on, until the strategy's
+// finishing depth (the leaf level, `target`, or a Cartesian-product depth once the level holds at
+// least the semi-BFS threshold of records).  A level that does not fit is redone for each half of
+// its input records in turn (one record that does not fit is counted depth-first), so the ring
+// size bounds memory, never the search.
+std::string Engine::search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,
+                           uint64_t top, uint32_t target, uint32_t res_slot) {
+    const uint8_t* cpi = qs.plan.cp_info[i];
+    while (count) {
+        const bool leaf = depth + 1 >= d.n;
+        const bool cp_here = qs.tech.cartesian_product && cpi[depth] == kCpTreeProduct && count >= opt_.semi_bfs_threshold;
+        if (leaf || depth >= target || (target == d.n && cp_here)) return finish_records(w, qs, d, i, depth, count, rec, res_slot);
+        uint32_t* out = w.ring + top;
+        bool ok;
+        uint32_t n_out;
+        std::string err = extend_to(w, d, rec, count, depth, depth + 1, out, (w.ring_words - top) / (depth + 1), ok, n_out);
+        if (!err.empty()) return err;
+        if (!ok) {
+            if (count == 1) return finish_records(w, qs, d, i, depth, count, rec, res_slot);
+            const uint32_t h = count / 2;
+            err = search(w, qs, d, i, rec, h, depth, top, target, res_slot);
+            if (!err.empty()) return err;
+            rec += static_cast<uint64_t>(h) * depth;
+            count -= h;
+            continue;
+        }
+        rec = out;
+        count = n_out;
+        top += static_cast<uint64_t>(n_out) * (depth + 1);
+        depth += 1;
+    }
+    return "";
+}
+
+std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, const uint32_t* rec,
+                                   uint32_t res_slot) {
     const uint32_t ph = d.phase;
     MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr};
-    const int sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
+    uint32_t* rec_count = w.counters + 3;
+    fill_u32(rec_count, count, 1, w.s);
     bool use_cp = qs.tech.cartesian_product && depth < d.n && qs.plan.cp_info[i][depth] == kCpTreeProduct && count <= opt_.cp_max_records;
     if (use_cp) {
         // match_gpu.cu:2046-2085: flatten unless the products are uniformly large
-        CSM_HIP(hipMemsetAsync(cp_.lists_top, 0, 16, s_));
-        prof_begin(kKCpPrepare);
-        cp_prepare(g_, d, depth, rec, rec_count, count, cp_, s_);
-        prof_end(kKCpPrepare);
+        CSM_HIP(hipMemsetAsync(w.cp.lists_top, 0, 24, w.s));  // lists_top, max_prod, err
+        prof_begin(w.prof, w.s, kKCpPrepare);
+        cp_prepare(g_, d, depth, rec, rec_count, count, w.cp, w.s);
+        prof_end(w.prof, w.s, kKCpPrepare);
         unsigned long long v[2];
-        CSM_HIP(hipMemcpyAsync(h_pinned_, cp_.max_prod, 8, hipMemcpyDeviceToHost, s_));
-        CSM_HIP(hipMemcpyAsync(h_pinned_ + 2, cp_.prod + count, 8, hipMemcpyDeviceToHost, s_));
-        CSM_HIP(hipStreamSynchronize(s_));
-        std::memcpy(&v[0], h_pinned_, 8);
-        std::memcpy(&v[1], h_pinned_ + 2, 8);
-        uint32_t e = read_u32(g_.err);
-        if (e & (kErrScratchFull | kErrProductOverflow)) {
-            CSM_HIP(hipMemsetAsync(g_.err, 0, 4, s_));
-            use_cp = false;  // fall back to counting by DFS: same embeddings
+        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.cp.max_prod, 8, hipMemcpyDeviceToHost, w.s));
+        CSM_HIP(hipMemcpyAsync(w.h_pinned + 2, w.cp.prod + count, 8, hipMemcpyDeviceToHost, w.s));
+        CSM_HIP(hipMemcpyAsync(w.h_pinned + 4, w.cp.err, 4, hipMemcpyDeviceToHost, w.s));
+        CSM_HIP(hipStreamSynchronize(w.s));
+        std::memcpy(&v[0], w.h_pinned, 8);
+        std::memcpy(&v[1], w.h_pinned + 2, 8);
+        if (w.h_pinned[4] & (kErrScratchFull | kErrProductOverflow)) {
+            use_cp = false;  // fall back to counting: same embeddings
         } else {
             const float avg = static_cast<float>(v[1]) / static_cast<float>(count);
             const float ratio = static_cast<float>(v[0]) / avg;
             if (ratio < 50.f && avg >= static_cast<float>(kSemiBfsThreshold)) use_cp = false;
         }
-        if (trace_) trace_line_ += ", \"cp_total\": " + std::to_string(v[1]) + ", \"cp_max\": " + std::to_string(v[0]) + ", \"cp_used\": " + std::to_string(use_cp ? 1 : 0);
+        if (trace_) w.trace_line += ", \"cp_total\": " + std::to_string(v[1]) + ", \"cp_max\": " + std::to_string(v[0]) + ", \"cp_used\": " + std::to_string(use_cp ? 1 : 0);
         if (use_cp) {
-            prof_begin(kKCpEnumerate);
-            cp_enumerate(g_, d, depth, rec, rec_count, count, cp_, out, s_);
-            prof_end(kKCpEnumerate);
+            prof_begin(w.prof, w.s, kKCpEnumerate);
+            cp_enumerate(g_, d, depth, rec, rec_count, count, w.cp, out, w.s);
+            prof_end(w.prof, w.s, kKCpEnumerate);
             return "";
         }
     }
-    fill_u32(counters_ + 8, 0u, 2, s_);
+    fill_u32(w.counters + 8, 0u, 2, w.s);
     Ring r{rec, rec_count, count, nullptr, nullptr, 0, nullptr};
-    if (trace_) trace_line_ += ", \"count_from\": " + std::to_string(depth) + ", \"records\": " + std::to_string(count);
-    if (opt_.level_kernel && depth + 1 == d.n) {
+    if (trace_) w.trace_line += ", \"count_from\": " + std::to_string(depth) + ", \"records\": " + std::to_string(count);
+    if (depth + 1 == d.n && w.sched == 1) {
         out.bytes = bytes_ptr(kKLevelCount);
         for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
             const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
             const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
-            prof_begin(kKLevelCount);
-            level_prepare(g_, d, depth, rc, nc, cp_.prod, cp_.rec_hash, cp_.scan_tmp, out.bytes, s_);
-            level_flat(g_, d, depth, rc, nc, kCountAtEnd, cp_.prod, cp_.rec_hash, r, out, s_);
-            prof_end(kKLevelCount);
+            prof_begin(w.prof, w.s, kKLevelPrepare);
+            level_prepare(g_, d, depth, rc, nc, w.cp.prod, w.cp.rec_hash, w.cp.scan_tmp, out.bytes, w.s);
+            prof_end(w.prof, w.s, kKLevelPrepare);
+            prof_begin(w.prof, w.s, kKLevelCount);
+            level_flat(g_, d, depth, rc, nc, kCountAtEnd, w.cp.prod, w.cp.rec_hash, r, out, w.s);
+            prof_end(w.prof, w.s, kKLevelCount);
         }
         return "";
     }
     out.bytes = bytes_ptr(kKExtendCount);
-    // Root splitting (dynamic schedules, few records: a handful of heavy records would otherwise
+    // Root splitting (dynamic schedule, few records: a handful of heavy records would otherwise
     // serialise on single waves).  The CP offsets buffer is free here.
-    const bool split = opt_.split_roots && sched == 1 && count <= kSplitMaxRecords;
-    prof_begin(kKExtendCount);
-    if (split) root_chunks(g_, d, depth, rec, rec_count, count, cp_.prod, cp_.scan_tmp, s_);
-    extend(g_, d, depth, d.n, kCountAtEnd, r, out, split ? cp_.prod : nullptr, work_counter(), sched, s_);
-    prof_end(kKExtendCount);
+    const bool split = opt_.split_roots && w.sched == 1 && count <= kSplitMaxRecords;
+    prof_begin(w.prof, w.s, kKExtendCount);
+    if (split) root_chunks(g_, d, depth, rec, rec_count, count, w.cp.prod, w.cp.scan_tmp, w.s);
+    extend(g_, d, depth, d.n, kCountAtEnd, r, out, split ? w.cp.prod : nullptr, w.work_counter(), w.sched, w.s);
+    prof_end(w.prof, w.s, kKExtendCount);
     return "";
 }
 
 std::string Engine::run_phase(uint32_t ph) {
+    std::vector<std::pair<uint32_t, uint32_t>> steps;  // (query, edge)
     for (size_t qi = 0; qi < qs_.size(); qi++) {
-        QState& qs = *qs_[qi];
+        const QState& qs = *qs_[qi];
         if (!qs.planned || !qs.matchable) continue;
-        for (uint32_t i = 0; i < qs.q.m; i++) {
-            const auto t0 = std::chrono::steady_clock::now();
-            trace_line_.clear();
-            std::string e = run_step(qs, i, ph);
-            if (trace_) {
-                (void)hipStreamSynchronize(s_);
-                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                std::fprintf(trace_, "{\"query\": %zu, \"step\": %u, \"phase\": %u, \"ms\": %.3f, \"empty\": %d%s}\n", qi, i, ph, ms,
-                             e == "empty" ? 1 : 0, trace_line_.c_str());
-            }
-            if (e == "empty") continue;
-            if (!e.empty()) return e;
-        }
+        for (uint32_t i = 0; i < qs.q.m; i++) steps.push_back({static_cast<uint32_t>(qi), i});
     }
+    if (steps.empty()) return "";
+    CSM_HIP(hipEventRecord(phase_start_, s_));
+    std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::vector<std::string> errs(lanes_.size());
+    auto worker = [&](size_t li) {
+        Lane& w = lanes_[li];
+        auto run = [&]() -> std::string {
+            CSM_HIP(hipSetDevice(opt_.device));
+            CSM_HIP(hipStreamWaitEvent(w.s, phase_start_, 0));
+            while (!failed.load(std::memory_order_relaxed)) {
+                const size_t k = next.fetch_add(1);
+                if (k >= steps.size()) break;
+                QState& qs = *qs_[steps[k].first];
+                const auto t0 = std::chrono::steady_clock::now();
+                w.trace_line.clear();
+                std::string e = run_step(w, qs, steps[k].second, ph);
+                if (trace_) {
+                    (void)hipStreamSynchronize(w.s);
+                    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    std::lock_guard<std::mutex> lock(trace_mu_);
+                    std::fprintf(trace_, "{\"query\": %u, \"step\": %u, \"phase\": %u, \"lane\": %zu, \"ms\": %.3f, \"empty\": %d%s}\n",
+                                 steps[k].first, steps[k].second, ph, li, ms, e == "empty" ? 1 : 0, w.trace_line.c_str());
+                }
+                if (e == "empty") continue;
+                if (!e.empty()) return e;
+            }
+            CSM_HIP(hipEventRecord(w.done, w.s));
+            return "";
+        };
+        errs[li] = run();
+        if (!errs[li].empty()) failed.store(true);
+    };
+    std::vector<std::thread> th;
+    for (size_t li = 1; li < lanes_.size(); li++) th.emplace_back(worker, li);
+    worker(0);
+    for (auto& t : th) t.join();
     if (trace_) std::fflush(trace_);
+    for (size_t li = 0; li < lanes_.size(); li++)
+        if (!errs[li].empty()) {
+            for (Lane& w : lanes_) (void)hipStreamSynchronize(w.s);
+            return errs[li];
+        }
+    for (Lane& w : lanes_) CSM_HIP(hipStreamWaitEvent(s_, w.done, 0));
     return "";
 }
 
@@ -827,9 +869,9 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t max_arcs = static_cast<uint32_t>(2 * n);
     if (n == 0) { out.assign(qs_.size(), QueryResult{}); return ""; }
-    prof_begin(kKBatchBuild);
+    prof_begin(prof_main_, s_, kKBatchBuild);
     for (int d = 0; d < ndirs_; d++) batch_build(g_, d, ndirs_ == 1 ? 0 : d + 1, bin_, static_cast<uint32_t>(lo), static_cast<uint32_t>(n), bs_[d], s_);
-    prof_end(kKBatchBuild);
+    prof_end(prof_main_, s_, kKBatchBuild);
     for (int d = 0; d < ndirs_; d++)
         nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 0, s_);
     fill_u32(reinterpret_cast<uint32_t*>(results_), 0u, qs_.size() * 8, s_);
@@ -844,12 +886,12 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
         if (!err.empty()) return err;
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-        prof_begin(kKApply);
+        prof_begin(prof_main_, s_, kKApply);
         for (int d = 0; d < ndirs_; d++) {
             batch_apply_deletions(g_, d, max_arcs, bs_[d], s_);
             slot_dir_update(g_, d, bs_[d].touched, bs_[d].n_touched, max_arcs, s_);
         }
-        prof_end(kKApply);
+        prof_end(prof_main_, s_, kKApply);
     }
     if (info.inserts) {
         CSM_HIP(hipStreamSynchronize(s_));
@@ -858,12 +900,12 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
         if (!err.empty()) return err;
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-        prof_begin(kKApply);
+        prof_begin(prof_main_, s_, kKApply);
         for (int d = 0; d < ndirs_; d++) {
             batch_apply_insertions(g_, d, max_arcs, bs_[d], s_);
             slot_dir_update(g_, d, bs_[d].touched, bs_[d].n_touched, max_arcs, s_);
         }
-        prof_end(kKApply);
+        prof_end(prof_main_, s_, kKApply);
     }
     for (int d = 0; d < ndirs_; d++)
         nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 1, s_);
@@ -895,40 +937,47 @@ std::string Engine::run_edges(const uint32_t*, const uint32_t*, const uint32_t*,
     return "in-memory batches are not supported yet";
 }
 
-void Engine::prof_begin(uint32_t family) {
+void Engine::prof_begin(Prof& P, hipStream_t st, uint32_t family) {
     if (!(prof_mask_ >> family & 1u)) return;
-    if (ev_used_ + 2 > ev_pool_.size()) {
+    if (P.used + 2 > P.pool.size()) {
         for (int k = 0; k < 64; k++) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) return;
-            ev_pool_.push_back(e);
+            P.pool.push_back(e);
         }
     }
-    hipEvent_t a = ev_pool_[ev_used_++];
-    (void)hipEventRecord(a, s_);
-    ev_open_[family] = a;
+    hipEvent_t a = P.pool[P.used++];
+    (void)hipEventRecord(a, st);
+    P.open[family] = a;
 }
 
-void Engine::prof_end(uint32_t family) {
-    if (!(prof_mask_ >> family & 1u) || !ev_open_[family]) return;
-    hipEvent_t b = ev_pool_[ev_used_++];
-    (void)hipEventRecord(b, s_);
-    ev_pending_.push_back({family, {ev_open_[family], b}});
-    ev_open_[family] = nullptr;
+void Engine::prof_end(Prof& P, hipStream_t st, uint32_t family) {
+    if (!(prof_mask_ >> family & 1u) || !P.open[family]) return;
+    hipEvent_t b = P.pool[P.used++];
+    (void)hipEventRecord(b, st);
+    P.pending.push_back({family, {P.open[family], b}});
+    P.open[family] = nullptr;
 }
 
 std::string Engine::prof_collect() {
     if (!prof_mask_) return "";
     CSM_HIP(hipStreamSynchronize(s_));
-    for (const auto& p : ev_pending_) {
-        float ms = 0;
-        CSM_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
-        kstat_[p.first].launches += 1;
-        kstat_[p.first].ms += ms;
-    }
-    ev_pending_.clear();
-    ev_used_ = 0;
-    return "";
+    for (Lane& w : lanes_) CSM_HIP(hipStreamSynchronize(w.s));
+    auto drain = [&](Prof& P) -> std::string {
+        for (const auto& p : P.pending) {
+            float ms = 0;
+            CSM_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+            kstat_[p.first].launches += 1;
+            kstat_[p.first].ms += ms;
+        }
+        P.pending.clear();
+        P.used = 0;
+        return "";
+    };
+    std::string err = drain(prof_main_);
+    for (Lane& w : lanes_)
+        if (err.empty()) err = drain(w.prof);
+    return err;
 }
 
 std::string Engine::profile_enable(uint32_t mask) {
@@ -978,6 +1027,7 @@ std::string Engine::calibration_copy(uint64_t words, double& ms) {
 
 std::string Engine::synchronize() {
     CSM_HIP(hipStreamSynchronize(s_));
+    for (Lane& w : lanes_) CSM_HIP(hipStreamSynchronize(w.s));
     return "";
 }
 

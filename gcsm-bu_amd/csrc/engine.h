@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,19 +34,20 @@ struct Technique {
 struct EngineOptions {
     int device = 0;
     uint64_t pool_bytes = 0;          // HBM budget for the device pool (0: no cap)
-    uint64_t ring_bytes = 4ull << 30; // partial-result ring (two halves)
-    uint64_t cp_list_bytes = 1ull << 30;
+    uint64_t ring_bytes = 1ull << 30; // partial-result ring of each lane (a stack of levels)
+    uint64_t cp_list_bytes = 512ull << 20;  // per lane
     uint32_t cp_max_records = 1u << 22;
     uint32_t max_batch_edges = 0;     // 0: the stream's batch size
     uint32_t semi_bfs_threshold = kSemiBfsThreshold;
     bool directed = false;
     bool split_roots = true;          // spread heavy records of the counting DFS over many waves
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
-    bool level_kernel = true;         // single-level extensions load-balanced over candidates
+    uint32_t lanes = 8;               // concurrent matching steps (streams), each with its own scratch
 };
 
 constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
 constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
+constexpr uint32_t kMaxLanes = 16;
 
 struct QueryResult {
     uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;
@@ -115,7 +118,7 @@ class Engine {
 
     // Kernel families timed with HIP events on the engine's stream when profiling is enabled.
     enum KernelFamily : uint32_t { kKExtendCount = 0, kKExtendWrite, kKCpEnumerate, kKLocalIndex, kKBatchBuild, kKApply, kKCpPrepare,
-                                   kKAnchors, kKLevelCount, kKLevelWrite, kKFamilies };
+                                   kKAnchors, kKLevelCount, kKLevelWrite, kKLevelPrepare, kKFamilies };
     struct KernelStat {
         uint64_t launches = 0;
         double ms = 0;
@@ -144,18 +147,47 @@ class Engine {
         uint32_t entry0 = 0, entry1 = 0;    // NLF entries [entry0, entry1)
     };
 
-    std::string upload_graph();
-    std::string upload_stream();
-    std::string alloc_runtime();
+    // HIP-event timing of kernel families on one stream.
+    struct Prof {
+        std::vector<hipEvent_t> pool;
+        size_t used = 0;
+        std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> pending;
+        hipEvent_t open[kKFamilies]{};
+    };
+    // One lane of concurrent matching steps: its own stream, local index sets, anchor pairs,
+    // partial-result ring, counters, Cartesian-product scratch and pinned read-back buffer.
+    struct Lane {
+        hipStream_t s = nullptr;
+        VSet L[kMaxQV]{};
+        uint16_t L_used = 0;  // query vertices whose local set holds data
+        uint32_t* tmp_list = nullptr;
+        uint32_t* tmp_len = nullptr;
+        uint32_t* pairs = nullptr;
+        uint32_t* n_pairs = nullptr;
+        uint32_t* ring = nullptr;
+        uint64_t ring_words = 0;
+        uint32_t* counters = nullptr;  // [0]: input count, [1]: output count, [2]: overflow, [3]: finishing count, [8..9]: u64 work counter
+        CpScratch cp{};
+        uint32_t* h_pinned = nullptr;
+        hipEvent_t done = nullptr;     // end of the lane's work in a phase
+        Prof prof;
+        std::string trace_line;
+        int sched = 1;                 // DFS schedule of the current step: 0 static, 1 dynamic
+        unsigned long long* work_counter() { return reinterpret_cast<unsigned long long*>(counters + 8); }
+    };
+
     std::string run_phase(uint32_t ph);
-    std::string run_step(QState& qs, uint32_t i, uint32_t ph);
-    std::string finish_records(QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, uint32_t* rec,
-                               const uint32_t* rec_count, uint32_t res_slot);
+    std::string run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph);
+    std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,
+                       uint64_t top, uint32_t target, uint32_t res_slot);
+    std::string extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, uint32_t count, uint32_t depth, uint32_t end, uint32_t* out,
+                          uint64_t cap, bool& ok, uint32_t& n_out);
+    std::string finish_records(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, const uint32_t* rec,
+                               uint32_t res_slot);
     uint32_t view_of(const QState& qs, uint32_t rel, uint32_t i, uint32_t ph) const;
     uint32_t key_of_edge(uint32_t nbr, uint32_t elabel_raw) const;
     std::string check_err(const char* where);
-    uint32_t read_u32(const uint32_t* p);
-    unsigned long long* work_counter() { return reinterpret_cast<unsigned long long*>(counters_ + 8); }
+    static uint32_t read_u32(hipStream_t st, uint32_t* pinned, const uint32_t* p);
 
     EngineOptions opt_;
     hipStream_t s_ = nullptr;
@@ -184,36 +216,26 @@ class Engine {
     uint32_t n_entries_ = 0;
     std::vector<NlfEntry> h_entries_;
     std::vector<NlfGroup> h_groups_;
-    VSet L_[kMaxQV]{};
-    uint32_t* tmp_list_ = nullptr;
-    uint32_t* tmp_len_ = nullptr;
-    uint32_t* pairs_ = nullptr;
-    uint32_t* n_pairs_ = nullptr;
-    uint32_t* ring_ = nullptr;
-    uint64_t ring_words_ = 0;
-    uint32_t* counters_ = nullptr;  // [0]: ring A count, [1]: ring B count, [2]: overflow, [8..9]: u64 work counter
-    CpScratch cp_{};
+    std::vector<Lane> lanes_;
+    uint64_t ring_words_ = 0;                 // words of each lane's ring
+    hipEvent_t phase_start_ = nullptr;
     unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash
     unsigned long long* summary_ = nullptr;   // [2]
-    uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer
-    uint16_t L_used_ = 0;                     // query vertices whose local set holds data
+    uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer (main stream)
     uint32_t last_err_ = 0;
 
     // profiling
-    void prof_begin(uint32_t family);
-    void prof_end(uint32_t family);
+    void prof_begin(Prof& P, hipStream_t st, uint32_t family);
+    void prof_end(Prof& P, hipStream_t st, uint32_t family);
     std::string prof_collect();
     uint32_t prof_mask_ = 0;
     KernelStat kstat_[kKFamilies];
-    std::vector<hipEvent_t> ev_pool_;
-    size_t ev_used_ = 0;
-    std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> ev_pending_;
-    hipEvent_t ev_open_[kKFamilies]{};
+    Prof prof_main_;
     unsigned long long* prof_bytes_ = nullptr;  // device counters per family
     unsigned long long* bytes_ptr(uint32_t family) { return (prof_mask_ >> family & 1u) ? prof_bytes_ + family : nullptr; }
     // Per-step trace (environment variable CSM_TRACE = output path): one JSON line per step.
     FILE* trace_ = nullptr;
-    std::string trace_line_;
+    std::mutex trace_mu_;
 };
 
 }  // namespace csm

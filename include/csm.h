@@ -108,7 +108,8 @@ csm_status csm_synchronize(csm_session* s);
  * utils/cuda_helpers.h:78-102).  Kernel families: 0 depth-first enumeration (count), 1 depth-first
  * enumeration (write partial results), 2 Cartesian product, 3 local index, 4 batch build, 5 apply
  * batch, 6 Cartesian-product preparation, 7 anchors, 8 one-level extension (count), 9 one-level
- * extension (write a level).  Bit f of mask times family f with HIP events on the session's stream and,
+ * extension (write a level), 10 one-level preparation (run lengths and their scan).  The bytes of
+ * families 8 and 9 include the runs their preparation sizes.  Bit f of mask times family f with HIP events on the session's stream and,
  * for the enumeration kernels, counts algorithmic bytes (every view run read once plus candidates
  * written). */
 csm_status csm_profile_enable(csm_session* s, uint32_t mask);

@@ -175,18 +175,18 @@ def test_adjacency_after_stream(lib, tmp_path):
             assert len(set(got)) == len(got)
 
 
-@pytest.mark.parametrize("split,sdir,level", [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)])
-def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, level):
-    """Root splitting, the label-slot directory and the one-level kernel change scheduling and
-    lookup, never results."""
+@pytest.mark.parametrize("split,sdir,lanes", [(0, 0, 1), (1, 0, 3), (0, 1, 8), (1, 1, 16)])
+def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, lanes):
+    """Root splitting, the label-slot directory and the number of concurrent step lanes change
+    scheduling and lookup, never results."""
     monkeypatch.setenv("CSM_SPLIT_ROOTS", str(split))
     monkeypatch.setenv("CSM_SLOT_DIRECTORY", str(sdir))
-    monkeypatch.setenv("CSM_LEVEL_KERNEL", str(level))
+    monkeypatch.setenv("CSM_LANES", str(lanes))
     meta, queries = util.gen("tinyins", tmp_path, seed=7)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
     want, _ = util.oracle_stream(data, upd, 64, queries)
     for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 2, True, False), (0, 1, True, True)]:
-        say(f"case split={split} sdir={sdir} level={level} order={mo} strategy={ss} local={li} cp={cp}")
+        say(f"case split={split} sdir={sdir} lanes={lanes} order={mo} strategy={ss} local={li} cp={cp}")
         got = run_library(lib, str(data), str(upd), 64, queries, lib.technique(mo, ss, li, cp))
         if got is not None:
             assert got == want
@@ -194,7 +194,7 @@ def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir,
     meta, queries = util.gen("c1", tmp_path / "c1", seed=3)
     data, upd = tmp_path / "c1" / "data.graph", tmp_path / "c1" / "update.stream"
     want, _ = util.oracle_stream(data, upd, 2000, queries, threads=8)
-    say(f"case split={split} sdir={sdir} level={level} c1 batch=2000")
+    say(f"case split={split} sdir={sdir} lanes={lanes} c1 batch=2000")
     assert run_library(lib, str(data), str(upd), 2000, queries, lib.technique()) == want
 
 
@@ -227,8 +227,9 @@ def test_directed_stream(lib, tmp_path, seed):
 
 
 def test_small_ring_and_record_chunks(lib, tmp_path, monkeypatch):
-    """Levels that do not fit the ring are skipped or abandoned (depth-first from the level before),
-    and levels of more records than the per-launch scratch are processed in chunks: same results."""
+    """Levels that do not fit the ring are redone for halves of their input records (a single
+    record that does not fit is counted depth-first), and levels of more records than the
+    per-launch scratch are processed in chunks: same results."""
     monkeypatch.setenv("CSM_CP_MAX_RECORDS", "1024")
     meta, queries = util.gen("c2", tmp_path, seed=6, scale=0.02)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"

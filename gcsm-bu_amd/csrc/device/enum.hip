@@ -74,8 +74,7 @@ __device__ __forceinline__ bool test_candidate(const DevGraph& g, const StepDesc
     for (uint32_t e = d.boff[depth]; e < d.boff[depth + 1]; e++) {
         if (e == p) continue;
         const BackEntry be = d.be[e];
-        const uint32_t k2 = (be.slot << g.id_bits) | c;
-        if (!run_contains(g, entry_delta(g, d, be), W.run[e], be.view, k2)) return false;
+        if (!view_contains(g, d.phase, be.view, be.dir, be.slot, W.f[be.pos], (be.slot << g.id_bits) | c)) return false;
     }
     return true;
 }
@@ -215,7 +214,8 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
 // Algorithmic bytes (as the DFS kernel's frames): every view run of the depth's backward lists
 // read once per record, plus 4 per passing candidate (counted by k_level_flat).
 __global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                   unsigned long long* len, unsigned long long* ph, unsigned long long* bytes_out) {
+                                                   unsigned long long* len, uint4* runs, unsigned long long* ph,
+                                                   unsigned long long* bytes_out) {
     const uint32_t p = d.pivot[d0];
     const BackEntry pe = d.be[p];
     unsigned long long bytes = 0;
@@ -223,6 +223,7 @@ __global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint3
         const uint32_t* f = rec + static_cast<uint64_t>(i) * d0;
         const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
         len[i] = run_seq_len(rv, pe.view);
+        runs[i] = make_uint4(rv.lo, rv.hi, rv.dlo, rv.dhi);
         if (ph) {
             unsigned long long h = 0;
             for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
@@ -242,14 +243,20 @@ __global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint3
     }
 }
 
-// Each wave walks one contiguous range of the level's candidates (equal shares, multiples of 64).
-// It finds the record of its first candidate with a 64-ary search; per 64 candidates it then loads
-// the next 65 record offsets into LDS with one coalesced load and each lane finds its record there
-// (a global search only for lanes past the window, when records have under one candidate each).
+// Each wave walks one contiguous range of the level's candidates (equal shares, multiples of
+// kLevelStep).  It finds the record of its first candidate with a 64-ary search; per kLevelStep
+// candidates it loads the next kLevelStep + 1 record offsets into LDS (coalesced) and each lane
+// finds the records of its kLevelIlp candidates there (a global search only past the window, when
+// records have under one candidate each).  The kLevelIlp candidates of a lane are tested in
+// lock-step so their independent loads are in flight together: the kernel is latency-bound.
+constexpr uint32_t kLevelIlp = 4;
+constexpr uint32_t kLevelStep = 64 * kLevelIlp;
+
 template <int kMode>
 __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                    const unsigned long long* off, const unsigned long long* ph, Ring r, MatchOut out) {
-    __shared__ unsigned long long s_win[4][65];
+                                                    const unsigned long long* off, const uint4* runs, const unsigned long long* ph,
+                                                    Ring r, MatchOut out) {
+    __shared__ unsigned long long s_win[4][kLevelStep + 1];
     const uint32_t lane = lane_id();
     unsigned long long* win = s_win[threadIdx.x >> 6];
     const uint32_t p = d.pivot[d0];
@@ -261,78 +268,122 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
     const unsigned long long total = off[n];
     const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * (blockDim.x >> 6);
     const unsigned long long gw = static_cast<unsigned long long>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const unsigned long long per = ((total + waves - 1) / waves + 63) & ~63ull;
+    const unsigned long long per = ((total + waves - 1) / waves + kLevelStep - 1) / kLevelStep * kLevelStep;
     const unsigned long long w0 = gw * per, w1 = min(total, w0 + per);
     unsigned long long cnt = 0, hsum = 0, bytes = 0;
     uint32_t rb = w0 < total ? find_record(off, n, w0, lane) : 0u;  // off[rb] <= w0
-    for (unsigned long long t0 = w0; t0 < w1; t0 += 64) {
+    for (unsigned long long t0 = w0; t0 < w1; t0 += kLevelStep) {
         // a level that overflows is redone by the host with fewer records: stop early (polled every
-        // 64 iterations; a load per iteration would sit on every iteration's latency chain)
-        if (kMode == kWriteAtEnd && ((t0 - w0) & (64ull * 64 - 1)) == 0 && t0 != w0 &&
+        // 16 iterations; a load per iteration would sit on every iteration's latency chain)
+        if (kMode == kWriteAtEnd && ((t0 - w0) & (16ull * kLevelStep - 1)) == 0 && t0 != w0 &&
             uniform(__hip_atomic_load(r.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
             break;
-        win[lane] = off[min(rb + lane, n)];
-        if (lane == 0) win[64] = off[min(rb + 64, n)];
+#pragma unroll
+        for (uint32_t j = 0; j < kLevelIlp; j++) win[lane + 64 * j] = off[min(rb + lane + 64 * j, n)];
+        if (lane == 0) win[kLevelStep] = off[min(rb + kLevelStep, n)];
         wave_sync();
-        const unsigned long long t = t0 + lane;
-        bool ok = false;
-        uint32_t c = 0, k = rb;
-        if (t < w1) {
-            if (win[64] <= t) {  // past the window: last k in [rb + 64, n) with off[k] <= t
-                uint32_t lo = rb + 64, hi = n;
+        unsigned long long t[kLevelIlp];
+        uint32_t k[kLevelIlp], key[kLevelIlp], c[kLevelIlp];
+        bool ok[kLevelIlp];
+#pragma unroll
+        for (uint32_t j = 0; j < kLevelIlp; j++) {
+            t[j] = t0 + 64 * j + lane;
+            ok[j] = t[j] < w1;
+            k[j] = rb;
+            if (!ok[j]) continue;
+            if (win[kLevelStep] <= t[j]) {  // past the window: last k in [rb + kLevelStep, n) with off[k] <= t
+                uint32_t lo = rb + kLevelStep, hi = n;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (off[mid] <= t) lo = mid;
+                    if (off[mid] <= t[j]) lo = mid;
                     else hi = mid;
                 }
-                k = lo;
-            } else {  // last j < 64 with win[j] <= t (win[0] <= t0 <= t)
-                uint32_t lo = 0, hi = 64;
+                k[j] = lo;
+            } else {  // last i < kLevelStep with win[i] <= t (win[0] <= t0 <= t)
+                uint32_t lo = 0, hi = kLevelStep;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (win[mid] <= t) lo = mid;
+                    if (win[mid] <= t[j]) lo = mid;
                     else hi = mid;
                 }
-                k = rb + lo;
+                k[j] = rb + lo;
             }
-            const uint32_t* f = rec + static_cast<uint64_t>(k) * d0;
-            const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
-            const uint32_t key = run_elem(g, pdl, rv, run_base_len(rv, pe.view), static_cast<uint32_t>(t - off[k]));
-            c = key & g.id_mask;
-            ok = bit_test(filter, c) && run_elem_live(pdl, rv, pe.view, key);
-            for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != c;
-            for (uint32_t e = e0; e < e1 && ok; e++) {
-                if (e == p) continue;
-                const BackEntry be = d.be[e];
-                const RunView ov = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
-                ok = run_contains(g, phase_delta(g, be.dir, d.phase), ov, be.view, (be.slot << g.id_bits) | c);
-            }
-            if (out.bytes && ok) bytes += 4;
+        }
+        RunView rv[kLevelIlp];
+        unsigned long long base[kLevelIlp];
+#pragma unroll
+        for (uint32_t j = 0; j < kLevelIlp; j++) {
+            rv[j] = RunView{0, 0, 0, 0};
+            base[j] = 0;
+            if (!ok[j]) continue;
+            const uint4 x = runs[k[j]];
+            rv[j] = RunView{x.x, x.y, x.z, x.w};
+            base[j] = k[j] - rb < kLevelStep ? win[k[j] - rb] : off[k[j]];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLevelIlp; j++) {
+            key[j] = ok[j] ? run_elem(g, pdl, rv[j], run_base_len(rv[j], pe.view), static_cast<uint32_t>(t[j] - base[j])) : 0u;
+            c[j] = key[j] & g.id_mask;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && bit_test(filter, c[j]);
+        if (pe.view == kViewBaseMinusDelta) {
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && run_elem_live(pdl, rv[j], pe.view, key[j]);
+        }
+        for (uint32_t q = 0; q < d0; q++) {
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && rec[static_cast<uint64_t>(k[j]) * d0 + q] != c[j];
+        }
+        for (uint32_t e = e0; e < e1; e++) {
+            if (e == p) continue;
+            const BackEntry be = d.be[e];
+            const uint32_t hi_bits = be.slot << g.id_bits;
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++)
+                ok[j] = ok[j] && view_contains(g, d.phase, be.view, be.dir, be.slot, rec[static_cast<uint64_t>(k[j]) * d0 + be.pos], hi_bits | c[j]);
         }
         if (kMode == kCountAtEnd) {
-            if (ok) {
-                cnt += 1;
-                hsum += embedding_hash(ph[k] + pair_hash(qv, c));
-            }
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++)
+                if (ok[j]) {
+                    cnt += 1;
+                    hsum += embedding_hash(ph[k[j]] + pair_hash(qv, c[j]));
+                }
         } else {
-            const uint64_t m = __ballot(ok);
-            if (m) {
-                const uint32_t np = static_cast<uint32_t>(__popcll(m));
+            uint64_t m[kLevelIlp];
+            uint32_t np = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++) {
+                m[j] = __ballot(ok[j]);
+                np += static_cast<uint32_t>(__popcll(m[j]));
+            }
+            if (np) {
                 uint32_t wb = 0;
                 if (lane == 0) wb = atomicAdd(r.out_count, np);
                 wb = __shfl(wb, 0, 64);
                 if (static_cast<uint64_t>(wb) + np > r.out_cap) {
                     if (lane == 0) atomicOr(r.overflow, 1u);
-                } else if (ok) {
-                    uint32_t* o = r.out + static_cast<uint64_t>(wb + lanes_below(m)) * (d0 + 1);
-                    const uint32_t* f = rec + static_cast<uint64_t>(k) * d0;
-                    for (uint32_t j = 0; j < d0; j++) o[j] = f[j];
-                    o[d0] = c;
+                } else {
+#pragma unroll
+                    for (uint32_t j = 0; j < kLevelIlp; j++) {
+                        if (ok[j]) {
+                            uint32_t* o = r.out + static_cast<uint64_t>(wb + lanes_below(m[j])) * (d0 + 1);
+                            const uint32_t* f = rec + static_cast<uint64_t>(k[j]) * d0;
+                            for (uint32_t q = 0; q < d0; q++) o[q] = f[q];
+                            o[d0] = c[j];
+                        }
+                        wb += static_cast<uint32_t>(__popcll(m[j]));
+                    }
                 }
             }
         }
-        rb = uniform(__shfl(k, 63, 64));  // lane 63 holds the highest record of this window (all lanes valid unless last)
-        wave_sync();                     // the window is rewritten next iteration
+        if (out.bytes) {
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++) bytes += ok[j] ? 4 : 0;
+        }
+        rb = uniform(__shfl(k[kLevelIlp - 1], 63, 64));  // highest record of this step (all lanes valid unless last)
+        wave_sync();                                    // the window is rewritten next iteration
     }
     if (kMode == kCountAtEnd) {
         cnt = wave_sum_u64(cnt);
@@ -599,19 +650,19 @@ void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_dept
         k_extend<kWriteAtEnd><<<grid, kWaves * 64, lds, s>>>(g, d, d0, end_depth, r, out, split, work_counter, schedule);
 }
 
-void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, unsigned long long* off,
-                   unsigned long long* ph, unsigned long long* scan_tmp, unsigned long long* bytes, hipStream_t s) {
-    k_level_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, off, ph, bytes);
-    scan_exclusive<unsigned long long>(off, off, nullptr, n, true, scan_tmp, s);
+void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
+                   bool with_hash, unsigned long long* bytes, hipStream_t s) {
+    k_level_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, with_hash ? ls.ph : nullptr, bytes);
+    scan_exclusive<unsigned long long>(ls.off, ls.off, nullptr, n, true, ls.scan_tmp, s);
 }
 
-void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode,
-                const unsigned long long* off, const unsigned long long* ph, Ring r, MatchOut out, hipStream_t s) {
-    constexpr int kGrid = 4096;  // grid-stride over the level's candidates (total known on the device only)
+void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode, const LevelScratch& ls,
+                Ring r, MatchOut out, hipStream_t s) {
+    constexpr int kGrid = 2048;  // waves take equal contiguous shares of the level's candidates
     if (mode == kCountAtEnd)
-        k_level_flat<kCountAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, off, ph, r, out);
+        k_level_flat<kCountAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
     else
-        k_level_flat<kWriteAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, off, ph, r, out);
+        k_level_flat<kWriteAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
 }
 
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,

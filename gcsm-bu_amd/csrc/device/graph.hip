@@ -12,6 +12,7 @@
 //                    the capacity is exceeded (addTriesToGraph, indexing.cu:76-263; capacity rule
 //                    roundCapacities, indexing.cu:46-59, computed with integer arithmetic).
 #include "kernels.h"
+#include "view.h"
 
 namespace csm {
 namespace {

@@ -144,13 +144,19 @@ void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_dept
             MatchOut out, const unsigned long long* split, unsigned long long* work_counter, int schedule, hipStream_t s);
 // One level from the n records of depth d0 at rec (n known on the host), flattened over all its
 // candidates: level_prepare writes the scanned pivot-run lengths of the records to off[0..n]
-// (off[n] = total candidates) and, when ph is set, each record's prefix hash; level_flat then runs
-// one thread per candidate and counts/hashes complete embeddings (d0 + 1 == n) or writes records
-// of depth d0 + 1.
-void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, unsigned long long* off,
-                   unsigned long long* ph, unsigned long long* scan_tmp, unsigned long long* bytes, hipStream_t s);
-void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode,
-                const unsigned long long* off, const unsigned long long* ph, Ring r, MatchOut out, hipStream_t s);
+// (off[n] = total candidates), their pivot runs and, with_hash, their prefix hashes; level_flat then
+// tests every candidate, load-balanced over the grid, and counts/hashes complete embeddings
+// (d0 + 1 == n) or writes records of depth d0 + 1.
+struct LevelScratch {
+    unsigned long long* off;       // [n + 1] scanned candidate counts
+    uint4* runs;                   // [n] pivot view run of each record
+    unsigned long long* ph;        // [n] prefix hash of each record (counting levels)
+    unsigned long long* scan_tmp;
+};
+void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
+                   bool with_hash, unsigned long long* bytes, hipStream_t s);
+void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode, const LevelScratch& ls,
+                Ring r, MatchOut out, hipStream_t s);
 // chunks[i] = ceil(|pivot view run of record i at depth d0| / 64), then scanned in place (total at [n]).
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
                  uint32_t max_rec, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s);

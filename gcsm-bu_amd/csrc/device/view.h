@@ -62,4 +62,27 @@ __device__ __forceinline__ bool run_contains(const DevGraph& g, const uint32_t* 
     }
 }
 
+// Whether `key` (slot and id of a neighbour) is in the view of b's direction-dir list for the
+// phase: binary searches of b's slot run and of its short delta segment.
+__device__ __forceinline__ bool view_contains(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, uint32_t b,
+                                              uint32_t key) {
+    bool base = false;
+    if (view != kViewDelta) {
+        uint32_t lo, hi;
+        base_range(g, dir, b, slot, lo, hi);
+        base = contains_u32(g.arena, lo, hi, key);
+        if (view == kViewBase || (view == kViewBasePlusDelta && base) || (view == kViewBaseMinusDelta && !base)) return base;
+    }
+    const uint4 s = g.delta[dir].seg[b];
+    const uint32_t off = phase ? s.z : s.x, len = phase ? s.w : s.y;
+    bool in_delta = false;
+    if (len) {
+        const uint32_t* dl = phase_delta(g, dir, phase);
+        uint32_t lo, hi;
+        slot_range(dl, off, off + len, slot, g.id_bits, lo, hi);
+        in_delta = contains_u32(dl, lo, hi, key);
+    }
+    return view == kViewBaseMinusDelta ? !in_delta : in_delta;  // Minus: base is true here
+}
+
 }  // namespace csm

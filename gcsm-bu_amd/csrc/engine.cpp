@@ -217,6 +217,7 @@ std::string Engine::build() {
         add(ring_words_ * 4); add(256);
         add((cp_rec + 1) * 8); add(cp_rec * 8); add(scan_scratch_elems(static_cast<uint32_t>(cp_rec)) * 8);
         add(cp_rec * 4); add(cp_rec * kMaxQV * 4); add(cp_list_words * 4); add(64);
+        add(cp_rec * 16);                                          // pivot runs of a level's records
         add(64 << 10);                                             // alignment slack of the takes
     }
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
@@ -391,6 +392,8 @@ std::string Engine::build() {
         w.cp.lists_top = cpc;
         w.cp.max_prod = cpc + 1;
         w.cp.err = reinterpret_cast<uint32_t*>(cpc + 2);
+        w.level = LevelScratch{w.cp.prod, pool_.take<uint4>(cp_rec), w.cp.rec_hash, w.cp.scan_tmp};
+        if (!w.level.runs) return "device pool exhausted (lane scratch)";
     }
     opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
     results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);
@@ -686,10 +689,10 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
             const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
             const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
             prof_begin(w.prof, w.s, kKLevelPrepare);
-            level_prepare(g_, d, depth, rc, nc, w.cp.prod, nullptr, w.cp.scan_tmp, bytes_ptr(kKLevelWrite), w.s);
+            level_prepare(g_, d, depth, rc, nc, w.level, false, bytes_ptr(kKLevelWrite), w.s);
             prof_end(w.prof, w.s, kKLevelPrepare);
             prof_begin(w.prof, w.s, kKLevelWrite);
-            level_flat(g_, d, depth, rc, nc, kWriteAtEnd, w.cp.prod, nullptr, r, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
+            level_flat(g_, d, depth, rc, nc, kWriteAtEnd, w.level, r, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
             prof_end(w.prof, w.s, kKLevelWrite);
         }
     } else {
@@ -785,10 +788,10 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
             const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
             const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
             prof_begin(w.prof, w.s, kKLevelPrepare);
-            level_prepare(g_, d, depth, rc, nc, w.cp.prod, w.cp.rec_hash, w.cp.scan_tmp, out.bytes, w.s);
+            level_prepare(g_, d, depth, rc, nc, w.level, true, out.bytes, w.s);
             prof_end(w.prof, w.s, kKLevelPrepare);
             prof_begin(w.prof, w.s, kKLevelCount);
-            level_flat(g_, d, depth, rc, nc, kCountAtEnd, w.cp.prod, w.cp.rec_hash, r, out, w.s);
+            level_flat(g_, d, depth, rc, nc, kCountAtEnd, w.level, r, out, w.s);
             prof_end(w.prof, w.s, kKLevelCount);
         }
         return "";

@@ -168,6 +168,7 @@ class Engine {
         uint64_t ring_words = 0;
         uint32_t* counters = nullptr;  // [0]: input count, [1]: output count, [2]: overflow, [3]: finishing count, [8..9]: u64 work counter
         CpScratch cp{};
+        LevelScratch level{};          // offsets, hashes and scan scratch shared with cp
         uint32_t* h_pinned = nullptr;
         hipEvent_t done = nullptr;     // end of the lane's work in a phase
         Prof prof;

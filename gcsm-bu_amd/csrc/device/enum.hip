@@ -453,91 +453,117 @@ __global__ void __launch_bounds__(256) k_init_from_set(DevGraph g, uint32_t phas
 }
 
 // ---- Cartesian product -------------------------------------------------------------------------
-// Per record (wave): filtered candidate list of every trailing leaf (candidate bitmap, not in the
-// matched prefix, alive in its view), their product, and the record's prefix hash.
-__global__ void __launch_bounds__(256) k_cp_prepare(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
-                                                    uint32_t max_rec, CpScratch c) {
-    const uint32_t lane = lane_id();
-    const uint32_t wpb = blockDim.x >> 6;
-    const uint32_t n = min(*n_rec, max_rec);
-    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb) {
-        uint32_t f[kMaxQV];
-        for (uint32_t j = 0; j < d0; j++) f[j] = rec[static_cast<uint64_t>(i) * d0 + j];
-        uint32_t total = 0;
-        unsigned long long prod = 1;
-        bool overflow = false;
-        // pass 1: sizes
-        for (uint32_t k = d0; k < d.n; k++) {
-            const uint32_t e = d.boff[k];
-            const BackEntry be = d.be[e];
-            const uint32_t* dl = phase_delta(g, be.dir, d.phase);
-            const RunView rv = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
-            const uint32_t nb = run_base_len(rv, be.view), tot = run_seq_len(rv, be.view);
-            uint32_t s = 0;
-            for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
-                const uint32_t t = t0 + lane;
-                bool ok = t < tot;
-                const uint32_t key = ok ? run_elem(g, dl, rv, nb, t) : 0u;
-                const uint32_t x = key & g.id_mask;
-                if (ok) ok = bit_test(d.filter[k], x) && run_elem_live(dl, rv, be.view, key);
-                for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != x;
-                s += static_cast<uint32_t>(__popcll(__ballot(ok)));
-            }
-            if (lane == 0) c.leaf_len[static_cast<uint64_t>(i) * kMaxQV + (k - d0)] = s;
-            total += s;
-            if (s == 0) prod = 0;
-            else if (prod > (~0ull >> 1) / s) overflow = true;
-            else prod *= s;
-        }
-        unsigned long long h = 0;
-        for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
-        if (overflow) {
-            if (lane == 0) atomicOr(c.err, kErrProductOverflow);
-            prod = 0;
-        }
-        unsigned long long base = 0;
-        if (prod && lane == 0) {
-            base = atomicAdd(c.lists_top, static_cast<unsigned long long>(total));
-            if (base + total > c.lists_cap) { atomicOr(c.err, kErrScratchFull); base = ~0ull; }
-        }
-        base = __shfl(base, 0, 64);
-        if (base == ~0ull) prod = 0;
-        if (lane == 0) {
-            c.prod[i] = prod;
-            c.rec_hash[i] = h;
-            c.list_base[i] = prod ? static_cast<uint32_t>(base) : 0u;
-            if (prod) atomicMax(c.max_prod, prod);
-        }
-        if (!prod) continue;
-        // pass 2: write the lists
-        uint32_t woff = static_cast<uint32_t>(base);
-        for (uint32_t k = d0; k < d.n; k++) {
-            const uint32_t e = d.boff[k];
-            const BackEntry be = d.be[e];
-            const uint32_t* dl = phase_delta(g, be.dir, d.phase);
-            const RunView rv = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
-            const uint32_t nb = run_base_len(rv, be.view), tot = run_seq_len(rv, be.view);
-            for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
-                const uint32_t t = t0 + lane;
-                bool ok = t < tot;
-                const uint32_t key = ok ? run_elem(g, dl, rv, nb, t) : 0u;
-                const uint32_t x = key & g.id_mask;
-                if (ok) ok = bit_test(d.filter[k], x) && run_elem_live(dl, rv, be.view, key);
-                for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != x;
-                const uint64_t m = __ballot(ok);
-                if (ok) c.lists[woff + lanes_below(m)] = x;
-                woff += static_cast<uint32_t>(__popcll(m));
-            }
+// The trailing leaves d0 .. n-1 of a record each hang off one matched vertex.  A segment is one
+// (record, leaf) pair, segment i = record * leaves + leaf.  k_cp_len sizes every segment's view run
+// and the record's prefix hash; after a scan, k_cp_filter tests every run element (load-balanced
+// over the grid, as k_level_flat) against the leaf's candidate bitmap, deletion liveness and the
+// matched prefix, and appends survivors to the segment's list (order within a list is free: the
+// enumeration takes every combination once); k_cp_products multiplies the list lengths per record.
+__global__ void __launch_bounds__(256) k_cp_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n, CpScratch c) {
+    const uint32_t nl = d.n - d0;
+    const uint64_t nseg = static_cast<uint64_t>(n) * nl;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < nseg; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t r = static_cast<uint32_t>(i / nl), kk = static_cast<uint32_t>(i % nl);
+        const uint32_t* f = rec + static_cast<uint64_t>(r) * d0;
+        const BackEntry be = d.be[d.boff[d0 + kk]];
+        const RunView rv = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
+        c.seg_off[i] = run_seq_len(rv, be.view);
+        c.seg_run[i] = make_uint4(rv.lo, rv.hi, rv.dlo, rv.dhi);
+        c.seg_cnt[i] = 0;
+        if (kk == 0) {
+            unsigned long long h = 0;
+            for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
+            c.rec_hash[r] = h;
         }
     }
 }
 
+__global__ void __launch_bounds__(256) k_cp_filter(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t nseg, CpScratch c) {
+    __shared__ unsigned long long s_win[4][65];
+    const uint32_t lane = lane_id();
+    unsigned long long* win = s_win[threadIdx.x >> 6];
+    const uint32_t nl = d.n - d0;
+    const unsigned long long* off = c.seg_off;
+    const unsigned long long total = off[nseg];
+    if (total > c.lists_cap) {  // lists do not fit: the host counts these records another way
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(c.err, kErrScratchFull);
+        return;
+    }
+    const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * (blockDim.x >> 6);
+    const unsigned long long gw = static_cast<unsigned long long>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const unsigned long long per = ((total + waves - 1) / waves + 63) & ~63ull;
+    const unsigned long long w0 = gw * per, w1 = min(total, w0 + per);
+    uint32_t rb = w0 < total ? find_record(off, nseg, w0, lane) : 0u;  // off[rb] <= w0
+    for (unsigned long long t0 = w0; t0 < w1; t0 += 64) {
+        win[lane] = off[min(rb + lane, nseg)];
+        if (lane == 0) win[64] = off[min(rb + 64, nseg)];
+        wave_sync();
+        const unsigned long long t = t0 + lane;
+        uint32_t i = rb;
+        if (t < w1) {
+            if (win[64] <= t) {
+                uint32_t lo = rb + 64, hi = nseg;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (off[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                i = lo;
+            } else {
+                uint32_t lo = 0, hi = 64;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (win[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                i = rb + lo;
+            }
+            const uint32_t r = i / nl, kk = i % nl;
+            const BackEntry be = d.be[d.boff[d0 + kk]];
+            const uint32_t* dl = phase_delta(g, be.dir, d.phase);
+            const uint4 x4 = c.seg_run[i];
+            const RunView rv{x4.x, x4.y, x4.z, x4.w};
+            const unsigned long long base = i - rb < 64 ? win[i - rb] : off[i];
+            const uint32_t key = run_elem(g, dl, rv, run_base_len(rv, be.view), static_cast<uint32_t>(t - base));
+            const uint32_t x = key & g.id_mask;
+            bool ok = bit_test(d.filter[d0 + kk], x) && run_elem_live(dl, rv, be.view, key);
+            const uint32_t* f = rec + static_cast<uint64_t>(r) * d0;
+            for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != x;
+            if (ok) c.lists[base + atomicAdd(&c.seg_cnt[i], 1u)] = x;
+        }
+        rb = uniform(__shfl(i, 63, 64));
+        wave_sync();
+    }
+}
+
+// Thread per record: the product of its leaf list lengths (0 when a list is empty).
+__global__ void __launch_bounds__(256) k_cp_products(StepDesc d, uint32_t d0, uint32_t n, CpScratch c) {
+    const uint32_t nl = d.n - d0;
+    unsigned long long mx = 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        unsigned long long prod = 1;
+        bool overflow = false;
+        for (uint32_t kk = 0; kk < nl; kk++) {
+            const uint32_t sz = c.seg_cnt[static_cast<uint64_t>(r) * nl + kk];
+            if (sz == 0) { prod = 0; break; }
+            if (prod > (~0ull >> 1) / sz) overflow = true;
+            else prod *= sz;
+        }
+        if (overflow) {
+            atomicOr(c.err, kErrProductOverflow);
+            prod = 0;
+        }
+        c.prod[r] = prod;
+        mx = max(mx, prod);
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, static_cast<unsigned long long>(__shfl_xor(mx, o, 64)));
+    if (lane_id() == 0 && mx) atomicMax(c.max_prod, mx);
+}
+
 // Thread per flattened combination t: record r owns [off[r], off[r+1]); digits of t - off[r] in the
 // mixed radix of the leaf list sizes select one candidate per leaf; leaves must be distinct.
-__global__ void __launch_bounds__(256) k_cp_enumerate(StepDesc d, uint32_t d0, const uint32_t* n_rec, uint32_t max_rec, CpScratch c,
-                                                      MatchOut out) {
+__global__ void __launch_bounds__(256) k_cp_enumerate(StepDesc d, uint32_t d0, uint32_t n, CpScratch c, MatchOut out) {
     __shared__ unsigned long long sc[256], sh[256];
-    const uint32_t n = min(*n_rec, max_rec);
     const unsigned long long T = c.prod[n];
     const uint32_t nl = d.n - d0;
     unsigned long long cnt = 0, hsum = 0;
@@ -551,17 +577,15 @@ __global__ void __launch_bounds__(256) k_cp_enumerate(StepDesc d, uint32_t d0, c
         }
         const uint32_t r = lo;
         unsigned long long local = t - c.prod[r];
-        const uint32_t* ll = c.leaf_len + static_cast<uint64_t>(r) * kMaxQV;
         uint32_t x[kMaxQV];
-        uint32_t loff = c.list_base[r];
         unsigned long long h = c.rec_hash[r];
         bool ok = true;
         for (uint32_t k = 0; k < nl; k++) {
-            const uint32_t s = ll[k];
-            const uint32_t idx = static_cast<uint32_t>(local % s);
-            local /= s;
-            x[k] = c.lists[loff + idx];
-            loff += s;
+            const uint64_t i = static_cast<uint64_t>(r) * nl + k;
+            const uint32_t sz = c.seg_cnt[i];
+            const uint32_t idx = static_cast<uint32_t>(local % sz);
+            local /= sz;
+            x[k] = c.lists[c.seg_off[i] + idx];
             for (uint32_t j = 0; j < k; j++) ok = ok && x[j] != x[k];
             h += pair_hash(d.order[d0 + k], x[k]);
         }
@@ -671,17 +695,17 @@ void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32
     scan_exclusive<unsigned long long>(chunks, chunks, n_rec, max_rec, true, scan_tmp, s);
 }
 
-void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
-                CpScratch c, hipStream_t s) {
-    k_cp_prepare<<<grid_for(max_rec, 4, 8192), 256, 0, s>>>(g, d, d0, rec, n_rec, max_rec, c);
-    scan_exclusive<unsigned long long>(c.prod, c.prod, n_rec, max_rec, true, c.scan_tmp, s);
+void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, CpScratch c, hipStream_t s) {
+    const uint32_t nseg = n * (d.n - d0);
+    k_cp_len<<<grid_for(nseg, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, c);
+    scan_exclusive<unsigned long long>(c.seg_off, c.seg_off, nullptr, nseg, true, c.scan_tmp, s);
+    k_cp_filter<<<2048, 256, 0, s>>>(g, d, d0, rec, nseg, c);
+    k_cp_products<<<grid_for(n, 256, 4096), 256, 0, s>>>(d, d0, n, c);
+    scan_exclusive<unsigned long long>(c.prod, c.prod, nullptr, n, true, c.scan_tmp, s);
 }
 
-void cp_enumerate(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
-                  CpScratch c, MatchOut out, hipStream_t s) {
-    (void)g;
-    (void)rec;
-    k_cp_enumerate<<<4096, 256, 0, s>>>(d, d0, n_rec, max_rec, c, out);
+void cp_enumerate(const StepDesc& d, uint32_t d0, uint32_t n, CpScratch c, MatchOut out, hipStream_t s) {
+    k_cp_enumerate<<<4096, 256, 0, s>>>(d, d0, n, c, out);
 }
 
 }  // namespace csm

@@ -168,6 +168,72 @@ __global__ void k_local_filter(DevGraph g, uint32_t phase, uint32_t view, uint32
     }
 }
 
+// Expansion fused with the semi-joins: y (from a frontier vertex's view run, in cand_u) is decided
+// by its first visitor (atomicOr on out.bitmap): it joins out.list when it has a neighbour in every
+// filter's bitmap along the filter's view, else it goes to the fail list and k_clear_failed drops
+// its bit afterwards.  Later visitors of y see the bit and skip it.
+__global__ void __launch_bounds__(256) k_local_build(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                                                     const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
+                                                     const uint32_t* cand_u, LocalFilters fl, VSet out, uint32_t* fail, uint32_t* fail_len) {
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t F = min(*frontier_len, max_frontier);
+    const uint32_t* dl = phase_delta(g, dir, phase);
+    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < F; i += gridDim.x * wpb) {
+        const uint32_t x = frontier[i];
+        const RunView r = make_run(g, phase, view, dir, x, slot);
+        const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
+        for (uint32_t t = lane; t < tot; t += 64) {
+            const uint32_t key = run_elem(g, dl, r, nb, t);
+            const uint32_t y = key & g.id_mask;
+            if (!bit_test(cand_u, y) || !run_elem_live(dl, r, view, key)) continue;
+            const uint32_t bit = 1u << (y & 31);
+            if (atomicOr(&out.bitmap[y >> 5], bit) & bit) continue;  // decided by another visitor
+            bool pass = true;
+            for (uint32_t k = 0; k < fl.n && pass; k++) {
+                const LocalFilter f = fl.f[k];
+                const uint32_t* fdl = phase_delta(g, f.dir, phase);
+                const RunView fr = make_run(g, phase, f.view, f.dir, y, f.slot);
+                const uint32_t fnb = run_base_len(fr, f.view), ftot = run_seq_len(fr, f.view);
+                bool hit = false;
+                for (uint32_t u = 0; u < ftot && !hit; u++) {
+                    const uint32_t fk = run_elem(g, fdl, fr, fnb, u);
+                    hit = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
+                }
+                pass = hit;
+            }
+            if (pass) {
+                const uint32_t p = atomicAdd(out.len, 1u);
+                if (p < out.cap) out.list[p] = y;
+            } else {
+                fail[atomicAdd(fail_len, 1u)] = y;
+            }
+        }
+    }
+}
+
+// One block: drops the bits of the failed candidates, then resets the fail list.
+__global__ void __launch_bounds__(1024) k_clear_failed(uint32_t* bitmap, const uint32_t* fail, uint32_t* fail_len, uint32_t max_len) {
+    const uint32_t n = min(*fail_len, max_len);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAnd(&bitmap[fail[i] >> 5], ~(1u << (fail[i] & 31)));
+    __syncthreads();
+    if (threadIdx.x == 0) *fail_len = 0;
+}
+
+// Clears the bitmaps of every set in `mask` (whole words of its listed members).
+__global__ void __launch_bounds__(256) k_clear_sets(VSetTable t, uint32_t mask, uint32_t max_len) {
+    for (uint32_t u = 0; u < kMaxQV; u++) {
+        if (!(mask >> u & 1u)) continue;
+        const VSet v = t.s[u];
+        const uint32_t n = min(*v.len, max_len);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v.bitmap[v.list[i] >> 5] = 0u;
+    }
+}
+
+__global__ void k_store_words(uint32_t* p, Words w) {
+    if (threadIdx.x < w.n) p[threadIdx.x] = w.v[threadIdx.x];
+}
+
 __global__ void k_set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap) {
     const uint32_t n = min(*len, max_len);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -224,6 +290,20 @@ void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir
     k_local_expand<<<grid_for(max_frontier, 4, 8192), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier,
                                                                      cand_u, out);
 }
+
+void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
+                 const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
+                 uint32_t* fail, uint32_t* fail_len, hipStream_t s) {
+    k_local_build<<<grid_for(max_frontier, 4, 8192), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
+                                                                    fl, out, fail, fail_len);
+    k_clear_failed<<<1, 1024, 0, s>>>(out.bitmap, fail, fail_len, g.dv);
+}
+
+void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s) {
+    if (mask) k_clear_sets<<<grid_for(max_len, 256, 1024), 256, 0, s>>>(t, mask, max_len);
+}
+
+void store_words(uint32_t* p, const Words& w, hipStream_t s) { k_store_words<<<1, 64, 0, s>>>(p, w); }
 
 void local_filter(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* in,
                   const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b, uint32_t* out, uint32_t* out_len, hipStream_t s) {

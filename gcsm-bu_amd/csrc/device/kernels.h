@@ -100,6 +100,30 @@ void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir
 void local_filter(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                   const uint32_t* in, const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b,
                   uint32_t* out, uint32_t* out_len, hipStream_t s);
+// Local index of one query vertex u in one launch pair: expand from the frontier along
+// (view, dir, slot), keep the candidates of cand_u that have a neighbour in every filter's bitmap.
+struct LocalFilter {
+    uint32_t view, dir, slot;
+    const uint32_t* bitmap;
+};
+struct LocalFilters {
+    uint32_t n;
+    LocalFilter f[kMaxQV - 1];
+};
+void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
+                 const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
+                 uint32_t* fail, uint32_t* fail_len, hipStream_t s);
+struct VSetTable {
+    VSet s[kMaxQV];
+};
+// Clears the bitmaps of the sets in mask (their lengths stay; the caller zeroes them).
+void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s);
+// Stores w.v[0 .. w.n) at p (one tiny launch instead of several fills).
+struct Words {
+    uint32_t n;
+    uint32_t v[15];
+};
+void store_words(uint32_t* p, const Words& w, hipStream_t s);
 void set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s);
 void clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s);
 void copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len, hipStream_t s);
@@ -161,23 +185,24 @@ void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
                  uint32_t max_rec, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s);
 
-// Cartesian-product flattening of trailing tree leaves (cartesian_product.cu:193-284).
+// Cartesian-product flattening of trailing tree leaves (cartesian_product.cu:193-284).  A segment
+// is one (record, leaf) pair: segment i = record * leaves + leaf.
 struct CpScratch {
-    unsigned long long* prod;      // [max_records + 1] products, scanned in place into offsets
+    unsigned long long* seg_off;   // [max_segments + 1] scanned view-run lengths of the segments
+    uint4* seg_run;                // [max_segments] view run of each segment
+    uint32_t* seg_cnt;             // [max_segments] filtered list length of each segment
+    unsigned long long* prod;      // [max_records + 1] list-length products, scanned in place into offsets
     unsigned long long* rec_hash;  // [max_records] pair-hash sum of the record's prefix
     unsigned long long* scan_tmp;
-    uint32_t* list_base;           // [max_records] start of the record's leaf lists in `lists`
-    uint32_t* leaf_len;            // [max_records * kMaxQV]
-    uint32_t* lists;               // leaf candidates
-    unsigned long long* lists_top; // [1]
+    uint32_t* lists;               // segment i's list starts at lists[seg_off[i]]
     unsigned long long lists_cap;
     unsigned long long* max_prod;  // [1]
     uint32_t* err;                 // kErrProductOverflow / kErrScratchFull of this scratch's last preparation
 };
-void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
-                uint32_t max_rec, CpScratch c, hipStream_t s);
-void cp_enumerate(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
-                  uint32_t max_rec, CpScratch c, MatchOut out, hipStream_t s);
+// n records of depth d0 (host count), n * (d.n - d0) <= max_segments; max_prod and err zeroed by
+// the caller.  prod[n] ends up as the number of combinations.
+void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, CpScratch c, hipStream_t s);
+void cp_enumerate(const StepDesc& d, uint32_t d0, uint32_t n, CpScratch c, MatchOut out, hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // Plain copy of n words (calibration of the HBM traffic counters).

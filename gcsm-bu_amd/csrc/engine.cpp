@@ -216,8 +216,8 @@ std::string Engine::build() {
         add(max_arcs * 8 + 16);                                    // anchor pairs
         add(ring_words_ * 4); add(256);
         add((cp_rec + 1) * 8); add(cp_rec * 8); add(scan_scratch_elems(static_cast<uint32_t>(cp_rec)) * 8);
-        add(cp_rec * 4); add(cp_rec * kMaxQV * 4); add(cp_list_words * 4); add(64);
-        add(cp_rec * 16);                                          // pivot runs of a level's records
+        add((cp_rec + 1) * 8); add(cp_rec * 16); add(cp_rec * 4);  // CP segments (runs shared with levels)
+        add(cp_list_words * 4); add(64);
         add(64 << 10);                                             // alignment slack of the takes
     }
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
@@ -370,29 +370,29 @@ std::string Engine::build() {
             if (!w.L[u].bitmap || !w.L[u].list) return "device pool exhausted (local sets)";
             CSM_HIP(hipMemsetAsync(w.L[u].bitmap, 0, words_ * 4ull, s_));
         }
-        uint32_t* lens = pool_.take<uint32_t>(32);
+        uint32_t* lens = pool_.take<uint32_t>(kLaneWords);  // set lengths, list counters, step counters
         if (!lens) return "device pool exhausted (local sets)";
         for (uint32_t u = 0; u < kMaxQV; u++) w.L[u].len = lens + u;
         w.tmp_len = lens + kMaxQV;
         w.n_pairs = lens + kMaxQV + 1;
+        w.counters = lens + 32;
         w.tmp_list = pool_.take<uint32_t>(dv_);
         w.pairs = pool_.take<uint32_t>(2 * max_arcs);
         w.ring = pool_.take<uint32_t>(ring_words_);
         w.ring_words = ring_words_;
-        w.counters = pool_.take<uint32_t>(16);
         w.cp.prod = pool_.take<unsigned long long>(cp_rec + 1);
         w.cp.rec_hash = pool_.take<unsigned long long>(cp_rec);
         w.cp.scan_tmp = pool_.take<unsigned long long>(scan_scratch_elems(static_cast<uint32_t>(cp_rec)));
-        w.cp.list_base = pool_.take<uint32_t>(cp_rec);
-        w.cp.leaf_len = pool_.take<uint32_t>(cp_rec * kMaxQV);
+        w.cp.seg_off = pool_.take<unsigned long long>(cp_rec + 1);
+        w.cp.seg_run = pool_.take<uint4>(cp_rec);
+        w.cp.seg_cnt = pool_.take<uint32_t>(cp_rec);
         w.cp.lists = pool_.take<uint32_t>(cp_list_words);
         w.cp.lists_cap = cp_list_words;
         unsigned long long* cpc = pool_.take<unsigned long long>(4);
         if (!cpc) return "device pool exhausted (lane scratch)";
-        w.cp.lists_top = cpc;
-        w.cp.max_prod = cpc + 1;
-        w.cp.err = reinterpret_cast<uint32_t*>(cpc + 2);
-        w.level = LevelScratch{w.cp.prod, pool_.take<uint4>(cp_rec), w.cp.rec_hash, w.cp.scan_tmp};
+        w.cp.max_prod = cpc;
+        w.cp.err = reinterpret_cast<uint32_t*>(cpc + 1);
+        w.level = LevelScratch{w.cp.prod, w.cp.seg_run, w.cp.rec_hash, w.cp.scan_tmp};
         if (!w.level.runs) return "device pool exhausted (lane scratch)";
     }
     opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
@@ -522,11 +522,14 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
     const uint32_t max_arcs = 2 * max_batch_;
     auto cand = [&](uint32_t u) { return qs.cand + static_cast<uint64_t>(u) * words_; };
 
-    // clear the local sets of the previous step
-    for (uint32_t u = 0; u < kMaxQV; u++)
-        if (w.L_used >> u & 1u) clear_bits(w.L[u].list, w.L[u].len, dv_, w.L[u].bitmap, w.s);
+    // clear the local sets of the previous step, then zero their lengths and the step counters
+    if (w.L_used) {
+        VSetTable t;
+        for (uint32_t u = 0; u < kMaxQV; u++) t.s[u] = w.L[u];
+        clear_sets(t, w.L_used, dv_, w.s);
+    }
     w.L_used = 0;
-    fill_u32(w.L[0].len, 0u, kMaxQV + 2, w.s);
+    fill_u32(w.L[0].len, 0u, kLaneWords, w.s);
 
     const RelInfo& R = qs.rel[r01];
     prof_begin(w.prof, w.s, kKAnchors);
@@ -557,19 +560,21 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
             const uint32_t b0 = bsv[first];
             const uint32_t rb = q.rel(b0, u);
             prof_begin(w.prof, w.s, kKLocalIndex);
-            local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u], w.s);
-            w.L_used |= static_cast<uint16_t>(1u << u);
-            for (size_t k = 0; k < bsv.size(); k++) {
-                if (k == first) continue;
-                const uint32_t b = bsv[k];
-                const uint32_t rub = q.rel(u, b);
-                fill_u32(w.tmp_len, 0u, 1, w.s);
-                local_filter(g_, ph, view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[u].list, w.L[u].len, dv_, w.L[b].bitmap,
-                             w.tmp_list, w.tmp_len, w.s);
-                clear_bits(w.L[u].list, w.L[u].len, dv_, w.L[u].bitmap, w.s);
-                copy_list(w.tmp_list, w.tmp_len, dv_, w.L[u].list, w.L[u].len, w.s);
-                set_bits(w.L[u].list, w.L[u].len, dv_, w.L[u].bitmap, w.s);
+            if (bsv.size() == 1) {
+                local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u],
+                             w.s);
+            } else {
+                // semi-joins with the other built neighbours, fused into the expansion
+                LocalFilters fl{};
+                for (size_t k = 0; k < bsv.size(); k++) {
+                    if (k == first) continue;
+                    const uint32_t rub = q.rel(u, bsv[k]);
+                    fl.f[fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[bsv[k]].bitmap};
+                }
+                local_build(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), fl,
+                            w.L[u], w.tmp_list, w.tmp_len, w.s);
             }
+            w.L_used |= static_cast<uint16_t>(1u << u);
             prof_end(w.prof, w.s, kKLocalIndex);
             built |= 1u << u;
         }
@@ -610,7 +615,6 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
 
     // ---- initial partial results (depth 2) at the bottom of the lane's ring ----
     uint32_t* ring = w.ring;
-    fill_u32(w.counters, 0u, 10, w.s);
     const uint32_t a = o.vs[0], b = o.vs[1];
     uint32_t count;
     if ((a == u0 && b == u1) || (a == u1 && b == u0)) {
@@ -680,9 +684,8 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
 std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, uint32_t count, uint32_t depth, uint32_t end, uint32_t* out,
                               uint64_t cap, bool& ok, uint32_t& n_out) {
     cap = std::min<uint64_t>(cap, 0xFFFFFFFFull);
-    fill_u32(w.counters, count, 1, w.s);   // [0]: input count
-    fill_u32(w.counters + 1, 0u, 2, w.s);  // [1]: output count, [2]: overflow
-    fill_u32(w.counters + 8, 0u, 2, w.s);  // work counter
+    // [0]: input count, [1]: output count, [2]: overflow, [3]: finishing count, [8..9]: work counter
+    store_words(w.counters, Words{10, {count, 0, 0, 0, 0, 0, 0, 0, 0, 0}}, w.s);
     Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
     if (end == depth + 1) {
         for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
@@ -749,13 +752,14 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
     const uint32_t ph = d.phase;
     MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr};
     uint32_t* rec_count = w.counters + 3;
-    fill_u32(rec_count, count, 1, w.s);
-    bool use_cp = qs.tech.cartesian_product && depth < d.n && qs.plan.cp_info[i][depth] == kCpTreeProduct && count <= opt_.cp_max_records;
+    store_words(rec_count, Words{7, {count, 0, 0, 0, 0, 0, 0}}, w.s);  // [3]: count, [8..9]: work counter
+    const uint64_t n_seg = static_cast<uint64_t>(count) * (d.n - depth);
+    bool use_cp = qs.tech.cartesian_product && depth < d.n && qs.plan.cp_info[i][depth] == kCpTreeProduct && n_seg <= opt_.cp_max_records;
     if (use_cp) {
         // match_gpu.cu:2046-2085: flatten unless the products are uniformly large
-        CSM_HIP(hipMemsetAsync(w.cp.lists_top, 0, 24, w.s));  // lists_top, max_prod, err
+        CSM_HIP(hipMemsetAsync(w.cp.max_prod, 0, 16, w.s));  // max_prod, err
         prof_begin(w.prof, w.s, kKCpPrepare);
-        cp_prepare(g_, d, depth, rec, rec_count, count, w.cp, w.s);
+        cp_prepare(g_, d, depth, rec, count, w.cp, w.s);
         prof_end(w.prof, w.s, kKCpPrepare);
         unsigned long long v[2];
         CSM_HIP(hipMemcpyAsync(w.h_pinned, w.cp.max_prod, 8, hipMemcpyDeviceToHost, w.s));
@@ -774,12 +778,11 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
         if (trace_) w.trace_line += ", \"cp_total\": " + std::to_string(v[1]) + ", \"cp_max\": " + std::to_string(v[0]) + ", \"cp_used\": " + std::to_string(use_cp ? 1 : 0);
         if (use_cp) {
             prof_begin(w.prof, w.s, kKCpEnumerate);
-            cp_enumerate(g_, d, depth, rec, rec_count, count, w.cp, out, w.s);
+            cp_enumerate(d, depth, count, w.cp, out, w.s);
             prof_end(w.prof, w.s, kKCpEnumerate);
             return "";
         }
     }
-    fill_u32(w.counters + 8, 0u, 2, w.s);
     Ring r{rec, rec_count, count, nullptr, nullptr, 0, nullptr};
     if (trace_) w.trace_line += ", \"count_from\": " + std::to_string(depth) + ", \"records\": " + std::to_string(count);
     if (depth + 1 == d.n && w.sched == 1) {

@@ -48,6 +48,7 @@ struct EngineOptions {
 constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
 constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
 constexpr uint32_t kMaxLanes = 16;
+constexpr uint32_t kLaneWords = 48;  // per lane: 16 set lengths, list counters, then 16 step counters at +32
 
 struct QueryResult {
     uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;

@@ -243,6 +243,71 @@ __global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint3
     }
 }
 
+// Levels of under kSmallLevel records (most of them): sizes and scan in one single-block launch.
+constexpr uint32_t kSmallThreads = 1024;
+constexpr uint32_t kSmallItems = 4;
+constexpr uint32_t kSmallLevel = kSmallThreads * kSmallItems;
+
+__global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                                   unsigned long long* off, uint4* runs, unsigned long long* ph,
+                                                                   unsigned long long* bytes_out) {
+    __shared__ unsigned long long s_wave[kSmallThreads / 64];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const uint32_t p = d.pivot[d0];
+    const BackEntry pe = d.be[p];
+    unsigned long long len[kSmallItems], sum = 0, bytes = 0;
+    // thread tid owns records tid * kSmallItems + j (contiguous, so the scan is one block scan)
+#pragma unroll
+    for (uint32_t j = 0; j < kSmallItems; j++) {
+        const uint32_t i = tid * kSmallItems + j;
+        len[j] = 0;
+        if (i >= n) continue;
+        const uint32_t* f = rec + static_cast<uint64_t>(i) * d0;
+        const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
+        len[j] = run_seq_len(rv, pe.view);
+        runs[i] = make_uint4(rv.lo, rv.hi, rv.dlo, rv.dhi);
+        if (ph) {
+            unsigned long long h = 0;
+            for (uint32_t q = 0; q < d0; q++) h += pair_hash(d.order[q], f[q]);
+            ph[i] = h;
+        }
+        if (bytes_out) {
+            for (uint32_t e = d.boff[d0]; e < d.boff[d0 + 1]; e++) {
+                const BackEntry be = d.be[e];
+                const RunView ov = e == p ? rv : make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
+                bytes += 4ull * ((ov.hi - ov.lo) + (ov.dhi - ov.dlo));
+            }
+        }
+        sum += len[j];
+    }
+    // block exclusive scan of the per-thread sums
+    unsigned long long x = sum;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wv] = x;
+    __syncthreads();
+    unsigned long long before = 0, total = 0;
+    for (uint32_t k = 0; k < kSmallThreads / 64; k++) {
+        const unsigned long long t = s_wave[k];
+        if (k < wv) before += t;
+        total += t;
+    }
+    unsigned long long o = before + x - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < kSmallItems; j++) {
+        const uint32_t i = tid * kSmallItems + j;
+        if (i < n) off[i] = o;
+        o += len[j];
+    }
+    if (tid == 0) off[n] = total;
+    if (bytes_out) {
+        bytes = wave_sum_u64(bytes);
+        if (lane == 0 && bytes) atomicAdd(bytes_out, bytes);
+    }
+}
+
 // Each wave walks one contiguous range of the level's candidates (equal shares, multiples of
 // kLevelStep).  It finds the record of its first candidate with a 64-ary search; per kLevelStep
 // candidates it loads the next kLevelStep + 1 record offsets into LDS (coalesced) and each lane
@@ -252,13 +317,40 @@ __global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint3
 constexpr uint32_t kLevelIlp = 4;
 constexpr uint32_t kLevelStep = 64 * kLevelIlp;
 
+// Write mode stages passing (record, candidate) pairs per wave in LDS and reserves ring space once
+// per kOutFlush pairs: an atomic per iteration would put its round trip on every iteration.
+constexpr uint32_t kOutFlush = 256;
+constexpr uint32_t kOutBuf = kOutFlush + kLevelStep;
+
+__device__ __forceinline__ void flush_records(const uint32_t* rec, uint32_t d0, const uint32_t* sk, const uint32_t* sc, uint32_t cnt, Ring& r,
+                                              uint32_t lane) {
+    uint32_t wb = 0;
+    if (lane == 0) wb = atomicAdd(r.out_count, cnt);
+    wb = uniform(__shfl(wb, 0, 64));
+    if (static_cast<uint64_t>(wb) + cnt > r.out_cap) {
+        if (lane == 0) atomicOr(r.overflow, 1u);
+        return;
+    }
+    // consecutive words of the reserved region: coalesced stores
+    const uint32_t rw = d0 + 1;
+    uint32_t* o = r.out + static_cast<uint64_t>(wb) * rw;
+    for (uint32_t w = lane; w < cnt * rw; w += 64) {
+        const uint32_t e = w / rw, q = w - e * rw;
+        o[w] = q < d0 ? rec[static_cast<uint64_t>(sk[e]) * d0 + q] : sc[e];
+    }
+}
+
 template <int kMode>
 __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
                                                     const unsigned long long* off, const uint4* runs, const unsigned long long* ph,
                                                     Ring r, MatchOut out) {
     __shared__ unsigned long long s_win[4][kLevelStep + 1];
+    __shared__ uint32_t s_out[4][kMode == kWriteAtEnd ? 2 * kOutBuf : 1];
     const uint32_t lane = lane_id();
     unsigned long long* win = s_win[threadIdx.x >> 6];
+    uint32_t* const out_k = s_out[threadIdx.x >> 6];
+    uint32_t* const out_c = out_k + (kMode == kWriteAtEnd ? kOutBuf : 0);
+    uint32_t staged = 0;  // pairs in the wave's LDS buffer
     const uint32_t p = d.pivot[d0];
     const BackEntry pe = d.be[p];
     const uint32_t* pdl = phase_delta(g, pe.dir, d.phase);
@@ -351,31 +443,20 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
                     hsum += embedding_hash(ph[k[j]] + pair_hash(qv, c[j]));
                 }
         } else {
-            uint64_t m[kLevelIlp];
-            uint32_t np = 0;
 #pragma unroll
             for (uint32_t j = 0; j < kLevelIlp; j++) {
-                m[j] = __ballot(ok[j]);
-                np += static_cast<uint32_t>(__popcll(m[j]));
-            }
-            if (np) {
-                uint32_t wb = 0;
-                if (lane == 0) wb = atomicAdd(r.out_count, np);
-                wb = __shfl(wb, 0, 64);
-                if (static_cast<uint64_t>(wb) + np > r.out_cap) {
-                    if (lane == 0) atomicOr(r.overflow, 1u);
-                } else {
-#pragma unroll
-                    for (uint32_t j = 0; j < kLevelIlp; j++) {
-                        if (ok[j]) {
-                            uint32_t* o = r.out + static_cast<uint64_t>(wb + lanes_below(m[j])) * (d0 + 1);
-                            const uint32_t* f = rec + static_cast<uint64_t>(k[j]) * d0;
-                            for (uint32_t q = 0; q < d0; q++) o[q] = f[q];
-                            o[d0] = c[j];
-                        }
-                        wb += static_cast<uint32_t>(__popcll(m[j]));
-                    }
+                const uint64_t m = __ballot(ok[j]);
+                if (ok[j]) {
+                    const uint32_t idx = staged + lanes_below(m);
+                    out_k[idx] = k[j];
+                    out_c[idx] = c[j];
                 }
+                staged += static_cast<uint32_t>(__popcll(m));
+            }
+            if (staged >= kOutFlush || t0 + kLevelStep >= w1) {
+                wave_sync();
+                if (staged) flush_records(rec, d0, out_k, out_c, staged, r, lane);
+                staged = 0;
             }
         }
         if (out.bytes) {
@@ -650,7 +731,7 @@ void init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t ma
 
 void init_from_set(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* list,
                    const uint32_t* len, uint32_t max_len, const uint32_t* filter_b, Ring r, hipStream_t s) {
-    k_init_from_set<<<grid_for(max_len, 4, 8192), 256, 0, s>>>(g, phase, view, dir, slot, list, len, max_len, filter_b, r);
+    k_init_from_set<<<grid_for(max_len, 4, 1024), 256, 0, s>>>(g, phase, view, dir, slot, list, len, max_len, filter_b, r);
 }
 
 void count_records(const StepDesc& d, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec, MatchOut out, hipStream_t s) {
@@ -676,17 +757,24 @@ void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_dept
 
 void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
                    bool with_hash, unsigned long long* bytes, hipStream_t s) {
+    if (n < kSmallLevel) {
+        k_level_len_small<<<1, kSmallThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, with_hash ? ls.ph : nullptr, bytes);
+        return;
+    }
     k_level_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, with_hash ? ls.ph : nullptr, bytes);
     scan_exclusive<unsigned long long>(ls.off, ls.off, nullptr, n, true, ls.scan_tmp, s);
 }
 
 void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode, const LevelScratch& ls,
                 Ring r, MatchOut out, hipStream_t s) {
-    constexpr int kGrid = 2048;  // waves take equal contiguous shares of the level's candidates
+    // Waves take equal contiguous shares of the level's candidates (their total is known on the
+    // device only): the grid follows the record count, so small levels do not dispatch thousands
+    // of idle waves.
+    const int grid = static_cast<int>(std::min<uint32_t>(2048, std::max<uint32_t>(16, n / 2)));
     if (mode == kCountAtEnd)
-        k_level_flat<kCountAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
+        k_level_flat<kCountAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
     else
-        k_level_flat<kWriteAtEnd><<<kGrid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
+        k_level_flat<kWriteAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
 }
 
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
@@ -699,13 +787,13 @@ void cp_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_
     const uint32_t nseg = n * (d.n - d0);
     k_cp_len<<<grid_for(nseg, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, c);
     scan_exclusive<unsigned long long>(c.seg_off, c.seg_off, nullptr, nseg, true, c.scan_tmp, s);
-    k_cp_filter<<<2048, 256, 0, s>>>(g, d, d0, rec, nseg, c);
+    k_cp_filter<<<static_cast<int>(std::min<uint32_t>(2048, std::max<uint32_t>(16, nseg / 2))), 256, 0, s>>>(g, d, d0, rec, nseg, c);
     k_cp_products<<<grid_for(n, 256, 4096), 256, 0, s>>>(d, d0, n, c);
     scan_exclusive<unsigned long long>(c.prod, c.prod, nullptr, n, true, c.scan_tmp, s);
 }
 
 void cp_enumerate(const StepDesc& d, uint32_t d0, uint32_t n, CpScratch c, MatchOut out, hipStream_t s) {
-    k_cp_enumerate<<<4096, 256, 0, s>>>(d, d0, n, c, out);
+    k_cp_enumerate<<<static_cast<int>(std::min<uint32_t>(4096, std::max<uint32_t>(64, n))), 256, 0, s>>>(d, d0, n, c, out);
 }
 
 }  // namespace csm

@@ -10,8 +10,8 @@
 // Local index.  For batch step i the reference rebuilds candidate bitmaps along the indexing order
 // (BuildLocalIndex_v2, match_gpu.cu:1098-1343, kernels indexing.cu:645-1436): the anchors give
 // L(u0), L(u1); a later vertex u keeps the candidates that have a neighbour in L(b) for every
-// earlier neighbour b.  Here it is computed sparsely: expand from one earlier neighbour's list,
-// then semi-join the survivors against the other earlier neighbours' bitmaps.
+// earlier neighbour b.  Here it is computed sparsely: expand from one earlier neighbour's list and
+// semi-join each new candidate against the other earlier neighbours' bitmaps in the same pass.
 #include "kernels.h"
 #include "view.h"
 
@@ -151,23 +151,6 @@ __global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase
     }
 }
 
-__global__ void k_local_filter(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* in,
-                               const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b, uint32_t* out, uint32_t* out_len) {
-    const uint32_t n = min(*in_len, max_in);
-    const uint32_t* dl = phase_delta(g, dir, phase);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t y = in[i];
-        const RunView r = make_run(g, phase, view, dir, y, slot);
-        const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
-        bool hit = false;
-        for (uint32_t t = 0; t < tot && !hit; t++) {
-            const uint32_t key = run_elem(g, dl, r, nb, t);
-            hit = bit_test(bitmap_b, key & g.id_mask) && run_elem_live(dl, r, view, key);
-        }
-        if (hit) out[atomicAdd(out_len, 1u)] = y;
-    }
-}
-
 // Expansion fused with the semi-joins: y (from a frontier vertex's view run, in cand_u) is decided
 // by its first visitor (atomicOr on out.bitmap): it joins out.list when it has a neighbour in every
 // filter's bitmap along the filter's view, else it goes to the fail list and k_clear_failed drops
@@ -234,20 +217,6 @@ __global__ void k_store_words(uint32_t* p, Words w) {
     if (threadIdx.x < w.n) p[threadIdx.x] = w.v[threadIdx.x];
 }
 
-__global__ void k_set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap) {
-    const uint32_t n = min(*len, max_len);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        atomicOr(&bitmap[list[i] >> 5], 1u << (list[i] & 31));
-}
-__global__ void k_clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap) {
-    const uint32_t n = min(*len, max_len);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) bitmap[list[i] >> 5] = 0u;
-}
-__global__ void k_copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len) {
-    const uint32_t n = min(*in_len, max_len);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *out_len = n;
-}
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) p[i] = v;
 }
@@ -287,38 +256,24 @@ void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slo
 
 void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
                   const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, hipStream_t s) {
-    k_local_expand<<<grid_for(max_frontier, 4, 8192), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier,
+    k_local_expand<<<grid_for(max_frontier, 4, 1024), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier,
                                                                      cand_u, out);
 }
 
 void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
                  uint32_t* fail, uint32_t* fail_len, hipStream_t s) {
-    k_local_build<<<grid_for(max_frontier, 4, 8192), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
+    k_local_build<<<grid_for(max_frontier, 4, 1024), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
                                                                     fl, out, fail, fail_len);
     k_clear_failed<<<1, 1024, 0, s>>>(out.bitmap, fail, fail_len, g.dv);
 }
 
 void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s) {
-    if (mask) k_clear_sets<<<grid_for(max_len, 256, 1024), 256, 0, s>>>(t, mask, max_len);
+    if (mask) k_clear_sets<<<grid_for(max_len, 256, 256), 256, 0, s>>>(t, mask, max_len);
 }
 
 void store_words(uint32_t* p, const Words& w, hipStream_t s) { k_store_words<<<1, 64, 0, s>>>(p, w); }
 
-void local_filter(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* in,
-                  const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b, uint32_t* out, uint32_t* out_len, hipStream_t s) {
-    k_local_filter<<<grid_for(max_in, 256, 8192), 256, 0, s>>>(g, phase, view, dir, slot, in, in_len, max_in, bitmap_b, out, out_len);
-}
-
-void set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s) {
-    k_set_bits<<<grid_for(max_len, 256, 4096), 256, 0, s>>>(list, len, max_len, bitmap);
-}
-void clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s) {
-    k_clear_bits<<<grid_for(max_len, 256, 4096), 256, 0, s>>>(list, len, max_len, bitmap);
-}
-void copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len, hipStream_t s) {
-    k_copy_list<<<grid_for(max_len, 256, 4096), 256, 0, s>>>(in, in_len, max_len, out, out_len);
-}
 void copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n, hipStream_t s) {
     k_copy_u32<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, dst, n);
 }

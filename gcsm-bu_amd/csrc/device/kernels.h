@@ -97,9 +97,6 @@ void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir
                   const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
                   const uint32_t* cand_u, VSet out, hipStream_t s);
 // Keeps the vertices y of `in` with a neighbour in L(b) along the view (u -> b).
-void local_filter(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
-                  const uint32_t* in, const uint32_t* in_len, uint32_t max_in, const uint32_t* bitmap_b,
-                  uint32_t* out, uint32_t* out_len, hipStream_t s);
 // Local index of one query vertex u in one launch pair: expand from the frontier along
 // (view, dir, slot), keep the candidates of cand_u that have a neighbour in every filter's bitmap.
 struct LocalFilter {
@@ -124,9 +121,6 @@ struct Words {
     uint32_t v[15];
 };
 void store_words(uint32_t* p, const Words& w, hipStream_t s);
-void set_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s);
-void clear_bits(const uint32_t* list, const uint32_t* len, uint32_t max_len, uint32_t* bitmap, hipStream_t s);
-void copy_list(const uint32_t* in, const uint32_t* in_len, uint32_t max_len, uint32_t* out, uint32_t* out_len, hipStream_t s);
 
 // ---- enumeration ------------------------------------------------------------------------------
 struct MatchOut {

@@ -694,6 +694,13 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
             prof_begin(w.prof, w.s, kKLevelPrepare);
             level_prepare(g_, d, depth, rc, nc, w.level, false, bytes_ptr(kKLevelWrite), w.s);
             prof_end(w.prof, w.s, kKLevelPrepare);
+            if (trace_) {  // candidates of the level (the flat kernel's work), for the trace only
+                CSM_HIP(hipMemcpyAsync(w.h_pinned + 12, w.level.off + nc, 8, hipMemcpyDeviceToHost, w.s));
+                CSM_HIP(hipStreamSynchronize(w.s));
+                uint64_t t;
+                std::memcpy(&t, w.h_pinned + 12, 8);
+                w.trace_line += ", \"cands_" + std::to_string(end) + "\": " + std::to_string(t);
+            }
             prof_begin(w.prof, w.s, kKLevelWrite);
             level_flat(g_, d, depth, rc, nc, kWriteAtEnd, w.level, r, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
             prof_end(w.prof, w.s, kKLevelWrite);
@@ -793,6 +800,13 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
             prof_begin(w.prof, w.s, kKLevelPrepare);
             level_prepare(g_, d, depth, rc, nc, w.level, true, out.bytes, w.s);
             prof_end(w.prof, w.s, kKLevelPrepare);
+            if (trace_) {
+                CSM_HIP(hipMemcpyAsync(w.h_pinned + 12, w.level.off + nc, 8, hipMemcpyDeviceToHost, w.s));
+                CSM_HIP(hipStreamSynchronize(w.s));
+                uint64_t t;
+                std::memcpy(&t, w.h_pinned + 12, 8);
+                w.trace_line += ", \"cands_leaf\": " + std::to_string(t);
+            }
             prof_begin(w.prof, w.s, kKLevelCount);
             level_flat(g_, d, depth, rc, nc, kCountAtEnd, w.level, r, out, w.s);
             prof_end(w.prof, w.s, kKLevelCount);
@@ -818,7 +832,17 @@ std::string Engine::run_phase(uint32_t ph) {
         for (uint32_t i = 0; i < qs.q.m; i++) steps.push_back({static_cast<uint32_t>(qi), i});
     }
     if (steps.empty()) return "";
+    // Longest steps first (host time of the same step in the previous batch): the phase ends when
+    // its last step does, so the heavy ones must not start last.
+    std::vector<double>& cost = step_cost_[ph];
+    auto cost_of = [&](const std::pair<uint32_t, uint32_t>& st) {
+        const size_t k = static_cast<size_t>(st.first) * kMaxQE + st.second;
+        return k < cost.size() ? cost[k] : 0.0;
+    };
+    std::stable_sort(steps.begin(), steps.end(), [&](const auto& x, const auto& y) { return cost_of(x) > cost_of(y); });
+    cost.resize(qs_.size() * kMaxQE, 0.0);
     CSM_HIP(hipEventRecord(phase_start_, s_));
+    const auto phase_t0 = std::chrono::steady_clock::now();
     std::atomic<size_t> next{0};
     std::atomic<bool> failed{false};
     std::vector<std::string> errs(lanes_.size());
@@ -834,12 +858,15 @@ std::string Engine::run_phase(uint32_t ph) {
                 const auto t0 = std::chrono::steady_clock::now();
                 w.trace_line.clear();
                 std::string e = run_step(w, qs, steps[k].second, ph);
+                cost[static_cast<size_t>(steps[k].first) * kMaxQE + steps[k].second] =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 if (trace_) {
                     (void)hipStreamSynchronize(w.s);
                     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                     std::lock_guard<std::mutex> lock(trace_mu_);
-                    std::fprintf(trace_, "{\"query\": %u, \"step\": %u, \"phase\": %u, \"lane\": %zu, \"ms\": %.3f, \"empty\": %d%s}\n",
-                                 steps[k].first, steps[k].second, ph, li, ms, e == "empty" ? 1 : 0, w.trace_line.c_str());
+                    const double at = std::chrono::duration<double, std::milli>(t0 - phase_t0).count();
+                    std::fprintf(trace_, "{\"query\": %u, \"step\": %u, \"phase\": %u, \"lane\": %zu, \"at\": %.3f, \"ms\": %.3f, \"empty\": %d%s}\n",
+                                 steps[k].first, steps[k].second, ph, li, at, ms, e == "empty" ? 1 : 0, w.trace_line.c_str());
                 }
                 if (e == "empty") continue;
                 if (!e.empty()) return e;

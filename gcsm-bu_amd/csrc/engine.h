@@ -219,6 +219,7 @@ class Engine {
     std::vector<NlfEntry> h_entries_;
     std::vector<NlfGroup> h_groups_;
     std::vector<Lane> lanes_;
+    std::vector<double> step_cost_[2];        // per phase, per (query, edge): host seconds in the last batch
     uint64_t ring_words_ = 0;                 // words of each lane's ring
     hipEvent_t phase_start_ = nullptr;
     unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash

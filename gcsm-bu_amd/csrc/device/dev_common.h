@@ -78,6 +78,10 @@ struct StepDesc {
     uint8_t boff[kMaxQV + 1];         // backward entries of depth d: be[boff[d] .. boff[d+1])
     uint8_t pivot[kMaxQV];            // entry whose list generates the candidates of depth d
     uint8_t pad0[3];
+    // Matched positions that a candidate of depth d must be compared with for injectivity: the
+    // ones of the same label rank (candidates carry their query vertex's label) that are not
+    // backward neighbours of d (a neighbour of f[q] is never f[q]: data graphs have no self-loops).
+    uint16_t inj[kMaxQV];
     BackEntry be[kMaxQE];
     const uint32_t* filter[kMaxQV];   // candidate bitmap of the vertex at depth d (local or global)
 };

@@ -66,8 +66,8 @@ __device__ __forceinline__ void setup_frame(const DevGraph& g, const StepDesc& d
 __device__ __forceinline__ bool test_candidate(const DevGraph& g, const StepDesc& d, const WaveHdr& W, uint32_t depth, uint32_t key) {
     const uint32_t c = key & g.id_mask;
     if (!bit_test(d.filter[depth], c)) return false;
-    for (uint32_t j = 0; j < depth; j++)
-        if (W.f[j] == c) return false;
+    for (uint32_t im = d.inj[depth]; im; im &= im - 1)
+        if (W.f[__ffs(im) - 1] == c) return false;
     const uint32_t p = d.pivot[depth];
     const BackEntry pe = d.be[p];
     if (!run_elem_live(entry_delta(g, d, pe), W.run[p], pe.view, key)) return false;
@@ -423,7 +423,8 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
 #pragma unroll
             for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && run_elem_live(pdl, rv[j], pe.view, key[j]);
         }
-        for (uint32_t q = 0; q < d0; q++) {
+        for (uint32_t im = d.inj[d0]; im; im &= im - 1) {
+            const uint32_t q = static_cast<uint32_t>(__ffs(im)) - 1;
 #pragma unroll
             for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && rec[static_cast<uint64_t>(k[j]) * d0 + q] != c[j];
         }
@@ -609,7 +610,7 @@ __global__ void __launch_bounds__(256) k_cp_filter(DevGraph g, StepDesc d, uint3
             const uint32_t x = key & g.id_mask;
             bool ok = bit_test(d.filter[d0 + kk], x) && run_elem_live(dl, rv, be.view, key);
             const uint32_t* f = rec + static_cast<uint64_t>(r) * d0;
-            for (uint32_t j = 0; j < d0 && ok; j++) ok = f[j] != x;
+            for (uint32_t im = d.inj[d0 + kk] & ((1u << d0) - 1u); im && ok; im &= im - 1) ok = f[__ffs(im) - 1] != x;
             if (ok) c.lists[base + atomicAdd(&c.seg_cnt[i], 1u)] = x;
         }
         rb = uniform(__shfl(i, 63, 64));
@@ -667,7 +668,7 @@ __global__ void __launch_bounds__(256) k_cp_enumerate(StepDesc d, uint32_t d0, u
             const uint32_t idx = static_cast<uint32_t>(local % sz);
             local /= sz;
             x[k] = c.lists[c.seg_off[i] + idx];
-            for (uint32_t j = 0; j < k; j++) ok = ok && x[j] != x[k];
+            for (uint32_t j = 0; j < k; j++) ok = ok && (!(d.inj[d0 + k] >> (d0 + j) & 1u) || x[j] != x[k]);
             h += pair_hash(d.order[d0 + k], x[k]);
         }
         if (ok) { cnt += 1; hsum += embedding_hash(h); }

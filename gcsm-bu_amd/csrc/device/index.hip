@@ -151,13 +151,14 @@ __global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase
     }
 }
 
-// Expansion fused with the semi-joins: y (from a frontier vertex's view run, in cand_u) is decided
-// by its first visitor (atomicOr on out.bitmap): it joins out.list when it has a neighbour in every
-// filter's bitmap along the filter's view, else it goes to the fail list and k_clear_failed drops
-// its bit afterwards.  Later visitors of y see the bit and skip it.
-__global__ void __launch_bounds__(256) k_local_build(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+// Expansion with the semi-joins in a second pass: k_local_queue expands the frontier along
+// (view, dir, slot) into cand_u and queues every new candidate y once (its first visitor sets its
+// bit in out.bitmap); k_local_check gives each queued y one wave, whose lanes scan y's run of every
+// filter relation 64 keys at a time for a member of the filter's bitmap.  Survivors join out.list;
+// the bits of the others are cleared.
+__global__ void __launch_bounds__(256) k_local_queue(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                                                      const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
-                                                     const uint32_t* cand_u, LocalFilters fl, VSet out, uint32_t* fail, uint32_t* fail_len) {
+                                                     const uint32_t* cand_u, uint32_t* bitmap, uint32_t* queue, uint32_t* queue_len) {
     const uint32_t lane = lane_id();
     const uint32_t wpb = blockDim.x >> 6;
     const uint32_t F = min(*frontier_len, max_frontier);
@@ -171,36 +172,46 @@ __global__ void __launch_bounds__(256) k_local_build(DevGraph g, uint32_t phase,
             const uint32_t y = key & g.id_mask;
             if (!bit_test(cand_u, y) || !run_elem_live(dl, r, view, key)) continue;
             const uint32_t bit = 1u << (y & 31);
-            if (atomicOr(&out.bitmap[y >> 5], bit) & bit) continue;  // decided by another visitor
-            bool pass = true;
-            for (uint32_t k = 0; k < fl.n && pass; k++) {
-                const LocalFilter f = fl.f[k];
-                const uint32_t* fdl = phase_delta(g, f.dir, phase);
-                const RunView fr = make_run(g, phase, f.view, f.dir, y, f.slot);
-                const uint32_t fnb = run_base_len(fr, f.view), ftot = run_seq_len(fr, f.view);
-                bool hit = false;
-                for (uint32_t u = 0; u < ftot && !hit; u++) {
-                    const uint32_t fk = run_elem(g, fdl, fr, fnb, u);
-                    hit = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
-                }
-                pass = hit;
-            }
-            if (pass) {
-                const uint32_t p = atomicAdd(out.len, 1u);
-                if (p < out.cap) out.list[p] = y;
-            } else {
-                fail[atomicAdd(fail_len, 1u)] = y;
-            }
+            if (atomicOr(&bitmap[y >> 5], bit) & bit) continue;  // queued by another visitor
+            queue[atomicAdd(queue_len, 1u)] = y;
         }
     }
 }
 
-// One block: drops the bits of the failed candidates, then resets the fail list.
-__global__ void __launch_bounds__(1024) k_clear_failed(uint32_t* bitmap, const uint32_t* fail, uint32_t* fail_len, uint32_t max_len) {
-    const uint32_t n = min(*fail_len, max_len);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAnd(&bitmap[fail[i] >> 5], ~(1u << (fail[i] & 31)));
-    __syncthreads();
-    if (threadIdx.x == 0) *fail_len = 0;
+__global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase, const uint32_t* queue, const uint32_t* queue_len,
+                                                     uint32_t max_len, LocalFilters fl, VSet out) {
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t n = min(*queue_len, max_len);
+    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb) {
+        const uint32_t y = queue[i];
+        bool pass = true;
+        for (uint32_t k = 0; k < fl.n && pass; k++) {
+            const LocalFilter f = fl.f[k];
+            const uint32_t* fdl = phase_delta(g, f.dir, phase);
+            const RunView fr = make_run(g, phase, f.view, f.dir, y, f.slot);
+            const uint32_t fnb = run_base_len(fr, f.view), ftot = run_seq_len(fr, f.view);
+            bool hit = false;
+            for (uint32_t t0 = 0; t0 < ftot && !hit; t0 += 64) {
+                const uint32_t t = t0 + lane;
+                bool h = false;
+                if (t < ftot) {
+                    const uint32_t fk = run_elem(g, fdl, fr, fnb, t);
+                    h = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
+                }
+                hit = __ballot(h) != 0;
+            }
+            pass = hit;
+        }
+        if (lane == 0) {
+            if (pass) {
+                const uint32_t p = atomicAdd(out.len, 1u);
+                if (p < out.cap) out.list[p] = y;
+            } else {
+                atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
+            }
+        }
+    }
 }
 
 // Clears the bitmaps of every set in `mask` (whole words of its listed members).
@@ -262,10 +273,10 @@ void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir
 
 void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
-                 uint32_t* fail, uint32_t* fail_len, hipStream_t s) {
-    k_local_build<<<grid_for(max_frontier, 4, 1024), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
-                                                                    fl, out, fail, fail_len);
-    k_clear_failed<<<1, 1024, 0, s>>>(out.bitmap, fail, fail_len, g.dv);
+                 uint32_t* queue, uint32_t* queue_len, hipStream_t s) {
+    k_local_queue<<<grid_for(max_frontier, 4, 1024), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
+                                                                    out.bitmap, queue, queue_len);
+    k_local_check<<<grid_for(max_frontier, 4, 1024), 256, 0, s>>>(g, phase, queue, queue_len, g.dv, fl, out);
 }
 
 void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s) {

@@ -97,7 +97,7 @@ void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir
                   const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
                   const uint32_t* cand_u, VSet out, hipStream_t s);
 // Keeps the vertices y of `in` with a neighbour in L(b) along the view (u -> b).
-// Local index of one query vertex u in one launch pair: expand from the frontier along
+// Local index of one query vertex u in two launches: expand from the frontier along
 // (view, dir, slot), keep the candidates of cand_u that have a neighbour in every filter's bitmap.
 struct LocalFilter {
     uint32_t view, dir, slot;
@@ -107,9 +107,10 @@ struct LocalFilters {
     uint32_t n;
     LocalFilter f[kMaxQV - 1];
 };
+// queue (capacity dv) and *queue_len (zero on entry) hold the new candidates between the passes.
 void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
-                 uint32_t* fail, uint32_t* fail_len, hipStream_t s);
+                 uint32_t* queue, uint32_t* queue_len, hipStream_t s);
 struct VSetTable {
     VSet s[kMaxQV];
 };

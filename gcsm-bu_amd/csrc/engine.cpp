@@ -375,7 +375,7 @@ std::string Engine::build() {
         for (uint32_t u = 0; u < kMaxQV; u++) w.L[u].len = lens + u;
         w.tmp_len = lens + kMaxQV;
         w.n_pairs = lens + kMaxQV + 1;
-        w.counters = lens + 32;
+        w.counters = lens + kStepCounters;
         w.tmp_list = pool_.take<uint32_t>(dv_);
         w.pairs = pool_.take<uint32_t>(2 * max_arcs);
         w.ring = pool_.take<uint32_t>(ring_words_);
@@ -572,7 +572,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
                     fl.f[fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[bsv[k]].bitmap};
                 }
                 local_build(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), fl,
-                            w.L[u], w.tmp_list, w.tmp_len, w.s);
+                            w.L[u], w.tmp_list, w.L[0].len + kQueueCounters + u, w.s);
             }
             w.L_used |= static_cast<uint16_t>(1u << u);
             prof_end(w.prof, w.s, kKLocalIndex);
@@ -612,6 +612,12 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
         }
     }
     d.boff[q.n] = static_cast<uint8_t>(ne);
+    for (uint32_t k = 0; k < q.n; k++) {
+        uint32_t back = 0;
+        for (uint32_t e = d.boff[k]; e < d.boff[k + 1]; e++) back |= 1u << d.be[e].pos;
+        for (uint32_t j = 0; j < k; j++)
+            if (d.vrank[j] == d.vrank[k] && !(back >> j & 1u)) d.inj[k] |= static_cast<uint16_t>(1u << j);
+    }
 
     // ---- initial partial results (depth 2) at the bottom of the lane's ring ----
     uint32_t* ring = w.ring;

@@ -48,7 +48,11 @@ struct EngineOptions {
 constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
 constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
 constexpr uint32_t kMaxLanes = 16;
-constexpr uint32_t kLaneWords = 48;  // per lane: 16 set lengths, list counters, then 16 step counters at +32
+// Per lane, zeroed at every step: 16 set lengths, list counters (+16, +17), one local-index queue
+// counter per query vertex (+kQueueCounters), then the step counters (+kStepCounters).
+constexpr uint32_t kQueueCounters = 18;
+constexpr uint32_t kStepCounters = 48;
+constexpr uint32_t kLaneWords = 64;
 
 struct QueryResult {
     uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;

@@ -66,8 +66,8 @@ def workload(config, seed, data_dir):
     return meta, queries
 
 
-def open_session(ffi, data_dir, queries, batch):
-    s = ffi.Session(device=0)
+def open_session(ffi, data_dir, queries, batch, lanes=0):
+    s = ffi.Session(device=0, lanes=lanes)
     s.load_graph(os.path.join(data_dir, "data.graph"))
     for q in queries:
         s.add_query(q)
@@ -76,6 +76,44 @@ def open_session(ffi, data_dir, queries, batch):
     for q in s.queries:
         s.plan(q)
     return s, nb
+
+
+def traffic_of(kernel, config):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (lines "traffic <kernel> <bytes per dispatch>"), or None."""
+    path = os.path.join(ROOT, "profiles", f"{config}_kernel_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
+def roofline_pass(ffi, data_dir, queries, batch, config):
+    """The dominant enumeration kernel timed on one lane (one stream, no other stream's packets
+    between its HIP events), warm-up batch 0, then batches 1-2: algorithmic bytes / event time."""
+    mask = sum(1 << FAMILIES.index(k) for k in BYTE_FAMILIES)
+    s, nb = open_session(ffi, data_dir, queries, batch, lanes=1)
+    s.profile_enable(mask)
+    s.apply_batch(0)
+    s.profile_enable(mask)  # counters from here on
+    for b in range(1, min(3, nb)):
+        s.apply_batch(b)
+    s.synchronize()
+    stats = {k: s.profile_read(FAMILIES.index(k)) for k in BYTE_FAMILIES}
+    s.close()
+    dom = max(stats, key=lambda k: stats[k][1])
+    launches, ms, nbytes = stats[dom]
+    achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    traffic = traffic_of(KERNEL_OF[dom], config)
+    return {"bound": "hbm", "kernel": KERNEL_OF[dom], "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+            "avg_launch_us": round(ms * 1e3 / launches, 3) if launches else None, "launches": launches,
+            "algorithmic_bytes_per_launch": nbytes // launches if launches else 0,
+            "measured": "one-lane session, batches 1-2 after warm-up batch 0, HIP events on the lane's stream",
+            "traffic_source": f"profiles/{config}_kernel_traffic.md" if traffic is not None else None}
 
 
 def cpu_baseline(data_dir, queries, batch, seconds):
@@ -101,16 +139,20 @@ def cpu_baseline(data_dir, queries, batch, seconds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--data-dir", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-mask", type=int, default=0x7FF)
+    ap.add_argument("--profile-mask", type=int, default=0,
+                    help="kernel families to time with HIP events during the timed batches (off by default: "
+                         "the roofline comes from a separate one-lane pass)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of 2^26 words (HBM traffic counter calibration)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the one-lane roofline pass (what the committed kernel profiles cover)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU: the matching path is single-GPU "
@@ -121,6 +163,14 @@ def main():
     data_dir = args.data_dir or os.path.join("/tmp", f"csm_bench_{args.config}_s{args.seed}")
     meta, queries = workload(args.config, args.seed, data_dir)
     batch = int(meta["batch_size"])
+    if args.roofline_only:
+        r = roofline_pass(ffi, data_dir, queries, batch, args.config)
+        if args.calibrate:
+            s, _ = open_session(ffi, data_dir, queries[:1], batch, lanes=1)
+            r["calibration_copy_ms"] = s.calibration_copy(1 << 26)
+            s.close()
+        print(json.dumps({"roofline": r}))
+        return 0
 
     t = time.time()
     s, nb = open_session(ffi, data_dir, queries, batch)
@@ -158,25 +208,15 @@ def main():
         log(f"step {k}: {dt * 1e3:.1f} ms, {sum(r[0] + r[1] for r in res)} matches, {info.edges} edges")
     total = sum(lat)
     stats = {FAMILIES[f]: s.profile_read(f) for f in range(len(FAMILIES)) if args.profile_mask >> f & 1}
-    # dominant kernel: the byte-counting family with the most kernel time
-    cands = [k for k in BYTE_FAMILIES if k in stats and stats[k][1] > 0]
-    dom = max(cands, key=lambda k: stats[k][1]) if cands else None
-    roofline = None
-    if dom:
-        launches, ms, nbytes = stats[dom]
-        achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
-        roofline = {"bound": "hbm", "kernel": KERNEL_OF[dom], "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                    "avg_launch_us": round(ms * 1e3 / launches, 3) if launches else None, "launches": launches,
-                    "algorithmic_bytes": nbytes}
     calib = None
     if args.calibrate:
         words = 1 << 26
         ms = s.calibration_copy(words)
         calib = {"kernel": "k_copy_u32", "bytes_read": 4 * words, "bytes_written": 4 * words, "ms": ms,
                  "GB/s": round(8 * words / (ms / 1e3) / 1e9, 1)}
-    cpu = None if args.no_cpu_baseline else cpu_baseline(data_dir, queries, batch, args.cpu_seconds)
     s.close()
+    roofline = roofline_pass(ffi, data_dir, queries, batch, args.config)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(data_dir, queries, batch, args.cpu_seconds)
     lat_ms = sorted(x * 1e3 for x in lat)
     p50 = statistics.median(lat_ms)
     p99 = lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))]
@@ -185,12 +225,14 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": total * 1e3 / max(1, args.steps),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": WORKLOADS[args.config], "config": args.config, "seed": args.seed, "batch_edges": batch,
-                   "queries": len(queries), "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP)"},
+                   "queries": len(queries), "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP)",
+                   "lanes": int(os.environ.get("CSM_LANES", "8"))},
         "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": p50, "p99_batch_ms": p99,
         "matches": matches, "per_query_matches": per_query,
-        "kernel_ms": {k: round(v[1], 3) for k, v in stats.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if stats:
+        out["kernel_ms"] = {k: round(v[1], 3) for k, v in stats.items()}
     if calib:
         out["calibration_copy"] = calib
     print(json.dumps(out))

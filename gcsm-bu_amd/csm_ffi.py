@@ -26,7 +26,7 @@ STATUS = {0: "OK", 1: "ARGUMENT", 2: "IO", 3: "FORMAT", 4: "LIMIT", 5: "DEVICE",
 class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("directed", ctypes.c_int32), ("pool_bytes", ctypes.c_uint64),
                 ("ring_bytes", ctypes.c_uint64), ("cp_list_bytes", ctypes.c_uint64),
-                ("max_batch_edges", ctypes.c_uint32), ("semi_bfs_threshold", ctypes.c_uint32)]
+                ("max_batch_edges", ctypes.c_uint32), ("semi_bfs_threshold", ctypes.c_uint32), ("lanes", ctypes.c_uint32)]
 
 
 class Technique(ctypes.Structure):
@@ -102,10 +102,10 @@ class Session:
     """One device session: graph, standing queries, update stream."""
 
     def __init__(self, device=0, directed=False, pool_bytes=0, ring_bytes=0, cp_list_bytes=0, max_batch_edges=0,
-                 semi_bfs_threshold=0):
+                 semi_bfs_threshold=0, lanes=0):
         self.lib = load()
         self.h = ctypes.c_void_p()
-        o = Options(device, int(directed), pool_bytes, ring_bytes, cp_list_bytes, max_batch_edges, semi_bfs_threshold)
+        o = Options(device, int(directed), pool_bytes, ring_bytes, cp_list_bytes, max_batch_edges, semi_bfs_threshold, lanes)
         st = self.lib.csm_session_create(ctypes.byref(o), ctypes.byref(self.h))
         if st != 0:
             msg = self.lib.csm_last_error(self.h).decode() if self.h else "session create failed"

@@ -64,6 +64,7 @@ csm_status csm_session_create(const csm_options* options, csm_session** out) {
         if (options->cp_list_bytes) o.cp_list_bytes = options->cp_list_bytes;
         o.max_batch_edges = options->max_batch_edges;
         if (options->semi_bfs_threshold) o.semi_bfs_threshold = options->semi_bfs_threshold;
+        if (options->lanes) o.lanes = options->lanes;
     }
     s->opt = o;
     std::string err = s->engine.init(o);

@@ -5,7 +5,7 @@
 // "--flag value", first occurrence wins, booleans "true"/"True"/"false"/"False".  Output lines keep the
 // reference's formats: "<Phase>, time (ms): X(host), Y(kernel)" (cuda_helpers.h:98-102), the
 // "Orders:" block (plan.cpp:1200-1291) and "Num Positive Matches: N" (technique_permutation.h:224).
-// Extensions: --queries a,b,c (several standing queries), --directed, --pool_mb, --ring_mb,
+// Extensions: --queries a,b,c (several standing queries), --directed, --pool_mb, --ring_mb, --lanes,
 // --json <path> (one JSON record per batch and query).
 #include <algorithm>
 #include <chrono>
@@ -134,6 +134,7 @@ int main(int argc, char** argv) {
     o.directed = p.get_bool("--directed", false) ? 1 : 0;
     o.pool_bytes = static_cast<uint64_t>(p.get_u32("--pool_mb", 0)) << 20;
     o.ring_bytes = static_cast<uint64_t>(p.get_u32("--ring_mb", 0)) << 20;
+    o.lanes = p.get_u32("--lanes", 0);
     csm_session* s = nullptr;
     csm_status st = csm_session_create(&o, &s);
     if (st != CSM_OK) return die(s, st);

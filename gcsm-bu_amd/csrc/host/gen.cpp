@@ -453,6 +453,7 @@ int gen_main(int argc, char** argv) {
         sp.m = std::max<uint64_t>(256, static_cast<uint64_t>(sp.m * scale));
         sp.dmax = std::max(16.0, sp.dmax * std::sqrt(scale));
         if (sp.batch != 0xFFFFFFFFu) sp.batch = std::max<uint32_t>(64, static_cast<uint32_t>(sp.batch * scale));
+        sp.emb_cap *= std::sqrt(scale);  // scaled-down graphs get lighter queries
     }
     mkdir(out.c_str(), 0755);
     Rng rng(seed * 0x100000001B3ull + 7);

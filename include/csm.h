@@ -39,6 +39,8 @@ typedef struct {
     uint64_t cp_list_bytes;  /* Cartesian-product leaf lists (config.h:43 SIZE_SPACE), 0 = 1 GiB */
     uint32_t max_batch_edges;/* largest batch the scratch is sized for, 0 = the stream's batch */
     uint32_t semi_bfs_threshold; /* config.h:47 MIN_NUM_RESULTS_TO_GPU, 0 = 65536 */
+    uint32_t lanes;          /* extension: concurrent step lanes (streams + host threads, each with its own
+                                ring and scratch), 0 = 8, at most 16 */
 } csm_options;
 
 /* TechniqueOption (utils/technique_option.h:37-56); values as the csmg flags. */

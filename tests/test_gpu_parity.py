@@ -200,15 +200,15 @@ def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir,
 
 def test_c2_shape_scaled_stream(lib, tmp_path):
     """The benchmark's workload family (Chung-Lu, Zipf labels, 90/10 insert/delete stream, six
-    sparse and dense queries) at 5% scale: every batch bit-exact for the default method and the
-    QO-GAMMA switches."""
+    sparse and dense queries) at 5% scale: every batch bit-exact for the default method, the
+    QO-GAMMA switches and the DFS (static schedule) and BFS strategies."""
     meta, queries = util.gen("c2", tmp_path, seed=5, scale=0.05)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
     batch = int(meta["batch_size"])
     want, _ = util.oracle_stream(data, upd, batch, queries, threads=16)
     assert sum(v[0] + v[1] for v in want.values()) > 10 ** 6
     assert any(v[1] for v in want.values())  # deletions destroy matches too
-    for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True)]:
+    for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 1, True, True), (2, 3, True, True)]:
         say(f"case c2 scale=0.05 seed=5 order={mo} strategy={ss} local={li} cp={cp}")
         assert run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp)) == want
 
@@ -240,3 +240,26 @@ def test_small_ring_and_record_chunks(lib, tmp_path, monkeypatch):
             say(f"case c2 scale=0.02 seed=6 ring={ring} cp_max_records=1024 order={mo} strategy={ss} cp={cp}")
             got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp), ring_bytes=ring)
             assert got == want
+
+
+def test_c4_sixty_four_standing_queries(lib, tmp_path):
+    """64 standing queries (trees, sparse and dense cycles of 4-12 vertices) on one session, all
+    steps of a phase shared by the concurrent lanes: each query's counts and hashes equal the
+    oracle's for that query alone (first two batches of the 1%-scale workload)."""
+    meta, queries = util.gen("c4", tmp_path, seed=2, scale=0.01)
+    assert len(queries) == 64
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = int(meta["batch_size"])
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=16, extra=["--max-batches", "2"])
+    say("case c4 scale=0.01 seed=2 batches 0-1")
+    with lib.Session(device=0) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        s.open_updates(str(upd), batch)
+        s.build()
+        for q in qids:
+            s.plan(q)
+        for b in range(2):
+            res, _ = s.apply_batch(b)
+            for k, r in enumerate(res):
+                assert r == want[(b, k)], f"batch {b} query {k}"

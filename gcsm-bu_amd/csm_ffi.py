@@ -14,7 +14,8 @@ BIN_DIR = os.path.join(HERE, "bin")
 EXPORTED = [
     "csm_session_create", "csm_session_destroy", "csm_last_error", "csm_load_graph_file",
     "csm_load_query_file", "csm_open_update_file", "csm_build_index", "csm_plan",
-    "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_get_relation_edge_counts",
+    "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_apply_range", "csm_apply_edges", "csm_arena_usage",
+    "csm_get_relation_edge_counts",
     "csm_get_counts", "csm_pool_usage", "csm_read_candidates", "csm_read_adjacency",
     "csm_synchronize", "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
 ]
@@ -75,6 +76,9 @@ def load():
         lib.csm_get_plan_text.argtypes = [vp, u32, ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_size_t)]
         lib.csm_get_summary.argtypes = [vp, u32, P(u32), P(ctypes.c_float), u32, P(u32)]
         lib.csm_apply_batch.argtypes = [vp, u64, P(QueryResult), u32, P(BatchInfo)]
+        lib.csm_apply_range.argtypes = [vp, u64, u64, P(QueryResult), u32, P(BatchInfo)]
+        lib.csm_apply_edges.argtypes = [vp, P(u32), P(u32), P(u32), P(ctypes.c_uint8), u64, P(QueryResult), u32, P(BatchInfo)]
+        lib.csm_arena_usage.argtypes = [vp, P(u64), P(u64)]
         lib.csm_get_relation_edge_counts.argtypes = [vp, u32, u64, P(u64), u32, P(u32)]
         lib.csm_get_counts.argtypes = [vp, P(u32), P(u32)]
         lib.csm_pool_usage.argtypes = [vp, P(u64), P(u64)]
@@ -173,6 +177,29 @@ class Session:
         info = BatchInfo()
         self._check(self.lib.csm_apply_batch(self.h, b, res, nq, ctypes.byref(info)))
         return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
+
+    def apply_range(self, first, n):
+        nq = len(self.queries)
+        res = (QueryResult * max(1, nq))()
+        info = BatchInfo()
+        self._check(self.lib.csm_apply_range(self.h, first, n, res, nq, ctypes.byref(info)))
+        return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
+
+    def apply_edges(self, src, dst, elabel, op):
+        n = len(src)
+        nq = len(self.queries)
+        res = (QueryResult * max(1, nq))()
+        info = BatchInfo()
+        A = ctypes.c_uint32 * max(1, n)
+        st = self.lib.csm_apply_edges(self.h, A(*src), A(*dst), A(*elabel), (ctypes.c_uint8 * max(1, n))(*op), n, res, nq,
+                                      ctypes.byref(info))
+        self._check(st)
+        return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
+
+    def arena_usage(self):
+        u, c = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.lib.csm_arena_usage(self.h, ctypes.byref(u), ctypes.byref(c)))
+        return u.value, c.value
 
     def counts(self):
         nv, nq = ctypes.c_uint32(), ctypes.c_uint32()

@@ -27,7 +27,9 @@ csm_status fail(csm_session* s, csm_status st, const std::string& msg) {
 csm_status classify(csm_session* s, const std::string& err) {
     if (err.empty()) return CSM_OK;
     csm_status st = CSM_ERR_DEVICE;
-    if (err.rfind("Failed to open", 0) == 0) st = CSM_ERR_IO;
+    if (err.rfind("batch edge", 0) == 0) st = CSM_ERR_ARGUMENT;
+    else if (err.rfind("batch larger", 0) == 0) st = CSM_ERR_LIMIT;
+    else if (err.rfind("Failed to open", 0) == 0) st = CSM_ERR_IO;
     else if (err.find("malformed") != std::string::npos || err.find("vertex update") != std::string::npos ||
              err.find("undeclared") != std::string::npos || err.find("declared twice") != std::string::npos ||
              err.find("duplicate") != std::string::npos || err.find("self-loop") != std::string::npos ||
@@ -186,6 +188,57 @@ csm_status csm_apply_batch(csm_session* s, uint64_t batch_index, csm_query_resul
         info->ms_index = bi.ms_index;
         info->ms_match = bi.ms_match;
     }
+    return CSM_OK;
+}
+
+namespace {
+csm_status copy_results(const std::vector<QueryResult>& out, const BatchInfo& bi, csm_query_result* results, uint32_t num_queries,
+                        csm_batch_info* info) {
+    for (uint32_t q = 0; q < num_queries && q < out.size(); q++) {
+        results[q].positive = out[q].pos;
+        results[q].negative = out[q].neg;
+        results[q].positive_hash = out[q].pos_hash;
+        results[q].negative_hash = out[q].neg_hash;
+    }
+    if (info) {
+        info->edges = bi.edges;
+        info->inserts = bi.inserts;
+        info->deletes = bi.deletes;
+        info->ms_total = bi.ms_total;
+        info->ms_index = bi.ms_index;
+        info->ms_match = bi.ms_match;
+    }
+    return CSM_OK;
+}
+}  // namespace
+
+csm_status csm_apply_range(csm_session* s, uint64_t first, uint64_t n, csm_query_result* results, uint32_t num_queries,
+                           csm_batch_info* info) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (num_queries && !results) return fail(s, CSM_ERR_ARGUMENT, "results array missing");
+    std::vector<QueryResult> out;
+    BatchInfo bi;
+    std::string err = s->engine.run_batch(first, n, out, bi);
+    if (!err.empty()) return classify(s, err);
+    return copy_results(out, bi, results, num_queries, info);
+}
+
+csm_status csm_apply_edges(csm_session* s, const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op,
+                           uint64_t n, csm_query_result* results, uint32_t num_queries, csm_batch_info* info) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (n && (!src || !dst || !elabel || !op)) return fail(s, CSM_ERR_ARGUMENT, "edge arrays missing");
+    if (num_queries && !results) return fail(s, CSM_ERR_ARGUMENT, "results array missing");
+    std::vector<QueryResult> out;
+    BatchInfo bi;
+    std::string err = s->engine.run_edges(src, dst, elabel, op, n, out, bi);
+    if (!err.empty()) return classify(s, err);
+    return copy_results(out, bi, results, num_queries, info);
+}
+
+csm_status csm_arena_usage(csm_session* s, uint64_t* used_words, uint64_t* capacity_words) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (used_words) *used_words = s->engine.arena_used_words();
+    if (capacity_words) *capacity_words = s->engine.arena_words();
     return CSM_OK;
 }
 

@@ -79,6 +79,7 @@ std::string Engine::init(const EngineOptions& o) {
     // A/B switches of the scheduling and layout aids (all on by default)
     if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) opt_.split_roots = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_SLOT_DIRECTORY")) opt_.slot_directory = std::atoi(t) != 0;
+    if (const char* t = std::getenv("CSM_ARENA_SLACK_WORDS")) opt_.arena_slack_words = std::strtoull(t, nullptr, 10);
     if (const char* t = std::getenv("CSM_CP_MAX_RECORDS")) opt_.cp_max_records = static_cast<uint32_t>(std::max(1024L, std::atol(t)));
     return "";
 }
@@ -189,7 +190,10 @@ std::string Engine::build() {
             total_cap += cap_for(deg[d][v]);
         }
     uint64_t stream_arcs = 2ull * stream_.size();
-    const uint64_t arena_words = 3 * total_cap + 8 * stream_arcs + (1u << 20);
+    // growth room: three times the initial lists, eight words per streamed arc, and the same for
+    // in-memory batches when the caller sized the scratch for them (max_batch_edges)
+    const uint64_t live_arcs = opt_.max_batch_edges ? 2ull * opt_.max_batch_edges * kLiveBatchesOfRoom : 0;
+    const uint64_t arena_words = 3 * total_cap + 8 * (stream_arcs + live_arcs) + opt_.arena_slack_words;
     if (arena_words >= (1ull << 32)) return "adjacency arena exceeds 2^32 words";
 
     // ---- sizes of everything else, then one reservation ----
@@ -250,6 +254,14 @@ std::string Engine::build() {
         std::vector<uint32_t> vr(dv_);
         for (uint32_t v = 0; v < dv_; v++) vr[v] = host_g_.vlabel[v] == kNoVertex ? kNoVertex : rank_of(vlabel_rank_, host_g_.vlabel[v]);
         CSM_HIP(hipMemcpyAsync(d_vrank, vr.data(), dv_ * 4ull, hipMemcpyHostToDevice, s_));
+        for (int d = 0; d < 2; d++) { h_deg_[d].clear(); h_cap_[d].clear(); }
+        for (int d = 0; d < ndirs_; d++) {
+            h_deg_[d] = deg[d];
+            h_cap_[d].resize(dv_);
+            for (uint32_t v = 0; v < dv_; v++) h_cap_[d][v] = cap_for(deg[d][v]);
+        }
+        h_arena_top_ = total_cap;
+        h_arena_words_ = arena_words;
         std::vector<uint32_t> arena(total_cap, 0u);
         std::vector<std::vector<uint64_t>> pos = off;
         auto key = [&](uint32_t nbr, uint32_t el) {
@@ -900,11 +912,87 @@ std::string Engine::run_phase(uint32_t ph) {
 std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info) {
     if (!built_) return "index not built";
     if (lo + n > stream_.size()) return "batch range beyond the update stream";
+    return run_resident(lo, n, stream_.src.data() + lo, stream_.dst.data() + lo, stream_.op.data() + lo, out, info);
+}
+
+// Arena admission: the words the batch's relocated lists will take, replaying the device rule
+// (deletions first; a list whose new length exceeds its capacity moves to a fresh block of
+// max(8, 2 * next_pow2(length))) on the host mirrors of degrees and capacities.
+std::string Engine::admit(const uint32_t* src, const uint32_t* dst, const uint8_t* op, uint64_t n, std::vector<MirrorUpdate>& upd) {
+    upd.clear();
+    for (int d = 0; d < ndirs_; d++)
+        if (h_ins_[d].size() != dv_) { h_ins_[d].assign(dv_, 0); h_del_[d].assign(dv_, 0); }
+    std::vector<std::pair<int, uint32_t>> touched;
+    touched.reserve(2 * n);
+    auto bump = [&](int d, uint32_t v, bool del) {
+        if (h_ins_[d][v] == 0 && h_del_[d][v] == 0) touched.push_back({d, v});
+        (del ? h_del_[d][v] : h_ins_[d][v]) += 1;
+    };
+    for (uint64_t k = 0; k < n; k++) {
+        bump(0, src[k], op[k] != 0);
+        bump(opt_.directed ? 1 : 0, dst[k], op[k] != 0);
+    }
+    uint64_t need = 0;
+    std::string err;
+    for (const auto& t : touched) {
+        const int d = t.first;
+        const uint32_t v = t.second;
+        const uint32_t dels = h_del_[d][v], adds = h_ins_[d][v];
+        h_del_[d][v] = h_ins_[d][v] = 0;
+        if (dels > h_deg_[d][v]) { err = "batch precondition violated: deleted edge missing"; continue; }
+        const uint64_t nd = static_cast<uint64_t>(h_deg_[d][v] - dels) + adds;
+        uint32_t cap = h_cap_[d][v];
+        if (adds && nd > cap) {
+            uint64_t p2 = 1;
+            while (p2 < nd) p2 <<= 1;
+            cap = static_cast<uint32_t>(std::max<uint64_t>(kMinNbrSize, 2 * p2));
+            need += cap;
+        }
+        upd.push_back(MirrorUpdate{d, v, static_cast<uint32_t>(nd), cap});
+    }
+    if (!err.empty()) return err;
+    if (h_arena_top_ + need > h_arena_words_)
+        return "adjacency arena budget exhausted: the batch needs " + std::to_string(need) + " more words of list space";
+    h_pending_need_ = need;
+    return "";
+}
+
+std::string Engine::run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
+                              std::vector<QueryResult>& out, BatchInfo& info) {
+    if (!built_) return "index not built";
+    if (n > max_batch_) return "batch larger than the scratch was sized for (max_batch_edges)";
+    std::vector<uint32_t> kf(n), kb(n);
+    for (uint64_t k = 0; k < n; k++) {
+        const uint32_t a = src[k], b = dst[k];
+        if (a >= dv_ || b >= dv_) return "batch edge endpoint out of range";
+        if (a == b) return "self-loops are not supported";
+        if (host_g_.vlabel[a] == kNoVertex || host_g_.vlabel[b] == kNoVertex) return "batch edge endpoint was never declared";
+        if (rank_of(elabel_rank_, elabel[k]) == kNoVertex) return "batch edge label unknown to the session";
+        kf[k] = key_of_edge(b, elabel[k]);
+        kb[k] = key_of_edge(a, elabel[k]);
+    }
+    if (n) {  // the scratch region after the resident stream (the engine's own buffers)
+        CSM_HIP(hipMemcpyAsync(const_cast<uint32_t*>(bin_.src) + stream_cap_, src, n * 4, hipMemcpyHostToDevice, s_));
+        CSM_HIP(hipMemcpyAsync(const_cast<uint32_t*>(bin_.dst) + stream_cap_, dst, n * 4, hipMemcpyHostToDevice, s_));
+        CSM_HIP(hipMemcpyAsync(const_cast<uint32_t*>(bin_.key_fwd) + stream_cap_, kf.data(), n * 4, hipMemcpyHostToDevice, s_));
+        CSM_HIP(hipMemcpyAsync(const_cast<uint32_t*>(bin_.key_bwd) + stream_cap_, kb.data(), n * 4, hipMemcpyHostToDevice, s_));
+        CSM_HIP(hipMemcpyAsync(const_cast<uint8_t*>(bin_.op) + stream_cap_, op, n, hipMemcpyHostToDevice, s_));
+    }
+    return run_resident(stream_cap_, n, src, dst, op, out, info);
+}
+
+// One batch whose edges sit on the device at [lo, lo + n) of the stream buffers (host copies at
+// src / dst / op for admission and counting).
+std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, const uint32_t* hdst, const uint8_t* hop,
+                                 std::vector<QueryResult>& out, BatchInfo& info) {
     if (n > max_batch_) return "batch larger than the scratch was sized for";
     for (auto& qs : qs_) if (!qs->planned) return "every query must be planned before matching";
     info = BatchInfo{};
     info.edges = n;
-    for (uint64_t k = lo; k < lo + n; k++) (stream_.op[k] ? info.deletes : info.inserts)++;
+    for (uint64_t k = 0; k < n; k++) (hop[k] ? info.deletes : info.inserts)++;
+    std::vector<MirrorUpdate> upd;
+    std::string aerr = admit(hsrc, hdst, hop, n, upd);
+    if (!aerr.empty()) return aerr;
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t max_arcs = static_cast<uint32_t>(2 * n);
     if (n == 0) { out.assign(qs_.size(), QueryResult{}); return ""; }
@@ -963,6 +1051,11 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
         out[q].neg = res[4 * q + 2];
         out[q].neg_hash = res[4 * q + 3];
     }
+    for (const MirrorUpdate& m : upd) {
+        h_deg_[m.dir][m.v] = m.deg;
+        h_cap_[m.dir][m.v] = m.cap;
+    }
+    h_arena_top_ += h_pending_need_;
     const auto t2 = std::chrono::steady_clock::now();
     info.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
     info.ms_match = match_ms;
@@ -971,10 +1064,6 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
     return "";
 }
 
-std::string Engine::run_edges(const uint32_t*, const uint32_t*, const uint32_t*, const uint8_t*, uint64_t, std::vector<QueryResult>&,
-                              BatchInfo&) {
-    return "in-memory batches are not supported yet";
-}
 
 void Engine::prof_begin(Prof& P, hipStream_t st, uint32_t family) {
     if (!(prof_mask_ >> family & 1u)) return;

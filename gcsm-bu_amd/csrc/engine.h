@@ -43,11 +43,13 @@ struct EngineOptions {
     bool split_roots = true;          // spread heavy records of the counting DFS over many waves
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
     uint32_t lanes = 8;               // concurrent matching steps (streams), each with its own scratch
+    uint64_t arena_slack_words = 1u << 20;  // adjacency arena room beyond the growth rule
 };
 
 constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
 constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
 constexpr uint32_t kMaxLanes = 16;
+constexpr uint64_t kLiveBatchesOfRoom = 16;  // arena room for in-memory batches, in max-size batches
 // Per lane, zeroed at every step: 16 set lengths, list counters (+16, +17), one local-index queue
 // counter per query vertex (+kQueueCounters), then the step counters (+kStepCounters).
 constexpr uint32_t kQueueCounters = 18;
@@ -97,7 +99,9 @@ class Engine {
     std::string build();                                  // upload + offline index + summaries
     std::string plan(uint32_t qid, const Technique& t);   // plans need the summaries
     std::string run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info);
-    // In-memory batch (serving): edges are appended to the resident stream.
+    // In-memory batch (serving): the edges are copied to the device scratch region after the
+    // resident stream and applied like a stream batch.  Rejected before any device work when the
+    // adjacency arena cannot hold the lists it grows (admission).
     std::string run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
                           std::vector<QueryResult>& out, BatchInfo& info);
     std::string synchronize();
@@ -119,6 +123,8 @@ class Engine {
     std::string read_candidates(uint32_t qid, uint32_t u, std::vector<uint32_t>& bitmap);
     std::string read_adjacency(uint32_t v, uint32_t dir, std::vector<uint32_t>& ids, std::vector<uint32_t>& labels);
     uint32_t id_bits() const { return id_bits_; }
+    uint64_t arena_used_words() const { return h_arena_top_; }
+    uint64_t arena_words() const { return h_arena_words_; }
     uint32_t last_error_bits() const { return last_err_; }
 
     // Kernel families timed with HIP events on the engine's stream when profiling is enabled.
@@ -182,6 +188,13 @@ class Engine {
         unsigned long long* work_counter() { return reinterpret_cast<unsigned long long*>(counters + 8); }
     };
 
+    struct MirrorUpdate {
+        int dir;
+        uint32_t v, deg, cap;
+    };
+    std::string admit(const uint32_t* src, const uint32_t* dst, const uint8_t* op, uint64_t n, std::vector<MirrorUpdate>& upd);
+    std::string run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, const uint32_t* hdst, const uint8_t* hop,
+                             std::vector<QueryResult>& out, BatchInfo& info);
     std::string run_phase(uint32_t ph);
     std::string run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph);
     std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,
@@ -224,6 +237,10 @@ class Engine {
     std::vector<NlfGroup> h_groups_;
     std::vector<Lane> lanes_;
     std::vector<double> step_cost_[2];        // per phase, per (query, edge): host seconds in the last batch
+    // host mirrors of list degrees and capacities per direction, and of the arena bump pointer
+    std::vector<uint32_t> h_deg_[2], h_cap_[2];
+    std::vector<uint32_t> h_ins_[2], h_del_[2];  // per-batch arc counts of the admission check (kept zeroed)
+    uint64_t h_arena_top_ = 0, h_arena_words_ = 0, h_pending_need_ = 0;
     uint64_t ring_words_ = 0;                 // words of each lane's ring
     hipEvent_t phase_start_ = nullptr;
     unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash

@@ -92,6 +92,21 @@ csm_status csm_get_summary(csm_session* s, uint32_t query_id, uint32_t* cardinal
  * (BuildTries, UpdateGlobalIndex, BuildLocalIndex_v2 / GammaBuildLocalIndex, Match per query edge). */
 csm_status csm_apply_batch(csm_session* s, uint64_t batch_index, csm_query_result* results, uint32_t num_queries,
                            csm_batch_info* info);
+/* Extension: the update-file edges [first, first + n) as one batch, so batches of any size can
+ * run over the resident stream (the reference splits the file into fixed batches only,
+ * graph.cpp:511-516).  Same semantics and results as csm_apply_batch. */
+csm_status csm_apply_range(csm_session* s, uint64_t first, uint64_t n, csm_query_result* results, uint32_t num_queries,
+                           csm_batch_info* info);
+/* Extension (serving; no counterpart in the reference, whose batches come only from the update
+ * file, graph.cpp:482-563): one batch of n edges from host arrays; op[k] = 0 inserts, 1 deletes;
+ * labels are raw edge labels already known to the session.  n <= csm_options.max_batch_edges (set
+ * it when opening a session for in-memory batches; it also reserves arena room for 16 such
+ * batches).  Admission: a batch whose grown lists do not fit the adjacency arena is rejected with
+ * CSM_ERR_MEMORY before any device work, leaving the session as it was. */
+csm_status csm_apply_edges(csm_session* s, const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op,
+                           uint64_t n, csm_query_result* results, uint32_t num_queries, csm_batch_info* info);
+/* Extension: adjacency arena words in use (initial lists plus every relocated list) and capacity. */
+csm_status csm_arena_usage(csm_session* s, uint64_t* used_words, uint64_t* capacity_words);
 /* Per-relation edge counts of the reference's grouping (GraphFileIoManager::PrintNumDataEdges,
  * graph.cpp:470-480): batch_index = UINT64_MAX for the initial graph. */
 csm_status csm_get_relation_edge_counts(csm_session* s, uint32_t query_id, uint64_t batch_index, uint64_t* counts, uint32_t cap,

@@ -9,6 +9,7 @@
 //       |Emb(Q, G)|: injective, vertex-label and edge-label preserving, non-induced maps; every
 //       automorphic image counts (reference: no automorphism handling, match_gpu.cu:279-286).
 //   oracle stream  <data> <update> <batch> <query>... [--threads T] [--directed] [--max-batches K]
+//                  [--batch-sizes FILE]  (variable batches: one size per line, replaces <batch>)
 //                  [--anchor-stride S] [--time-limit-ms T]
 //       Per batch t and query: N_t (deletion-anchored) and P_t (insertion-anchored) with the
 //       embedding-set hashes, following the exactly-once interleaving of
@@ -511,6 +512,7 @@ int main(int argc, char** argv) {
     uint32_t max_batches = UINT32_MAX, stride = 1;
     bool literal = false;
     double time_limit_ms = 0;
+    const char* batch_sizes = nullptr;
     std::vector<const char*> pos;
     for (int i = 2; i < argc; i++) {
         if (!std::strcmp(argv[i], "--directed")) directed = true;
@@ -519,6 +521,7 @@ int main(int argc, char** argv) {
         else if (!std::strcmp(argv[i], "--max-batches") && i + 1 < argc) max_batches = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
         else if (!std::strcmp(argv[i], "--anchor-stride") && i + 1 < argc) stride = std::max(1u, (uint32_t)std::strtoul(argv[++i], nullptr, 10));
         else if (!std::strcmp(argv[i], "--time-limit-ms") && i + 1 < argc) time_limit_ms = std::atof(argv[++i]);
+        else if (!std::strcmp(argv[i], "--batch-sizes") && i + 1 < argc) batch_sizes = argv[++i];
         else pos.push_back(argv[i]);
     }
     if (cmd == "count" && pos.size() == 2) {
@@ -559,10 +562,22 @@ int main(int argc, char** argv) {
         const uint32_t bs = (uint32_t)std::strtoul(pos[2], nullptr, 10);
         std::vector<Query> qs(pos.size() - 3);
         for (size_t k = 3; k < pos.size(); k++) if (!load_query(pos[k], qs[k - 3])) return 1;
-        const size_t nb = bs ? (ups.size() + bs - 1) / bs : 0;
+        // batch b covers updates [bounds[b], bounds[b + 1]): fixed size bs, or the sizes listed one
+        // per line in --batch-sizes (variable batches, as a serving loop applies them)
+        std::vector<size_t> bounds{0};
+        if (batch_sizes) {
+            FILE* f = std::fopen(batch_sizes, "r");
+            if (!f) { std::fprintf(stderr, "cannot read %s\n", batch_sizes); return 1; }
+            unsigned long long x;
+            while (std::fscanf(f, "%llu", &x) == 1 && bounds.back() < ups.size()) bounds.push_back(std::min<size_t>(ups.size(), bounds.back() + x));
+            std::fclose(f);
+        } else if (bs) {
+            while (bounds.back() < ups.size()) bounds.push_back(std::min<size_t>(ups.size(), bounds.back() + bs));
+        }
+        const size_t nb = bounds.size() - 1;
         std::printf("threads %d batches %zu\n", threads, nb);
         for (size_t b = 0; b < nb && b < max_batches; b++) {
-            std::vector<Update> batch(ups.begin() + b * bs, ups.begin() + std::min(ups.size(), (b + 1) * (size_t)bs));
+            std::vector<Update> batch(ups.begin() + bounds[b], ups.begin() + bounds[b + 1]);
             auto t0 = std::chrono::steady_clock::now();
             if (time_limit_ms > 0) g_deadline = t0 + std::chrono::microseconds((long long)(time_limit_ms * 1000));
             g_anchors_done = 0;

@@ -263,3 +263,119 @@ def test_c4_sixty_four_standing_queries(lib, tmp_path):
             res, _ = s.apply_batch(b)
             for k, r in enumerate(res):
                 assert r == want[(b, k)], f"batch {b} query {k}"
+
+
+def _read_stream(path):
+    src, dst, lab, op = [], [], [], []
+    for line in open(path):
+        t = line.split()
+        if not t or t[0] not in ("e", "-e"):
+            continue
+        src.append(int(t[1]))
+        dst.append(int(t[2]))
+        lab.append(int(t[3]) if len(t) > 3 else 0)
+        op.append(1 if t[0] == "-e" else 0)
+    return src, dst, lab, op
+
+
+def _variable_sizes(n, nominal, seed):
+    """Seeded lognormal batch sizes clipped to [nominal / 16, 2 * nominal] covering n updates."""
+    import random
+    rng = random.Random(seed)
+    sizes, left = [], n
+    while left > 0:
+        x = int(min(2 * nominal, max(nominal // 16, rng.lognormvariate(0, 1.0) * nominal / 2)))
+        x = max(1, min(x, left))
+        sizes.append(x)
+        left -= x
+    return sizes
+
+
+def test_variable_batches_resident_and_in_memory(lib, tmp_path):
+    """Serving-style variable batches: the same lognormal batch sizes applied over the resident
+    stream (csm_apply_range) and as host arrays on a session without an update file
+    (csm_apply_edges) give the oracle's counts and hashes for exactly those batches."""
+    meta, queries = util.gen("c2", tmp_path, seed=6, scale=0.02)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    nominal = int(meta["batch_size"])
+    src, dst, lab, op = _read_stream(upd)
+    sizes = _variable_sizes(len(src), nominal, seed=11)
+    (tmp_path / "sizes.txt").write_text("".join(f"{x}\n" for x in sizes))
+    res, _ = util.oracle_stream(data, upd, nominal, queries, threads=16, extra=["--batch-sizes", str(tmp_path / "sizes.txt")])
+    assert len({k[0] for k in res}) == len(sizes)
+    say(f"case c2 scale=0.02 seed=6 variable batches {len(sizes)} via apply_range")
+    with lib.Session(device=0, max_batch_edges=max(sizes)) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        s.open_updates(str(upd), nominal)
+        s.build()
+        for q in qids:
+            s.plan(q)
+        lo = 0
+        for b, x in enumerate(sizes):
+            got, info = s.apply_range(lo, x)
+            assert info.edges == x
+            for k, r in enumerate(got):
+                assert r == res[(b, k)], f"range batch {b} query {k}"
+            lo += x
+    say(f"case c2 scale=0.02 seed=6 variable batches {len(sizes)} via apply_edges")
+    with lib.Session(device=0, max_batch_edges=max(sizes)) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        s.build()
+        for q in qids:
+            s.plan(q)
+        lo = 0
+        for b, x in enumerate(sizes):
+            got, info = s.apply_edges(src[lo:lo + x], dst[lo:lo + x], lab[lo:lo + x], op[lo:lo + x])
+            for k, r in enumerate(got):
+                assert r == res[(b, k)], f"edges batch {b} query {k}"
+            lo += x
+
+
+def test_arena_admission(lib, tmp_path, monkeypatch):
+    """A batch whose grown lists do not fit the adjacency arena is refused (CSM_ERR_MEMORY) before
+    any device work; the graph is then exactly the admitted batches' result."""
+    import random
+    monkeypatch.setenv("CSM_ARENA_SLACK_WORDS", "0")
+    meta, queries = util.gen("tinyins", tmp_path, seed=3)
+    data = tmp_path / "data.graph"
+    nv, present, labels = 0, set(), {}
+    for line in open(data):
+        t = line.split()
+        if t and t[0] == "v":
+            nv = max(nv, int(t[1]) + 1)
+        elif t and t[0] == "e":
+            a, b = int(t[1]), int(t[2])
+            present.add((min(a, b), max(a, b)))
+    rng = random.Random(5)
+    with lib.Session(device=0, max_batch_edges=64) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        s.build()
+        for q in qids:
+            s.plan(q)
+        used0, cap = s.arena_usage()
+        refused = False
+        for _ in range(400):
+            batch = []
+            while len(batch) < 64:
+                a, b = rng.randrange(nv), rng.randrange(nv)
+                e = (min(a, b), max(a, b))
+                if a != b and e not in present and e not in batch:
+                    batch.append(e)
+            try:
+                s.apply_edges([e[0] for e in batch], [e[1] for e in batch], [0] * 64, [0] * 64)
+            except lib.CsmError as err:
+                assert err.status == 6 and "arena" in str(err)  # CSM_ERR_MEMORY
+                refused = True
+                break
+            present.update(batch)
+        say(f"case arena admission: used {used0} -> {s.arena_usage()[0]} of {cap} words, refused={refused}")
+        assert refused
+        nbrs = {}
+        for a, b in present:
+            nbrs.setdefault(a, set()).add(b)
+            nbrs.setdefault(b, set()).add(a)
+        for v in range(nv):
+            assert sorted(s.adjacency(v)) == sorted(nbrs.get(v, ())), f"vertex {v}"

@@ -129,3 +129,27 @@ def test_directed_telescopes(tmp_path):
         c0, _ = util.oracle_count(data, q, directed=True)
         c1, _ = util.oracle_count(final, q, directed=True)
         assert pos == c1 - c0
+
+
+def test_variable_batch_sizes(tmp_path):
+    """--batch-sizes: equal sizes reproduce the fixed-size run line for line; uneven sizes still
+    telescope to the brute-force difference between the first and the final graph."""
+    util.ensure_built()
+    meta, queries = util.gen("tiny", tmp_path, seed=4)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    n = sum(1 for line in open(upd) if line.split() and line.split()[0] in ("e", "-e"))
+    fixed, _ = util.oracle_stream(data, upd, 37, queries)
+    (tmp_path / "equal.txt").write_text("37\n" * (n // 37 + 1))
+    same, _ = util.oracle_stream(data, upd, 1, queries, extra=["--batch-sizes", str(tmp_path / "equal.txt")])
+    assert same == fixed
+    sizes = [1, 50, 3, 90, 7, 200, 11, 5000]
+    (tmp_path / "uneven.txt").write_text("".join(f"{x}\n" for x in sizes))
+    res, _ = util.oracle_stream(data, upd, 1, queries, extra=["--batch-sizes", str(tmp_path / "uneven.txt")])
+    final = tmp_path / "final.graph"
+    util.write_final_graph(data, upd, final)
+    for k, q in enumerate(queries):
+        pos = sum(v[0] - v[1] for (b, kk), v in res.items() if kk == k)
+        hsh = sum(v[2] - v[3] for (b, kk), v in res.items() if kk == k) % (1 << 64)
+        c0, h0 = util.oracle_count(data, q)
+        c1, h1 = util.oracle_count(final, q)
+        assert (pos, hsh) == (c1 - c0, (h1 - h0) % (1 << 64))

@@ -12,6 +12,7 @@
 #include <functional>
 #include <thread>
 #include <unordered_set>
+#include <tuple>
 #include <vector>
 
 namespace csm {
@@ -119,6 +120,24 @@ std::vector<uint64_t> chung_lu(uint32_t n, uint64_t m, double gamma, double dmax
     return keys;
 }
 
+// Directed arcs a -> b: out-degree of every vertex lognormal(mu, sigma) (rounded, capped), targets
+// uniform, no self-loops or duplicate arcs; returned in random order.
+std::vector<uint64_t> lognormal_arcs(uint32_t n, double mu, double sigma, Rng& rng) {
+    std::vector<uint64_t> keys;
+    for (uint32_t a = 0; a < n; a++) {
+        const double x = std::exp(mu + sigma * rng.normal());
+        const uint64_t d = std::min<uint64_t>(std::min<uint64_t>(static_cast<uint64_t>(x + 0.5), n - 1), 100000);
+        for (uint64_t k = 0; k < d; k++) {
+            const uint32_t b = static_cast<uint32_t>(rng.below(n));
+            if (b != a) keys.push_back((static_cast<uint64_t>(a) << 32) | b);
+        }
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (size_t i = keys.size(); i > 1; i--) std::swap(keys[i - 1], keys[rng.below(i)]);
+    return keys;
+}
+
 std::vector<uint32_t> labels(uint32_t n, uint32_t k, bool zipf, Rng& rng) {
     std::vector<uint32_t> lab(n);
     if (!zipf) {
@@ -140,22 +159,28 @@ struct Csr {  // undirected adjacency of the final graph, sorted lists (used for
     std::vector<uint64_t> off;
     std::vector<uint32_t> nbr;
     std::vector<uint32_t> lab;  // edge label aligned with nbr
+    std::vector<uint8_t> out;   // 1 when the edge is the arc v -> nbr (directed graphs), aligned with nbr
     void build(uint32_t n, const std::vector<Edge>& es) {
         off.assign(static_cast<size_t>(n) + 1, 0);
         for (const auto& e : es) { off[e.a + 1]++; off[e.b + 1]++; }
         for (uint32_t i = 0; i < n; i++) off[i + 1] += off[i];
         nbr.resize(off[n]);
         lab.resize(off[n]);
+        out.resize(off[n]);
         std::vector<uint64_t> pos(off.begin(), off.end() - 1);
         for (const auto& e : es) {
-            nbr[pos[e.a]] = e.b; lab[pos[e.a]++] = e.l;
-            nbr[pos[e.b]] = e.a; lab[pos[e.b]++] = e.l;
+            nbr[pos[e.a]] = e.b; out[pos[e.a]] = 1; lab[pos[e.a]++] = e.l;
+            nbr[pos[e.b]] = e.a; out[pos[e.b]] = 0; lab[pos[e.b]++] = e.l;
         }
         for (uint32_t v = 0; v < n; v++) {
-            std::vector<std::pair<uint32_t, uint32_t>> t;
-            for (uint64_t k = off[v]; k < off[v + 1]; k++) t.emplace_back(nbr[k], lab[k]);
+            std::vector<std::tuple<uint32_t, uint32_t, uint8_t>> t;
+            for (uint64_t k = off[v]; k < off[v + 1]; k++) t.emplace_back(nbr[k], lab[k], out[k]);
             std::sort(t.begin(), t.end());
-            for (uint64_t k = off[v]; k < off[v + 1]; k++) { nbr[k] = t[k - off[v]].first; lab[k] = t[k - off[v]].second; }
+            for (uint64_t k = off[v]; k < off[v + 1]; k++) {
+                nbr[k] = std::get<0>(t[k - off[v]]);
+                lab[k] = std::get<1>(t[k - off[v]]);
+                out[k] = std::get<2>(t[k - off[v]]);
+            }
         }
     }
     uint64_t deg(uint32_t v) const { return off[v + 1] - off[v]; }
@@ -269,9 +294,14 @@ bool extract(const Csr& g, const std::vector<uint32_t>& vlab, uint32_t nq, Shape
         }
         q.verts = vs;
         q.edges.clear();
-        for (auto& t : tree) q.edges.push_back(Edge{t.first, t.second, g.lab[g.find(vs[t.first], vs[t.second])]});
-        for (size_t k = 0; k < take; k++)
-            q.edges.push_back(Edge{extra[k].first, extra[k].second, g.lab[g.find(vs[extra[k].first], vs[extra[k].second])]});
+        // a query edge takes the label and (for directed graphs) the direction of its data edge
+        auto add = [&](uint32_t i, uint32_t j) {
+            const int64_t p = g.find(vs[i], vs[j]);
+            if (g.out[p]) q.edges.push_back(Edge{i, j, g.lab[p]});
+            else q.edges.push_back(Edge{j, i, g.lab[p]});
+        };
+        for (auto& t : tree) add(t.first, t.second);
+        for (size_t k = 0; k < take; k++) add(extra[k].first, extra[k].second);
         (void)vlab;
         return true;
     }
@@ -400,6 +430,12 @@ struct Spec {
     uint32_t batch;
     std::vector<std::pair<Shape, uint32_t>> queries;  // (shape, vertices)
     double emb_cap = 0;  // reject queries with more estimated embeddings in the final graph (0: none)
+    // directed configurations: arcs a -> b, out-degree lognormal(mu, sigma), uniform edge labels, and
+    // a batch-size file of lognormal sizes clipped to [batch_min, batch_max]
+    bool directed = false;
+    uint32_t nelabels = 1;
+    double mu = 0, sigma = 0;
+    uint32_t batch_min = 0, batch_max = 0;
 };
 
 Spec spec_of(const std::string& c) {
@@ -423,6 +459,17 @@ Spec spec_of(const std::string& c) {
             for (int i = 0; i < 24; i++) s.queries.push_back({kSparse, 4 + (i % 9)});
             for (int i = 0; i < 24; i++) s.queries.push_back({kDense, 4 + (i % 9)});
         }
+    } else if (c == "c5") {
+        s = Spec{"c5", 3000000, 0, 0, 0, 4, false, 0.10, 0.15, 16384, {}};
+        s.directed = true;
+        s.nelabels = 8;
+        s.mu = 1.5;
+        s.sigma = 1.2;
+        s.batch_min = 1024;
+        s.batch_max = 262144;
+        s.emb_cap = 2e8;
+        // uniform random targets leave few cycles: sparse shapes fall back to trees when none is found
+        for (int i = 0; i < 32; i++) s.queries.push_back({i % 2 == 0 ? kTree : kSparse, 4 + (i % 5)});
     } else if (c == "c3") {
         s = Spec{"c3", 4000000, 35000000, 2.0, 20000, 16, false, 0.05, 0.05, 1048576, {}};
         s.queries = {{kCycles, 8}, {kCycles, 9}, {kCycles, 10}, {kCycles, 8}, {kCycles, 9}, {kCycles, 10}};
@@ -435,7 +482,7 @@ Spec spec_of(const std::string& c) {
 
 int gen_main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: gen <tiny|c1|c2|c3|c4> <out_dir> [--seed S] [--scale F]\n");
+        std::fprintf(stderr, "usage: gen <tiny|tinyins|c1|c2|c3|c4|c5> <out_dir> [--seed S] [--scale F]\n");
         return 2;
     }
     const std::string cfg = argv[1];
@@ -453,28 +500,33 @@ int gen_main(int argc, char** argv) {
         sp.m = std::max<uint64_t>(256, static_cast<uint64_t>(sp.m * scale));
         sp.dmax = std::max(16.0, sp.dmax * std::sqrt(scale));
         if (sp.batch != 0xFFFFFFFFu) sp.batch = std::max<uint32_t>(64, static_cast<uint32_t>(sp.batch * scale));
+        sp.batch_min = std::max<uint32_t>(16, static_cast<uint32_t>(sp.batch_min * scale));
+        sp.batch_max = std::max<uint32_t>(sp.batch_min, static_cast<uint32_t>(sp.batch_max * scale));
         sp.emb_cap *= std::sqrt(scale);  // scaled-down graphs get lighter queries
     }
     mkdir(out.c_str(), 0755);
     Rng rng(seed * 0x100000001B3ull + 7);
 
-    std::vector<uint64_t> keys = chung_lu(sp.n, sp.m, sp.gamma, sp.dmax, rng);
+    std::vector<uint64_t> keys = sp.directed ? lognormal_arcs(sp.n, sp.mu, sp.sigma, rng) : chung_lu(sp.n, sp.m, sp.gamma, sp.dmax, rng);
     std::vector<uint32_t> vlab = labels(sp.n, sp.nlabels, sp.zipf, rng);
+    std::vector<uint32_t> elab(keys.size(), 0);  // edge label of keys[k]
+    if (sp.nelabels > 1)
+        for (auto& x : elab) x = static_cast<uint32_t>(rng.below(sp.nelabels));
     const size_t n_hold = static_cast<size_t>(keys.size() * sp.holdout);
     const size_t n_init = keys.size() - n_hold;
     // Stream: all held-out edges are inserted; deletions are distinct initial edges, so every
     // deletion removes an existing edge and nothing is reinserted.
     const size_t n_del = sp.delete_frac > 0 ? static_cast<size_t>(n_hold * sp.delete_frac / (1.0 - sp.delete_frac)) : 0;
-    std::vector<std::pair<uint64_t, uint8_t>> stream;
+    std::vector<std::pair<size_t, uint8_t>> stream;  // (index into keys, 1 = deletion)
     stream.reserve(n_hold + n_del);
-    for (size_t k = n_init; k < keys.size(); k++) stream.emplace_back(keys[k], 0);
+    for (size_t k = n_init; k < keys.size(); k++) stream.emplace_back(k, 0);
     {
         std::vector<size_t> pick(n_init);
         for (size_t k = 0; k < n_init; k++) pick[k] = k;
         for (size_t k = 0; k < n_del && k < n_init; k++) {
             const size_t j = k + rng.below(n_init - k);
             std::swap(pick[k], pick[j]);
-            stream.emplace_back(keys[pick[k]], 1);
+            stream.emplace_back(pick[k], 1);
         }
     }
     for (size_t i = stream.size(); i > 1; i--) std::swap(stream[i - 1], stream[rng.below(i)]);
@@ -486,23 +538,24 @@ int gen_main(int argc, char** argv) {
         std::fprintf(f, "t # 0\n");
         for (uint32_t v = 0; v < sp.n; v++) std::fprintf(f, "v %u %u\n", v, vlab[v]);
         for (size_t k = 0; k < n_init; k++)
-            std::fprintf(f, "e %u %u 0\n", static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]));
+            std::fprintf(f, "e %u %u %u\n", static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), elab[k]);
         std::fclose(f);
     }
     {
         FILE* f = std::fopen((out + "/update.stream").c_str(), "w");
         for (const auto& e : stream)
-            std::fprintf(f, "%se %u %u 0\n", e.second ? "-" : "", static_cast<uint32_t>(e.first >> 32), static_cast<uint32_t>(e.first));
+            std::fprintf(f, "%se %u %u %u\n", e.second ? "-" : "", static_cast<uint32_t>(keys[e.first] >> 32), static_cast<uint32_t>(keys[e.first]),
+                         elab[e.first]);
         std::fclose(f);
     }
     // queries are extracted from the final graph (initial minus deletions plus insertions)
     std::vector<Edge> fin;
     {
         std::unordered_set<uint64_t> dels;
-        for (const auto& e : stream) if (e.second) dels.insert(e.first);
+        for (const auto& e : stream) if (e.second) dels.insert(keys[e.first]);
         fin.reserve(keys.size());
         for (size_t k = 0; k < keys.size(); k++)
-            if (k >= n_init || !dels.count(keys[k])) fin.push_back(Edge{static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), 0});
+            if (k >= n_init || !dels.count(keys[k])) fin.push_back(Edge{static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), elab[k]});
     }
     Csr g;
     g.build(sp.n, fin);
@@ -544,6 +597,9 @@ int gen_main(int argc, char** argv) {
             for (size_t i = 0; i < cands.size(); i++)
                 if (ok[i]) { q = cands[i]; found = true; break; }
         }
+        if (!found && sp.directed && qs.first != kTree) {  // no cycle of this size: a tree instead
+            for (int t = 0; t < 64 && !found; t++) found = extract(g, vlab, qs.second, kTree, fin_keys, rng, q);
+        }
         if (!found) {
             std::fprintf(stderr, "could not extract query %u\n", qk);
             return 1;
@@ -551,9 +607,20 @@ int gen_main(int argc, char** argv) {
         write_query(out + "/q" + std::to_string(qk) + ".graph", q, vlab);
         qk++;
     }
+    if (sp.batch_max) {  // variable batches: lognormal sizes (median batch, sigma 1) clipped to [min, max]
+        FILE* bf = std::fopen((out + "/batches.txt").c_str(), "w");
+        for (size_t left = stream.size(); left > 0;) {
+            const double x = sp.batch * std::exp(rng.normal());
+            const size_t b = std::min<size_t>(left, std::min<double>(sp.batch_max, std::max<double>(sp.batch_min, x)));
+            std::fprintf(bf, "%zu\n", b);
+            left -= b;
+        }
+        std::fclose(bf);
+    }
     FILE* f = std::fopen((out + "/meta.txt").c_str(), "w");
     std::fprintf(f, "config %s\nseed %llu\nn %u\nm_initial %zu\nstream %zu\ninserts %zu\ndeletes %zu\nbatch_size %u\nqueries %u\n",
                  sp.name, (unsigned long long)seed, sp.n, n_init, stream.size(), n_hold, n_del, sp.batch, qk);
+    std::fprintf(f, "directed %d\nbatch_max %u\n", sp.directed ? 1 : 0, sp.batch_max ? sp.batch_max : sp.batch);
     std::fclose(f);
     return 0;
 }

@@ -379,3 +379,31 @@ def test_arena_admission(lib, tmp_path, monkeypatch):
             nbrs.setdefault(b, set()).add(a)
         for v in range(nv):
             assert sorted(s.adjacency(v)) == sorted(nbrs.get(v, ())), f"vertex {v}"
+
+
+def test_c5_directed_serving(lib, tmp_path):
+    """The serving workload family: a directed graph (lognormal out-degrees, 4 vertex labels x 8 edge
+    labels), 32 standing queries, 85/15 insert/delete, lognormal batch sizes, applied as in-memory
+    batches on a session without an update file.  Every batch of every query equals the oracle's
+    directed counts for that query alone."""
+    meta, queries = util.gen("c5", tmp_path, seed=1, scale=0.005)
+    assert meta["directed"] == "1" and len(queries) == 32
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    sizes = [int(x) for x in open(tmp_path / "batches.txt")]
+    res, _ = util.oracle_stream(data, upd, 1, queries, threads=16, directed=True,
+                                extra=["--batch-sizes", str(tmp_path / "batches.txt")])
+    assert sum(v[1] for v in res.values()) > 0  # some deletions destroy matches
+    src, dst, lab, op = _read_stream(upd)
+    say(f"case c5 scale=0.005 seed=1 directed, {len(sizes)} variable batches via apply_edges")
+    with lib.Session(device=0, directed=True, max_batch_edges=max(sizes)) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        s.build()
+        for q in qids:
+            s.plan(q)
+        lo = 0
+        for b, x in enumerate(sizes):
+            got, _ = s.apply_edges(src[lo:lo + x], dst[lo:lo + x], lab[lo:lo + x], op[lo:lo + x])
+            for k, r in enumerate(got):
+                assert r == res[(b, k)], f"batch {b} query {k}"
+            lo += x

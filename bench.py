@@ -28,10 +28,14 @@ FAMILIES = ["enumerate_count", "enumerate_write", "cartesian_product", "local_in
 BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_write"]
 KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat<0>",
              "level_write": "k_level_flat<1>"}
+SERVING_METRIC = "serving: update edges/s and p50/p99 latency of in-memory batches (host arrays, copy included)"
 WORKLOADS = {
     "c2": "plain method on one GPU: Chung-Lu 400K vertices / ~3.4M edges, gamma 2.1, max degree ~3000, "
           "Zipf(1.0) labels over 32, 10% held out, 90/10 insert/delete stream, batches of 65,536 edges, "
           "6 queries of 6-8 vertices (3 sparse, 3 dense)",
+    "c5": "serving: directed, 3M vertices / ~25M arcs, lognormal(1.5, 1.2) out-degree, 4 vertex x 8 edge labels, "
+          "10% held out, 85/15 insert/delete, lognormal batch sizes in [1K, 256K] (median 16K), "
+          "32 queries of 4-8 vertices",
 }
 
 
@@ -76,6 +80,49 @@ def open_session(ffi, data_dir, queries, batch, lanes=0):
     for q in s.queries:
         s.plan(q)
     return s, nb
+
+
+def serving(ffi, data_dir, queries, meta, max_batches):
+    """In-memory batches of the directed serving workload: lognormal batch sizes from batches.txt,
+    each handed over as host arrays (csm_apply_edges: host-to-device copy, arena admission, index
+    update, matching of every standing query).  Latency is wall clock per batch."""
+    import numpy as np
+    cols = {"src": [], "dst": [], "lab": [], "op": []}
+    for line in open(os.path.join(data_dir, "update.stream")):
+        t = line.split()
+        if t and t[0] in ("e", "-e"):
+            cols["src"].append(int(t[1])); cols["dst"].append(int(t[2])); cols["lab"].append(int(t[3]))
+            cols["op"].append(1 if t[0] == "-e" else 0)
+    src, dst = np.array(cols["src"], dtype=np.uint32), np.array(cols["dst"], dtype=np.uint32)
+    lab, op = np.array(cols["lab"], dtype=np.uint32), np.array(cols["op"], dtype=np.uint8)
+    sizes = [int(x) for x in open(os.path.join(data_dir, "batches.txt"))][:max_batches]
+    s = ffi.Session(device=0, directed=True, max_batch_edges=int(meta["batch_max"]))
+    s.load_graph(os.path.join(data_dir, "data.graph"))
+    for q in queries:
+        s.add_query(q)
+    s.build()
+    for q in s.queries:
+        s.plan(q)
+    lat, edges, matches, lo = [], 0, 0, 0
+    for b, x in enumerate(sizes):
+        t0 = time.perf_counter()
+        res, info = s.apply_edges(src[lo:lo + x], dst[lo:lo + x], lab[lo:lo + x], op[lo:lo + x])
+        s.synchronize()
+        if b > 0:  # batch 0 warms up
+            lat.append(time.perf_counter() - t0)
+            edges += x
+            matches += sum(r[0] + r[1] for r in res)
+        lo += x
+    used, cap = s.arena_usage()
+    s.close()
+    lat_ms = sorted(v * 1e3 for v in lat)
+    total = sum(lat)
+    return {"metric": SERVING_METRIC, "value": edges / total, "unit": "update edges/s", "batches": len(lat),
+            "batch_sizes": {"min": min(sizes[1:]), "median": sorted(sizes[1:])[len(sizes[1:]) // 2], "max": max(sizes[1:])},
+            "p50_batch_ms": statistics.median(lat_ms),
+            "p99_batch_ms": lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))],
+            "matches_per_s": matches / total, "queries": len(queries), "arena_words_used": used, "arena_words": cap,
+            "config": {"workload": WORKLOADS[meta["config"]], "seed": int(meta["seed"])}}
 
 
 def traffic_of(kernel, config):
@@ -141,7 +188,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="c2", choices=["c2"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--data-dir", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -151,6 +198,9 @@ def main():
                          "the roofline comes from a separate one-lane pass)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of 2^26 words (HBM traffic counter calibration)")
+    ap.add_argument("--serving", action="store_true",
+                    help="run the in-memory serving workload (c5) instead and print its own JSON line")
+    ap.add_argument("--serving-batches", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the one-lane roofline pass (what the committed kernel profiles cover)")
     args = ap.parse_args()
@@ -160,9 +210,14 @@ def main():
         return 2
     ensure_built()
     ffi = load_ffi()
+    if args.serving:
+        args.config = "c5"
     data_dir = args.data_dir or os.path.join("/tmp", f"csm_bench_{args.config}_s{args.seed}")
     meta, queries = workload(args.config, args.seed, data_dir)
     batch = int(meta["batch_size"])
+    if args.serving:
+        print(json.dumps(serving(ffi, data_dir, queries, meta, args.serving_batches + 1)))
+        return 0
     if args.roofline_only:
         r = roofline_pass(ffi, data_dir, queries, batch, args.config)
         if args.calibrate:

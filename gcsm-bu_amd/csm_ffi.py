@@ -186,13 +186,19 @@ class Session:
         return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
 
     def apply_edges(self, src, dst, elabel, op):
+        """One in-memory batch.  Lists, or contiguous numpy arrays (uint32 src/dst/elabel, uint8 op),
+        which are passed without copying."""
         n = len(src)
         nq = len(self.queries)
         res = (QueryResult * max(1, nq))()
         info = BatchInfo()
-        A = ctypes.c_uint32 * max(1, n)
-        st = self.lib.csm_apply_edges(self.h, A(*src), A(*dst), A(*elabel), (ctypes.c_uint8 * max(1, n))(*op), n, res, nq,
-                                      ctypes.byref(info))
+        if hasattr(src, "ctypes"):
+            P32, P8 = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8)
+            args = (src.ctypes.data_as(P32), dst.ctypes.data_as(P32), elabel.ctypes.data_as(P32), op.ctypes.data_as(P8))
+        else:
+            A = ctypes.c_uint32 * max(1, n)
+            args = (A(*src), A(*dst), A(*elabel), (ctypes.c_uint8 * max(1, n))(*op))
+        st = self.lib.csm_apply_edges(self.h, *args, n, res, nq, ctypes.byref(info))
         self._check(st)
         return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
 

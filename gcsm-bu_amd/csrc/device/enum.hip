@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, S
 // finds the records of its kLevelIlp candidates there (a global search only past the window, when
 // records have under one candidate each).  The kLevelIlp candidates of a lane are tested in
 // lock-step so their independent loads are in flight together: the kernel is latency-bound.
-constexpr uint32_t kLevelIlp = 4;
+constexpr uint32_t kLevelIlp = 8;
 constexpr uint32_t kLevelStep = 64 * kLevelIlp;
 
 // Write mode stages passing (record, candidate) pairs per wave in LDS and reserves ring space once

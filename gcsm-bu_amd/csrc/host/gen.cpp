@@ -160,6 +160,7 @@ struct Csr {  // undirected adjacency of the final graph, sorted lists (used for
     std::vector<uint32_t> nbr;
     std::vector<uint32_t> lab;  // edge label aligned with nbr
     std::vector<uint8_t> out;   // 1 when the edge is the arc v -> nbr (directed graphs), aligned with nbr
+    bool directed = false;
     void build(uint32_t n, const std::vector<Edge>& es) {
         off.assign(static_cast<size_t>(n) + 1, 0);
         for (const auto& e : es) { off[e.a + 1]++; off[e.b + 1]++; }
@@ -297,7 +298,7 @@ bool extract(const Csr& g, const std::vector<uint32_t>& vlab, uint32_t nq, Shape
         // a query edge takes the label and (for directed graphs) the direction of its data edge
         auto add = [&](uint32_t i, uint32_t j) {
             const int64_t p = g.find(vs[i], vs[j]);
-            if (g.out[p]) q.edges.push_back(Edge{i, j, g.lab[p]});
+            if (!g.directed || g.out[p]) q.edges.push_back(Edge{i, j, g.lab[p]});
             else q.edges.push_back(Edge{j, i, g.lab[p]});
         };
         for (auto& t : tree) add(t.first, t.second);
@@ -559,6 +560,7 @@ int gen_main(int argc, char** argv) {
     }
     Csr g;
     g.build(sp.n, fin);
+    g.directed = sp.directed;
     std::vector<uint64_t> fin_keys;
     fin_keys.reserve(fin.size());
     for (const auto& e : fin) fin_keys.push_back((static_cast<uint64_t>(e.a) << 32) | e.b);

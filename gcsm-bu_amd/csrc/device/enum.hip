@@ -344,10 +344,11 @@ template <int kMode>
 __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
                                                     const unsigned long long* off, const uint4* runs, const unsigned long long* ph,
                                                     Ring r, MatchOut out) {
-    __shared__ unsigned long long s_win[4][kLevelStep + 1];
+    // window of record offsets, relative to off[rb] (a window spans fewer than 2^32 candidates)
+    __shared__ uint32_t s_win[4][kLevelStep + 1];
     __shared__ uint32_t s_out[4][kMode == kWriteAtEnd ? 2 * kOutBuf : 1];
     const uint32_t lane = lane_id();
-    unsigned long long* win = s_win[threadIdx.x >> 6];
+    uint32_t* win = s_win[threadIdx.x >> 6];
     uint32_t* const out_k = s_out[threadIdx.x >> 6];
     uint32_t* const out_c = out_k + (kMode == kWriteAtEnd ? kOutBuf : 0);
     uint32_t staged = 0;  // pairs in the wave's LDS buffer
@@ -370,9 +371,13 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
         if (kMode == kWriteAtEnd && ((t0 - w0) & (16ull * kLevelStep - 1)) == 0 && t0 != w0 &&
             uniform(__hip_atomic_load(r.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
             break;
+        const unsigned long long wbase = off[rb];
+        auto rel = [&](uint32_t i) {
+            return static_cast<uint32_t>(min(off[min(rb + i, n)] - wbase, 0xFFFFFFFFull));
+        };
 #pragma unroll
-        for (uint32_t j = 0; j < kLevelIlp; j++) win[lane + 64 * j] = off[min(rb + lane + 64 * j, n)];
-        if (lane == 0) win[kLevelStep] = off[min(rb + kLevelStep, n)];
+        for (uint32_t j = 0; j < kLevelIlp; j++) win[lane + 64 * j] = rel(lane + 64 * j);
+        if (lane == 0) win[kLevelStep] = rel(kLevelStep);
         wave_sync();
         unsigned long long t[kLevelIlp];
         uint32_t k[kLevelIlp], key[kLevelIlp], c[kLevelIlp];
@@ -383,7 +388,8 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
             ok[j] = t[j] < w1;
             k[j] = rb;
             if (!ok[j]) continue;
-            if (win[kLevelStep] <= t[j]) {  // past the window: last k in [rb + kLevelStep, n) with off[k] <= t
+            const unsigned long long tr = t[j] - wbase;  // < 2^32: inside record rb or the window
+            if (win[kLevelStep] <= tr) {  // past the window: last k in [rb + kLevelStep, n) with off[k] <= t
                 uint32_t lo = rb + kLevelStep, hi = n;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
@@ -395,7 +401,7 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
                 uint32_t lo = 0, hi = kLevelStep;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (win[mid] <= t[j]) lo = mid;
+                    if (win[mid] <= tr) lo = mid;
                     else hi = mid;
                 }
                 k[j] = rb + lo;
@@ -410,7 +416,7 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
             if (!ok[j]) continue;
             const uint4 x = runs[k[j]];
             rv[j] = RunView{x.x, x.y, x.z, x.w};
-            base[j] = k[j] - rb < kLevelStep ? win[k[j] - rb] : off[k[j]];
+            base[j] = k[j] - rb < kLevelStep ? wbase + win[k[j] - rb] : off[k[j]];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kLevelIlp; j++) {

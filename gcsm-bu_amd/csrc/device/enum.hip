@@ -208,35 +208,71 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
 }
 
 // ---- One level, flattened over the whole level ------------------------------------------------
-// k_level_len: per record, the length of its pivot view run (and, when counting, its prefix hash);
-// after an exclusive scan, k_level_flat gives one thread per candidate of the whole level: thread t
-// finds its record by binary search over the offsets, so skewed records spread over the grid.
+// k_level_len: per record, the view runs of all backward entries of depth d0 (entry order, stored
+// at runs[i * E ..], E = entries of the depth) and the entry whose view sequence is the shortest:
+// that one generates the record's candidates, the others are probed (the candidate set is their
+// intersection whichever entry generates it).  Its length goes to len[i] and, when counting, the
+// record's prefix hash to ph[i].  After an exclusive scan, k_level_flat gives one thread per
+// candidate of the whole level: thread t finds its record by binary search over the offsets, so
+// skewed records spread over the grid, and probes the precomputed runs (no directory or delta
+// segment lookups per candidate).
 // Algorithmic bytes (as the DFS kernel's frames): every view run of the depth's backward lists
 // read once per record, plus 4 per passing candidate (counted by k_level_flat).
-__global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                   unsigned long long* len, uint4* runs, unsigned long long* ph,
-                                                   unsigned long long* bytes_out) {
-    const uint32_t p = d.pivot[d0];
-    const BackEntry pe = d.be[p];
-    unsigned long long bytes = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t* f = rec + static_cast<uint64_t>(i) * d0;
-        const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
-        len[i] = run_seq_len(rv, pe.view);
-        runs[i] = make_uint4(rv.lo, rv.hi, rv.dlo, rv.dhi);
-        if (ph) {
-            unsigned long long h = 0;
-            for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
-            ph[i] = h;
+constexpr uint32_t kRunsUnroll = 4;  // entries whose lookups are all issued before the first store
+
+__device__ __forceinline__ uint32_t record_runs(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* f, uint4* runs,
+                                                uint4* prun, uint8_t* piv, unsigned long long* ph, unsigned long long& bytes,
+                                                bool count_bytes) {
+    const uint32_t e0 = d.boff[d0], ne = d.boff[d0 + 1] - e0;
+    uint32_t best = 0xFFFFFFFFu, tag = 0;
+    uint4 brun = make_uint4(0, 0, 0, 0);
+    auto take = [&](uint32_t e, const RunView& rv) {
+        const BackEntry be = d.be[e0 + e];
+        const uint4 x = make_uint4(rv.lo, rv.hi, rv.dlo, rv.dhi);
+        if (ne > 1) runs[e] = x;
+        const uint32_t len = run_seq_len(rv, be.view);
+        if (len < best) {
+            best = len;
+            brun = x;
+            tag = e | static_cast<uint32_t>(be.view) << 4 | static_cast<uint32_t>(be.dir) << 6;
         }
-        if (bytes_out) {
-            for (uint32_t e = d.boff[d0]; e < d.boff[d0 + 1]; e++) {
-                const BackEntry be = d.be[e];
-                const RunView ov = e == p ? rv : make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
-                bytes += 4ull * ((ov.hi - ov.lo) + (ov.dhi - ov.dlo));
-            }
-        }
+        if (count_bytes) bytes += 4ull * ((rv.hi - rv.lo) + (rv.dhi - rv.dlo));
+    };
+    // the stores below may alias the graph for the compiler: look the first entries up first, so
+    // their dependent loads overlap
+    RunView rv[kRunsUnroll];
+#pragma unroll
+    for (uint32_t e = 0; e < kRunsUnroll; e++) {
+        if (e >= ne) break;
+        const BackEntry be = d.be[e0 + e];
+        rv[e] = make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
     }
+    unsigned long long h = 0;
+    if (ph)
+        for (uint32_t j = 0; j < d0; j++) h += pair_hash(d.order[j], f[j]);
+#pragma unroll
+    for (uint32_t e = 0; e < kRunsUnroll; e++) {
+        if (e >= ne) break;
+        take(e, rv[e]);
+    }
+    for (uint32_t e = kRunsUnroll; e < ne; e++) {
+        const BackEntry be = d.be[e0 + e];
+        take(e, make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot));
+    }
+    *prun = brun;
+    *piv = static_cast<uint8_t>(tag);
+    if (ph) *ph = h;
+    return best;
+}
+
+__global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                   unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv,
+                                                   unsigned long long* ph, unsigned long long* bytes_out) {
+    const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
+    unsigned long long bytes = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        len[i] = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i, piv + i,
+                             ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
     if (bytes_out) {
         bytes = wave_sum_u64(bytes);
         if (lane_id() == 0 && bytes) atomicAdd(bytes_out, bytes);
@@ -249,59 +285,43 @@ constexpr uint32_t kSmallItems = 4;
 constexpr uint32_t kSmallLevel = kSmallThreads * kSmallItems;
 
 __global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                                   unsigned long long* off, uint4* runs, unsigned long long* ph,
-                                                                   unsigned long long* bytes_out) {
+                                                                   unsigned long long* off, uint4* prun, uint4* runs, uint8_t* piv,
+                                                                   unsigned long long* ph, unsigned long long* bytes_out) {
     __shared__ unsigned long long s_wave[kSmallThreads / 64];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-    const uint32_t p = d.pivot[d0];
-    const BackEntry pe = d.be[p];
-    unsigned long long len[kSmallItems], sum = 0, bytes = 0;
-    // thread tid owns records tid * kSmallItems + j (contiguous, so the scan is one block scan)
+    const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
+    unsigned long long len[kSmallItems], bytes = 0;
+    // thread tid owns records j * kSmallThreads + tid (independent lookups in flight together)
 #pragma unroll
     for (uint32_t j = 0; j < kSmallItems; j++) {
-        const uint32_t i = tid * kSmallItems + j;
+        const uint32_t i = j * kSmallThreads + tid;
         len[j] = 0;
         if (i >= n) continue;
-        const uint32_t* f = rec + static_cast<uint64_t>(i) * d0;
-        const RunView rv = make_run(g, d.phase, pe.view, pe.dir, f[pe.pos], pe.slot);
-        len[j] = run_seq_len(rv, pe.view);
-        runs[i] = make_uint4(rv.lo, rv.hi, rv.dlo, rv.dhi);
-        if (ph) {
-            unsigned long long h = 0;
-            for (uint32_t q = 0; q < d0; q++) h += pair_hash(d.order[q], f[q]);
-            ph[i] = h;
+        len[j] = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i, piv + i,
+                             ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
+    }
+    // exclusive scan in record order: one block scan per item row, carried over the rows
+    unsigned long long carry = 0;
+    for (uint32_t j = 0; j < kSmallItems && j * kSmallThreads < n; j++) {
+        unsigned long long x = len[j];
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const unsigned long long y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
         }
-        if (bytes_out) {
-            for (uint32_t e = d.boff[d0]; e < d.boff[d0 + 1]; e++) {
-                const BackEntry be = d.be[e];
-                const RunView ov = e == p ? rv : make_run(g, d.phase, be.view, be.dir, f[be.pos], be.slot);
-                bytes += 4ull * ((ov.hi - ov.lo) + (ov.dhi - ov.dlo));
-            }
+        if (lane == 63) s_wave[wv] = x;
+        __syncthreads();
+        unsigned long long before = 0, total = 0;
+        for (uint32_t k = 0; k < kSmallThreads / 64; k++) {
+            const unsigned long long t = s_wave[k];
+            if (k < wv) before += t;
+            total += t;
         }
-        sum += len[j];
+        const uint32_t i = j * kSmallThreads + tid;
+        if (i < n) off[i] = carry + before + x - len[j];
+        carry += total;
+        __syncthreads();  // s_wave is rewritten by the next row
     }
-    // block exclusive scan of the per-thread sums
-    unsigned long long x = sum;
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-        const unsigned long long y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) s_wave[wv] = x;
-    __syncthreads();
-    unsigned long long before = 0, total = 0;
-    for (uint32_t k = 0; k < kSmallThreads / 64; k++) {
-        const unsigned long long t = s_wave[k];
-        if (k < wv) before += t;
-        total += t;
-    }
-    unsigned long long o = before + x - sum;
-#pragma unroll
-    for (uint32_t j = 0; j < kSmallItems; j++) {
-        const uint32_t i = tid * kSmallItems + j;
-        if (i < n) off[i] = o;
-        o += len[j];
-    }
-    if (tid == 0) off[n] = total;
+    if (tid == 0) off[n] = carry;
     if (bytes_out) {
         bytes = wave_sum_u64(bytes);
         if (lane == 0 && bytes) atomicAdd(bytes_out, bytes);
@@ -342,8 +362,8 @@ __device__ __forceinline__ void flush_records(const uint32_t* rec, uint32_t d0, 
 
 template <int kMode>
 __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                    const unsigned long long* off, const uint4* runs, const unsigned long long* ph,
-                                                    Ring r, MatchOut out) {
+                                                    const unsigned long long* off, const uint4* prun, const uint4* runs,
+                                                    const uint8_t* piv, const unsigned long long* ph, Ring r, MatchOut out) {
     // window of record offsets, relative to off[rb] (a window spans fewer than 2^32 candidates)
     __shared__ uint32_t s_win[4][kLevelStep + 1];
     __shared__ uint32_t s_out[4][kMode == kWriteAtEnd ? 2 * kOutBuf : 1];
@@ -352,10 +372,7 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
     uint32_t* const out_k = s_out[threadIdx.x >> 6];
     uint32_t* const out_c = out_k + (kMode == kWriteAtEnd ? kOutBuf : 0);
     uint32_t staged = 0;  // pairs in the wave's LDS buffer
-    const uint32_t p = d.pivot[d0];
-    const BackEntry pe = d.be[p];
-    const uint32_t* pdl = phase_delta(g, pe.dir, d.phase);
-    const uint32_t e0 = d.boff[d0], e1 = d.boff[d0 + 1];
+    const uint32_t e0 = d.boff[d0], ne = d.boff[d0 + 1] - e0;
     const uint32_t* filter = d.filter[d0];
     const uint32_t qv = d.order[d0];
     const unsigned long long total = off[n];
@@ -407,40 +424,50 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
                 k[j] = rb + lo;
             }
         }
+        // the record's generating run (prun) and its entry | view << 4 | dir << 6 (piv): independent loads
         RunView rv[kLevelIlp];
         unsigned long long base[kLevelIlp];
+        uint32_t pv[kLevelIlp];
 #pragma unroll
         for (uint32_t j = 0; j < kLevelIlp; j++) {
             rv[j] = RunView{0, 0, 0, 0};
             base[j] = 0;
+            pv[j] = 0;
             if (!ok[j]) continue;
-            const uint4 x = runs[k[j]];
+            const uint4 x = prun[k[j]];
             rv[j] = RunView{x.x, x.y, x.z, x.w};
+            pv[j] = piv[k[j]];
             base[j] = k[j] - rb < kLevelStep ? wbase + win[k[j] - rb] : off[k[j]];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kLevelIlp; j++) {
-            key[j] = ok[j] ? run_elem(g, pdl, rv[j], run_base_len(rv[j], pe.view), static_cast<uint32_t>(t[j] - base[j])) : 0u;
+            const uint32_t view = (pv[j] >> 4) & 3u;
+            key[j] = ok[j] ? run_elem(g, phase_delta(g, pv[j] >> 6, d.phase), rv[j], run_base_len(rv[j], view),
+                                      static_cast<uint32_t>(t[j] - base[j]))
+                           : 0u;
             c[j] = key[j] & g.id_mask;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && bit_test(filter, c[j]);
-        if (pe.view == kViewBaseMinusDelta) {
 #pragma unroll
-            for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && run_elem_live(pdl, rv[j], pe.view, key[j]);
-        }
+        for (uint32_t j = 0; j < kLevelIlp; j++)
+            ok[j] = ok[j] && run_elem_live(phase_delta(g, pv[j] >> 6, d.phase), rv[j], (pv[j] >> 4) & 3u, key[j]);
         for (uint32_t im = d.inj[d0]; im; im &= im - 1) {
             const uint32_t q = static_cast<uint32_t>(__ffs(im)) - 1;
 #pragma unroll
             for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && rec[static_cast<uint64_t>(k[j]) * d0 + q] != c[j];
         }
-        for (uint32_t e = e0; e < e1; e++) {
-            if (e == p) continue;
-            const BackEntry be = d.be[e];
+        // probe the other entries' precomputed runs (each with its own slot bits)
+        for (uint32_t e = 0; e < ne; e++) {
+            const BackEntry be = d.be[e0 + e];
+            const uint32_t* dl = phase_delta(g, be.dir, d.phase);
             const uint32_t hi_bits = be.slot << g.id_bits;
 #pragma unroll
-            for (uint32_t j = 0; j < kLevelIlp; j++)
-                ok[j] = ok[j] && view_contains(g, d.phase, be.view, be.dir, be.slot, rec[static_cast<uint64_t>(k[j]) * d0 + be.pos], hi_bits | c[j]);
+            for (uint32_t j = 0; j < kLevelIlp; j++) {
+                if (!ok[j] || (pv[j] & 15u) == e) continue;
+                const uint4 x = runs[static_cast<uint64_t>(k[j]) * ne + e];
+                ok[j] = run_contains(g, dl, RunView{x.x, x.y, x.z, x.w}, be.view, hi_bits | c[j]);
+            }
         }
         if (kMode == kCountAtEnd) {
 #pragma unroll
@@ -765,10 +792,12 @@ void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_dept
 void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
                    bool with_hash, unsigned long long* bytes, hipStream_t s) {
     if (n < kSmallLevel) {
-        k_level_len_small<<<1, kSmallThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, with_hash ? ls.ph : nullptr, bytes);
+        k_level_len_small<<<1, kSmallThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, with_hash ? ls.ph : nullptr,
+                                                      bytes);
         return;
     }
-    k_level_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, with_hash ? ls.ph : nullptr, bytes);
+    k_level_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, with_hash ? ls.ph : nullptr,
+                                                       bytes);
     scan_exclusive<unsigned long long>(ls.off, ls.off, nullptr, n, true, ls.scan_tmp, s);
 }
 
@@ -779,9 +808,14 @@ void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_
     // of idle waves.
     const int grid = static_cast<int>(std::min<uint32_t>(2048, std::max<uint32_t>(16, n / 2)));
     if (mode == kCountAtEnd)
-        k_level_flat<kCountAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
+        k_level_flat<kCountAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, r, out);
     else
-        k_level_flat<kWriteAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.runs, ls.ph, r, out);
+        k_level_flat<kWriteAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, r, out);
+}
+
+uint32_t level_max_records(const StepDesc& d, uint32_t d0, const LevelScratch& ls) {
+    const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
+    return static_cast<uint32_t>(std::min<uint64_t>(ls.cap, ne > 1 ? ls.runs_cap / ne : ls.cap));
 }
 
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,

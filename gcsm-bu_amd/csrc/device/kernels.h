@@ -162,20 +162,28 @@ enum ExtendMode : int { kCountAtEnd = 0, kWriteAtEnd = 1 };
 void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r,
             MatchOut out, const unsigned long long* split, unsigned long long* work_counter, int schedule, hipStream_t s);
 // One level from the n records of depth d0 at rec (n known on the host), flattened over all its
-// candidates: level_prepare writes the scanned pivot-run lengths of the records to off[0..n]
-// (off[n] = total candidates), their pivot runs and, with_hash, their prefix hashes; level_flat then
-// tests every candidate, load-balanced over the grid, and counts/hashes complete embeddings
-// (d0 + 1 == n) or writes records of depth d0 + 1.
+// candidates: level_prepare writes, per record, the view runs of the depth's backward entries, the
+// entry with the shortest view sequence (it generates the candidates; the others are probed), the
+// scanned lengths of those sequences to off[0..n] (off[n] = total candidates) and, with_hash, the
+// record's prefix hash; level_flat then tests every candidate, load-balanced over the grid, and
+// counts/hashes complete embeddings (d0 + 1 == n) or writes records of depth d0 + 1.
+// n <= level_max_records(d, d0, ls).
 struct LevelScratch {
-    unsigned long long* off;       // [n + 1] scanned candidate counts
-    uint4* runs;                   // [n] pivot view run of each record
-    unsigned long long* ph;        // [n] prefix hash of each record (counting levels)
+    unsigned long long* off;       // [cap + 1] scanned candidate counts
+    uint4* prun;                   // [cap] generating view run of each record
+    uint4* runs;                   // [runs_cap] view runs of every backward entry (record-major; depths with one entry skip it)
+    uint8_t* piv;                  // [cap] generating entry | view << 4 | dir << 6
+    unsigned long long* ph;        // [cap] prefix hash of each record (counting levels)
     unsigned long long* scan_tmp;
+    uint64_t cap;
+    uint64_t runs_cap;
 };
 void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
                    bool with_hash, unsigned long long* bytes, hipStream_t s);
 void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode, const LevelScratch& ls,
                 Ring r, MatchOut out, hipStream_t s);
+// Most records one level call takes at depth d0 (the per-entry runs must fit).
+uint32_t level_max_records(const StepDesc& d, uint32_t d0, const LevelScratch& ls);
 // chunks[i] = ceil(|pivot view run of record i at depth d0| / 64), then scanned in place (total at [n]).
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
                  uint32_t max_rec, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s);

@@ -221,6 +221,7 @@ std::string Engine::build() {
         add(ring_words_ * 4); add(256);
         add((cp_rec + 1) * 8); add(cp_rec * 8); add(scan_scratch_elems(static_cast<uint32_t>(cp_rec)) * 8);
         add((cp_rec + 1) * 8); add(cp_rec * 16); add(cp_rec * 4);  // CP segments (runs shared with levels)
+        add(kLevelRunsPerRecord * cp_rec * 16);                    // level: per-entry runs
         add(cp_list_words * 4); add(64);
         add(64 << 10);                                             // alignment slack of the takes
     }
@@ -404,8 +405,11 @@ std::string Engine::build() {
         if (!cpc) return "device pool exhausted (lane scratch)";
         w.cp.max_prod = cpc;
         w.cp.err = reinterpret_cast<uint32_t*>(cpc + 1);
-        w.level = LevelScratch{w.cp.prod, w.cp.seg_run, w.cp.rec_hash, w.cp.scan_tmp};
-        if (!w.level.runs) return "device pool exhausted (lane scratch)";
+        // level scratch: offsets, generating runs, entry tags and hashes shared with the CP scratch
+        uint4* level_runs = pool_.take<uint4>(kLevelRunsPerRecord * cp_rec);
+        if (!level_runs) return "device pool exhausted (lane scratch)";
+        w.level = LevelScratch{w.cp.prod, w.cp.seg_run, level_runs, reinterpret_cast<uint8_t*>(w.cp.seg_cnt), w.cp.rec_hash,
+                               w.cp.scan_tmp, cp_rec, kLevelRunsPerRecord * cp_rec};
     }
     opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
     results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);
@@ -706,8 +710,9 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
     store_words(w.counters, Words{10, {count, 0, 0, 0, 0, 0, 0, 0, 0, 0}}, w.s);
     Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
     if (end == depth + 1) {
-        for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
-            const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
+        const uint32_t chunk = level_max_records(d, depth, w.level);
+        for (uint32_t s0 = 0; s0 < count; s0 += chunk) {
+            const uint32_t nc = std::min(chunk, count - s0);
             const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
             prof_begin(w.prof, w.s, kKLevelPrepare);
             level_prepare(g_, d, depth, rc, nc, w.level, false, bytes_ptr(kKLevelWrite), w.s);
@@ -812,8 +817,9 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
     if (trace_) w.trace_line += ", \"count_from\": " + std::to_string(depth) + ", \"records\": " + std::to_string(count);
     if (depth + 1 == d.n && w.sched == 1) {
         out.bytes = bytes_ptr(kKLevelCount);
-        for (uint32_t s0 = 0; s0 < count; s0 += opt_.cp_max_records) {
-            const uint32_t nc = std::min(opt_.cp_max_records, count - s0);
+        const uint32_t chunk = level_max_records(d, depth, w.level);
+        for (uint32_t s0 = 0; s0 < count; s0 += chunk) {
+            const uint32_t nc = std::min(chunk, count - s0);
             const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
             prof_begin(w.prof, w.s, kKLevelPrepare);
             level_prepare(g_, d, depth, rc, nc, w.level, true, out.bytes, w.s);

@@ -49,6 +49,9 @@ struct EngineOptions {
 constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
 constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
 constexpr uint32_t kMaxLanes = 16;
+// Per-entry view runs a lane's level scratch holds, in records' worth of cp_max_records: levels
+// whose depth has more backward entries than this take fewer records per call.
+constexpr uint64_t kLevelRunsPerRecord = 3;
 constexpr uint64_t kLiveBatchesOfRoom = 16;  // arena room for in-memory batches, in max-size batches
 // Per lane, zeroed at every step: 16 set lengths, list counters (+16, +17), one local-index queue
 // counter per query vertex (+kQueueCounters), then the step counters (+kStepCounters).

@@ -218,6 +218,44 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
 // segment lookups per candidate).
 // Algorithmic bytes (as the DFS kernel's frames): every view run of the depth's backward lists
 // read once per record, plus 4 per passing candidate (counted by k_level_flat).
+// Membership of key[j] in the sorted a[lo[j], hi[j]) for the active j, N searches advancing in
+// lock-step (each round issues all N loads before using any).  A present key is always probed: a
+// lower-bound search probes the position it ends on unless that position is hi.
+template <uint32_t N>
+__device__ __forceinline__ void contains_many(const uint32_t* a, const uint32_t* lo0, const uint32_t* hi0, const uint32_t* key,
+                                              const bool* act, bool* found) {
+    uint32_t lo[N], n[N];
+    bool more = false;
+#pragma unroll
+    for (uint32_t j = 0; j < N; j++) {
+        lo[j] = lo0[j];
+        n[j] = act[j] ? hi0[j] - lo0[j] : 0u;
+        found[j] = false;
+        more = more || n[j] != 0;
+    }
+    while (more) {
+        uint32_t v[N];
+#pragma unroll
+        for (uint32_t j = 0; j < N; j++) v[j] = n[j] ? a[lo[j] + (n[j] >> 1)] : 0u;
+        more = false;
+#pragma unroll
+        for (uint32_t j = 0; j < N; j++) {
+            if (!n[j]) continue;
+            const uint32_t h = n[j] >> 1;
+            if (v[j] == key[j]) {
+                found[j] = true;
+                n[j] = 0;
+            } else if (v[j] < key[j]) {
+                lo[j] += h + 1;
+                n[j] -= h + 1;
+            } else {
+                n[j] = h;
+            }
+            more = more || n[j] != 0;
+        }
+    }
+}
+
 constexpr uint32_t kRunsUnroll = 4;  // entries whose lookups are all issued before the first store
 
 __device__ __forceinline__ uint32_t record_runs(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* f, uint4* runs,
@@ -399,14 +437,31 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
         unsigned long long t[kLevelIlp];
         uint32_t k[kLevelIlp], key[kLevelIlp], c[kLevelIlp];
         bool ok[kLevelIlp];
+        // record of each candidate: last i < kLevelStep with win[i] <= t - off[rb] (win[0] <= t0 <= t),
+        // a fixed-step search run for all kLevelIlp candidates at once so their LDS reads overlap
+        uint32_t tr[kLevelIlp], pos[kLevelIlp];
+        bool past = false;
 #pragma unroll
         for (uint32_t j = 0; j < kLevelIlp; j++) {
             t[j] = t0 + 64 * j + lane;
             ok[j] = t[j] < w1;
-            k[j] = rb;
-            if (!ok[j]) continue;
-            const unsigned long long tr = t[j] - wbase;  // < 2^32: inside record rb or the window
-            if (win[kLevelStep] <= tr) {  // past the window: last k in [rb + kLevelStep, n) with off[k] <= t
+            const unsigned long long x = ok[j] ? t[j] - wbase : 0ull;  // < 2^32: inside record rb or the window
+            tr[j] = static_cast<uint32_t>(min(x, 0xFFFFFFFFull));
+            past = past || (ok[j] && win[kLevelStep] <= x);
+            pos[j] = 0;
+        }
+#pragma unroll
+        for (uint32_t st = kLevelStep / 2; st; st >>= 1) {
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++)
+                if (win[pos[j] + st] <= tr[j]) pos[j] += st;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLevelIlp; j++) k[j] = rb + pos[j];
+        if (past) {  // records with fewer than one candidate each: last k in [rb + kLevelStep, n) with off[k] <= t
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++) {
+                if (!ok[j] || win[kLevelStep] > t[j] - wbase) continue;
                 uint32_t lo = rb + kLevelStep, hi = n;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
@@ -414,14 +469,6 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
                     else hi = mid;
                 }
                 k[j] = lo;
-            } else {  // last i < kLevelStep with win[i] <= t (win[0] <= t0 <= t)
-                uint32_t lo = 0, hi = kLevelStep;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (win[mid] <= tr) lo = mid;
-                    else hi = mid;
-                }
-                k[j] = rb + lo;
             }
         }
         // the record's generating run (prun) and its entry | view << 4 | dir << 6 (piv): independent loads
@@ -457,16 +504,54 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
 #pragma unroll
             for (uint32_t j = 0; j < kLevelIlp; j++) ok[j] = ok[j] && rec[static_cast<uint64_t>(k[j]) * d0 + q] != c[j];
         }
-        // probe the other entries' precomputed runs (each with its own slot bits)
+        // probe the other entries' precomputed runs (each with its own slot bits); the searches of
+        // the kLevelIlp candidates advance in lock-step so their loads are in flight together
         for (uint32_t e = 0; e < ne; e++) {
             const BackEntry be = d.be[e0 + e];
-            const uint32_t* dl = phase_delta(g, be.dir, d.phase);
             const uint32_t hi_bits = be.slot << g.id_bits;
+            // runs as {lo, hi} / {dlo, dhi} halves: the delta half is loaded only for views that use it
+            const uint2* rh = reinterpret_cast<const uint2*>(runs);
+            uint32_t pk[kLevelIlp];
+            bool act[kLevelIlp], inb[kLevelIlp], ind[kLevelIlp];
 #pragma unroll
             for (uint32_t j = 0; j < kLevelIlp; j++) {
-                if (!ok[j] || (pv[j] & 15u) == e) continue;
-                const uint4 x = runs[static_cast<uint64_t>(k[j]) * ne + e];
-                ok[j] = run_contains(g, dl, RunView{x.x, x.y, x.z, x.w}, be.view, hi_bits | c[j]);
+                act[j] = ok[j] && (pv[j] & 15u) != e;
+                pk[j] = hi_bits | c[j];
+                inb[j] = false;
+                ind[j] = false;
+            }
+            if (be.view != kViewDelta) {
+                uint32_t lo[kLevelIlp], hi[kLevelIlp];
+#pragma unroll
+                for (uint32_t j = 0; j < kLevelIlp; j++) {
+                    const uint2 x = act[j] ? rh[2 * (static_cast<uint64_t>(k[j]) * ne + e)] : make_uint2(0, 0);
+                    lo[j] = x.x;
+                    hi[j] = x.y;
+                }
+                contains_many<kLevelIlp>(g.arena, lo, hi, pk, act, inb);
+            }
+            if (be.view != kViewBase) {
+                const uint32_t* dl = phase_delta(g, be.dir, d.phase);
+                uint32_t lo[kLevelIlp], hi[kLevelIlp];
+                bool need[kLevelIlp];
+#pragma unroll
+                for (uint32_t j = 0; j < kLevelIlp; j++) {
+                    need[j] = act[j] && (be.view == kViewDelta || (be.view == kViewBasePlusDelta ? !inb[j] : inb[j]));
+                    const uint2 x = need[j] ? rh[2 * (static_cast<uint64_t>(k[j]) * ne + e) + 1] : make_uint2(0, 0);
+                    lo[j] = x.x;
+                    hi[j] = x.y;
+                }
+                contains_many<kLevelIlp>(dl, lo, hi, pk, need, ind);
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kLevelIlp; j++) {
+                if (!act[j]) continue;
+                switch (be.view) {
+                    case kViewBase: ok[j] = inb[j]; break;
+                    case kViewBasePlusDelta: ok[j] = inb[j] || ind[j]; break;
+                    case kViewDelta: ok[j] = ind[j]; break;
+                    default: ok[j] = inb[j] && !ind[j]; break;
+                }
             }
         }
         if (kMode == kCountAtEnd) {

@@ -35,8 +35,8 @@ typedef struct {
     int32_t device;          /* main.cpp:168 --device */
     int32_t directed;        /* extension: arcs src -> dst only (the reference is undirected) */
     uint64_t pool_bytes;     /* HBM budget for the device pool, 0 = no cap */
-    uint64_t ring_bytes;     /* partial-result ring (config.h:39 RES_SPACE), 0 = 4 GiB */
-    uint64_t cp_list_bytes;  /* Cartesian-product leaf lists (config.h:43 SIZE_SPACE), 0 = 1 GiB */
+    uint64_t ring_bytes;     /* partial-result ring of each lane (config.h:39 RES_SPACE), 0 = 1 GiB */
+    uint64_t cp_list_bytes;  /* Cartesian-product leaf lists of each lane (config.h:43 SIZE_SPACE), 0 = 512 MiB */
     uint32_t max_batch_edges;/* largest batch the scratch is sized for, 0 = the stream's batch */
     uint32_t semi_bfs_threshold; /* config.h:47 MIN_NUM_RESULTS_TO_GPU, 0 = 65536 */
     uint32_t lanes;          /* extension: concurrent step lanes (streams + host threads, each with its own
@@ -58,7 +58,10 @@ typedef struct {
 typedef struct {
     uint64_t positive;
     uint64_t negative;
-    uint64_t positive_hash;  /* sum over new embeddings of H(f), see DESIGN */
+    /* Embedding hash H(f) = fmix64(sum over query vertices q of fmix64(((f(q) << 4) | q) +
+     * 0x9E3779B97F4A7C15)), fmix64 = MurmurHash3's 64-bit finaliser; the two hashes are sums of
+     * H(f) over the new / destroyed embeddings, all mod 2^64. */
+    uint64_t positive_hash;
     uint64_t negative_hash;
 } csm_query_result;
 

@@ -303,30 +303,95 @@ __device__ __forceinline__ uint32_t record_runs(const DevGraph& g, const StepDes
     return best;
 }
 
-__global__ void __launch_bounds__(256) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                   unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv,
-                                                   unsigned long long* ph, unsigned long long* bytes_out) {
+// Block-wide exclusive scan of one value per thread over kSmallThreads threads (s_wave holds
+// kSmallThreads / 64 words); returns the exclusive prefix and the block total.
+constexpr uint32_t kSmallThreads = 1024;
+__device__ __forceinline__ unsigned long long block_scan_1024(unsigned long long v, unsigned long long* s_wave,
+                                                               unsigned long long& total) {
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    unsigned long long x = v;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wv] = x;
+    __syncthreads();
+    unsigned long long before = 0;
+    total = 0;
+    for (uint32_t k = 0; k < kSmallThreads / 64; k++) {
+        const unsigned long long t = s_wave[k];
+        if (k < wv) before += t;
+        total += t;
+    }
+    __syncthreads();  // s_wave may be rewritten by the caller's next scan
+    return before + x - v;
+}
+
+// Records a level call takes per block of the multi-block preparation (two per thread).
+constexpr uint32_t kLenItems = 2;
+constexpr uint32_t kLenTile = kSmallThreads * kLenItems;
+
+// Multi-block levels: lengths and runs of tile blockIdx.x's records, and the tile's candidate
+// total in tile_sum[blockIdx.x]; k_level_scan then turns the lengths into offsets.
+__global__ void __launch_bounds__(kSmallThreads) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                             unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv,
+                                                             unsigned long long* ph, unsigned long long* tile_sum, uint32_t* zero,
+                                                             unsigned long long* bytes_out) {
+    __shared__ unsigned long long s_wave[kSmallThreads / 64];
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) { zero[0] = 0; zero[1] = 0; }
     const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
-    unsigned long long bytes = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        len[i] = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i, piv + i,
-                             ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
+    unsigned long long sum = 0, bytes = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kLenItems; j++) {
+        const uint32_t i = blockIdx.x * kLenTile + j * kSmallThreads + threadIdx.x;
+        if (i >= n) continue;
+        const uint32_t l = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i,
+                                       piv + i, ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
+        len[i] = l;
+        sum += l;
+    }
+    unsigned long long total;
+    block_scan_1024(sum, s_wave, total);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
     if (bytes_out) {
         bytes = wave_sum_u64(bytes);
         if (lane_id() == 0 && bytes) atomicAdd(bytes_out, bytes);
     }
 }
 
+// Exclusive scan of the lengths in place (total at off[n]): block b adds the totals of the tiles
+// before it (at most a few thousand words, read by the whole block) to the scan of its own tile.
+__global__ void __launch_bounds__(kSmallThreads) k_level_scan(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
+    __shared__ unsigned long long s_wave[kSmallThreads / 64];
+    unsigned long long carry = 0, total;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kSmallThreads) carry += tile_sum[t];
+    carry = wave_sum_u64(carry);
+    if (lane_id() == 0) s_wave[threadIdx.x >> 6] = carry;
+    __syncthreads();
+    carry = 0;
+    for (uint32_t k = 0; k < kSmallThreads / 64; k++) carry += s_wave[k];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kLenItems; j++) {
+        const uint32_t i = blockIdx.x * kLenTile + j * kSmallThreads + threadIdx.x;
+        const unsigned long long v = i < n ? off[i] : 0ull;
+        const unsigned long long ex = block_scan_1024(v, s_wave, total);
+        if (i < n) off[i] = carry + ex;
+        carry += total;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[n] = carry;
+}
+
 // Levels of under kSmallLevel records (most of them): sizes and scan in one single-block launch.
-constexpr uint32_t kSmallThreads = 1024;
 constexpr uint32_t kSmallItems = 4;
 constexpr uint32_t kSmallLevel = kSmallThreads * kSmallItems;
 
 __global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
                                                                    unsigned long long* off, uint4* prun, uint4* runs, uint8_t* piv,
-                                                                   unsigned long long* ph, unsigned long long* bytes_out) {
+                                                                   unsigned long long* ph, uint32_t* zero, unsigned long long* bytes_out) {
     __shared__ unsigned long long s_wave[kSmallThreads / 64];
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const uint32_t tid = threadIdx.x;
+    if (zero && tid == 0) { zero[0] = 0; zero[1] = 0; }
     const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
     unsigned long long len[kSmallItems], bytes = 0;
     // thread tid owns records j * kSmallThreads + tid (independent lookups in flight together)
@@ -341,28 +406,16 @@ __global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, S
     // exclusive scan in record order: one block scan per item row, carried over the rows
     unsigned long long carry = 0;
     for (uint32_t j = 0; j < kSmallItems && j * kSmallThreads < n; j++) {
-        unsigned long long x = len[j];
-        for (uint32_t o = 1; o < 64; o <<= 1) {
-            const unsigned long long y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) s_wave[wv] = x;
-        __syncthreads();
-        unsigned long long before = 0, total = 0;
-        for (uint32_t k = 0; k < kSmallThreads / 64; k++) {
-            const unsigned long long t = s_wave[k];
-            if (k < wv) before += t;
-            total += t;
-        }
+        unsigned long long total;
+        const unsigned long long ex = block_scan_1024(len[j], s_wave, total);
         const uint32_t i = j * kSmallThreads + tid;
-        if (i < n) off[i] = carry + before + x - len[j];
+        if (i < n) off[i] = carry + ex;
         carry += total;
-        __syncthreads();  // s_wave is rewritten by the next row
     }
     if (tid == 0) off[n] = carry;
     if (bytes_out) {
         bytes = wave_sum_u64(bytes);
-        if (lane == 0 && bytes) atomicAdd(bytes_out, bytes);
+        if (lane_id() == 0 && bytes) atomicAdd(bytes_out, bytes);
     }
 }
 
@@ -374,6 +427,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, S
 // lock-step so their independent loads are in flight together: the kernel is latency-bound.
 constexpr uint32_t kLevelIlp = 8;
 constexpr uint32_t kLevelStep = 64 * kLevelIlp;
+static_assert((kLevelStep & (kLevelStep - 1)) == 0, "the window search takes power-of-two steps");
 
 // Write mode stages passing (record, candidate) pairs per wave in LDS and reserves ring space once
 // per kOutFlush pairs: an atomic per iteration would put its round trip on every iteration.
@@ -875,15 +929,15 @@ void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_dept
 }
 
 void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
-                   bool with_hash, unsigned long long* bytes, hipStream_t s) {
+                   bool with_hash, uint32_t* zero, unsigned long long* bytes, hipStream_t s) {
+    unsigned long long* ph = with_hash ? ls.ph : nullptr;
     if (n < kSmallLevel) {
-        k_level_len_small<<<1, kSmallThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, with_hash ? ls.ph : nullptr,
-                                                      bytes);
+        k_level_len_small<<<1, kSmallThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ph, zero, bytes);
         return;
     }
-    k_level_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, with_hash ? ls.ph : nullptr,
-                                                       bytes);
-    scan_exclusive<unsigned long long>(ls.off, ls.off, nullptr, n, true, ls.scan_tmp, s);
+    const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
+    k_level_len<<<tiles, kSmallThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ph, ls.scan_tmp, zero, bytes);
+    k_level_scan<<<tiles, kSmallThreads, 0, s>>>(ls.off, n, ls.scan_tmp);
 }
 
 void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode, const LevelScratch& ls,

@@ -174,12 +174,14 @@ struct LevelScratch {
     uint4* runs;                   // [runs_cap] view runs of every backward entry (record-major; depths with one entry skip it)
     uint8_t* piv;                  // [cap] generating entry | view << 4 | dir << 6
     unsigned long long* ph;        // [cap] prefix hash of each record (counting levels)
-    unsigned long long* scan_tmp;
+    unsigned long long* scan_tmp;  // [cap / 2048 + 1] tile totals
     uint64_t cap;
     uint64_t runs_cap;
 };
+// zero (nullable): two device words cleared before the level runs (a writing level's output count
+// and overflow flag).
 void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
-                   bool with_hash, unsigned long long* bytes, hipStream_t s);
+                   bool with_hash, uint32_t* zero, unsigned long long* bytes, hipStream_t s);
 void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode, const LevelScratch& ls,
                 Ring r, MatchOut out, hipStream_t s);
 // Most records one level call takes at depth d0 (the per-entry runs must fit).

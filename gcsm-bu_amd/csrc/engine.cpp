@@ -706,16 +706,15 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
 std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, uint32_t count, uint32_t depth, uint32_t end, uint32_t* out,
                               uint64_t cap, bool& ok, uint32_t& n_out) {
     cap = std::min<uint64_t>(cap, 0xFFFFFFFFull);
-    // [0]: input count, [1]: output count, [2]: overflow, [3]: finishing count, [8..9]: work counter
-    store_words(w.counters, Words{10, {count, 0, 0, 0, 0, 0, 0, 0, 0, 0}}, w.s);
-    Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
     if (end == depth + 1) {
+        // [1]: output count, [2]: overflow, cleared by the first chunk's preparation
+        Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
         const uint32_t chunk = level_max_records(d, depth, w.level);
         for (uint32_t s0 = 0; s0 < count; s0 += chunk) {
             const uint32_t nc = std::min(chunk, count - s0);
             const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
             prof_begin(w.prof, w.s, kKLevelPrepare);
-            level_prepare(g_, d, depth, rc, nc, w.level, false, bytes_ptr(kKLevelWrite), w.s);
+            level_prepare(g_, d, depth, rc, nc, w.level, false, s0 == 0 ? w.counters + 1 : nullptr, bytes_ptr(kKLevelWrite), w.s);
             prof_end(w.prof, w.s, kKLevelPrepare);
             if (trace_) {  // candidates of the level (the flat kernel's work), for the trace only
                 CSM_HIP(hipMemcpyAsync(w.h_pinned + 12, w.level.off + nc, 8, hipMemcpyDeviceToHost, w.s));
@@ -728,16 +727,23 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
             level_flat(g_, d, depth, rc, nc, kWriteAtEnd, w.level, r, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
             prof_end(w.prof, w.s, kKLevelWrite);
         }
+        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
+        CSM_HIP(hipStreamSynchronize(w.s));
+        ok = w.h_pinned[2] == 0;
+        n_out = w.h_pinned[1];
     } else {
+        // [0]: input count, [1]: output count, [2]: overflow, [3]: finishing count, [8..9]: work counter
+        store_words(w.counters, Words{10, {count, 0, 0, 0, 0, 0, 0, 0, 0, 0}}, w.s);
+        Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
         prof_begin(w.prof, w.s, kKExtendWrite);
         extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, nullptr, w.work_counter(), w.sched,
                w.s);
         prof_end(w.prof, w.s, kKExtendWrite);
+        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
+        CSM_HIP(hipStreamSynchronize(w.s));
+        ok = w.h_pinned[2] == 0;
+        n_out = w.h_pinned[1];
     }
-    CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
-    CSM_HIP(hipStreamSynchronize(w.s));
-    ok = w.h_pinned[2] == 0;
-    n_out = w.h_pinned[1];
     if (trace_) w.trace_line += ", \"level_" + std::to_string(end) + "\": " + (ok ? std::to_string(n_out) : std::string("\"overflow\""));
     return "";
 }
@@ -782,7 +788,6 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
     const uint32_t ph = d.phase;
     MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr};
     uint32_t* rec_count = w.counters + 3;
-    store_words(rec_count, Words{7, {count, 0, 0, 0, 0, 0, 0}}, w.s);  // [3]: count, [8..9]: work counter
     const uint64_t n_seg = static_cast<uint64_t>(count) * (d.n - depth);
     bool use_cp = qs.tech.cartesian_product && depth < d.n && qs.plan.cp_info[i][depth] == kCpTreeProduct && n_seg <= opt_.cp_max_records;
     if (use_cp) {
@@ -822,7 +827,7 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
             const uint32_t nc = std::min(chunk, count - s0);
             const uint32_t* rc = rec + static_cast<uint64_t>(s0) * depth;
             prof_begin(w.prof, w.s, kKLevelPrepare);
-            level_prepare(g_, d, depth, rc, nc, w.level, true, out.bytes, w.s);
+            level_prepare(g_, d, depth, rc, nc, w.level, true, nullptr, out.bytes, w.s);
             prof_end(w.prof, w.s, kKLevelPrepare);
             if (trace_) {
                 CSM_HIP(hipMemcpyAsync(w.h_pinned + 12, w.level.off + nc, 8, hipMemcpyDeviceToHost, w.s));
@@ -837,6 +842,7 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
         }
         return "";
     }
+    store_words(rec_count, Words{7, {count, 0, 0, 0, 0, 0, 0}}, w.s);  // [3]: count, [8..9]: work counter
     out.bytes = bytes_ptr(kKExtendCount);
     // Root splitting (dynamic schedule, few records: a handful of heavy records would otherwise
     // serialise on single waves).  The CP offsets buffer is free here.

@@ -330,56 +330,110 @@ __device__ __forceinline__ unsigned long long block_scan_1024(unsigned long long
 // Records a level call takes per block of the multi-block preparation (two per thread).
 constexpr uint32_t kLenItems = 2;
 constexpr uint32_t kLenTile = kSmallThreads * kLenItems;
+constexpr int kChainLenGrid = 128;    // blocks of a chained level's preparation (they loop over tiles)
+constexpr int kChainFlatGrid = 1024;  // blocks of a chained level's candidate kernel
 
-// Multi-block levels: lengths and runs of tile blockIdx.x's records, and the tile's candidate
-// total in tile_sum[blockIdx.x]; k_level_scan then turns the lengths into offsets.
-__global__ void __launch_bounds__(kSmallThreads) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                             unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv,
-                                                             unsigned long long* ph, unsigned long long* tile_sum, uint32_t* zero,
-                                                             unsigned long long* bytes_out) {
-    __shared__ unsigned long long s_wave[kSmallThreads / 64];
-    if (zero && blockIdx.x == 0 && threadIdx.x == 0) { zero[0] = 0; zero[1] = 0; }
-    const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
-    unsigned long long sum = 0, bytes = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kLenItems; j++) {
-        const uint32_t i = blockIdx.x * kLenTile + j * kSmallThreads + threadIdx.x;
-        if (i >= n) continue;
-        const uint32_t l = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i,
-                                       piv + i, ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
-        len[i] = l;
-        sum += l;
+// Where level k of a device chain reads its records: false when the level does not run (an
+// earlier level stopped the chain or overflowed, or its own input is empty, larger than one call
+// takes, or large enough for the host's Cartesian-product decision).  Every thread computes the
+// same answer from the host-known start and the counters of the levels before k.
+__device__ __forceinline__ bool chain_at(const LevelChain& c, uint32_t k, uint64_t& off, uint32_t& n) {
+    off = c.off0;
+    n = c.n0;
+    for (uint32_t j = 0;; j++) {
+        if (n == 0 || n > c.cap[j] || n >= c.stop_at[j]) return false;
+        if (j == k) return true;
+        if (c.ctr[2 * j + 1]) return false;
+        off += static_cast<uint64_t>(n) * (c.depth0 + j);
+        n = c.ctr[2 * j];
     }
-    unsigned long long total;
-    block_scan_1024(sum, s_wave, total);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// Multi-block levels: lengths and runs of the records of tiles blockIdx.x, blockIdx.x + gridDim.x,
+// ..., and each tile's candidate total in tile_sum[tile]; level_scan then turns the lengths into
+// offsets.
+__device__ __forceinline__ void level_len_tiles(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv, unsigned long long* ph,
+                                                unsigned long long* tile_sum, unsigned long long* bytes_out) {
+    __shared__ unsigned long long s_wave[kSmallThreads / 64];
+    const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
+    const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
+    unsigned long long bytes = 0;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        unsigned long long sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kLenItems; j++) {
+            const uint32_t i = tile * kLenTile + j * kSmallThreads + threadIdx.x;
+            if (i >= n) continue;
+            const uint32_t l = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i,
+                                           piv + i, ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
+            len[i] = l;
+            sum += l;
+        }
+        unsigned long long total;
+        block_scan_1024(sum, s_wave, total);
+        if (threadIdx.x == 0) tile_sum[tile] = total;
+    }
     if (bytes_out) {
         bytes = wave_sum_u64(bytes);
         if (lane_id() == 0 && bytes) atomicAdd(bytes_out, bytes);
     }
 }
 
-// Exclusive scan of the lengths in place (total at off[n]): block b adds the totals of the tiles
-// before it (at most a few thousand words, read by the whole block) to the scan of its own tile.
-__global__ void __launch_bounds__(kSmallThreads) k_level_scan(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
+__global__ void __launch_bounds__(kSmallThreads) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                             unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv,
+                                                             unsigned long long* ph, unsigned long long* tile_sum, uint32_t* zero,
+                                                             unsigned long long* bytes_out) {
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) { zero[0] = 0; zero[1] = 0; }
+    level_len_tiles(g, d, d0, rec, n, len, prun, runs, piv, ph, tile_sum, bytes_out);
+}
+
+__global__ void __launch_bounds__(kSmallThreads) k_level_len_chain(DevGraph g, StepDesc d, LevelChain c, uint32_t k, unsigned long long* len,
+                                                                   uint4* prun, uint4* runs, uint8_t* piv, unsigned long long* ph,
+                                                                   unsigned long long* tile_sum, unsigned long long* bytes_out) {
+    if (k == 0 && blockIdx.x == 0 && threadIdx.x < 2 * kMaxQV) c.ctr[threadIdx.x] = 0;
+    uint64_t off;
+    uint32_t n;
+    if (!chain_at(c, k, off, n)) return;
+    level_len_tiles(g, d, c.depth0 + k, c.ring + off, n, len, prun, runs, piv, ph, tile_sum, bytes_out);
+}
+
+// Exclusive scan of the lengths in place (total at off[n]): the block of a tile adds the totals of
+// the tiles before it (at most a few thousand words, read by the whole block) to the tile's scan.
+__device__ __forceinline__ void level_scan_tiles(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
     __shared__ unsigned long long s_wave[kSmallThreads / 64];
-    unsigned long long carry = 0, total;
-    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kSmallThreads) carry += tile_sum[t];
-    carry = wave_sum_u64(carry);
-    if (lane_id() == 0) s_wave[threadIdx.x >> 6] = carry;
-    __syncthreads();
-    carry = 0;
-    for (uint32_t k = 0; k < kSmallThreads / 64; k++) carry += s_wave[k];
-    __syncthreads();
+    const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        unsigned long long carry = 0, total;
+        for (uint32_t t = threadIdx.x; t < tile; t += kSmallThreads) carry += tile_sum[t];
+        carry = wave_sum_u64(carry);
+        if (lane_id() == 0) s_wave[threadIdx.x >> 6] = carry;
+        __syncthreads();
+        carry = 0;
+        for (uint32_t w = 0; w < kSmallThreads / 64; w++) carry += s_wave[w];
+        __syncthreads();
 #pragma unroll
-    for (uint32_t j = 0; j < kLenItems; j++) {
-        const uint32_t i = blockIdx.x * kLenTile + j * kSmallThreads + threadIdx.x;
-        const unsigned long long v = i < n ? off[i] : 0ull;
-        const unsigned long long ex = block_scan_1024(v, s_wave, total);
-        if (i < n) off[i] = carry + ex;
-        carry += total;
+        for (uint32_t j = 0; j < kLenItems; j++) {
+            const uint32_t i = tile * kLenTile + j * kSmallThreads + threadIdx.x;
+            const unsigned long long v = i < n ? off[i] : 0ull;
+            const unsigned long long ex = block_scan_1024(v, s_wave, total);
+            if (i < n) off[i] = carry + ex;
+            carry += total;
+        }
+        if (tile == tiles - 1 && threadIdx.x == 0) off[n] = carry;
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[n] = carry;
+}
+
+__global__ void __launch_bounds__(kSmallThreads) k_level_scan(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
+    level_scan_tiles(off, n, tile_sum);
+}
+
+__global__ void __launch_bounds__(kSmallThreads) k_level_scan_chain(LevelChain c, uint32_t k, unsigned long long* off,
+                                                                    const unsigned long long* tile_sum) {
+    uint64_t o;
+    uint32_t n;
+    if (!chain_at(c, k, o, n)) return;
+    level_scan_tiles(off, n, tile_sum);
 }
 
 // Levels of under kSmallLevel records (most of them): sizes and scan in one single-block launch.
@@ -453,9 +507,9 @@ __device__ __forceinline__ void flush_records(const uint32_t* rec, uint32_t d0, 
 }
 
 template <int kMode>
-__global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
-                                                    const unsigned long long* off, const uint4* prun, const uint4* runs,
-                                                    const uint8_t* piv, const unsigned long long* ph, Ring r, MatchOut out) {
+__device__ __forceinline__ void level_flat_body(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                const unsigned long long* off, const uint4* prun, const uint4* runs, const uint8_t* piv,
+                                                const unsigned long long* ph, Ring r, const MatchOut& out) {
     // window of record offsets, relative to off[rb] (a window spans fewer than 2^32 candidates)
     __shared__ uint32_t s_win[4][kLevelStep + 1];
     __shared__ uint32_t s_out[4][kMode == kWriteAtEnd ? 2 * kOutBuf : 1];
@@ -648,6 +702,30 @@ __global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint
         bytes = wave_sum_u64(bytes);
         if (lane == 0 && bytes) atomicAdd(out.bytes, bytes);
     }
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(256) k_level_flat(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+                                                    const unsigned long long* off, const uint4* prun, const uint4* runs,
+                                                    const uint8_t* piv, const unsigned long long* ph, Ring r, MatchOut out) {
+    level_flat_body<kMode>(g, d, d0, rec, n, off, prun, runs, piv, ph, r, out);
+}
+
+// Level k of a device chain: reads its records where chain_at says and writes the records of the
+// next depth right after them ({count, overflow} in ctr[2k], ctr[2k + 1]).
+template <int kMode>
+__global__ void __launch_bounds__(256) k_level_flat_chain(DevGraph g, StepDesc d, LevelChain c, uint32_t k, const unsigned long long* off,
+                                                          const uint4* prun, const uint4* runs, const uint8_t* piv,
+                                                          const unsigned long long* ph, MatchOut out) {
+    uint64_t o;
+    uint32_t n;
+    if (!chain_at(c, k, o, n)) return;
+    const uint32_t d0 = c.depth0 + k;
+    const uint64_t out_off = o + static_cast<uint64_t>(n) * d0;
+    const uint64_t room = out_off < c.ring_words ? (c.ring_words - out_off) / (d0 + 1) : 0ull;
+    const Ring r{c.ring + o, nullptr, n, c.ring + out_off, c.ctr + 2 * k, static_cast<uint32_t>(min(room, 0xFFFFFFFFull)),
+                 c.ctr + 2 * k + 1};
+    level_flat_body<kMode>(g, d, d0, c.ring + o, n, off, prun, runs, piv, ph, r, out);
 }
 
 __global__ void __launch_bounds__(256) k_root_chunks(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
@@ -950,6 +1028,22 @@ void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_
         k_level_flat<kCountAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, r, out);
     else
         k_level_flat<kWriteAtEnd><<<grid, 256, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, r, out);
+}
+
+void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, const LevelScratch& ls, bool with_hash,
+                         unsigned long long* bytes, hipStream_t s) {
+    k_level_len_chain<<<kChainLenGrid, kSmallThreads, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, with_hash ? ls.ph : nullptr,
+                                                             ls.scan_tmp, bytes);
+    k_level_scan_chain<<<kChainLenGrid, kSmallThreads, 0, s>>>(c, k, ls.off, ls.scan_tmp);
+}
+
+void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,
+                      MatchOut out, hipStream_t s) {
+    // one full residency of four-wave blocks at four waves per SIMD: level sizes are device-side
+    if (mode == kCountAtEnd)
+        k_level_flat_chain<kCountAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
+    else
+        k_level_flat_chain<kWriteAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
 }
 
 uint32_t level_max_records(const StepDesc& d, uint32_t d0, const LevelScratch& ls) {

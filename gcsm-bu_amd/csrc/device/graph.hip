@@ -372,6 +372,46 @@ __global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, B
     }
 }
 
+// Admission of a built batch before anything changes: the list space its insertions will take
+// under k_apply_insertions' relocation rule (adm[0]) and whether some vertex would lose more arcs
+// than it has (adm[1]).
+__global__ void __launch_bounds__(256) k_admit(DevGraph g, int dir, BatchScratch w, unsigned long long* adm) {
+    const uint32_t T = *w.n_touched;
+    const DevAdj A = g.dir[dir];
+    const DevDelta D = g.delta[dir];
+    unsigned long long need = 0;
+    bool bad = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        const uint32_t v = w.touched[i];
+        const uint4 s = D.seg[v];
+        const uint32_t deg = A.adj[v].y;
+        if (s.w > deg) { bad = true; continue; }
+        const uint32_t nd = deg - s.w + s.y;
+        if (s.y && nd > A.cap[v]) {
+            uint32_t p2 = 1;
+            while (p2 < nd) p2 <<= 1;
+            need += max(kMinNbrSize, 2u * p2);
+        }
+    }
+    need = wave_sum_u64(need);
+    const bool any_bad = __ballot(bad) != 0;
+    if (lane_id() == 0) {
+        if (need) atomicAdd(&adm[0], need);
+        if (any_bad) atomicOr(reinterpret_cast<uint32_t*>(&adm[1]), 1u);
+    }
+}
+
+// adm[2..5] = {need, bad, arena top, error bits} for the host; adm[0..1] cleared for the next batch.
+__global__ void k_admit_finish(DevGraph g, unsigned long long* adm) {
+    if (threadIdx.x != 0) return;
+    adm[2] = adm[0];
+    adm[3] = adm[1];
+    adm[4] = *g.arena_top;
+    adm[5] = *g.err;
+    adm[0] = 0;
+    adm[1] = 0;
+}
+
 __global__ void k_batch_cleanup(BatchScratch w, DevDelta d) {
     const uint32_t T = *w.n_touched;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
@@ -440,6 +480,13 @@ void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, co
     const int grid = static_cast<int>(std::min<uint32_t>(8192, (max_touched + 3) / 4 + 1));
     k_apply_insertions<<<grid, 256, 0, s>>>(g, dir, w);
 }
+
+void batch_admit(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, unsigned long long* adm, hipStream_t s) {
+    const int grid = static_cast<int>(std::min<uint32_t>(4096, (max_touched + 255) / 256 + 1));
+    k_admit<<<grid, 256, 0, s>>>(g, dir, w, adm);
+}
+
+void batch_admit_finish(const DevGraph& g, unsigned long long* adm, hipStream_t s) { k_admit_finish<<<1, 64, 0, s>>>(g, adm); }
 
 void batch_cleanup(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s) {
     const int grid = static_cast<int>(std::min<uint32_t>(4096, (max_touched + 255) / 256 + 1));

@@ -45,6 +45,11 @@ uint64_t scan_scratch_elems(uint32_t n_max);
 
 void batch_build(const DevGraph& g, int dir, int dirsel, const BatchInput& b, uint32_t lo, uint32_t n,
                  const BatchScratch& w, hipStream_t s);
+// Admission of the built batch (per direction, then finish once): adm[0] += list words the
+// insertions will allocate, adm[1] |= a vertex losing more arcs than it has; finish moves
+// {need, bad, arena top, error bits} to adm[2..5] and clears adm[0..1] (zero before the first batch).
+void batch_admit(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, unsigned long long* adm, hipStream_t s);
+void batch_admit_finish(const DevGraph& g, unsigned long long* adm, hipStream_t s);
 // Recomputes label-slot directory rows of direction dir: every vertex when list is null, else
 // the vertices list[0, min(*n_list, max_n)).
 void slot_dir_update(const DevGraph& g, int dir, const uint32_t* list, const uint32_t* n_list, uint32_t max_n, hipStream_t s);
@@ -186,6 +191,27 @@ void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_
                 Ring r, MatchOut out, hipStream_t s);
 // Most records one level call takes at depth d0 (the per-entry runs must fit).
 uint32_t level_max_records(const StepDesc& d, uint32_t d0, const LevelScratch& ls);
+
+// The levels of one step chained on the device, with no host round trip between them: level k
+// extends the records of depth depth0 + k.  Its input follows from the host-known start and the
+// counters of the levels before it (level k writes its records right after its input, with
+// {count, overflow} in ctr[2k], ctr[2k + 1]; level 0's preparation clears ctr).  A level whose
+// input is empty, holds more than cap[k] records or at least stop_at[k], or that follows an
+// overflow, does not run, and neither does any later one: the host reads ctr and resumes there.
+struct LevelChain {
+    uint32_t* ring;
+    uint64_t ring_words;
+    uint32_t* ctr;                 // [2 * kMaxQV]
+    uint64_t off0;                 // ring word offset of level 0's input records
+    uint32_t n0;                   // their count
+    uint32_t depth0;
+    uint32_t cap[kMaxQV];
+    uint32_t stop_at[kMaxQV];
+};
+void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, const LevelScratch& ls, bool with_hash,
+                         unsigned long long* bytes, hipStream_t s);
+void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,
+                      MatchOut out, hipStream_t s);
 // chunks[i] = ceil(|pivot view run of record i at depth d0| / 64), then scanned in place (total at [n]).
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
                  uint32_t max_rec, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s);

@@ -79,6 +79,7 @@ std::string Engine::init(const EngineOptions& o) {
     // A/B switches of the scheduling and layout aids (all on by default)
     if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) opt_.split_roots = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_SLOT_DIRECTORY")) opt_.slot_directory = std::atoi(t) != 0;
+    if (const char* t = std::getenv("CSM_CHAIN_LEVELS")) opt_.chain_levels = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_ARENA_SLACK_WORDS")) opt_.arena_slack_words = std::strtoull(t, nullptr, 10);
     if (const char* t = std::getenv("CSM_CP_MAX_RECORDS")) opt_.cp_max_records = static_cast<uint32_t>(std::max(1024L, std::atol(t)));
     return "";
@@ -222,12 +223,14 @@ std::string Engine::build() {
         add((cp_rec + 1) * 8); add(cp_rec * 8); add(scan_scratch_elems(static_cast<uint32_t>(cp_rec)) * 8);
         add((cp_rec + 1) * 8); add(cp_rec * 16); add(cp_rec * 4);  // CP segments (runs shared with levels)
         add(kLevelRunsPerRecord * cp_rec * 16);                    // level: per-entry runs
+        add(2 * kMaxQV * 4);                                       // level chain counters
         add(cp_list_words * 4); add(64);
         add(64 << 10);                                             // alignment slack of the takes
     }
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
     add(qs_.size() * 32 + 64);
     add(kKFamilies * 8);
+    add(64);                                                    // batch admission words
     add(kMaxQV * qs_.size() * sizeof(NlfEntry) + kMaxRel * qs_.size() * sizeof(NlfGroup) + 512);
     const uint64_t sdir_bytes = (nslots + 1) * dv_ * 4ull;
     const bool use_sdir = opt_.slot_directory && sdir_bytes * ndirs_ <= kMaxSlotDirectoryBytes;
@@ -255,13 +258,6 @@ std::string Engine::build() {
         std::vector<uint32_t> vr(dv_);
         for (uint32_t v = 0; v < dv_; v++) vr[v] = host_g_.vlabel[v] == kNoVertex ? kNoVertex : rank_of(vlabel_rank_, host_g_.vlabel[v]);
         CSM_HIP(hipMemcpyAsync(d_vrank, vr.data(), dv_ * 4ull, hipMemcpyHostToDevice, s_));
-        for (int d = 0; d < 2; d++) { h_deg_[d].clear(); h_cap_[d].clear(); }
-        for (int d = 0; d < ndirs_; d++) {
-            h_deg_[d] = deg[d];
-            h_cap_[d].resize(dv_);
-            for (uint32_t v = 0; v < dv_; v++) h_cap_[d][v] = cap_for(deg[d][v]);
-        }
-        h_arena_top_ = total_cap;
         h_arena_words_ = arena_words;
         std::vector<uint32_t> arena(total_cap, 0u);
         std::vector<std::vector<uint64_t>> pos = off;
@@ -407,7 +403,8 @@ std::string Engine::build() {
         w.cp.err = reinterpret_cast<uint32_t*>(cpc + 1);
         // level scratch: offsets, generating runs, entry tags and hashes shared with the CP scratch
         uint4* level_runs = pool_.take<uint4>(kLevelRunsPerRecord * cp_rec);
-        if (!level_runs) return "device pool exhausted (lane scratch)";
+        w.chain_ctr = pool_.take<uint32_t>(2 * kMaxQV);
+        if (!level_runs || !w.chain_ctr) return "device pool exhausted (lane scratch)";
         w.level = LevelScratch{w.cp.prod, w.cp.seg_run, level_runs, reinterpret_cast<uint8_t*>(w.cp.seg_cnt), w.cp.rec_hash,
                                w.cp.scan_tmp, cp_rec, kLevelRunsPerRecord * cp_rec};
     }
@@ -416,6 +413,9 @@ std::string Engine::build() {
     prof_bytes_ = pool_.take<unsigned long long>(kKFamilies);
     CSM_HIP(hipMemsetAsync(prof_bytes_, 0, kKFamilies * 8, s_));
     summary_ = pool_.take<unsigned long long>(4);
+    adm_ = pool_.take<unsigned long long>(8);
+    if (!adm_) return "device pool exhausted (admission words)";
+    CSM_HIP(hipMemsetAsync(adm_, 0, 64, s_));
 
     // ---- queries: relation slots, NLF programs, candidate bitmaps ----
     for (auto& qsp : qs_) {
@@ -506,6 +506,15 @@ std::string Engine::plan(uint32_t qid, const Technique& t) {
     if (!err.empty()) return err;
     qs.planned = true;
     return "";
+}
+
+uint64_t Engine::arena_used_words() {
+    if (!built_) return 0;
+    unsigned long long top = 0;
+    if (hipMemcpyAsync(h_pinned_ + 64, g_.arena_top, 8, hipMemcpyDeviceToHost, s_) != hipSuccess || hipStreamSynchronize(s_) != hipSuccess)
+        return 0;
+    std::memcpy(&top, h_pinned_ + 64, 8);
+    return top;
 }
 
 std::string Engine::check_err(const char* where) {
@@ -679,6 +688,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
     const uint64_t top = 2ull * count;  // first free word of the ring
     switch (qs.tech.strategy) {
         case SearchStrategy::kDynamicBfsDfs:  // match_gpu.cu:1916-2127
+            if (opt_.chain_levels && !trace_) return search_chained(w, qs, d, i, count, res_slot);
             return search(w, qs, d, i, ring, count, 2, top, q.n, res_slot);
         case SearchStrategy::kBfs: {  // match_gpu.cu:2129-2294: breadth-first to the Cartesian-product tail
             const uint32_t target = qs.tech.cartesian_product ? std::min(tail_from(2), q.n - 1) : q.n - 1;
@@ -746,6 +756,61 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
     }
     if (trace_) w.trace_line += ", \"level_" + std::to_string(end) + "\": " + (ok ? std::to_string(n_out) : std::string("\"overflow\""));
     return "";
+}
+
+// The semi-BFS search of `search` from the initial records (depth 2 at the bottom of the ring)
+// with its levels chained on the device: every writing level and the final counting level are
+// enqueued at once, each reading its size from the counters of the levels before it, and the host
+// waits once.  The chain runs exactly the levels `search` would run while they need no host
+// decision (a level larger than one call, a Cartesian-product depth with enough records for the
+// semi-BFS threshold, a Cartesian-product leaf, a ring overflow); at the first level that does,
+// the chain stops and `search` goes on from that level's records.
+std::string Engine::search_chained(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot) {
+    const uint32_t depth0 = 2, L = d.n - 1 - depth0;  // writing levels; level L counts the leaf
+    const uint8_t* cpi = qs.plan.cp_info[i];
+    LevelChain c{};
+    c.ring = w.ring;
+    c.ring_words = w.ring_words;
+    c.ctr = w.chain_ctr;
+    c.off0 = 0;
+    c.n0 = count;
+    c.depth0 = depth0;
+    for (uint32_t k = 0; k <= L; k++) {
+        const uint32_t depth = depth0 + k;
+        const bool cp_depth = qs.tech.cartesian_product && cpi[depth] == kCpTreeProduct;
+        c.cap[k] = level_max_records(d, depth, w.level);
+        c.stop_at[k] = !cp_depth ? 0xFFFFFFFFu : k < L ? opt_.semi_bfs_threshold : 0u;
+    }
+    for (uint32_t k = 0; k < L; k++) {
+        prof_begin(w.prof, w.s, kKLevelPrepare);
+        level_prepare_chain(g_, d, c, k, w.level, false, bytes_ptr(kKLevelWrite), w.s);
+        prof_end(w.prof, w.s, kKLevelPrepare);
+        prof_begin(w.prof, w.s, kKLevelWrite);
+        level_flat_chain(g_, d, c, k, kWriteAtEnd, w.level, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
+        prof_end(w.prof, w.s, kKLevelWrite);
+    }
+    const uint32_t ph = d.phase;
+    const MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, bytes_ptr(kKLevelCount)};
+    prof_begin(w.prof, w.s, kKLevelPrepare);
+    level_prepare_chain(g_, d, c, L, w.level, true, out.bytes, w.s);
+    prof_end(w.prof, w.s, kKLevelPrepare);
+    prof_begin(w.prof, w.s, kKLevelCount);
+    level_flat_chain(g_, d, c, L, kCountAtEnd, w.level, out, w.s);
+    prof_end(w.prof, w.s, kKLevelCount);
+    if (L) CSM_HIP(hipMemcpyAsync(w.h_pinned, w.chain_ctr, 8ull * L, hipMemcpyDeviceToHost, w.s));
+    CSM_HIP(hipStreamSynchronize(w.s));
+    // replay the chain's decisions (chain_at) to find where it stopped
+    uint64_t off = 0;
+    uint32_t n = count;
+    for (uint32_t k = 0;; k++) {
+        const uint32_t depth = depth0 + k;
+        if (n == 0) return "";
+        if (n > c.cap[k] || n >= c.stop_at[k] || (k < L && w.h_pinned[2 * k + 1]))
+            return search(w, qs, d, i, w.ring + off, n, depth, off + static_cast<uint64_t>(n) * depth, d.n, res_slot);
+        if (k == L) return "";  // the leaf level counted
+        off += static_cast<uint64_t>(n) * depth;
+        n = w.h_pinned[2 * k];
+    }
 }
 
 // Breadth-first levels kept as a stack in the lane's ring: the `count` records of depth `depth` at
@@ -930,45 +995,6 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
 // Arena admission: the words the batch's relocated lists will take, replaying the device rule
 // (deletions first; a list whose new length exceeds its capacity moves to a fresh block of
 // max(8, 2 * next_pow2(length))) on the host mirrors of degrees and capacities.
-std::string Engine::admit(const uint32_t* src, const uint32_t* dst, const uint8_t* op, uint64_t n, std::vector<MirrorUpdate>& upd) {
-    upd.clear();
-    for (int d = 0; d < ndirs_; d++)
-        if (h_ins_[d].size() != dv_) { h_ins_[d].assign(dv_, 0); h_del_[d].assign(dv_, 0); }
-    std::vector<std::pair<int, uint32_t>> touched;
-    touched.reserve(2 * n);
-    auto bump = [&](int d, uint32_t v, bool del) {
-        if (h_ins_[d][v] == 0 && h_del_[d][v] == 0) touched.push_back({d, v});
-        (del ? h_del_[d][v] : h_ins_[d][v]) += 1;
-    };
-    for (uint64_t k = 0; k < n; k++) {
-        bump(0, src[k], op[k] != 0);
-        bump(opt_.directed ? 1 : 0, dst[k], op[k] != 0);
-    }
-    uint64_t need = 0;
-    std::string err;
-    for (const auto& t : touched) {
-        const int d = t.first;
-        const uint32_t v = t.second;
-        const uint32_t dels = h_del_[d][v], adds = h_ins_[d][v];
-        h_del_[d][v] = h_ins_[d][v] = 0;
-        if (dels > h_deg_[d][v]) { err = "batch precondition violated: deleted edge missing"; continue; }
-        const uint64_t nd = static_cast<uint64_t>(h_deg_[d][v] - dels) + adds;
-        uint32_t cap = h_cap_[d][v];
-        if (adds && nd > cap) {
-            uint64_t p2 = 1;
-            while (p2 < nd) p2 <<= 1;
-            cap = static_cast<uint32_t>(std::max<uint64_t>(kMinNbrSize, 2 * p2));
-            need += cap;
-        }
-        upd.push_back(MirrorUpdate{d, v, static_cast<uint32_t>(nd), cap});
-    }
-    if (!err.empty()) return err;
-    if (h_arena_top_ + need > h_arena_words_)
-        return "adjacency arena budget exhausted: the batch needs " + std::to_string(need) + " more words of list space";
-    h_pending_need_ = need;
-    return "";
-}
-
 std::string Engine::run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
                               std::vector<QueryResult>& out, BatchInfo& info) {
     if (!built_) return "index not built";
@@ -999,24 +1025,35 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
                                  std::vector<QueryResult>& out, BatchInfo& info) {
     if (n > max_batch_) return "batch larger than the scratch was sized for";
     for (auto& qs : qs_) if (!qs->planned) return "every query must be planned before matching";
+    (void)hsrc;
+    (void)hdst;
     info = BatchInfo{};
     info.edges = n;
     for (uint64_t k = 0; k < n; k++) (hop[k] ? info.deletes : info.inserts)++;
-    std::vector<MirrorUpdate> upd;
-    std::string aerr = admit(hsrc, hdst, hop, n, upd);
-    if (!aerr.empty()) return aerr;
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t max_arcs = static_cast<uint32_t>(2 * n);
     if (n == 0) { out.assign(qs_.size(), QueryResult{}); return ""; }
     prof_begin(prof_main_, s_, kKBatchBuild);
     for (int d = 0; d < ndirs_; d++) batch_build(g_, d, ndirs_ == 1 ? 0 : d + 1, bin_, static_cast<uint32_t>(lo), static_cast<uint32_t>(n), bs_[d], s_);
+    // admission before anything changes: list space of the insertions, deletions of missing arcs
+    for (int d = 0; d < ndirs_; d++) batch_admit(g_, d, max_arcs, bs_[d], adm_, s_);
+    batch_admit_finish(g_, adm_, s_);
     prof_end(prof_main_, s_, kKBatchBuild);
+    CSM_HIP(hipMemcpyAsync(h_pinned_, adm_ + 2, 32, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    unsigned long long a[4];
+    std::memcpy(a, h_pinned_, 32);
+    std::string err = a[3] ? check_err("batch build") : std::string();
+    if (!err.empty()) return err;
+    if (a[1] || a[2] + a[0] > h_arena_words_) {  // rejected: drop the built batch, nothing else changed
+        for (int d = 0; d < ndirs_; d++) batch_cleanup(g_, d, max_arcs, bs_[d], s_);
+        CSM_HIP(hipStreamSynchronize(s_));
+        if (a[1]) return "batch precondition violated: deleted edge missing";
+        return "adjacency arena budget exhausted: the batch needs " + std::to_string(a[0]) + " more words of list space";
+    }
     for (int d = 0; d < ndirs_; d++)
         nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 0, s_);
     fill_u32(reinterpret_cast<uint32_t*>(results_), 0u, qs_.size() * 8, s_);
-    CSM_HIP(hipStreamSynchronize(s_));
-    std::string err = check_err("batch build");
-    if (!err.empty()) return err;
     const auto t1 = std::chrono::steady_clock::now();
     double match_ms = 0;
     if (info.deletes) {
@@ -1063,11 +1100,6 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
         out[q].neg = res[4 * q + 2];
         out[q].neg_hash = res[4 * q + 3];
     }
-    for (const MirrorUpdate& m : upd) {
-        h_deg_[m.dir][m.v] = m.deg;
-        h_cap_[m.dir][m.v] = m.cap;
-    }
-    h_arena_top_ += h_pending_need_;
     const auto t2 = std::chrono::steady_clock::now();
     info.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
     info.ms_match = match_ms;

@@ -42,6 +42,7 @@ struct EngineOptions {
     bool directed = false;
     bool split_roots = true;          // spread heavy records of the counting DFS over many waves
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
+    bool chain_levels = true;         // semi-BFS levels of a step chained on the device (one host wait per step)
     uint32_t lanes = 8;               // concurrent matching steps (streams), each with its own scratch
     uint64_t arena_slack_words = 1u << 20;  // adjacency arena room beyond the growth rule
 };
@@ -126,7 +127,7 @@ class Engine {
     std::string read_candidates(uint32_t qid, uint32_t u, std::vector<uint32_t>& bitmap);
     std::string read_adjacency(uint32_t v, uint32_t dir, std::vector<uint32_t>& ids, std::vector<uint32_t>& labels);
     uint32_t id_bits() const { return id_bits_; }
-    uint64_t arena_used_words() const { return h_arena_top_; }
+    uint64_t arena_used_words();  // words of the adjacency arena in use (device bump pointer)
     uint64_t arena_words() const { return h_arena_words_; }
     uint32_t last_error_bits() const { return last_err_; }
 
@@ -184,6 +185,7 @@ class Engine {
         CpScratch cp{};
         LevelScratch level{};          // offsets, hashes and scan scratch shared with cp
         uint32_t* h_pinned = nullptr;
+        uint32_t* chain_ctr = nullptr; // [2 * kMaxQV] per-level {count, overflow} of a device chain
         hipEvent_t done = nullptr;     // end of the lane's work in a phase
         Prof prof;
         std::string trace_line;
@@ -191,15 +193,11 @@ class Engine {
         unsigned long long* work_counter() { return reinterpret_cast<unsigned long long*>(counters + 8); }
     };
 
-    struct MirrorUpdate {
-        int dir;
-        uint32_t v, deg, cap;
-    };
-    std::string admit(const uint32_t* src, const uint32_t* dst, const uint8_t* op, uint64_t n, std::vector<MirrorUpdate>& upd);
     std::string run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, const uint32_t* hdst, const uint8_t* hop,
                              std::vector<QueryResult>& out, BatchInfo& info);
     std::string run_phase(uint32_t ph);
     std::string run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph);
+    std::string search_chained(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot);
     std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,
                        uint64_t top, uint32_t target, uint32_t res_slot);
     std::string extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, uint32_t count, uint32_t depth, uint32_t end, uint32_t* out,
@@ -241,9 +239,8 @@ class Engine {
     std::vector<Lane> lanes_;
     std::vector<double> step_cost_[2];        // per phase, per (query, edge): host seconds in the last batch
     // host mirrors of list degrees and capacities per direction, and of the arena bump pointer
-    std::vector<uint32_t> h_deg_[2], h_cap_[2];
-    std::vector<uint32_t> h_ins_[2], h_del_[2];  // per-batch arc counts of the admission check (kept zeroed)
-    uint64_t h_arena_top_ = 0, h_arena_words_ = 0, h_pending_need_ = 0;
+    uint64_t h_arena_words_ = 0;
+    unsigned long long* adm_ = nullptr;  // [8] batch admission words (batch_admit)
     uint64_t ring_words_ = 0;                 // words of each lane's ring
     hipEvent_t phase_start_ = nullptr;
     unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash

@@ -335,7 +335,8 @@ def test_variable_batches_resident_and_in_memory(lib, tmp_path):
 
 def test_arena_admission(lib, tmp_path, monkeypatch):
     """A batch whose grown lists do not fit the adjacency arena is refused (CSM_ERR_MEMORY) before
-    any device work; the graph is then exactly the admitted batches' result."""
+    anything changes (the device admission check runs right after the batch is built); the graph
+    is then exactly the admitted batches' result."""
     import random
     monkeypatch.setenv("CSM_ARENA_SLACK_WORDS", "0")
     meta, queries = util.gen("tinyins", tmp_path, seed=3)

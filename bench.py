@@ -26,8 +26,10 @@ FAMILIES = ["enumerate_count", "enumerate_write", "cartesian_product", "local_in
             "cartesian_prepare", "anchors", "level_count", "level_write", "level_prepare"]
 # families whose kernels count algorithmic bytes on the device (the roofline candidates)
 BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_write"]
-KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat<0>",
-             "level_write": "k_level_flat<1>"}
+# (the level families' launches are the device-chained kernels; levels the host resumes after a
+# chain stop use k_level_flat<0>/<1>, the same kernel body, and are timed with them)
+KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat_chain<0>",
+             "level_write": "k_level_flat_chain<1>"}
 SERVING_METRIC = "serving: update edges/s and p50/p99 latency of in-memory batches (host arrays, copy included)"
 WORKLOADS = {
     "c2": "plain method on one GPU: Chung-Lu 400K vertices / ~3.4M edges, gamma 2.1, max degree ~3000, "

@@ -303,11 +303,10 @@ __device__ __forceinline__ uint32_t record_runs(const DevGraph& g, const StepDes
     return best;
 }
 
-// Block-wide exclusive scan of one value per thread over kSmallThreads threads (s_wave holds
-// kSmallThreads / 64 words); returns the exclusive prefix and the block total.
-constexpr uint32_t kSmallThreads = 1024;
-__device__ __forceinline__ unsigned long long block_scan_1024(unsigned long long v, unsigned long long* s_wave,
-                                                               unsigned long long& total) {
+// Block-wide exclusive scan of one value per thread over T threads (s_wave holds T / 64 words);
+// returns the exclusive prefix and the block total.
+template <uint32_t T>
+__device__ __forceinline__ unsigned long long block_scan(unsigned long long v, unsigned long long* s_wave, unsigned long long& total) {
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
     unsigned long long x = v;
     for (uint32_t o = 1; o < 64; o <<= 1) {
@@ -318,7 +317,7 @@ __device__ __forceinline__ unsigned long long block_scan_1024(unsigned long long
     __syncthreads();
     unsigned long long before = 0;
     total = 0;
-    for (uint32_t k = 0; k < kSmallThreads / 64; k++) {
+    for (uint32_t k = 0; k < T / 64; k++) {
         const unsigned long long t = s_wave[k];
         if (k < wv) before += t;
         total += t;
@@ -327,10 +326,14 @@ __device__ __forceinline__ unsigned long long block_scan_1024(unsigned long long
     return before + x - v;
 }
 
-// Records a level call takes per block of the multi-block preparation (two per thread).
+constexpr uint32_t kSmallThreads = 1024;
+// Multi-block level preparation: 256-thread blocks (small enough to find room beside the other
+// lanes' kernels), two records per thread.
+constexpr uint32_t kLenThreads = 256;
 constexpr uint32_t kLenItems = 2;
-constexpr uint32_t kLenTile = kSmallThreads * kLenItems;
-constexpr int kChainLenGrid = 128;    // blocks of a chained level's preparation (they loop over tiles)
+constexpr uint32_t kLenTile = kLenThreads * kLenItems;
+static_assert(kLenTile == kLevelTile, "tile totals are sized by kLevelTile");
+constexpr int kChainLenGrid = 512;    // blocks of a chained level's preparation (they loop over tiles)
 constexpr int kChainFlatGrid = 1024;  // blocks of a chained level's candidate kernel
 
 // Where level k of a device chain reads its records: false when the level does not run (an
@@ -355,7 +358,7 @@ __device__ __forceinline__ bool chain_at(const LevelChain& c, uint32_t k, uint64
 __device__ __forceinline__ void level_len_tiles(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n,
                                                 unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv, unsigned long long* ph,
                                                 unsigned long long* tile_sum, unsigned long long* bytes_out) {
-    __shared__ unsigned long long s_wave[kSmallThreads / 64];
+    __shared__ unsigned long long s_wave[kLenThreads / 64];
     const uint32_t ne = d.boff[d0 + 1] - d.boff[d0];
     const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
     unsigned long long bytes = 0;
@@ -363,7 +366,7 @@ __device__ __forceinline__ void level_len_tiles(const DevGraph& g, const StepDes
         unsigned long long sum = 0;
 #pragma unroll
         for (uint32_t j = 0; j < kLenItems; j++) {
-            const uint32_t i = tile * kLenTile + j * kSmallThreads + threadIdx.x;
+            const uint32_t i = tile * kLenTile + j * kLenThreads + threadIdx.x;
             if (i >= n) continue;
             const uint32_t l = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i,
                                            piv + i, ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
@@ -371,7 +374,7 @@ __device__ __forceinline__ void level_len_tiles(const DevGraph& g, const StepDes
             sum += l;
         }
         unsigned long long total;
-        block_scan_1024(sum, s_wave, total);
+        block_scan<kLenThreads>(sum, s_wave, total);
         if (threadIdx.x == 0) tile_sum[tile] = total;
     }
     if (bytes_out) {
@@ -380,7 +383,7 @@ __device__ __forceinline__ void level_len_tiles(const DevGraph& g, const StepDes
     }
 }
 
-__global__ void __launch_bounds__(kSmallThreads) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
+__global__ void __launch_bounds__(kLenThreads) k_level_len(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, uint32_t n,
                                                              unsigned long long* len, uint4* prun, uint4* runs, uint8_t* piv,
                                                              unsigned long long* ph, unsigned long long* tile_sum, uint32_t* zero,
                                                              unsigned long long* bytes_out) {
@@ -388,7 +391,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_level_len(DevGraph g, StepDes
     level_len_tiles(g, d, d0, rec, n, len, prun, runs, piv, ph, tile_sum, bytes_out);
 }
 
-__global__ void __launch_bounds__(kSmallThreads) k_level_len_chain(DevGraph g, StepDesc d, LevelChain c, uint32_t k, unsigned long long* len,
+__global__ void __launch_bounds__(kLenThreads) k_level_len_chain(DevGraph g, StepDesc d, LevelChain c, uint32_t k, unsigned long long* len,
                                                                    uint4* prun, uint4* runs, uint8_t* piv, unsigned long long* ph,
                                                                    unsigned long long* tile_sum, unsigned long long* bytes_out) {
     if (k == 0 && blockIdx.x == 0 && threadIdx.x < 2 * kMaxQV) c.ctr[threadIdx.x] = 0;
@@ -401,22 +404,22 @@ __global__ void __launch_bounds__(kSmallThreads) k_level_len_chain(DevGraph g, S
 // Exclusive scan of the lengths in place (total at off[n]): the block of a tile adds the totals of
 // the tiles before it (at most a few thousand words, read by the whole block) to the tile's scan.
 __device__ __forceinline__ void level_scan_tiles(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
-    __shared__ unsigned long long s_wave[kSmallThreads / 64];
+    __shared__ unsigned long long s_wave[kLenThreads / 64];
     const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         unsigned long long carry = 0, total;
-        for (uint32_t t = threadIdx.x; t < tile; t += kSmallThreads) carry += tile_sum[t];
+        for (uint32_t t = threadIdx.x; t < tile; t += kLenThreads) carry += tile_sum[t];
         carry = wave_sum_u64(carry);
         if (lane_id() == 0) s_wave[threadIdx.x >> 6] = carry;
         __syncthreads();
         carry = 0;
-        for (uint32_t w = 0; w < kSmallThreads / 64; w++) carry += s_wave[w];
+        for (uint32_t w = 0; w < kLenThreads / 64; w++) carry += s_wave[w];
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < kLenItems; j++) {
-            const uint32_t i = tile * kLenTile + j * kSmallThreads + threadIdx.x;
+            const uint32_t i = tile * kLenTile + j * kLenThreads + threadIdx.x;
             const unsigned long long v = i < n ? off[i] : 0ull;
-            const unsigned long long ex = block_scan_1024(v, s_wave, total);
+            const unsigned long long ex = block_scan<kLenThreads>(v, s_wave, total);
             if (i < n) off[i] = carry + ex;
             carry += total;
         }
@@ -424,11 +427,11 @@ __device__ __forceinline__ void level_scan_tiles(unsigned long long* off, uint32
     }
 }
 
-__global__ void __launch_bounds__(kSmallThreads) k_level_scan(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
+__global__ void __launch_bounds__(kLenThreads) k_level_scan(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
     level_scan_tiles(off, n, tile_sum);
 }
 
-__global__ void __launch_bounds__(kSmallThreads) k_level_scan_chain(LevelChain c, uint32_t k, unsigned long long* off,
+__global__ void __launch_bounds__(kLenThreads) k_level_scan_chain(LevelChain c, uint32_t k, unsigned long long* off,
                                                                     const unsigned long long* tile_sum) {
     uint64_t o;
     uint32_t n;
@@ -461,7 +464,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, S
     unsigned long long carry = 0;
     for (uint32_t j = 0; j < kSmallItems && j * kSmallThreads < n; j++) {
         unsigned long long total;
-        const unsigned long long ex = block_scan_1024(len[j], s_wave, total);
+        const unsigned long long ex = block_scan<kSmallThreads>(len[j], s_wave, total);
         const uint32_t i = j * kSmallThreads + tid;
         if (i < n) off[i] = carry + ex;
         carry += total;
@@ -1014,8 +1017,8 @@ void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint
         return;
     }
     const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
-    k_level_len<<<tiles, kSmallThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ph, ls.scan_tmp, zero, bytes);
-    k_level_scan<<<tiles, kSmallThreads, 0, s>>>(ls.off, n, ls.scan_tmp);
+    k_level_len<<<tiles, kLenThreads, 0, s>>>(g, d, d0, rec, n, ls.off, ls.prun, ls.runs, ls.piv, ph, ls.tile_sum, zero, bytes);
+    k_level_scan<<<tiles, kLenThreads, 0, s>>>(ls.off, n, ls.tile_sum);
 }
 
 void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, int mode, const LevelScratch& ls,
@@ -1032,9 +1035,9 @@ void level_flat(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_
 
 void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, const LevelScratch& ls, bool with_hash,
                          unsigned long long* bytes, hipStream_t s) {
-    k_level_len_chain<<<kChainLenGrid, kSmallThreads, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, with_hash ? ls.ph : nullptr,
-                                                             ls.scan_tmp, bytes);
-    k_level_scan_chain<<<kChainLenGrid, kSmallThreads, 0, s>>>(c, k, ls.off, ls.scan_tmp);
+    k_level_len_chain<<<kChainLenGrid, kLenThreads, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, with_hash ? ls.ph : nullptr,
+                                                           ls.tile_sum, bytes);
+    k_level_scan_chain<<<kChainLenGrid, kLenThreads, 0, s>>>(c, k, ls.off, ls.tile_sum);
 }
 
 void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,

@@ -173,13 +173,14 @@ void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_dept
 // record's prefix hash; level_flat then tests every candidate, load-balanced over the grid, and
 // counts/hashes complete embeddings (d0 + 1 == n) or writes records of depth d0 + 1.
 // n <= level_max_records(d, d0, ls).
+constexpr uint32_t kLevelTile = 512;  // records per tile of the multi-block level preparation
 struct LevelScratch {
     unsigned long long* off;       // [cap + 1] scanned candidate counts
     uint4* prun;                   // [cap] generating view run of each record
     uint4* runs;                   // [runs_cap] view runs of every backward entry (record-major; depths with one entry skip it)
     uint8_t* piv;                  // [cap] generating entry | view << 4 | dir << 6
     unsigned long long* ph;        // [cap] prefix hash of each record (counting levels)
-    unsigned long long* scan_tmp;  // [cap / 2048 + 1] tile totals
+    unsigned long long* tile_sum;  // [cap / kLevelTile + 2] candidate totals of the preparation's tiles
     uint64_t cap;
     uint64_t runs_cap;
 };

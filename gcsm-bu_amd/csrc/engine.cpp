@@ -224,6 +224,7 @@ std::string Engine::build() {
         add((cp_rec + 1) * 8); add(cp_rec * 16); add(cp_rec * 4);  // CP segments (runs shared with levels)
         add(kLevelRunsPerRecord * cp_rec * 16);                    // level: per-entry runs
         add(2 * kMaxQV * 4);                                       // level chain counters
+        add((cp_rec / kLevelTile + 2) * 8);                        // level tile totals
         add(cp_list_words * 4); add(64);
         add(64 << 10);                                             // alignment slack of the takes
     }
@@ -403,10 +404,11 @@ std::string Engine::build() {
         w.cp.err = reinterpret_cast<uint32_t*>(cpc + 1);
         // level scratch: offsets, generating runs, entry tags and hashes shared with the CP scratch
         uint4* level_runs = pool_.take<uint4>(kLevelRunsPerRecord * cp_rec);
+        unsigned long long* tile_sum = pool_.take<unsigned long long>(cp_rec / kLevelTile + 2);
         w.chain_ctr = pool_.take<uint32_t>(2 * kMaxQV);
-        if (!level_runs || !w.chain_ctr) return "device pool exhausted (lane scratch)";
+        if (!level_runs || !tile_sum || !w.chain_ctr) return "device pool exhausted (lane scratch)";
         w.level = LevelScratch{w.cp.prod, w.cp.seg_run, level_runs, reinterpret_cast<uint8_t*>(w.cp.seg_cnt), w.cp.rec_hash,
-                               w.cp.scan_tmp, cp_rec, kLevelRunsPerRecord * cp_rec};
+                               tile_sum, cp_rec, kLevelRunsPerRecord * cp_rec};
     }
     opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
     results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);

@@ -334,7 +334,7 @@ constexpr uint32_t kLenItems = 2;
 constexpr uint32_t kLenTile = kLenThreads * kLenItems;
 static_assert(kLenTile == kLevelTile, "tile totals are sized by kLevelTile");
 constexpr int kChainLenGrid = 512;    // blocks of a chained level's preparation (they loop over tiles)
-constexpr int kChainFlatGrid = 1024;  // blocks of a chained level's candidate kernel
+constexpr int kChainFlatGrid = 768;  // blocks of a chained level's candidate kernel (three per CU)
 
 // Where level k of a device chain reads its records: false when the level does not run (an
 // earlier level stopped the chain or overflowed, or its own input is empty, larger than one call
@@ -1042,7 +1042,9 @@ void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain&
 
 void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,
                       MatchOut out, hipStream_t s) {
-    // one full residency of four-wave blocks at four waves per SIMD: level sizes are device-side
+    // Level sizes are device-side, so the grid is fixed: three four-wave blocks per CU, short of a
+    // full residency (four per CU) so the other lanes' kernels keep finding room (measured on C2:
+    // 512-768 blocks beat 256, 1024 and 2048)
     if (mode == kCountAtEnd)
         k_level_flat_chain<kCountAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
     else

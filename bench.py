@@ -35,6 +35,8 @@ WORKLOADS = {
     "c2": "plain method on one GPU: Chung-Lu 400K vertices / ~3.4M edges, gamma 2.1, max degree ~3000, "
           "Zipf(1.0) labels over 32, 10% held out, 90/10 insert/delete stream, batches of 65,536 edges, "
           "6 queries of 6-8 vertices (3 sparse, 3 dense)",
+    "c4": "multi-query: the C2 graph and stream (Chung-Lu 400K vertices / ~3.4M edges, batches of 65,536 edges) "
+          "with 64 standing queries of 4-12 vertices (16 trees, 24 sparse cyclic, 24 dense)",
     "c5": "serving: directed, 3M vertices / ~25M arcs, lognormal(1.5, 1.2) out-degree, 4 vertex x 8 edge labels, "
           "10% held out, 85/15 insert/delete, lognormal batch sizes in [1K, 256K] (median 16K), "
           "32 queries of 4-8 vertices",
@@ -60,12 +62,12 @@ def ensure_built():
         m.build()
 
 
-def workload(config, seed, data_dir):
+def workload(config, seed, data_dir, scale=1.0):
     meta_path = os.path.join(data_dir, "meta.txt")
     if not os.path.exists(meta_path):
-        log(f"generating workload {config} seed {seed} into {data_dir}")
+        log(f"generating workload {config} seed {seed} scale {scale} into {data_dir}")
         t = time.time()
-        subprocess.run([TOOL, "gen", config, data_dir, "--seed", str(seed)], check=True)
+        subprocess.run([TOOL, "gen", config, data_dir, "--seed", str(seed), "--scale", str(scale)], check=True)
         log(f"generated in {time.time() - t:.1f}s")
     meta = dict(line.split() for line in open(meta_path))
     queries = [os.path.join(data_dir, f"q{k}.graph") for k in range(int(meta["queries"]))]
@@ -190,8 +192,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c2"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c4"],
+                    help="c2: the headline workload; c4: 64 standing queries on the same graph and stream, also "
+                         "timed with one lane (the steps one after another) for comparison")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="graph and stream scale of the c4 workload (c2 and serving always run at full size)")
     ap.add_argument("--data-dir", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -214,8 +220,11 @@ def main():
     ffi = load_ffi()
     if args.serving:
         args.config = "c5"
-    data_dir = args.data_dir or os.path.join("/tmp", f"csm_bench_{args.config}_s{args.seed}")
-    meta, queries = workload(args.config, args.seed, data_dir)
+    if args.config != "c4":
+        args.scale = 1.0
+    tag = "" if args.scale == 1.0 else f"_x{args.scale:g}"
+    data_dir = args.data_dir or os.path.join("/tmp", f"csm_bench_{args.config}_s{args.seed}{tag}")
+    meta, queries = workload(args.config, args.seed, data_dir, args.scale)
     batch = int(meta["batch_size"])
     if args.serving:
         print(json.dumps(serving(ffi, data_dir, queries, meta, args.serving_batches + 1)))
@@ -272,6 +281,19 @@ def main():
         calib = {"kernel": "k_copy_u32", "bytes_read": 4 * words, "bytes_written": 4 * words, "ms": ms,
                  "GB/s": round(8 * words / (ms / 1e3) / 1e9, 1)}
     s.close()
+    one_lane = None
+    if args.config == "c4":  # the same batches with one lane: every (query, edge) step in turn
+        s1, _ = open_session(ffi, data_dir, queries, batch, lanes=1)
+        for b in range(args.warmup):
+            s1.apply_batch(b)
+        t1 = []
+        for b in range(args.warmup, args.warmup + args.steps):
+            t0 = time.perf_counter()
+            s1.apply_batch(b % nb)
+            s1.synchronize()
+            t1.append(time.perf_counter() - t0)
+        s1.close()
+        one_lane = {"ms_per_step": sum(t1) * 1e3 / len(t1), "lanes": 1}
     roofline = roofline_pass(ffi, data_dir, queries, batch, args.config)
     cpu = None if args.no_cpu_baseline else cpu_baseline(data_dir, queries, batch, args.cpu_seconds)
     lat_ms = sorted(x * 1e3 for x in lat)
@@ -281,13 +303,16 @@ def main():
         "metric": METRIC, "value": matches / total if total > 0 else 0.0, "unit": "matches/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": total * 1e3 / max(1, args.steps),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.config], "config": args.config, "seed": args.seed, "batch_edges": batch,
+        "config": {"workload": WORKLOADS[args.config], "config": args.config, "seed": args.seed, "scale": args.scale,
+                   "batch_edges": batch,
                    "queries": len(queries), "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP)",
                    "lanes": int(os.environ.get("CSM_LANES", "8"))},
         "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": p50, "p99_batch_ms": p99,
         "matches": matches, "per_query_matches": per_query,
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if one_lane:
+        out["one_lane"] = one_lane
     if stats:
         out["kernel_ms"] = {k: round(v[1], 3) for k, v in stats.items()}
     if calib:

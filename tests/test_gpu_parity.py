@@ -175,18 +175,19 @@ def test_adjacency_after_stream(lib, tmp_path):
             assert len(set(got)) == len(got)
 
 
-@pytest.mark.parametrize("split,sdir,lanes", [(0, 0, 1), (1, 0, 3), (0, 1, 8), (1, 1, 16)])
-def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, lanes):
-    """Root splitting, the label-slot directory and the number of concurrent step lanes change
-    scheduling and lookup, never results."""
+@pytest.mark.parametrize("split,sdir,lanes,chain", [(0, 0, 1, 0), (1, 0, 3, 1), (0, 1, 8, 0), (1, 1, 16, 1)])
+def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, lanes, chain):
+    """Root splitting, the label-slot directory, the number of concurrent step lanes and chaining
+    a step's levels on the device change scheduling and lookup, never results."""
     monkeypatch.setenv("CSM_SPLIT_ROOTS", str(split))
     monkeypatch.setenv("CSM_SLOT_DIRECTORY", str(sdir))
     monkeypatch.setenv("CSM_LANES", str(lanes))
+    monkeypatch.setenv("CSM_CHAIN_LEVELS", str(chain))
     meta, queries = util.gen("tinyins", tmp_path, seed=7)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
     want, _ = util.oracle_stream(data, upd, 64, queries)
     for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 2, True, False), (0, 1, True, True)]:
-        say(f"case split={split} sdir={sdir} lanes={lanes} order={mo} strategy={ss} local={li} cp={cp}")
+        say(f"case split={split} sdir={sdir} lanes={lanes} chain={chain} order={mo} strategy={ss} local={li} cp={cp}")
         got = run_library(lib, str(data), str(upd), 64, queries, lib.technique(mo, ss, li, cp))
         if got is not None:
             assert got == want
@@ -194,7 +195,7 @@ def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir,
     meta, queries = util.gen("c1", tmp_path / "c1", seed=3)
     data, upd = tmp_path / "c1" / "data.graph", tmp_path / "c1" / "update.stream"
     want, _ = util.oracle_stream(data, upd, 2000, queries, threads=8)
-    say(f"case split={split} sdir={sdir} lanes={lanes} c1 batch=2000")
+    say(f"case split={split} sdir={sdir} lanes={lanes} chain={chain} c1 batch=2000")
     assert run_library(lib, str(data), str(upd), 2000, queries, lib.technique()) == want
 
 
@@ -226,18 +227,21 @@ def test_directed_stream(lib, tmp_path, seed):
             assert got == want
 
 
-def test_small_ring_and_record_chunks(lib, tmp_path, monkeypatch):
+@pytest.mark.parametrize("chain", [1, 0])
+def test_small_ring_and_record_chunks(lib, tmp_path, monkeypatch, chain):
     """Levels that do not fit the ring are redone for halves of their input records (a single
     record that does not fit is counted depth-first), and levels of more records than the
-    per-launch scratch are processed in chunks: same results."""
+    per-launch scratch are processed in chunks: same results.  With the levels chained on the
+    device, both make the chain stop and the host resume from that level."""
     monkeypatch.setenv("CSM_CP_MAX_RECORDS", "1024")
+    monkeypatch.setenv("CSM_CHAIN_LEVELS", str(chain))
     meta, queries = util.gen("c2", tmp_path, seed=6, scale=0.02)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
     batch = int(meta["batch_size"])
     want, _ = util.oracle_stream(data, upd, batch, queries, threads=16)
     for ring in [0, 4 << 20]:
         for (mo, ss, li, cp) in [(2, 2, True, True), (2, 3, True, False)]:
-            say(f"case c2 scale=0.02 seed=6 ring={ring} cp_max_records=1024 order={mo} strategy={ss} cp={cp}")
+            say(f"case c2 scale=0.02 seed=6 ring={ring} cp_max_records=1024 chain={chain} order={mo} strategy={ss} cp={cp}")
             got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp), ring_bytes=ring)
             assert got == want
 

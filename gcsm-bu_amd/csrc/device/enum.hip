@@ -834,7 +834,9 @@ __global__ void __launch_bounds__(256) k_cp_filter(DevGraph g, StepDesc d, uint3
         if (lane == 0) win[64] = off[min(rb + 64, nseg)];
         wave_sync();
         const unsigned long long t = t0 + lane;
-        uint32_t i = rb;
+        uint32_t i = rb, x = 0;
+        unsigned long long base = 0;
+        bool ok = false;
         if (t < w1) {
             if (win[64] <= t) {
                 uint32_t lo = rb + 64, hi = nseg;
@@ -858,14 +860,29 @@ __global__ void __launch_bounds__(256) k_cp_filter(DevGraph g, StepDesc d, uint3
             const uint32_t* dl = phase_delta(g, be.dir, d.phase);
             const uint4 x4 = c.seg_run[i];
             const RunView rv{x4.x, x4.y, x4.z, x4.w};
-            const unsigned long long base = i - rb < 64 ? win[i - rb] : off[i];
+            base = i - rb < 64 ? win[i - rb] : off[i];
             const uint32_t key = run_elem(g, dl, rv, run_base_len(rv, be.view), static_cast<uint32_t>(t - base));
-            const uint32_t x = key & g.id_mask;
-            bool ok = bit_test(d.filter[d0 + kk], x) && run_elem_live(dl, rv, be.view, key);
+            x = key & g.id_mask;
+            ok = bit_test(d.filter[d0 + kk], x) && run_elem_live(dl, rv, be.view, key);
             const uint32_t* f = rec + static_cast<uint64_t>(r) * d0;
             for (uint32_t im = d.inj[d0 + kk] & ((1u << d0) - 1u); im && ok; im &= im - 1) ok = f[__ffs(im) - 1] != x;
-            if (ok) c.lists[base + atomicAdd(&c.seg_cnt[i], 1u)] = x;
         }
+        // One counter reservation per segment per wave: the lanes of a segment are consecutive
+        // (i never decreases along the lanes), so each group's first lane reserves for the group.
+        const uint64_t pm = __ballot(ok);
+        const uint64_t starts = __ballot(lane == 0 || __shfl_up(i, 1, 64) != i);
+        const uint64_t lt = (1ull << lane) - 1ull;
+        const uint32_t leader = 63u - static_cast<uint32_t>(__clzll(starts & (lt | (1ull << lane))));
+        const uint64_t after = starts & ~(lt | (1ull << lane));  // group starts above this lane
+        const uint64_t grp_end = after ? (after & (0ull - after)) : 0ull;  // lowest later start (0: lane 63 ends)
+        const uint64_t grp = (grp_end ? grp_end - 1ull : ~0ull) & ~((1ull << leader) - 1ull);
+        uint32_t wb = 0;
+        if (lane == leader) {
+            const uint32_t cnt = static_cast<uint32_t>(__popcll(pm & grp));
+            if (cnt) wb = atomicAdd(&c.seg_cnt[i], cnt);
+        }
+        wb = __shfl(wb, static_cast<int>(leader), 64);
+        if (ok) c.lists[base + wb + static_cast<uint32_t>(__popcll(pm & lt & grp))] = x;
         rb = uniform(__shfl(i, 63, 64));
         wave_sync();
     }

@@ -143,10 +143,24 @@ __global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase
         const uint32_t x = frontier[i];
         const RunView r = make_run(g, phase, view, dir, x, slot);
         const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
-        for (uint32_t t = lane; t < tot; t += 64) {
-            const uint32_t key = run_elem(g, dl, r, nb, t);
-            const uint32_t y = key & g.id_mask;
-            if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) vset_add(out, y);
+        for (uint32_t t0 = 0; t0 < tot; t0 += 64) {  // wave-uniform trip count: one list reservation per pass
+            const uint32_t t = t0 + lane;
+            bool fresh = false;
+            uint32_t y = 0;
+            if (t < tot) {
+                const uint32_t key = run_elem(g, dl, r, nb, t);
+                y = key & g.id_mask;
+                if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) {
+                    const uint32_t bit = 1u << (y & 31);
+                    fresh = !(atomicOr(&out.bitmap[y >> 5], bit) & bit);
+                }
+            }
+            const uint64_t m = __ballot(fresh);
+            if (!m) continue;
+            uint32_t p = 0;
+            if (lane == 0) p = atomicAdd(out.len, static_cast<uint32_t>(__popcll(m)));
+            p = __shfl(p, 0, 64) + lanes_below(m);
+            if (fresh && p < out.cap) out.list[p] = y;
         }
     }
 }
@@ -167,13 +181,24 @@ __global__ void __launch_bounds__(256) k_local_queue(DevGraph g, uint32_t phase,
         const uint32_t x = frontier[i];
         const RunView r = make_run(g, phase, view, dir, x, slot);
         const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
-        for (uint32_t t = lane; t < tot; t += 64) {
-            const uint32_t key = run_elem(g, dl, r, nb, t);
-            const uint32_t y = key & g.id_mask;
-            if (!bit_test(cand_u, y) || !run_elem_live(dl, r, view, key)) continue;
-            const uint32_t bit = 1u << (y & 31);
-            if (atomicOr(&bitmap[y >> 5], bit) & bit) continue;  // queued by another visitor
-            queue[atomicAdd(queue_len, 1u)] = y;
+        for (uint32_t t0 = 0; t0 < tot; t0 += 64) {  // wave-uniform trip count: one queue reservation per pass
+            const uint32_t t = t0 + lane;
+            bool fresh = false;
+            uint32_t y = 0;
+            if (t < tot) {
+                const uint32_t key = run_elem(g, dl, r, nb, t);
+                y = key & g.id_mask;
+                if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) {
+                    const uint32_t bit = 1u << (y & 31);
+                    fresh = !(atomicOr(&bitmap[y >> 5], bit) & bit);  // else queued by another visitor
+                }
+            }
+            const uint64_t m = __ballot(fresh);
+            if (!m) continue;
+            uint32_t p = 0;
+            if (lane == 0) p = atomicAdd(queue_len, static_cast<uint32_t>(__popcll(m)));
+            p = __shfl(p, 0, 64);
+            if (fresh) queue[p + lanes_below(m)] = y;
         }
     }
 }

@@ -697,9 +697,17 @@ __device__ __forceinline__ void level_flat_body(const DevGraph& g, const StepDes
         wave_sync();                                    // the window is rewritten next iteration
     }
     if (kMode == kCountAtEnd) {
+        // one pair of device atomics per block (they serialise on one address across the grid)
+        __shared__ unsigned long long s_cnt[4], s_hash[4];
         cnt = wave_sum_u64(cnt);
         hsum = wave_sum_u64(hsum);
-        if (lane == 0 && cnt) { atomicAdd(out.count, cnt); atomicAdd(out.hash, hsum); }
+        if (lane == 0) { s_cnt[threadIdx.x >> 6] = cnt; s_hash[threadIdx.x >> 6] = hsum; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long c = 0, h = 0;
+            for (uint32_t w = 0; w < (blockDim.x >> 6); w++) { c += s_cnt[w]; h += s_hash[w]; }
+            if (c) { atomicAdd(out.count, c); atomicAdd(out.hash, h); }
+        }
     }
     if (out.bytes) {
         bytes = wave_sum_u64(bytes);

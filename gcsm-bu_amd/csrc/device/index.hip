@@ -205,9 +205,22 @@ __global__ void __launch_bounds__(256) k_local_queue(DevGraph g, uint32_t phase,
 
 __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase, const uint32_t* queue, const uint32_t* queue_len,
                                                      uint32_t max_len, LocalFilters fl, VSet out) {
+    // survivors are staged per wave and reserved 64 at a time (one atomic on the list length)
+    __shared__ uint32_t s_keep[4][64];
     const uint32_t lane = lane_id();
     const uint32_t wpb = blockDim.x >> 6;
     const uint32_t n = min(*queue_len, max_len);
+    uint32_t* keep = s_keep[threadIdx.x >> 6];
+    uint32_t kept = 0;
+    auto flush = [&]() {
+        wave_sync();
+        uint32_t p = 0;
+        if (lane == 0) p = atomicAdd(out.len, kept);
+        p = __shfl(p, 0, 64) + lane;
+        if (lane < kept && p < out.cap) out.list[p] = keep[lane];
+        kept = 0;
+        wave_sync();
+    };
     for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb) {
         const uint32_t y = queue[i];
         bool pass = true;
@@ -228,15 +241,14 @@ __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase,
             }
             pass = hit;
         }
-        if (lane == 0) {
-            if (pass) {
-                const uint32_t p = atomicAdd(out.len, 1u);
-                if (p < out.cap) out.list[p] = y;
-            } else {
-                atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
-            }
+        if (pass) {
+            if (lane == 0) keep[kept] = y;
+            if (++kept == 64) flush();
+        } else if (lane == 0) {
+            atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
         }
     }
+    if (kept) flush();
 }
 
 // Clears the bitmaps of every set in `mask` (whole words of its listed members).

@@ -8,12 +8,18 @@
 //                                      end_depth (a BFS level when end_depth = d0 + 1).  Replaces
 //                                      extendBFSDFSRegTwo (enumeration.cu:407-781) and gamma_enumerate
 //                                      (gamma_enumeration.cu:359-880).
+//   level_prepare / level_flat         one breadth-first level flattened over its candidates: per
+//   (and the _chain variants)          record, the runs of every backward entry and the shortest as
+//                                      the generator, then eight candidates per lane tested in
+//                                      lock-step; the _chain variants read their size from the
+//                                      levels before them, so a step's levels run without the host.
 //   cp_prepare / cp_enumerate          Cartesian-product flattening of trailing tree leaves
 //                                      (GetNumTree + enumerateCartesianProductTree,
 //                                      cartesian_product.cu:193-284).
-// Candidates of depth d come from the pivot backward neighbour's view run; each lane tests one
-// candidate: candidate bitmap, injectivity against the matched prefix, and membership in every other
-// backward neighbour's view run by binary search (search.cuh:7-25 semantics).
+// In the depth-first kernel, candidates of depth d come from the pivot backward neighbour's view
+// run; each lane tests one candidate: candidate bitmap, injectivity against the matched prefix, and
+// membership in every other backward neighbour's view run by binary search (search.cuh:7-25
+// semantics).
 #include "kernels.h"
 #include "view.h"
 

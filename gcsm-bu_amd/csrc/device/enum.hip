@@ -205,10 +205,17 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
             item = __shfl(t2, 0, 64);
         }
     }
-    if (kMode == kCountAtEnd) {
+    if (kMode == kCountAtEnd) {  // one pair of device atomics per block
+        __shared__ unsigned long long s_cnt[kWaves], s_hash[kWaves];
         my_cnt = wave_sum_u64(my_cnt);
         my_hash = wave_sum_u64(my_hash);
-        if (lane == 0 && my_cnt) { atomicAdd(out.count, my_cnt); atomicAdd(out.hash, my_hash); }
+        if (lane == 0) { s_cnt[threadIdx.x >> 6] = my_cnt; s_hash[threadIdx.x >> 6] = my_hash; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long c = 0, h = 0;
+            for (uint32_t w = 0; w < kWaves; w++) { c += s_cnt[w]; h += s_hash[w]; }
+            if (c) { atomicAdd(out.count, c); atomicAdd(out.hash, h); }
+        }
     }
     if (out.bytes && lane == 0 && my_bytes) atomicAdd(out.bytes, my_bytes);
 }

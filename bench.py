@@ -35,7 +35,7 @@ WORKLOADS = {
     "c2": "plain method on one GPU: Chung-Lu 400K vertices / ~3.4M edges, gamma 2.1, max degree ~3000, "
           "Zipf(1.0) labels over 32, 10% held out, 90/10 insert/delete stream, batches of 65,536 edges, "
           "6 queries of 6-8 vertices (3 sparse, 3 dense)",
-    "c4": "multi-query: the C2 graph and stream (Chung-Lu 400K vertices / ~3.4M edges, batches of 65,536 edges) "
+    "c4": "multi-query: the c2 graph and stream (Chung-Lu 400K vertices / ~3.4M edges, batches of 65,536 edges) "
           "with 64 standing queries of 4-12 vertices (16 trees, 24 sparse cyclic, 24 dense)",
     "c5": "serving: directed, 3M vertices / ~25M arcs, lognormal(1.5, 1.2) out-degree, 4 vertex x 8 edge labels, "
           "10% held out, 85/15 insert/delete, lognormal batch sizes in [1K, 256K] (median 16K), "

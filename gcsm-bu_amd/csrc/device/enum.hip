@@ -1081,8 +1081,8 @@ void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain&
 void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,
                       MatchOut out, hipStream_t s) {
     // Level sizes are device-side, so the grid is fixed: three four-wave blocks per CU, short of a
-    // full residency (four per CU) so the other lanes' kernels keep finding room (measured on C2:
-    // 512-768 blocks beat 256, 1024 and 2048)
+    // full residency (four per CU) so the other lanes' kernels keep finding room (measured on a
+    // 400K-vertex Chung-Lu graph with 65,536-edge batches: 512-768 blocks beat 256, 1024 and 2048)
     if (mode == kCountAtEnd)
         k_level_flat_chain<kCountAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
     else

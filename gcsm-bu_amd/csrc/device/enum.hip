@@ -10,7 +10,7 @@
 //                                      (gamma_enumeration.cu:359-880).
 //   level_prepare / level_flat         one breadth-first level flattened over its candidates: per
 //   (and the _chain variants)          record, the runs of every backward entry and the shortest as
-//                                      the generator, then eight candidates per lane tested in
+//                                      the generator, then four candidates per lane tested in
 //                                      lock-step; the _chain variants read their size from the
 //                                      levels before them, so a step's levels run without the host.
 //   cp_prepare / cp_enumerate          Cartesian-product flattening of trailing tree leaves
@@ -347,7 +347,7 @@ constexpr uint32_t kLenItems = 2;
 constexpr uint32_t kLenTile = kLenThreads * kLenItems;
 static_assert(kLenTile == kLevelTile, "tile totals are sized by kLevelTile");
 constexpr int kChainLenGrid = 512;    // blocks of a chained level's preparation (they loop over tiles)
-constexpr int kChainFlatGrid = 768;  // blocks of a chained level's candidate kernel (three per CU)
+constexpr int kChainFlatGrid = 1024;  // blocks of a chained level's candidate kernel (four per CU)
 
 // Where level k of a device chain reads its records: false when the level does not run (an
 // earlier level stopped the chain or overflowed, or its own input is empty, larger than one call
@@ -494,8 +494,10 @@ __global__ void __launch_bounds__(kSmallThreads) k_level_len_small(DevGraph g, S
 // candidates it loads the next kLevelStep + 1 record offsets into LDS (coalesced) and each lane
 // finds the records of its kLevelIlp candidates there (a global search only past the window, when
 // records have under one candidate each).  The kLevelIlp candidates of a lane are tested in
-// lock-step so their independent loads are in flight together: the kernel is latency-bound.
-constexpr uint32_t kLevelIlp = 8;
+// lock-step so their independent loads are in flight together: the kernel is latency-bound.  Four
+// candidates per lane keep the kernel at 76 VGPRs, six waves per SIMD; with the lock-step searches
+// that beat eight (four waves per SIMD) and two on the 400K-vertex Chung-Lu workload.
+constexpr uint32_t kLevelIlp = 4;
 constexpr uint32_t kLevelStep = 64 * kLevelIlp;
 static_assert((kLevelStep & (kLevelStep - 1)) == 0, "the window search takes power-of-two steps");
 
@@ -1080,9 +1082,10 @@ void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain&
 
 void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,
                       MatchOut out, hipStream_t s) {
-    // Level sizes are device-side, so the grid is fixed: three four-wave blocks per CU, short of a
-    // full residency (four per CU) so the other lanes' kernels keep finding room (measured on a
-    // 400K-vertex Chung-Lu graph with 65,536-edge batches: 512-768 blocks beat 256, 1024 and 2048)
+    // Level sizes are device-side, so the grid is fixed: four four-wave blocks per CU, short of a
+    // full residency (six per CU at four candidates per lane) so the other lanes' kernels keep
+    // finding room (measured on a 400K-vertex Chung-Lu graph with 65,536-edge batches: 1024 blocks
+    // beat 768 and 1536 at four candidates per lane, as 512-768 beat 256, 1024 and 2048 at eight)
     if (mode == kCountAtEnd)
         k_level_flat_chain<kCountAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
     else

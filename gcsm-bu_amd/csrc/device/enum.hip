@@ -376,15 +376,33 @@ __device__ __forceinline__ void level_len_tiles(const DevGraph& g, const StepDes
     const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
     unsigned long long bytes = 0;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        unsigned long long sum = 0;
+        unsigned long long l[kLenItems], sum = 0;
 #pragma unroll
         for (uint32_t j = 0; j < kLenItems; j++) {
             const uint32_t i = tile * kLenTile + j * kLenThreads + threadIdx.x;
+            l[j] = 0;
             if (i >= n) continue;
-            const uint32_t l = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i,
-                                           piv + i, ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
-            len[i] = l;
-            sum += l;
+            l[j] = record_runs(g, d, d0, rec + static_cast<uint64_t>(i) * d0, runs + static_cast<uint64_t>(i) * ne, prun + i, piv + i,
+                               ph ? ph + i : nullptr, bytes, bytes_out != nullptr);
+            sum += l[j];
+        }
+        if (tiles == 1) {  // a one-tile level is scanned here (item rows in record order); level_scan skips it
+            unsigned long long carry = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kLenItems; j++) {
+                unsigned long long row;
+                const unsigned long long ex = block_scan<kLenThreads>(l[j], s_wave, row);
+                const uint32_t i = j * kLenThreads + threadIdx.x;
+                if (i < n) len[i] = carry + ex;
+                carry += row;
+            }
+            if (threadIdx.x == 0) len[n] = carry;
+            continue;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLenItems; j++) {
+            const uint32_t i = tile * kLenTile + j * kLenThreads + threadIdx.x;
+            if (i < n) len[i] = l[j];
         }
         unsigned long long total;
         block_scan<kLenThreads>(sum, s_wave, total);
@@ -419,6 +437,7 @@ __global__ void __launch_bounds__(kLenThreads) k_level_len_chain(DevGraph g, Ste
 __device__ __forceinline__ void level_scan_tiles(unsigned long long* off, uint32_t n, const unsigned long long* tile_sum) {
     __shared__ unsigned long long s_wave[kLenThreads / 64];
     const uint32_t tiles = (n + kLenTile - 1) / kLenTile;
+    if (tiles == 1) return;  // scanned by level_len_tiles
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         unsigned long long carry = 0, total;
         for (uint32_t t = threadIdx.x; t < tile; t += kLenThreads) carry += tile_sum[t];

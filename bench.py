@@ -47,11 +47,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def load_ffi():
-    spec = importlib.util.spec_from_file_location("csm_ffi", os.path.join(PKG, "csm_ffi.py"))
+def load_module(name):
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG, name + ".py"))
     m = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(m)
     return m
+
+
+def load_ffi():
+    return load_module("csm_ffi")
 
 
 def ensure_built():
@@ -74,8 +78,8 @@ def workload(config, seed, data_dir, scale=1.0):
     return meta, queries
 
 
-def open_session(ffi, data_dir, queries, batch, lanes=0):
-    s = ffi.Session(device=0, lanes=lanes)
+def open_session(ffi, data_dir, queries, batch, lanes=0, semi_bfs_threshold=0):
+    s = ffi.Session(device=0, lanes=lanes, semi_bfs_threshold=semi_bfs_threshold)
     s.load_graph(os.path.join(data_dir, "data.graph"))
     for q in queries:
         s.add_query(q)
@@ -209,6 +213,9 @@ def main():
     ap.add_argument("--serving", action="store_true",
                     help="run the in-memory serving workload (c5) instead and print its own JSON line")
     ap.add_argument("--serving-batches", type=int, default=64)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick lanes and the semi-BFS threshold first: one fresh session per candidate on "
+                         "batches 0-2 (batch 0 untimed), results checked equal; then measure with the fastest")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the one-lane roofline pass (what the committed kernel profiles cover)")
     args = ap.parse_args()
@@ -238,8 +245,16 @@ def main():
         print(json.dumps({"roofline": r}))
         return 0
 
+    opts, tuned = {}, None
+    if args.autotune:
+        at = load_module("autotune")
+        t = time.time()
+        best, table = at.tune(lambda o: open_session(ffi, data_dir, queries, batch, **o), warmup=1, batches=2)
+        opts = best.options()
+        tuned = {"chosen": opts, "candidates": table, "batches": "1-2 after warm-up batch 0"}
+        log(f"autotuned in {time.time() - t:.1f}s: {opts}")
     t = time.time()
-    s, nb = open_session(ffi, data_dir, queries, batch)
+    s, nb = open_session(ffi, data_dir, queries, batch, **opts)
     log(f"loaded + indexed + planned in {time.time() - t:.1f}s, {nb} batches, pool {s.pool_usage()[0] >> 20} MiB")
     s.profile_enable(args.profile_mask)
     nxt = 0
@@ -248,7 +263,7 @@ def main():
         nonlocal s, nxt
         if nxt >= nb:  # stream exhausted: reload (outside any timed interval)
             s.close()
-            s, _ = open_session(ffi, data_dir, queries, batch)
+            s, _ = open_session(ffi, data_dir, queries, batch, **opts)
             s.profile_enable(args.profile_mask)
             nxt = 0
         r = s.apply_batch(nxt)
@@ -306,13 +321,16 @@ def main():
         "config": {"workload": WORKLOADS[args.config], "config": args.config, "seed": args.seed, "scale": args.scale,
                    "batch_edges": batch,
                    "queries": len(queries), "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP)",
-                   "lanes": int(os.environ.get("CSM_LANES", "8"))},
+                   "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "8"))),
+                   "semi_bfs_threshold": opts.get("semi_bfs_threshold", 1 << 16)},
         "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": p50, "p99_batch_ms": p99,
         "matches": matches, "per_query_matches": per_query,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if one_lane:
         out["one_lane"] = one_lane
+    if tuned:
+        out["autotune"] = tuned
     if stats:
         out["kernel_ms"] = {k: round(v[1], 3) for k, v in stats.items()}
     if calib:

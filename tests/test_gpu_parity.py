@@ -111,6 +111,53 @@ def test_c1_batch_size_invariance(lib, tmp_path):
         assert hsh == (h1 - h0) % (1 << 64)
 
 
+@pytest.mark.parametrize("threshold", [64, 4096])
+def test_semi_bfs_threshold_changes_no_result(lib, tmp_path, threshold):
+    """The semi-BFS threshold (config.h:47 MIN_NUM_RESULTS_TO_GPU, a session option here) decides
+    where the levels stop expanding and Cartesian-product flattening starts. Thresholds this small
+    make the c1 and tiny workloads cross it; the results must still equal the oracle's."""
+    for config, seed, batch in [("c1", 1, 1000), ("tiny", 2, 37)]:
+        d = tmp_path / f"{config}{seed}"
+        d.mkdir()
+        meta, queries = util.gen(config, d, seed=seed)
+        data, upd = d / "data.graph", d / "update.stream"
+        want, _ = util.oracle_stream(data, upd, batch, queries, threads=8)
+        for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 3, True, True), (3, 2, True, True)]:
+            say(f"case {config} seed={seed} batch={batch} threshold={threshold} order={mo} strategy={ss} local={li} cp={cp}")
+            got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp),
+                              semi_bfs_threshold=threshold)
+            if got is None:
+                continue
+            assert got == want, f"threshold={threshold} order={mo} strategy={ss}"
+
+
+def test_autotune_candidates_agree(lib, tmp_path):
+    """The autotuner opens one session per candidate on the same stream prefix; their results must
+    agree (it raises otherwise), and the chosen settings reproduce the oracle's results."""
+    at = util.autotune()
+    meta, queries = util.gen("c1", tmp_path, seed=2)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = 1000
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=8)
+
+    def open_session(opts):
+        s = lib.Session(device=0, **opts)
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        nb = s.open_updates(str(upd), batch)
+        s.build()
+        for q in qids:
+            s.plan(q)
+        return s, nb
+
+    cands = [at.Candidate(8, 1 << 16), at.Candidate(3, 64), at.Candidate(1, 4096)]
+    say("case autotune c1 seed=2 batch=1000 candidates=3")
+    best, table = at.tune(open_session, cands, warmup=1, batches=2)
+    assert best in cands and len(table) == 3 and all(r["ms_per_batch"] > 0 for r in table)
+    got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(), **best.options())
+    assert got == want
+
+
 def _oracle_summary(data, query):
     out = util.run([util.ORACLE, "summary", str(data), str(query)])
     card, avg = [], []

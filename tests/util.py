@@ -28,6 +28,13 @@ def ffi():
     return mod
 
 
+def autotune():
+    spec = importlib.util.spec_from_file_location("autotune", os.path.join(PKG, "autotune.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def run(cmd, timeout=600):
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:

@@ -12,6 +12,7 @@ The settings change scheduling only, never results. Every candidate's per-batch 
 hashes of every query) must equal the first candidate's, and a difference raises `TuneMismatch`.
 """
 import dataclasses
+import os
 import time
 
 MAX_LANES = 16  # kMaxLanes of the library
@@ -44,6 +45,9 @@ def _check(candidates, warmup, batches):
             raise ValueError(f"semi_bfs_threshold {c.semi_bfs_threshold} outside 1..2^32-1")
     if warmup < 0 or batches < 1:
         raise ValueError("need warmup >= 0 and batches >= 1")
+    if os.environ.get("CSM_LANES") and len({c.lanes for c in candidates}) > 1:
+        # the library reads CSM_LANES at session creation and it overrides csm_options.lanes
+        raise ValueError("CSM_LANES is set: it would override every candidate's lanes")
 
 
 def tune(open_session, candidates=DEFAULT_CANDIDATES, warmup=1, batches=2, clock=time.perf_counter):

@@ -98,3 +98,15 @@ def test_short_stream_and_bad_candidates_rejected():
 def test_default_candidates_cover_the_library_defaults():
     assert at.Candidate(8, 1 << 16) in at.DEFAULT_CANDIDATES
     assert all(1 <= c.lanes <= at.MAX_LANES for c in at.DEFAULT_CANDIDATES)
+
+
+def test_lanes_environment_override_rejected(monkeypatch):
+    a, b = at.Candidate(4, 1 << 16), at.Candidate(8, 1 << 16)
+    clock = FakeClock()
+    open_session, log = factory(clock, {a: [0] * 2, b: [0] * 2}, [[(0, 0, 0, 0)]] * 2, nb=2)
+    monkeypatch.setenv("CSM_LANES", "4")
+    with pytest.raises(ValueError, match="CSM_LANES"):
+        at.tune(open_session, [a, b], warmup=1, batches=1, clock=clock)
+    assert log == []  # no session opened
+    best, _ = at.tune(open_session, [a], warmup=1, batches=1, clock=clock)  # one lane count: allowed
+    assert best == a

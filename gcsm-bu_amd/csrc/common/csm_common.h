@@ -18,7 +18,9 @@ constexpr uint32_t kMaxQE = 42;          // config.h:10 MAX_QE_COUNT
 constexpr uint32_t kMaxRel = 2 * kMaxQE; // directed query edges (relations)
 constexpr uint32_t kMinNbrSize = 8;      // config.h:11 MIN_NBR_SIZE
 constexpr uint32_t kNoVertex = 0xFFFFFFFFu;
-constexpr uint32_t kSemiBfsThreshold = 1u << 16;  // config.h:47 MIN_NUM_RESULTS_TO_GPU
+// config.h:47 MIN_NUM_RESULTS_TO_GPU, kept for the flattening rule it also sets (match_gpu.cu:2081);
+// where the levels stop is the session's semi_bfs_threshold
+constexpr uint32_t kSemiBfsThreshold = 1u << 16;
 
 // Relation views of one directed query edge while step i of a batch is processed
 // (technique_permutation.h:133-211 interleaving; deletions mirror it):

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/csm.h"
 #include "device/kernels.h"
 #include "host/graph_io.h"
 #include "host/plan.h"
@@ -38,7 +39,7 @@ struct EngineOptions {
     uint64_t cp_list_bytes = 512ull << 20;  // per lane
     uint32_t cp_max_records = 1u << 22;
     uint32_t max_batch_edges = 0;     // 0: the stream's batch size
-    uint32_t semi_bfs_threshold = kSemiBfsThreshold;
+    uint32_t semi_bfs_threshold = CSM_DEFAULT_SEMI_BFS_THRESHOLD;
     bool directed = false;
     bool split_roots = true;          // spread heavy records of the counting DFS over many waves
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)

@@ -6,7 +6,7 @@
 // reference's formats: "<Phase>, time (ms): X(host), Y(kernel)" (cuda_helpers.h:98-102), the
 // "Orders:" block (plan.cpp:1200-1291) and "Num Positive Matches: N" (technique_permutation.h:224).
 // Extensions: --queries a,b,c (several standing queries), --directed, --pool_mb, --ring_mb, --lanes,
-// --json <path> (one JSON record per batch and query).
+// --semi_bfs_threshold, --json <path> (one JSON record per batch and query).
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -58,7 +58,8 @@ class InputParser {
 
 const char* kRule = "--------------------------------------------------------------------";
 
-void print_info(const std::string& q, const std::string& d, const std::string& u, int dev, uint32_t bs, bool mem, const csm_technique& t) {
+void print_info(const std::string& q, const std::string& d, const std::string& u, int dev, uint32_t bs, bool mem, const csm_technique& t,
+                uint32_t semi_bfs) {
     static const char* orders[] = {"BFS Order", "Gamma Order", "RapidCSM Order", "RI Order"};
     static const char* strat[] = {"DFS w/ Work Stealing", "DFS w/o Work Stealing", "Dynamic BFS DFS", "BFS"};
     std::cout << kRule << std::endl;
@@ -73,7 +74,7 @@ void print_info(const std::string& q, const std::string& d, const std::string& u
               << "\n\tUse Relation Switch?: " << (t.relation_switch ? "w/ Relation Switch" : "w/o Use Relation Switch") << '\n';
     std::cout << kRule << std::endl;
     std::cout << "Macros (Items after the third one are for GPU configuration.):\nMAX_QV_COUNT: 16\nMAX_QE_COUNT: 42\nMIN_NBR_SIZE: 8\n"
-              << "WAVE_SIZE: 64\nMIN_NUM_RESULTS_TO_GPU: 65536\n";
+              << "WAVE_SIZE: 64\nMIN_NUM_RESULTS_TO_GPU: " << semi_bfs << '\n';
     std::cout << kRule << std::endl;
 }
 
@@ -127,7 +128,8 @@ int main(int argc, char** argv) {
         std::string item;
         while (std::getline(ss, item, ',')) if (!item.empty()) queries.push_back(item);
     }
-    print_info(query, data, update, device, batch_size, show_mem, t);
+    const uint32_t semi_bfs = p.get_u32("--semi_bfs_threshold", CSM_DEFAULT_SEMI_BFS_THRESHOLD);
+    print_info(query, data, update, device, batch_size, show_mem, t, semi_bfs);
 
     csm_options o{};
     o.device = device;
@@ -135,6 +137,7 @@ int main(int argc, char** argv) {
     o.pool_bytes = static_cast<uint64_t>(p.get_u32("--pool_mb", 0)) << 20;
     o.ring_bytes = static_cast<uint64_t>(p.get_u32("--ring_mb", 0)) << 20;
     o.lanes = p.get_u32("--lanes", 0);
+    o.semi_bfs_threshold = semi_bfs;
     csm_session* s = nullptr;
     csm_status st = csm_session_create(&o, &s);
     if (st != CSM_OK) return die(s, st);

@@ -322,7 +322,7 @@ def main():
                    "batch_edges": batch,
                    "queries": len(queries), "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP)",
                    "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "8"))),
-                   "semi_bfs_threshold": opts.get("semi_bfs_threshold", 1 << 18)},
+                   "semi_bfs_threshold": opts.get("semi_bfs_threshold", 1 << 20)},
         "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": p50, "p99_batch_ms": p99,
         "matches": matches, "per_query_matches": per_query,
         "roofline": roofline, "cpu_baseline": cpu,

@@ -11,9 +11,10 @@
 #define CSM_H_
 
 /* Records at a Cartesian-product depth at which the semi-BFS levels stop expanding, when
- * csm_options.semi_bfs_threshold is 0. The reference compiles in 65536 (config.h:47); 262144 ran
- * 1.8% faster per batch on the 400K-vertex, 65,536-edge-batch workload. Results do not depend on it. */
-#define CSM_DEFAULT_SEMI_BFS_THRESHOLD 262144u
+ * csm_options.semi_bfs_threshold is 0. The reference compiles in 65536 (config.h:47); on the
+ * 400K-vertex, 65,536-edge-batch workload 262144 ran 1.8% faster per batch and 1048576 another 1.4%.
+ * Results do not depend on it. */
+#define CSM_DEFAULT_SEMI_BFS_THRESHOLD 1048576u
 
 #include <stddef.h>
 #include <stdint.h>

@@ -1,7 +1,8 @@
 """The oracle, pinned before it is trusted.
 
 count:   brute force vs closed forms computed independently with numpy (edges, wedges, stars,
-         triangles = trace(A^3), 4-cycles from trace(A^4)), and the labelled edge count.
+         triangles = trace(A^3), 4-cycles from trace(A^4)), the labelled edge count, and labelled
+         5-10-vertex cycles (with and without a chord) against a plain depth-first walk.
 stream:  per-batch anchored counts telescope to brute-force differences, with and without
          deletions, and the sum does not depend on the batch size.
 summary: closed form == literal replay of the reference's offline UpdateGlobalIndex.
@@ -71,6 +72,62 @@ def test_labelled_edge_count(tmp_path):
     write_query(q, 2, [(0, 1)], [0, 1])
     want = sum(1 for a, b in edges if {labels[a], labels[b]} == {0, 1})
     assert util.oracle_count(g, q)[0] == want
+
+
+def _brute_cycle_embeddings(n, edges, labels, qlabels, chord=None):
+    """Injective maps of the k-cycle q0-q1-...-q(k-1)-q0 (plus an optional chord between query
+    vertices) into the graph that keep vertex labels: every sequence of distinct vertices, walked
+    by plain depth-first search. Automorphic images count separately."""
+    adj = [set() for _ in range(n)]
+    for a, b in edges:
+        adj[a].add(b)
+        adj[b].add(a)
+    k, total = len(qlabels), 0
+
+    def dfs(path, used):
+        nonlocal total
+        v = path[-1]
+        if len(path) == k:
+            if path[0] in adj[v] and (chord is None or path[chord[1]] in adj[path[chord[0]]]):
+                total += 1
+            return
+        for w in adj[v]:
+            if not used >> w & 1 and labels[w] == qlabels[len(path)]:
+                path.append(w)
+                dfs(path, used | 1 << w)
+                path.pop()
+
+    for s in range(n):
+        if labels[s] == qlabels[0]:
+            dfs([s], 1 << s)
+    return total
+
+
+@pytest.mark.parametrize("k,seed", [(5, 0), (8, 1), (8, 2), (10, 3)])
+def test_long_cycles_against_brute_force(tmp_path, k, seed):
+    """8-10-vertex cycle queries with labels (the cycle workload's query shape): the oracle's count
+    equals a brute-force walk over vertex sequences, with and without a chord."""
+    util.ensure_built()
+    rng = np.random.default_rng(seed)
+    n = 18
+    edges, _ = random_graph(rng, n, 0.4)
+    labels = [int(x) for x in rng.integers(0, 2, n)]
+    g = tmp_path / "g.graph"
+    write_graph(g, n, edges, labels)
+    qlabels = [int(x) for x in rng.integers(0, 2, k)]
+    ring = [(i, (i + 1) % k) for i in range(k)]
+    for chord in (None, (0, k // 2)):
+        q = tmp_path / f"c{k}_{chord is not None}.graph"
+        write_query(q, k, ring + ([chord] if chord else []), qlabels)
+        want = _brute_cycle_embeddings(n, edges, labels, qlabels, chord)
+        got, _ = util.oracle_count(g, q)
+        assert got == want, f"k={k} chord={chord}"
+        if chord is None:
+            assert want > 0  # the case is not vacuous
+    # planted mistake: the cycle with its closing edge dropped (an open path) must not match
+    q = tmp_path / f"p{k}.graph"
+    write_query(q, k, ring[:-1], qlabels)
+    assert util.oracle_count(g, q)[0] != _brute_cycle_embeddings(n, edges, labels, qlabels)
 
 
 def test_planted_mistake_is_caught(tmp_path):

@@ -47,3 +47,15 @@ def test_csmg_reports_errors_cleanly(tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 1
     assert "error (" in r.stdout
+
+
+def test_csmg_prints_the_semi_bfs_threshold_in_use(tmp_path):
+    """csmg prints the reference's settings block (main.cpp:22) with the threshold the session will
+    use: the header's default, or --semi_bfs_threshold."""
+    util.ensure_built()
+    text = open(util.HEADER).read()
+    default = int(re.search(r"#define CSM_DEFAULT_SEMI_BFS_THRESHOLD (\d+)u", text).group(1))
+    for extra, want in (([], default), (["--semi_bfs_threshold", "4096"], 4096)):
+        r = subprocess.run([util.CSMG, "--query", str(tmp_path / "missing"), "--data", "x", "--update", "y"] + extra,
+                           capture_output=True, text=True)
+        assert f"MIN_NUM_RESULTS_TO_GPU: {want}\n" in r.stdout

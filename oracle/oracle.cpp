@@ -220,9 +220,19 @@ struct Brute {
 enum ViewKind { kBase, kBasePlus, kDelta, kBaseMinus };
 
 struct DeltaSet {  // batch arcs of one kind (insertions or deletions) with adjacency for iteration
-    std::unordered_set<uint64_t> arcs;                     // (a << 32 | b) ^ (l * K): keyed with label
+    // Membership is exact: an arc is the pair (a << 32 | b, label) and set equality compares both
+    // members; the hash only picks the bucket.
+    struct Arc {
+        uint64_t ab;
+        uint32_t l;
+        bool operator==(const Arc& o) const { return ab == o.ab && l == o.l; }
+    };
+    struct ArcHash {
+        size_t operator()(const Arc& x) const { return static_cast<size_t>(fmix64(x.ab ^ ((uint64_t)x.l << 1))); }
+    };
+    std::unordered_set<Arc, ArcHash> arcs;
     std::unordered_map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>> out, in;
-    static uint64_t key(uint32_t a, uint32_t b, uint32_t l) { return ((uint64_t)a << 32 | b) * 31 + (uint64_t)l * 0x9E3779B97F4A7C15ull; }
+    static Arc key(uint32_t a, uint32_t b, uint32_t l) { return Arc{(uint64_t)a << 32 | b, l}; }
     void add_arc(uint32_t a, uint32_t b, uint32_t l) { arcs.insert(key(a, b, l)); out[a].push_back({b, l}); in[b].push_back({a, l}); }
     bool has(uint32_t a, uint32_t b, uint32_t l) const { return arcs.count(key(a, b, l)) != 0; }
 };

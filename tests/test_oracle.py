@@ -210,3 +210,151 @@ def test_variable_batch_sizes(tmp_path):
         c0, h0 = util.oracle_count(data, q)
         c1, h1 = util.oracle_count(final, q)
         assert (pos, hsh) == (c1 - c0, (h1 - h0) % (1 << 64))
+
+
+def _random_digraph(rng, n, p, nlabels):
+    """Arcs a -> b (a != b) drawn independently for every ordered pair, each with a uniform edge
+    label; both a -> b and b -> a may be present.  Returns the arcs and one adjacency matrix per
+    edge label."""
+    arcs = [(a, b, int(rng.integers(0, nlabels))) for a in range(n) for b in range(n) if a != b and rng.random() < p]
+    A = np.zeros((nlabels, n, n), dtype=np.int64)
+    for a, b, l in arcs:
+        A[l, a, b] = 1
+    return arcs, A
+
+
+def _write_arcs(path, n, arcs, labels=None):
+    with open(path, "w") as f:
+        for v in range(n):
+            f.write(f"v {v} {labels[v] if labels else 0}\n")
+        for a, b, l in arcs:
+            f.write(f"e {a} {b} {l}\n")
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_directed_and_edge_labelled_closed_forms(tmp_path, seed):
+    """Directed and multi-edge-label counting against matrix closed forms (A_l = arcs of label l):
+    directed 3-cycle = tr(A^3); directed 2-path with labels (l1, l2) = sum(A_l1 A_l2) -
+    tr(A_l1 A_l2) (the closed walks a -> b -> a are not injective); undirected triangle with three
+    edge labels = tr(S_0 S_1 S_2) over the symmetric per-label adjacencies.  Planted mistakes (a
+    reversed arc, a dropped edge label) change the count."""
+    util.ensure_built()
+    rng = np.random.default_rng(100 + seed)
+    n = 16
+    # directed, one edge label
+    arcs, A = _random_digraph(rng, n, 0.3, 1)
+    g = tmp_path / "d1.graph"
+    _write_arcs(g, n, arcs)
+    q = tmp_path / "dcycle3.graph"
+    _write_arcs(q, 3, [(0, 1, 0), (1, 2, 0), (2, 0, 0)])
+    A0 = A[0]
+    assert util.oracle_count(g, q, directed=True)[0] == int(np.trace(A0 @ A0 @ A0))
+    # planted mistake: one arc of the cycle reversed gives a different count
+    q = tmp_path / "dcycle3_rev.graph"
+    _write_arcs(q, 3, [(0, 1, 0), (2, 1, 0), (2, 0, 0)])
+    assert util.oracle_count(g, q, directed=True)[0] != int(np.trace(A0 @ A0 @ A0))
+    # directed, three edge labels: every labelled 2-path
+    arcs, A = _random_digraph(rng, n, 0.35, 3)
+    g = tmp_path / "d3.graph"
+    _write_arcs(g, n, arcs)
+    for l1 in range(3):
+        for l2 in range(3):
+            P = A[l1] @ A[l2]
+            want = int(P.sum() - np.trace(P))
+            q = tmp_path / f"dpath_{l1}{l2}.graph"
+            _write_arcs(q, 3, [(0, 1, l1), (1, 2, l2)])
+            assert util.oracle_count(g, q, directed=True)[0] == want, (l1, l2)
+    # planted mistake: the same path read with its edge labels ignored (all arcs of any label)
+    Aall = A.sum(0)
+    Pall = Aall @ Aall
+    q = tmp_path / "dpath_01.graph"
+    assert util.oracle_count(g, q, directed=True)[0] != int(Pall.sum() - np.trace(Pall))
+    # undirected, three edge labels: one label per unordered pair
+    pairs = [(a, b, int(rng.integers(0, 3))) for a in range(n) for b in range(a + 1, n) if rng.random() < 0.45]
+    S = np.zeros((3, n, n), dtype=np.int64)
+    for a, b, l in pairs:
+        S[l, a, b] = S[l, b, a] = 1
+    g = tmp_path / "u3.graph"
+    _write_arcs(g, n, pairs)
+    q = tmp_path / "tri012.graph"
+    _write_arcs(q, 3, [(0, 1, 0), (1, 2, 1), (0, 2, 2)])
+    want = int(np.trace(S[0] @ S[1] @ S[2]))
+    assert want > 0
+    assert util.oracle_count(g, q)[0] == want
+
+
+def _brute_directed_cycles(n, arcs, labels, qlabels, qelabels):
+    """Injective maps of the directed cycle q0 -> q1 -> ... -> q(k-1) -> q0 whose arc i carries edge
+    label qelabels[i], keeping vertex labels: a plain depth-first walk over vertex sequences."""
+    out = {}
+    for a, b, l in arcs:
+        out.setdefault(a, set()).add((b, l))
+    k, total = len(qlabels), 0
+
+    def dfs(path, used):
+        nonlocal total
+        v = path[-1]
+        if len(path) == k:
+            if (path[0], qelabels[k - 1]) in out.get(v, ()):
+                total += 1
+            return
+        for w, l in out.get(v, ()):
+            if l == qelabels[len(path) - 1] and not used >> w & 1 and labels[w] == qlabels[len(path)]:
+                path.append(w)
+                dfs(path, used | 1 << w)
+                path.pop()
+
+    for s in range(n):
+        if labels[s] == qlabels[0]:
+            dfs([s], 1 << s)
+    return total
+
+
+@pytest.mark.parametrize("k,seed", [(4, 0), (5, 1), (6, 2)])
+def test_directed_labelled_cycles_against_brute_force(tmp_path, k, seed):
+    """Directed cycles with vertex and edge labels (the serving workload's query kind): the oracle's
+    directed count equals a depth-first walk over arc sequences; a query with one edge label
+    changed is a planted mistake the walk tells apart."""
+    util.ensure_built()
+    rng = np.random.default_rng(200 + seed)
+    n = 14
+    arcs, _ = _random_digraph(rng, n, 0.45, 2)
+    labels = [int(x) for x in rng.integers(0, 2, n)]
+    g = tmp_path / "g.graph"
+    _write_arcs(g, n, arcs, labels)
+    qlabels = [int(x) for x in rng.integers(0, 2, k)]
+    qel = [int(x) for x in rng.integers(0, 2, k)]
+    q = tmp_path / "q.graph"
+    _write_arcs(q, k, [(i, (i + 1) % k, qel[i]) for i in range(k)], qlabels)
+    want = _brute_directed_cycles(n, arcs, labels, qlabels, qel)
+    assert want > 0
+    assert util.oracle_count(g, q, directed=True)[0] == want
+    wrong = list(qel)
+    wrong[0] ^= 1
+    assert _brute_directed_cycles(n, arcs, labels, qlabels, wrong) != want
+    q2 = tmp_path / "q2.graph"
+    _write_arcs(q2, k, [(i, (i + 1) % k, wrong[i]) for i in range(k)], qlabels)
+    assert util.oracle_count(g, q2, directed=True)[0] != want
+
+
+def test_directed_edge_labelled_stream_telescopes(tmp_path):
+    """The serving workload family (directed, 4 vertex x 8 edge labels, 85/15 insert/delete,
+    variable batches) at a small scale: per-batch anchored counts and hashes telescope to the
+    brute-force difference of the first and the final graph, for every query."""
+    util.ensure_built()
+    meta, queries = util.gen("c5", tmp_path, seed=2, scale=0.0005)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    final = tmp_path / "final.graph"
+    util.write_final_graph(data, upd, final, directed=True)
+    res, _ = util.oracle_stream(data, upd, 1, queries, threads=8, directed=True,
+                                extra=["--batch-sizes", str(tmp_path / "batches.txt")])
+    assert any(v[1] for v in res.values())
+    nonzero = 0
+    for k, q in enumerate(queries[:12]):
+        pos = sum(v[0] - v[1] for (b, kk), v in res.items() if kk == k)
+        hsh = sum(v[2] - v[3] for (b, kk), v in res.items() if kk == k) % (1 << 64)
+        c0, h0 = util.oracle_count(data, q, directed=True)
+        c1, h1 = util.oracle_count(final, q, directed=True)
+        assert (pos, hsh) == (c1 - c0, (h1 - h0) % (1 << 64)), f"query {k}"
+        nonzero += pos != 0
+    assert nonzero > 0

@@ -78,9 +78,14 @@ def oracle_count(data, query, directed=False):
     return int(t[1]), int(t[3], 16)
 
 
-def write_final_graph(data, update, out):
-    """Applies the whole update stream to the data graph file (plain Python, small inputs)."""
+def write_final_graph(data, update, out, directed=False):
+    """Applies the whole update stream to the data graph file (plain Python, small inputs).  Edges
+    are keyed by their unordered pair, or by the ordered pair when `directed`."""
     vs, es = [], {}
+
+    def key(a, b):
+        return (a, b) if directed else (min(a, b), max(a, b))
+
     for line in open(data):
         t = line.split()
         if not t:
@@ -89,17 +94,16 @@ def write_final_graph(data, update, out):
             vs.append(line.strip())
         elif t[0] == "e":
             a, b, l = map(int, t[1:])
-            es[(min(a, b), max(a, b))] = l
+            es[key(a, b)] = l
     for line in open(update):
         t = line.split()
         if not t:
             continue
         a, b, l = map(int, t[1:])
-        key = (min(a, b), max(a, b))
         if t[0] == "-e":
-            del es[key]
+            del es[key(a, b)]
         else:
-            es[key] = l
+            es[key(a, b)] = l
     with open(out, "w") as f:
         f.write("\n".join(vs) + "\n")
         for (a, b), l in sorted(es.items()):

@@ -43,6 +43,15 @@ WORKLOADS = {
 }
 
 
+def header_default_threshold():
+    """CSM_DEFAULT_SEMI_BFS_THRESHOLD from include/csm.h (the value a session uses when given 0)."""
+    for line in open(os.path.join(ROOT, "include", "csm.h")):
+        f = line.split()
+        if len(f) == 3 and f[0] == "#define" and f[1] == "CSM_DEFAULT_SEMI_BFS_THRESHOLD":
+            return int(f[2].rstrip("uU"))
+    raise RuntimeError("CSM_DEFAULT_SEMI_BFS_THRESHOLD missing from include/csm.h")
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -255,38 +264,42 @@ def main():
         log(f"autotuned in {time.time() - t:.1f}s: {opts}")
     t = time.time()
     s, nb = open_session(ffi, data_dir, queries, batch, **opts)
+    s.snapshot()  # the initial graph, restored whenever the stream runs out
     log(f"loaded + indexed + planned in {time.time() - t:.1f}s, {nb} batches, pool {s.pool_usage()[0] >> 20} MiB")
     s.profile_enable(args.profile_mask)
     nxt = 0
 
-    def step():
-        nonlocal s, nxt
-        if nxt >= nb:  # stream exhausted: reload (outside any timed interval)
-            s.close()
-            s, _ = open_session(ffi, data_dir, queries, batch, **opts)
-            s.profile_enable(args.profile_mask)
+    def rewind_if_done():
+        """Back to the initial graph once the stream is exhausted (device copy; never timed)."""
+        nonlocal nxt
+        if nxt >= nb:
+            s.restore()
             nxt = 0
-        r = s.apply_batch(nxt)
-        nxt += 1
-        return r
 
     for w in range(args.warmup):
-        step()
-        log(f"warmup {w} done")
+        rewind_if_done()
+        t0 = time.perf_counter()
+        s.apply_batch(nxt)
+        s.synchronize()
+        log(f"warmup {w} (batch {nxt}) done in {(time.perf_counter() - t0) * 1e3:.1f} ms")
+        nxt += 1
     s.profile_enable(args.profile_mask)  # reset counters: only timed batches below
     lat, matches, edges, per_query = [], 0, 0, [0] * len(queries)
     s.synchronize()
     for k in range(args.steps):
+        rewind_if_done()
+        b = nxt
         t0 = time.perf_counter()
-        res, info = step()
+        res, info = s.apply_batch(b)
         s.synchronize()
         dt = time.perf_counter() - t0
+        nxt += 1
         lat.append(dt)
         for i, r in enumerate(res):
             matches += r[0] + r[1]
             per_query[i] += r[0] + r[1]
         edges += info.edges
-        log(f"step {k}: {dt * 1e3:.1f} ms, {sum(r[0] + r[1] for r in res)} matches, {info.edges} edges")
+        log(f"step {k} (batch {b}): {dt * 1e3:.1f} ms, {sum(r[0] + r[1] for r in res)} matches, {info.edges} edges")
     total = sum(lat)
     stats = {FAMILIES[f]: s.profile_read(f) for f in range(len(FAMILIES)) if args.profile_mask >> f & 1}
     calib = None
@@ -322,7 +335,7 @@ def main():
                    "batch_edges": batch,
                    "queries": len(queries), "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP)",
                    "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "8"))),
-                   "semi_bfs_threshold": opts.get("semi_bfs_threshold", 1 << 20)},
+                   "semi_bfs_threshold": opts.get("semi_bfs_threshold") or header_default_threshold()},
         "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": p50, "p99_batch_ms": p99,
         "matches": matches, "per_query_matches": per_query,
         "roofline": roofline, "cpu_baseline": cpu,

@@ -17,7 +17,7 @@ EXPORTED = [
     "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_apply_range", "csm_apply_edges", "csm_arena_usage",
     "csm_get_relation_edge_counts",
     "csm_get_counts", "csm_pool_usage", "csm_read_candidates", "csm_read_adjacency",
-    "csm_synchronize", "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
+    "csm_synchronize", "csm_snapshot", "csm_restore", "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
 ]
 
 STATUS = {0: "OK", 1: "ARGUMENT", 2: "IO", 3: "FORMAT", 4: "LIMIT", 5: "DEVICE", 6: "MEMORY", 7: "STATE",
@@ -85,6 +85,8 @@ def load():
         lib.csm_read_candidates.argtypes = [vp, u32, u32, P(u32), u32]
         lib.csm_read_adjacency.argtypes = [vp, u32, u32, P(u32), u32, P(u32)]
         lib.csm_synchronize.argtypes = [vp]
+        lib.csm_snapshot.argtypes = [vp]
+        lib.csm_restore.argtypes = [vp]
         lib.csm_profile_enable.argtypes = [vp, u32]
         lib.csm_profile_read.argtypes = [vp, u32, P(u64), P(ctypes.c_double), P(u64)]
         lib.csm_calibration_copy.argtypes = [vp, u64, P(ctypes.c_double)]
@@ -228,6 +230,14 @@ class Session:
 
     def synchronize(self):
         self._check(self.lib.csm_synchronize(self.h))
+
+    def snapshot(self):
+        """Device copy of the graph state between batches (csm_snapshot)."""
+        self._check(self.lib.csm_snapshot(self.h))
+
+    def restore(self):
+        """Back to the last snapshot's graph state (csm_restore)."""
+        self._check(self.lib.csm_restore(self.h))
 
     def profile_enable(self, mask):
         self._check(self.lib.csm_profile_enable(self.h, mask))

@@ -24,29 +24,10 @@ csm_status fail(csm_session* s, csm_status st, const std::string& msg) {
     return st;
 }
 
+// The status a failing engine or parser call carries in its message tag (untagged: HIP failure).
 csm_status classify(csm_session* s, const std::string& err) {
     if (err.empty()) return CSM_OK;
-    csm_status st = CSM_ERR_DEVICE;
-    if (err.rfind("batch edge", 0) == 0) st = CSM_ERR_ARGUMENT;
-    else if (err.rfind("batch larger", 0) == 0) st = CSM_ERR_LIMIT;
-    else if (err.rfind("Failed to open", 0) == 0) st = CSM_ERR_IO;
-    else if (err.find("malformed") != std::string::npos || err.find("vertex update") != std::string::npos ||
-             err.find("undeclared") != std::string::npos || err.find("declared twice") != std::string::npos ||
-             err.find("duplicate") != std::string::npos || err.find("self-loop") != std::string::npos ||
-             err.find("contiguous") != std::string::npos || err.find("connected") != std::string::npos ||
-             err.find("less than 2") != std::string::npos)
-        st = CSM_ERR_FORMAT;
-    else if (err.find("at most 16") != std::string::npos || err.find("32-bit") != std::string::npos ||
-             err.find("2^32") != std::string::npos || err.find("out of range") != std::string::npos)
-        st = CSM_ERR_LIMIT;
-    else if (err.find("pool") != std::string::npos || err.find("arena") != std::string::npos || err.find("ring") != std::string::npos)
-        st = CSM_ERR_MEMORY;
-    else if (err.find("precondition") != std::string::npos)
-        st = CSM_ERR_PRECONDITION;
-    else if (err.find("before") != std::string::npos || err.find("already") != std::string::npos ||
-             err.find("not built") != std::string::npos || err.find("must be") != std::string::npos)
-        st = CSM_ERR_STATE;
-    return fail(s, st, err);
+    return fail(s, static_cast<csm_status>(code_of(err) - '0'), text_of(err));
 }
 
 }  // namespace
@@ -320,6 +301,16 @@ csm_status csm_calibration_copy(csm_session* s, uint64_t words, double* ms) {
     if (!err.empty()) return classify(s, err);
     if (ms) *ms = m;
     return CSM_OK;
+}
+
+csm_status csm_snapshot(csm_session* s) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.snapshot());
+}
+
+csm_status csm_restore(csm_session* s) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.restore());
 }
 
 csm_status csm_synchronize(csm_session* s) {

@@ -372,30 +372,46 @@ __global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, B
     }
 }
 
-// Admission of a built batch before anything changes: the list space its insertions will take
-// under k_apply_insertions' relocation rule (adm[0]) and whether some vertex would lose more arcs
-// than it has (adm[1]).
+// Admission of a built batch before anything changes, one wave per touched vertex: the list space
+// its insertions will take under k_apply_insertions' relocation rule (adm[0]) and whether the batch
+// breaks a precondition of the update semantics (adm[1]): an inserted arc that is already in the
+// list (this also catches a delete and re-insert of one arc in the same batch), a deleted arc that
+// is not in the list, or the same arc twice in the batch (equal neighbouring keys in the sorted
+// segments: an undirected edge given as both (a, b) and (b, a) lands in a's segment twice).
 __global__ void __launch_bounds__(256) k_admit(DevGraph g, int dir, BatchScratch w, unsigned long long* adm) {
     const uint32_t T = *w.n_touched;
     const DevAdj A = g.dir[dir];
     const DevDelta D = g.delta[dir];
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x >> 6;
     unsigned long long need = 0;
     bool bad = false;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < T; i += gridDim.x * wpb) {
         const uint32_t v = w.touched[i];
         const uint4 s = D.seg[v];
-        const uint32_t deg = A.adj[v].y;
-        if (s.w > deg) { bad = true; continue; }
-        const uint32_t nd = deg - s.w + s.y;
-        if (s.y && nd > A.cap[v]) {
-            uint32_t p2 = 1;
-            while (p2 < nd) p2 <<= 1;
-            need += max(kMinNbrSize, 2u * p2);
+        const uint2 a = A.adj[v];
+        for (uint32_t j = lane; j < s.y; j += 64) {
+            const uint32_t k = D.ins[s.x + j];
+            if (j > 0 && D.ins[s.x + j - 1] == k) bad = true;
+            if (contains_u32(g.arena, a.x, a.x + a.y, k)) bad = true;
+        }
+        for (uint32_t j = lane; j < s.w; j += 64) {
+            const uint32_t k = D.del[s.z + j];
+            if (j > 0 && D.del[s.z + j - 1] == k) bad = true;
+            if (!contains_u32(g.arena, a.x, a.x + a.y, k)) bad = true;
+        }
+        if (lane == 0 && s.w <= a.y) {
+            const uint32_t nd = a.y - s.w + s.y;
+            if (s.y && nd > A.cap[v]) {
+                uint32_t p2 = 1;
+                while (p2 < nd) p2 <<= 1;
+                need += max(kMinNbrSize, 2u * p2);
+            }
         }
     }
     need = wave_sum_u64(need);
     const bool any_bad = __ballot(bad) != 0;
-    if (lane_id() == 0) {
+    if (lane == 0) {
         if (need) atomicAdd(&adm[0], need);
         if (any_bad) atomicOr(reinterpret_cast<uint32_t*>(&adm[1]), 1u);
     }
@@ -482,7 +498,7 @@ void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, co
 }
 
 void batch_admit(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, unsigned long long* adm, hipStream_t s) {
-    const int grid = static_cast<int>(std::min<uint32_t>(4096, (max_touched + 255) / 256 + 1));
+    const int grid = static_cast<int>(std::min<uint32_t>(8192, (max_touched + 3) / 4 + 1));
     k_admit<<<grid, 256, 0, s>>>(g, dir, w, adm);
 }
 

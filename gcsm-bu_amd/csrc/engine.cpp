@@ -25,11 +25,11 @@ DevicePool::~DevicePool() {
 }
 
 std::string DevicePool::reserve(uint64_t bytes) {
-    if (base_) return "device pool already reserved";
+    if (base_) return tagged(Code::kState, "device pool already reserved");
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&base_), bytes);
     if (e != hipSuccess) {
         base_ = nullptr;
-        return std::string("device pool allocation of ") + std::to_string(bytes >> 20) + " MiB failed: " + hipGetErrorString(e);
+        return tagged(Code::kMemory, std::string("device pool allocation of ") + std::to_string(bytes >> 20) + " MiB failed: " + hipGetErrorString(e));
     }
     cap_ = bytes;
     used_ = 0;
@@ -59,6 +59,7 @@ Engine::~Engine() {
         if (w.s) (void)hipStreamDestroy(w.s);
     }
     if (phase_start_) (void)hipEventDestroy(phase_start_);
+    if (snap_) (void)hipFree(snap_);
     if (trace_) std::fclose(trace_);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
     if (s_) (void)hipStreamDestroy(s_);
@@ -68,8 +69,8 @@ std::string Engine::init(const EngineOptions& o) {
     opt_ = o;
     int n = 0;
     CSM_HIP(hipGetDeviceCount(&n));
-    if (n == 0) return "no HIP device available";
-    if (o.device < 0 || o.device >= n) return "device id out of range";
+    if (n == 0) return tagged(Code::kDevice, "no HIP device available");
+    if (o.device < 0 || o.device >= n) return tagged(Code::kArgument, "device id out of range");
     CSM_HIP(hipSetDevice(o.device));
     CSM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096, hipHostMallocDefault));
@@ -82,25 +83,27 @@ std::string Engine::init(const EngineOptions& o) {
     if (const char* t = std::getenv("CSM_CHAIN_LEVELS")) opt_.chain_levels = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_ARENA_SLACK_WORDS")) opt_.arena_slack_words = std::strtoull(t, nullptr, 10);
     if (const char* t = std::getenv("CSM_CP_MAX_RECORDS")) opt_.cp_max_records = static_cast<uint32_t>(std::max(1024L, std::atol(t)));
+    // test hook: an exact (possibly tiny) partial-result ring per lane, in words
+    if (const char* t = std::getenv("CSM_RING_WORDS")) ring_words_exact_ = std::max<uint64_t>(8, std::strtoull(t, nullptr, 10)) & ~1ull;
     return "";
 }
 
 std::string Engine::set_graph(const DataGraph& g) {
-    if (built_) return "graph already uploaded";
+    if (built_) return tagged(Code::kState, "graph already uploaded");
     host_g_ = g;
     return "";
 }
 
 std::string Engine::set_stream(const UpdateStream& s, uint32_t batch_size) {
-    if (built_) return "stream must be set before the index is built";
-    if (batch_size == 0) return "batch size must be positive";
+    if (built_) return tagged(Code::kState, "stream must be set before the index is built");
+    if (batch_size == 0) return tagged(Code::kArgument, "batch size must be positive");
     stream_ = s;
     batch_size_ = batch_size;
     return "";
 }
 
 std::string Engine::add_query(const Query& q, uint32_t& qid) {
-    if (built_) return "queries must be added before the index is built";
+    if (built_) return tagged(Code::kState, "queries must be added before the index is built");
     auto st = std::make_unique<QState>();
     st->q = q;
     qid = static_cast<uint32_t>(qs_.size());
@@ -145,9 +148,9 @@ uint32_t Engine::read_u32(hipStream_t st, uint32_t* pinned, const uint32_t* p) {
 }
 
 std::string Engine::build() {
-    if (built_) return "index already built";
-    if (host_g_.dv == 0) return "no data graph loaded";
-    if (qs_.empty()) return "no query loaded";
+    if (built_) return tagged(Code::kState, "index already built");
+    if (host_g_.dv == 0) return tagged(Code::kState, "no data graph loaded");
+    if (qs_.empty()) return tagged(Code::kState, "no query loaded");
     host_g_.directed = opt_.directed;
     ndirs_ = opt_.directed ? 2 : 1;
     dv_ = host_g_.dv;
@@ -170,7 +173,7 @@ std::string Engine::build() {
     while ((1ull << id_bits_) < dv_) id_bits_++;
     const uint64_t nslots = static_cast<uint64_t>(nvl_) * nel_;
     if ((nslots << id_bits_) > (1ull << 32))
-        return "label ranks times vertex ids exceed 32-bit adjacency keys (too many labels for this vertex count)";
+        return tagged(Code::kLimit, "label ranks times vertex ids exceed 32-bit adjacency keys (too many labels for this vertex count)");
 
     max_batch_ = opt_.max_batch_edges ? opt_.max_batch_edges : std::max<uint32_t>(1, std::min<uint64_t>(batch_size_, stream_.size()));
     if (batch_size_ > 0 && batch_size_ < 0xFFFFFFFFu) max_batch_ = std::max<uint32_t>(max_batch_, std::min<uint64_t>(batch_size_, stream_.size()));
@@ -195,11 +198,11 @@ std::string Engine::build() {
     // in-memory batches when the caller sized the scratch for them (max_batch_edges)
     const uint64_t live_arcs = opt_.max_batch_edges ? 2ull * opt_.max_batch_edges * kLiveBatchesOfRoom : 0;
     const uint64_t arena_words = 3 * total_cap + 8 * (stream_arcs + live_arcs) + opt_.arena_slack_words;
-    if (arena_words >= (1ull << 32)) return "adjacency arena exceeds 2^32 words";
+    if (arena_words >= (1ull << 32)) return tagged(Code::kLimit, "adjacency arena exceeds 2^32 words");
 
     // ---- sizes of everything else, then one reservation ----
     const uint64_t max_arcs = 2ull * max_batch_;
-    ring_words_ = std::max<uint64_t>(opt_.ring_bytes / 4, 1u << 20) & ~1ull;
+    ring_words_ = ring_words_exact_ ? ring_words_exact_ : std::max<uint64_t>(opt_.ring_bytes / 4, 1u << 20) & ~1ull;
     const uint64_t cp_rec = std::max<uint32_t>(1024, opt_.cp_max_records);
     const uint64_t cp_list_words = std::max<uint64_t>(opt_.cp_list_bytes / 4, 1u << 20);
     uint64_t need = 0;
@@ -239,7 +242,7 @@ std::string Engine::build() {
         for (int d = 0; d < ndirs_; d++) add(sdir_bytes);
     need += 1 << 20;
     if (opt_.pool_bytes && need > opt_.pool_bytes)
-        return "device pool budget too small: need " + std::to_string(need >> 20) + " MiB, budget " + std::to_string(opt_.pool_bytes >> 20) + " MiB";
+        return tagged(Code::kMemory, "device pool budget too small: need " + std::to_string(need >> 20) + " MiB, budget " + std::to_string(opt_.pool_bytes >> 20) + " MiB");
     std::string err = pool_.reserve(need);
     if (!err.empty()) return err;
 
@@ -285,7 +288,7 @@ std::string Engine::build() {
             for (uint32_t v = 0; v < dv_; v++)
                 if (deg[d][v] > 1)
                     for (uint64_t i = off[d][v] + 1; i < off[d][v] + deg[d][v]; i++)
-                        if (arena[i] == arena[i - 1]) return "duplicate edge in the data graph";
+                        if (arena[i] == arena[i - 1]) return tagged(Code::kFormat, "duplicate edge in the data graph");
         CSM_HIP(hipMemcpyAsync(g_.arena, arena.data(), total_cap * 4, hipMemcpyHostToDevice, s_));
         for (int d = 0; d < ndirs_; d++) {
             g_.dir[d].adj = pool_.take<uint2>(dv_);
@@ -344,7 +347,7 @@ std::string Engine::build() {
         stream_cap_ = stream_.size();
         uint32_t* sb = pool_.take<uint32_t>(4 * (stream_cap_ + max_batch_));
         uint8_t* ob = pool_.take<uint8_t>(stream_cap_ + max_batch_);
-        if (!sb || !ob) return "device pool exhausted (stream)";
+        if (!sb || !ob) return tagged(Code::kMemory, "device pool exhausted (stream)");
         const uint64_t S = stream_cap_ + max_batch_;
         bin_.src = sb;
         bin_.dst = sb + S;
@@ -377,11 +380,11 @@ std::string Engine::build() {
             w.L[u].bitmap = pool_.take<uint32_t>(words_);
             w.L[u].list = pool_.take<uint32_t>(dv_);
             w.L[u].cap = dv_;
-            if (!w.L[u].bitmap || !w.L[u].list) return "device pool exhausted (local sets)";
+            if (!w.L[u].bitmap || !w.L[u].list) return tagged(Code::kMemory, "device pool exhausted (local sets)");
             CSM_HIP(hipMemsetAsync(w.L[u].bitmap, 0, words_ * 4ull, s_));
         }
         uint32_t* lens = pool_.take<uint32_t>(kLaneWords);  // set lengths, list counters, step counters
-        if (!lens) return "device pool exhausted (local sets)";
+        if (!lens) return tagged(Code::kMemory, "device pool exhausted (local sets)");
         for (uint32_t u = 0; u < kMaxQV; u++) w.L[u].len = lens + u;
         w.tmp_len = lens + kMaxQV;
         w.n_pairs = lens + kMaxQV + 1;
@@ -399,14 +402,14 @@ std::string Engine::build() {
         w.cp.lists = pool_.take<uint32_t>(cp_list_words);
         w.cp.lists_cap = cp_list_words;
         unsigned long long* cpc = pool_.take<unsigned long long>(4);
-        if (!cpc) return "device pool exhausted (lane scratch)";
+        if (!cpc) return tagged(Code::kMemory, "device pool exhausted (lane scratch)");
         w.cp.max_prod = cpc;
         w.cp.err = reinterpret_cast<uint32_t*>(cpc + 1);
         // level scratch: offsets, generating runs, entry tags and hashes shared with the CP scratch
         uint4* level_runs = pool_.take<uint4>(kLevelRunsPerRecord * cp_rec);
         unsigned long long* tile_sum = pool_.take<unsigned long long>(cp_rec / kLevelTile + 2);
         w.chain_ctr = pool_.take<uint32_t>(2 * kMaxQV);
-        if (!level_runs || !tile_sum || !w.chain_ctr) return "device pool exhausted (lane scratch)";
+        if (!level_runs || !tile_sum || !w.chain_ctr) return tagged(Code::kMemory, "device pool exhausted (lane scratch)");
         w.level = LevelScratch{w.cp.prod, w.cp.seg_run, level_runs, reinterpret_cast<uint8_t*>(w.cp.seg_cnt), w.cp.rec_hash,
                                tile_sum, cp_rec, kLevelRunsPerRecord * cp_rec};
     }
@@ -416,7 +419,7 @@ std::string Engine::build() {
     CSM_HIP(hipMemsetAsync(prof_bytes_, 0, kKFamilies * 8, s_));
     summary_ = pool_.take<unsigned long long>(4);
     adm_ = pool_.take<unsigned long long>(8);
-    if (!adm_) return "device pool exhausted (admission words)";
+    if (!adm_) return tagged(Code::kMemory, "device pool exhausted (admission words)");
     CSM_HIP(hipMemsetAsync(adm_, 0, 64, s_));
 
     // ---- queries: relation slots, NLF programs, candidate bitmaps ----
@@ -442,7 +445,7 @@ std::string Engine::build() {
             if (!ri.valid) qs.matchable = false;
         }
         qs.cand = pool_.take<uint32_t>(static_cast<uint64_t>(q.n) * words_);
-        if (!qs.cand) return "device pool exhausted (candidates)";
+        if (!qs.cand) return tagged(Code::kMemory, "device pool exhausted (candidates)");
         CSM_HIP(hipMemsetAsync(qs.cand, 0, static_cast<uint64_t>(q.n) * words_ * 4, s_));
         qs.entry0 = static_cast<uint32_t>(h_entries_.size());
         for (uint32_t u = 0; u < q.n; u++) {
@@ -462,7 +465,7 @@ std::string Engine::build() {
     n_entries_ = static_cast<uint32_t>(h_entries_.size());
     d_entries_ = pool_.take<NlfEntry>(std::max<size_t>(1, h_entries_.size()));
     d_groups_ = pool_.take<NlfGroup>(std::max<size_t>(1, h_groups_.size()));
-    if (pool_.failed()) return "device pool exhausted";
+    if (pool_.failed()) return tagged(Code::kMemory, "device pool exhausted");
     CSM_HIP(hipMemcpyAsync(d_entries_, h_entries_.data(), h_entries_.size() * sizeof(NlfEntry), hipMemcpyHostToDevice, s_));
     if (!h_groups_.empty())
         CSM_HIP(hipMemcpyAsync(d_groups_, h_groups_.data(), h_groups_.size() * sizeof(NlfGroup), hipMemcpyHostToDevice, s_));
@@ -500,8 +503,8 @@ std::string Engine::build() {
 }
 
 std::string Engine::plan(uint32_t qid, const Technique& t) {
-    if (!built_) return "build the index before planning";
-    if (qid >= qs_.size()) return "query id out of range";
+    if (!built_) return tagged(Code::kState, "build the index before planning");
+    if (qid >= qs_.size()) return tagged(Code::kArgument, "query id out of range");
     QState& qs = *qs_[qid];
     qs.tech = t;
     std::string err = make_plan(qs.q, t.order, qs.card.data(), qs.avg.data(), qs.plan);
@@ -530,7 +533,11 @@ std::string Engine::check_err(const char* where) {
     if (e & kErrSegmentTooLong) msg += " [too many batch arcs on one vertex]";
     if (e & kErrProductOverflow) msg += " [Cartesian product exceeds 2^63]";
     if (e & kErrScratchFull) msg += " [Cartesian-product scratch full]";
-    return msg;
+    const Code c = (e & kErrPrecondition) ? Code::kPrecondition
+                   : (e & kErrArenaFull) ? Code::kMemory
+                   : (e & kErrSegmentTooLong) ? Code::kLimit
+                                              : Code::kDevice;
+    return tagged(c, msg);
 }
 
 uint32_t Engine::view_of(const QState& qs, uint32_t rel, uint32_t i, uint32_t ph) const {
@@ -651,7 +658,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
     const uint32_t a = o.vs[0], b = o.vs[1];
     uint32_t count;
     if ((a == u0 && b == u1) || (a == u1 && b == u0)) {
-        if (2ull * np > w.ring_words) return "partial-result ring too small for the batch's anchors";
+        if (2ull * np > w.ring_words) return tagged(Code::kMemory, "partial-result ring too small for the batch's anchors");
         init_from_pairs(w.pairs, w.n_pairs, max_arcs, a == u1, ring, w.counters, w.s);
         count = np;
     } else {
@@ -670,7 +677,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
         init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, w.s);
         CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
         CSM_HIP(hipStreamSynchronize(w.s));
-        if (w.h_pinned[2]) return "partial-result ring too small for the initial partial results";
+        if (w.h_pinned[2]) return tagged(Code::kMemory, "partial-result ring too small for the initial partial results");
         count = w.h_pinned[0];
     }
     if (trace_) w.trace_line += ", \"anchors\": " + std::to_string(np) + ", \"initial\": " + std::to_string(count);
@@ -989,8 +996,8 @@ std::string Engine::run_phase(uint32_t ph) {
 }
 
 std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info) {
-    if (!built_) return "index not built";
-    if (lo + n > stream_.size()) return "batch range beyond the update stream";
+    if (!built_) return tagged(Code::kState, "index not built");
+    if (lo + n > stream_.size()) return tagged(Code::kArgument, "batch range beyond the update stream");
     return run_resident(lo, n, stream_.src.data() + lo, stream_.dst.data() + lo, stream_.op.data() + lo, out, info);
 }
 
@@ -999,15 +1006,15 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
 // max(8, 2 * next_pow2(length))) on the host mirrors of degrees and capacities.
 std::string Engine::run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
                               std::vector<QueryResult>& out, BatchInfo& info) {
-    if (!built_) return "index not built";
-    if (n > max_batch_) return "batch larger than the scratch was sized for (max_batch_edges)";
+    if (!built_) return tagged(Code::kState, "index not built");
+    if (n > max_batch_) return tagged(Code::kLimit, "batch larger than the scratch was sized for (max_batch_edges)");
     std::vector<uint32_t> kf(n), kb(n);
     for (uint64_t k = 0; k < n; k++) {
         const uint32_t a = src[k], b = dst[k];
-        if (a >= dv_ || b >= dv_) return "batch edge endpoint out of range";
-        if (a == b) return "self-loops are not supported";
-        if (host_g_.vlabel[a] == kNoVertex || host_g_.vlabel[b] == kNoVertex) return "batch edge endpoint was never declared";
-        if (rank_of(elabel_rank_, elabel[k]) == kNoVertex) return "batch edge label unknown to the session";
+        if (a >= dv_ || b >= dv_) return tagged(Code::kArgument, "batch edge endpoint out of range");
+        if (a == b) return tagged(Code::kArgument, "self-loops are not supported");
+        if (host_g_.vlabel[a] == kNoVertex || host_g_.vlabel[b] == kNoVertex) return tagged(Code::kArgument, "batch edge endpoint was never declared");
+        if (rank_of(elabel_rank_, elabel[k]) == kNoVertex) return tagged(Code::kArgument, "batch edge label unknown to the session");
         kf[k] = key_of_edge(b, elabel[k]);
         kb[k] = key_of_edge(a, elabel[k]);
     }
@@ -1025,8 +1032,9 @@ std::string Engine::run_edges(const uint32_t* src, const uint32_t* dst, const ui
 // src / dst / op for admission and counting).
 std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, const uint32_t* hdst, const uint8_t* hop,
                                  std::vector<QueryResult>& out, BatchInfo& info) {
-    if (n > max_batch_) return "batch larger than the scratch was sized for";
-    for (auto& qs : qs_) if (!qs->planned) return "every query must be planned before matching";
+    if (!failed_.empty()) return failed_;
+    if (n > max_batch_) return tagged(Code::kLimit, "batch larger than the scratch was sized for");
+    for (auto& qs : qs_) if (!qs->planned) return tagged(Code::kState, "every query must be planned before matching");
     (void)hsrc;
     (void)hdst;
     info = BatchInfo{};
@@ -1045,13 +1053,26 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
     CSM_HIP(hipStreamSynchronize(s_));
     unsigned long long a[4];
     std::memcpy(a, h_pinned_, 32);
-    std::string err = a[3] ? check_err("batch build") : std::string();
-    if (!err.empty()) return err;
-    if (a[1] || a[2] + a[0] > h_arena_words_) {  // rejected: drop the built batch, nothing else changed
+    // Every failure from here on drops the built batch (delta segments, touched flags, counters),
+    // so the next batch starts clean; a failure after the deletions were applied also leaves the
+    // session refusing further batches, since the graph then holds half a batch.
+    bool graph_changed = false;
+    auto fail = [&](const std::string& why) -> std::string {
+        for (Lane& w : lanes_) (void)hipStreamSynchronize(w.s);
         for (int d = 0; d < ndirs_; d++) batch_cleanup(g_, d, max_arcs, bs_[d], s_);
-        CSM_HIP(hipStreamSynchronize(s_));
-        if (a[1]) return "batch precondition violated: deleted edge missing";
-        return "adjacency arena budget exhausted: the batch needs " + std::to_string(a[0]) + " more words of list space";
+        (void)hipStreamSynchronize(s_);
+        if (graph_changed) {
+            failed_ = tagged(Code::kState, "an earlier batch failed after changing the graph (" + text_of(why) + "); reopen the session");
+            return why + " (the session is now unusable: the graph holds part of the batch)";
+        }
+        return why;
+    };
+    std::string err = a[3] ? check_err("batch build") : std::string();
+    if (!err.empty()) return fail(err);
+    if (a[1] || a[2] + a[0] > h_arena_words_) {  // rejected: drop the built batch, nothing else changed
+        if (a[1])
+            return fail(tagged(Code::kPrecondition, "batch precondition violated: an inserted edge exists, a deleted edge is missing, or an edge occurs twice in the batch"));
+        return fail(tagged(Code::kMemory, "adjacency arena budget exhausted: the batch needs " + std::to_string(a[0]) + " more words of list space"));
     }
     for (int d = 0; d < ndirs_; d++)
         nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 0, s_);
@@ -1061,7 +1082,7 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
     if (info.deletes) {
         const auto a = std::chrono::steady_clock::now();
         err = run_phase(1);
-        if (!err.empty()) return err;
+        if (!err.empty()) return fail(err);
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         prof_begin(prof_main_, s_, kKApply);
@@ -1070,12 +1091,13 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
             slot_dir_update(g_, d, bs_[d].touched, bs_[d].n_touched, max_arcs, s_);
         }
         prof_end(prof_main_, s_, kKApply);
+        graph_changed = true;
     }
     if (info.inserts) {
         CSM_HIP(hipStreamSynchronize(s_));
         const auto a = std::chrono::steady_clock::now();
         err = run_phase(0);
-        if (!err.empty()) return err;
+        if (!err.empty()) return fail(err);
         CSM_HIP(hipStreamSynchronize(s_));
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         prof_begin(prof_main_, s_, kKApply);
@@ -1092,7 +1114,10 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
     CSM_HIP(hipMemcpyAsync(res.data(), results_, res.size() * 8, hipMemcpyDeviceToHost, s_));
     CSM_HIP(hipStreamSynchronize(s_));
     err = check_err("batch apply");
-    if (!err.empty()) return err;
+    if (!err.empty()) {
+        failed_ = tagged(Code::kState, "an earlier batch failed while it was applied to the graph (" + text_of(err) + "); reopen the session");
+        return err;
+    }
     err = prof_collect();
     if (!err.empty()) return err;
     out.assign(qs_.size(), QueryResult{});
@@ -1155,7 +1180,7 @@ std::string Engine::prof_collect() {
 }
 
 std::string Engine::profile_enable(uint32_t mask) {
-    if (!built_) return "build the index before profiling";
+    if (!built_) return tagged(Code::kState, "build the index before profiling");
     std::string err = prof_collect();
     if (!err.empty()) return err;
     prof_mask_ = mask;
@@ -1166,7 +1191,7 @@ std::string Engine::profile_enable(uint32_t mask) {
 }
 
 std::string Engine::profile_read(uint32_t family, KernelStat& st) {
-    if (family >= kKFamilies) return "kernel family out of range";
+    if (family >= kKFamilies) return tagged(Code::kArgument, "kernel family out of range");
     std::string err = prof_collect();
     if (!err.empty()) return err;
     unsigned long long b[kKFamilies];
@@ -1205,8 +1230,78 @@ std::string Engine::synchronize() {
     return "";
 }
 
+// The regions snapshot() saves, in order: arena words [0, top), per direction the list headers,
+// capacities and label-slot directory, then every query's candidate bitmaps.
+namespace {
+struct Region {
+    void* p;
+    uint64_t bytes;
+};
+}  // namespace
+
+std::string Engine::snapshot() {
+    if (!built_) return tagged(Code::kState, "build the index before taking a snapshot");
+    if (!failed_.empty()) return failed_;
+    CSM_HIP(hipStreamSynchronize(s_));
+    const uint64_t top = arena_used_words();
+    std::vector<Region> rs{{g_.arena, top * 4}};
+    for (int d = 0; d < ndirs_; d++) {
+        rs.push_back({g_.dir[d].adj, dv_ * 8ull});
+        rs.push_back({g_.dir[d].cap, dv_ * 4ull});
+        if (g_.sdir[d]) rs.push_back({g_.sdir[d], (g_.nslots + 1ull) * dv_ * 4});
+    }
+    for (auto& qs : qs_) rs.push_back({qs->cand, static_cast<uint64_t>(qs->q.n) * words_ * 4});
+    uint64_t need = 0;
+    for (const Region& r : rs) need += (r.bytes + 255) & ~255ull;
+    // the arena part may grow between snapshots: keep room for the whole arena
+    const uint64_t cap = need - ((top * 4 + 255) & ~255ull) + ((h_arena_words_ * 4 + 255) & ~255ull);
+    if (!snap_ || snap_bytes_ < cap) {
+        if (snap_) CSM_HIP(hipFree(snap_));
+        snap_ = nullptr;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&snap_), cap);
+        if (e != hipSuccess) {
+            snap_ = nullptr;
+            snap_bytes_ = 0;
+            return tagged(Code::kMemory, std::string("snapshot allocation of ") + std::to_string(cap >> 20) + " MiB failed: " + hipGetErrorString(e));
+        }
+        snap_bytes_ = cap;
+    }
+    uint64_t off = 0;
+    for (const Region& r : rs) {
+        if (r.bytes) CSM_HIP(hipMemcpyAsync(snap_ + off, r.p, r.bytes, hipMemcpyDeviceToDevice, s_));
+        off += (r.bytes + 255) & ~255ull;
+    }
+    CSM_HIP(hipStreamSynchronize(s_));
+    snap_top_ = top;
+    return "";
+}
+
+std::string Engine::restore() {
+    if (!snap_) return tagged(Code::kState, "no snapshot taken");
+    CSM_HIP(hipStreamSynchronize(s_));
+    for (Lane& w : lanes_) CSM_HIP(hipStreamSynchronize(w.s));
+    std::vector<Region> rs{{g_.arena, snap_top_ * 4}};
+    for (int d = 0; d < ndirs_; d++) {
+        rs.push_back({g_.dir[d].adj, dv_ * 8ull});
+        rs.push_back({g_.dir[d].cap, dv_ * 4ull});
+        if (g_.sdir[d]) rs.push_back({g_.sdir[d], (g_.nslots + 1ull) * dv_ * 4});
+    }
+    for (auto& qs : qs_) rs.push_back({qs->cand, static_cast<uint64_t>(qs->q.n) * words_ * 4});
+    uint64_t off = 0;
+    for (const Region& r : rs) {
+        if (r.bytes) CSM_HIP(hipMemcpyAsync(r.p, snap_ + off, r.bytes, hipMemcpyDeviceToDevice, s_));
+        off += (r.bytes + 255) & ~255ull;
+    }
+    const unsigned long long top = snap_top_;
+    std::memcpy(h_pinned_ + 64, &top, 8);
+    CSM_HIP(hipMemcpyAsync(g_.arena_top, h_pinned_ + 64, 8, hipMemcpyHostToDevice, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    failed_.clear();
+    return check_err("restore");
+}
+
 std::string Engine::read_candidates(uint32_t qid, uint32_t u, std::vector<uint32_t>& bitmap) {
-    if (!built_ || qid >= qs_.size() || u >= qs_[qid]->q.n) return "bad arguments";
+    if (!built_ || qid >= qs_.size() || u >= qs_[qid]->q.n) return tagged(Code::kArgument, "bad arguments");
     bitmap.resize(words_);
     CSM_HIP(hipMemcpyAsync(bitmap.data(), qs_[qid]->cand + static_cast<uint64_t>(u) * words_, words_ * 4ull, hipMemcpyDeviceToHost, s_));
     CSM_HIP(hipStreamSynchronize(s_));
@@ -1214,7 +1309,7 @@ std::string Engine::read_candidates(uint32_t qid, uint32_t u, std::vector<uint32
 }
 
 std::string Engine::read_adjacency(uint32_t v, uint32_t dir, std::vector<uint32_t>& ids, std::vector<uint32_t>& labels) {
-    if (!built_ || v >= dv_ || dir >= static_cast<uint32_t>(ndirs_)) return "bad arguments";
+    if (!built_ || v >= dv_ || dir >= static_cast<uint32_t>(ndirs_)) return tagged(Code::kArgument, "bad arguments");
     uint2 a;
     CSM_HIP(hipMemcpyAsync(&a, g_.dir[dir].adj + v, 8, hipMemcpyDeviceToHost, s_));
     CSM_HIP(hipStreamSynchronize(s_));

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/csm.h"
+#include "common/errors.h"
 #include "device/kernels.h"
 #include "host/graph_io.h"
 #include "host/plan.h"
@@ -110,6 +111,11 @@ class Engine {
     std::string run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
                           std::vector<QueryResult>& out, BatchInfo& info);
     std::string synchronize();
+    // Device copy of the graph state (adjacency arena in use, list headers, label-slot directory,
+    // candidate bitmaps, arena bump pointer) taken between batches, and its restoration: the session
+    // then continues as if the batches after the snapshot had never been applied.
+    std::string snapshot();
+    std::string restore();
 
     uint32_t num_queries() const { return static_cast<uint32_t>(qs_.size()); }
     uint64_t num_batches() const;
@@ -243,11 +249,16 @@ class Engine {
     uint64_t h_arena_words_ = 0;
     unsigned long long* adm_ = nullptr;  // [8] batch admission words (batch_admit)
     uint64_t ring_words_ = 0;                 // words of each lane's ring
+    uint64_t ring_words_exact_ = 0;           // CSM_RING_WORDS: ring size without the 1M-word minimum
     hipEvent_t phase_start_ = nullptr;
     unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash
     unsigned long long* summary_ = nullptr;   // [2]
     uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer (main stream)
     uint32_t last_err_ = 0;
+    std::string failed_;  // set when a batch failed after changing the graph: every later batch is refused
+    // snapshot(): one device allocation outside the pool holding the graph state
+    char* snap_ = nullptr;
+    uint64_t snap_bytes_ = 0, snap_top_ = 0;
 
     // profiling
     void prof_begin(Prof& P, hipStream_t st, uint32_t family);

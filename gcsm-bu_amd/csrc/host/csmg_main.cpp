@@ -128,7 +128,8 @@ int main(int argc, char** argv) {
         std::string item;
         while (std::getline(ss, item, ',')) if (!item.empty()) queries.push_back(item);
     }
-    const uint32_t semi_bfs = p.get_u32("--semi_bfs_threshold", CSM_DEFAULT_SEMI_BFS_THRESHOLD);
+    uint32_t semi_bfs = p.get_u32("--semi_bfs_threshold", CSM_DEFAULT_SEMI_BFS_THRESHOLD);
+    if (semi_bfs == 0) semi_bfs = CSM_DEFAULT_SEMI_BFS_THRESHOLD;  // the library's meaning of 0: print the value in use
     print_info(query, data, update, device, batch_size, show_mem, t, semi_bfs);
 
     csm_options o{};

@@ -1,5 +1,7 @@
 #include "graph_io.h"
 
+#include "../common/errors.h"
+
 #include <cstdio>
 #include <sstream>
 #include <unordered_map>
@@ -41,7 +43,7 @@ struct Tok {
 
 std::string load_data_graph(const std::string& path, DataGraph& g) {
     std::string buf;
-    if (!read_file(path, buf)) return "Failed to open: " + path;
+    if (!read_file(path, buf)) return tagged(Code::kIo, "Failed to open: " + path);
     g = DataGraph{};
     Tok t{buf.data(), buf.data() + buf.size()};
     char type;
@@ -51,17 +53,17 @@ std::string load_data_graph(const std::string& path, DataGraph& g) {
             t.chr(c); t.u32(x);
         } else if (type == 'v') {
             uint32_t id, label;
-            if (!t.u32(id) || !t.u32(label)) return "malformed vertex line in data graph";
-            if (id == kNoVertex) return "vertex id out of range";
+            if (!t.u32(id) || !t.u32(label)) return tagged(Code::kFormat, "malformed vertex line in data graph");
+            if (id == kNoVertex) return tagged(Code::kLimit, "vertex id out of range");
             if (id >= g.vlabel.size()) g.vlabel.resize(static_cast<size_t>(id) + 1, kNoVertex);
-            if (g.vlabel[id] != kNoVertex) return "data vertex declared twice";
+            if (g.vlabel[id] != kNoVertex) return tagged(Code::kFormat, "data vertex declared twice");
             g.vlabel[id] = label;
         } else {
             uint32_t a, b, label;
-            if (!t.u32(a) || !t.u32(b) || !t.u32(label)) return "malformed edge line in data graph";
+            if (!t.u32(a) || !t.u32(b) || !t.u32(label)) return tagged(Code::kFormat, "malformed edge line in data graph");
             if (a >= g.vlabel.size() || b >= g.vlabel.size() || g.vlabel[a] == kNoVertex || g.vlabel[b] == kNoVertex)
-                return "data edge references an undeclared vertex";
-            if (a == b) return "data graph self-loops are not supported";
+                return tagged(Code::kFormat, "data edge references an undeclared vertex");
+            if (a == b) return tagged(Code::kFormat, "data graph self-loops are not supported");
             g.src.push_back(a);
             g.dst.push_back(b);
             g.elabel.push_back(label);
@@ -73,7 +75,7 @@ std::string load_data_graph(const std::string& path, DataGraph& g) {
 
 std::string load_update_stream(const std::string& path, const DataGraph& g, UpdateStream& s) {
     std::string buf;
-    if (!read_file(path, buf)) return "Failed to open: " + path;
+    if (!read_file(path, buf)) return tagged(Code::kIo, "Failed to open: " + path);
     s = UpdateStream{};
     Tok t{buf.data(), buf.data() + buf.size()};
     char type;
@@ -82,19 +84,19 @@ std::string load_update_stream(const std::string& path, const DataGraph& g, Upda
             char c; uint32_t x;
             t.chr(c); t.u32(x);
         } else if (type == 'v') {
-            return "vertex update.";
+            return tagged(Code::kFormat, "vertex update.");
         } else {
             uint8_t op = 0;
             if (type == '-') {
                 char e;
-                if (!t.chr(e) || e != 'e') return "malformed deletion line in update stream";
+                if (!t.chr(e) || e != 'e') return tagged(Code::kFormat, "malformed deletion line in update stream");
                 op = 1;
             }
             uint32_t a, b, label;
-            if (!t.u32(a) || !t.u32(b) || !t.u32(label)) return "malformed edge line in update stream";
+            if (!t.u32(a) || !t.u32(b) || !t.u32(label)) return tagged(Code::kFormat, "malformed edge line in update stream");
             if (a >= g.dv || b >= g.dv || g.vlabel[a] == kNoVertex || g.vlabel[b] == kNoVertex)
-                return "update edge references an undeclared vertex";
-            if (a == b) return "self-loops are not supported";
+                return tagged(Code::kFormat, "update edge references an undeclared vertex");
+            if (a == b) return tagged(Code::kFormat, "self-loops are not supported");
             s.src.push_back(a);
             s.dst.push_back(b);
             s.elabel.push_back(label);

@@ -1,5 +1,7 @@
 #include "plan.h"
 
+#include "../common/errors.h"
+
 #include <algorithm>
 #include <cstring>
 #include <limits>
@@ -193,7 +195,7 @@ std::string rebuild_vertex_flags_check(const Query& q, uint32_t index, const Pla
         for (uint32_t k = io.bni_offs[i]; k < io.bni_offs[i + 1]; k++)
             sum += flags[io.vs[io.bni[k]] * q.n + u] ? 1u : 0u;
         const uint32_t cnt = io.bni_offs[i + 1] - io.bni_offs[i];
-        if (sum != 0 && sum != cnt) return "Error in order generation!";
+        if (sum != 0 && sum != cnt) return tagged(Code::kState, "Error in order generation!");
     }
     return "";
 }
@@ -240,7 +242,7 @@ std::string extend_order(const Query& q, std::vector<uint32_t>& order, uint32_t&
                     }
                 }
             }
-            if (cand.empty()) return "planner: no extendable query vertex (query not connected)";
+            if (cand.empty()) return tagged(Code::kState, "planner: no extendable query vertex (query not connected)");
             insertion_sort(cand.data(), cand.data() + cand.size(),
                            [](const std::pair<uint8_t, uint32_t>& a, const std::pair<uint8_t, uint32_t>& b) {
                                return a.second < b.second;
@@ -315,7 +317,7 @@ std::string extend_order(const Query& q, std::vector<uint32_t>& order, uint32_t&
             std::swap(fur, sel);
             fur.clear();
         }
-        if (sel.empty()) return "planner: empty candidate set (reference behaviour undefined)";
+        if (sel.empty()) return tagged(Code::kState, "planner: empty candidate set (reference behaviour undefined)");
         order.push_back(sel[0]);
         visited |= 1u << sel[0];
     }
@@ -394,7 +396,7 @@ std::string gcsm_start_edge(const Query& q, uint32_t index, const uint32_t* card
         fur.clear();
     }
     if (sel.empty()) {
-        if (stale_first.first == kNoVertex) return "planner: empty starting-edge set";
+        if (stale_first.first == kNoVertex) return tagged(Code::kState, "planner: empty starting-edge set");
         out = stale_first;
         return "";
     }
@@ -535,7 +537,7 @@ std::string make_plan(const Query& q, MatchingOrder kind, const uint32_t* card, 
             break;
         }
         default:
-            return "unknown matching order";
+            return tagged(Code::kArgument, "unknown matching order");
     }
     return "";
 }

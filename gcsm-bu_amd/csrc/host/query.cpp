@@ -1,5 +1,7 @@
 #include "query.h"
 
+#include "../common/errors.h"
+
 #include <algorithm>
 #include <fstream>
 #include <map>
@@ -25,7 +27,7 @@ namespace {
 // graph.cpp:271-357: CSR, eidx, NLF groups, step order.
 std::string set_query_meta(Query& q) {
     if (q.n > kMaxQV || q.m > kMaxQE) {
-        return "The query graph should have at most 16 vertices and 42 edges.";
+        return tagged(Code::kLimit, "The query graph should have at most 16 vertices and 42 edges.");
     }
     q.eidx.fill(0xFF);
     uint32_t pos = 0;
@@ -107,27 +109,27 @@ std::string parse_stream(std::istream& is, Query& q) {
             is >> c >> x;
         } else if (type == 'v') {
             uint32_t id, label;
-            if (!(is >> id >> label)) return "malformed vertex line in query";
+            if (!(is >> id >> label)) return tagged(Code::kFormat, "malformed vertex line in query");
             auto it = vmap.find(label);
             if (it == vmap.end()) it = vmap.emplace(label, static_cast<uint32_t>(vmap.size())).first;
-            if (id >= kMaxQV) return "query vertex id out of range (ids must be 0..15)";
+            if (id >= kMaxQV) return tagged(Code::kLimit, "query vertex id out of range (ids must be 0..15)");
             if (id >= declared.size()) {
                 declared.resize(id + 1, 0);
                 q.vlabel.resize(id + 1, kNoVertex);
                 q.vlabel_raw.resize(id + 1, kNoVertex);
                 q.nbrs.resize(id + 1);
             }
-            if (declared[id]) return "query vertex declared twice";
+            if (declared[id]) return tagged(Code::kFormat, "query vertex declared twice");
             declared[id] = 1;
             q.vlabel[id] = it->second;
             q.vlabel_raw[id] = label;
             q.n += 1;
         } else {
             uint32_t a, b, label;
-            if (!(is >> a >> b >> label)) return "malformed edge line in query";
+            if (!(is >> a >> b >> label)) return tagged(Code::kFormat, "malformed edge line in query");
             if (a >= declared.size() || b >= declared.size() || !declared[a] || !declared[b])
-                return "query edge references an undeclared vertex";
-            if (a == b) return "query self-loops are not supported";
+                return tagged(Code::kFormat, "query edge references an undeclared vertex");
+            if (a == b) return tagged(Code::kFormat, "query self-loops are not supported");
             auto it = emap.find(label);
             if (it == emap.end()) {
                 it = emap.emplace(label, static_cast<uint32_t>(emap.size())).first;
@@ -135,7 +137,7 @@ std::string parse_stream(std::istream& is, Query& q) {
             }
             const uint32_t l = it->second;
             for (const auto& e : q.nbrs[a])
-                if (e.first == b) return "duplicate query edge";
+                if (e.first == b) return tagged(Code::kFormat, "duplicate query edge");
             q.out_mask[a] |= static_cast<uint16_t>(1u << b);
             auto p = std::make_pair(b, l);
             q.nbrs[a].insert(std::lower_bound(q.nbrs[a].begin(), q.nbrs[a].end(), p), p);
@@ -144,11 +146,11 @@ std::string parse_stream(std::istream& is, Query& q) {
             q.m += 1;
         }
     }
-    if (q.n < 2) return "the number of query verticex should not be less than 2!";
-    if (declared.size() != q.n) return "query vertex ids must be contiguous from 0";
+    if (q.n < 2) return tagged(Code::kFormat, "the number of query verticex should not be less than 2!");
+    if (declared.size() != q.n) return tagged(Code::kFormat, "query vertex ids must be contiguous from 0");
     std::string err = set_query_meta(q);
     if (!err.empty()) return err;
-    if (!connected(q)) return "query graph must be connected";
+    if (!connected(q)) return tagged(Code::kFormat, "query graph must be connected");
     return "";
 }
 
@@ -156,7 +158,7 @@ std::string parse_stream(std::istream& is, Query& q) {
 
 std::string parse_query(const std::string& path, Query& q) {
     std::ifstream ifs(path);
-    if (!ifs) return "Failed to open: " + path;
+    if (!ifs) return tagged(Code::kIo, "Failed to open: " + path);
     std::string err = parse_stream(ifs, q);
     q.path = path;
     return err;

@@ -10,11 +10,12 @@
 #ifndef CSM_H_
 #define CSM_H_
 
-/* Records at a Cartesian-product depth at which the semi-BFS levels stop expanding, when
- * csm_options.semi_bfs_threshold is 0. The reference compiles in 65536 (config.h:47); on the
- * 400K-vertex, 65,536-edge-batch workload 262144 ran 1.8% faster per batch and 1048576 another 1.4%.
- * Results do not depend on it. */
-#define CSM_DEFAULT_SEMI_BFS_THRESHOLD 1048576u
+/* Records at a Cartesian-product depth at which the semi-BFS levels stop expanding and the
+ * Cartesian-product flattening takes over, when csm_options.semi_bfs_threshold is 0: the
+ * reference's compiled-in 65536 (config.h:47 MIN_NUM_RESULTS_TO_GPU).  Results do not depend on
+ * it; the autotuner (gcsm-bu_amd/autotune.py) may pick a larger value per workload, which moves
+ * work from flattening to further levels. */
+#define CSM_DEFAULT_SEMI_BFS_THRESHOLD 65536u
 
 #include <stddef.h>
 #include <stdint.h>
@@ -129,6 +130,14 @@ csm_status csm_read_candidates(csm_session* s, uint32_t query_id, uint32_t u, ui
 /* Read-back for parity tests: current neighbour ids of v (sorted by label run, then id). */
 csm_status csm_read_adjacency(csm_session* s, uint32_t v, uint32_t direction, uint32_t* ids, uint32_t cap, uint32_t* degree);
 csm_status csm_synchronize(csm_session* s);
+/* Extension (no counterpart in the reference, whose graph only moves forward): a device copy of the
+ * graph state between batches (adjacency lists, label-slot directory, candidate bitmaps), and its
+ * restoration, after which the session behaves as if the batches applied since the snapshot had
+ * never run.  Lets a benchmark or a serving replica replay an update stream without reloading the
+ * data graph.  The copy is one device allocation outside the pool (about the adjacency arena's
+ * size), reused by later snapshots.  restore also clears a failed session's refusal state. */
+csm_status csm_snapshot(csm_session* s);
+csm_status csm_restore(csm_session* s);
 
 /* Measurement (no counterpart in the reference, whose timing is the TIME_* macros of
  * utils/cuda_helpers.h:78-102).  Kernel families: 0 depth-first enumeration (count), 1 depth-first

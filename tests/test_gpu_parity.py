@@ -64,6 +64,11 @@ TECHS = [
     (0, 1, True, True),
     (3, 2, True, True),
     (3, 1, False, False),
+    # (..., heavy-vertex avoidance, relation switch): the ablations main.cpp exposes as flags
+    (2, 2, True, True, False, True),
+    (2, 2, True, True, True, False),
+    (2, 3, True, True, False, False),
+    (1, 1, True, False, False, False),
 ]
 
 
@@ -73,13 +78,12 @@ def test_tiny_stream_all_techniques(lib, tmp_path, seed, batch):
     meta, queries = util.gen("tiny", tmp_path, seed=seed)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
     want, _ = util.oracle_stream(data, upd, batch, queries)
-    for (mo, ss, li, cp) in TECHS:
-        say(f"case tiny seed={seed} batch={batch} order={mo} strategy={ss} local={li} cp={cp}")
-        tech = lib.technique(mo, ss, li, cp)
-        got = run_library(lib, str(data), str(upd), batch, queries, tech)
+    for t in TECHS:
+        say(f"case tiny seed={seed} batch={batch} technique={t}")
+        got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(*t))
         if got is None:
             continue
-        assert got == want, f"order={mo} strategy={ss} local_index={li} cp={cp}"
+        assert got == want, f"technique (order, strategy, local index, cp, hva, relation switch) = {t}"
 
 
 def test_c1_single_batch(lib, tmp_path):
@@ -459,3 +463,153 @@ def test_c5_directed_serving(lib, tmp_path):
             for k, r in enumerate(got):
                 assert r == res[(b, k)], f"batch {b} query {k}"
             lo += x
+
+
+def _edges_of(path):
+    es = {}
+    for line in open(path):
+        t = line.split()
+        if t and t[0] == "e":
+            es[(int(t[1]), int(t[2]))] = int(t[3])
+    return es
+
+
+def test_precondition_violations_refused(lib, tmp_path):
+    """Batches that break the update preconditions (an inserted edge that exists, the same new edge
+    twice, also as (a, b) and (b, a), a deletion of a missing edge, a deletion twice) are refused with
+    CSM_ERR_PRECONDITION before anything changes; the valid batches around them still equal the
+    oracle's results batch for batch."""
+    meta, queries = util.gen("tiny", tmp_path, seed=8)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = 60
+    want, _ = util.oracle_stream(data, upd, batch, queries)
+    src, dst, lab, op = _read_stream(upd)
+    initial = _edges_of(data)
+    (a, b), la = next(iter(initial.items()))
+    nv = max(max(x) for x in initial) + 1
+    present = set(initial) | {(x, y) for x, y in zip(src, dst)}
+    new = next((x, y) for x in range(nv) for y in range(x + 1, nv) if (x, y) not in present and (y, x) not in present)
+    bad_batches = [
+        ([a], [b], [la], [0]),                          # inserted edge already present
+        ([new[0], new[0]], [new[1], new[1]], [0, 0], [0, 0]),  # the same new edge twice
+        ([new[0], new[1]], [new[1], new[0]], [0, 0], [0, 0]),  # (a, b) and (b, a) in one batch
+        ([new[0]], [new[1]], [0], [1]),                 # deletion of a missing edge
+        ([a, b], [b, a], [la, la], [1, 1]),             # one deletion twice
+    ]
+    with lib.Session(device=0, max_batch_edges=batch) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        nb = s.open_updates(str(upd), batch)
+        s.build()
+        for q in qids:
+            s.plan(q)
+        for bi in range(nb):
+            if bi < len(bad_batches):
+                say(f"case precondition violation {bi} before batch {bi}")
+                with pytest.raises(lib.CsmError) as e:
+                    s.apply_edges(*bad_batches[bi])
+                assert e.value.status == 8, str(e.value)  # CSM_ERR_PRECONDITION
+            got, _ = s.apply_batch(bi)
+            for k, r in enumerate(got):
+                assert r == want[(bi, k)], f"batch {bi} query {k}"
+
+
+def test_failed_batch_leaves_session_refusing(lib, tmp_path, monkeypatch):
+    """A batch that fails after its deletions were applied leaves the graph holding half a batch:
+    the error says so, every later batch is refused with CSM_ERR_STATE instead of giving silently
+    wrong counts, and restoring a snapshot clears the refusal.  The failure is forced from the host:
+    an 8-word partial-result ring holds the deletion phase's 2 anchor arcs (one deleted edge) but
+    not the insertion phase's 20 (ten new edges) of a one-edge query that every arc matches."""
+    import random
+    monkeypatch.setenv("CSM_RING_WORDS", "8")
+    rng = random.Random(4)
+    n, edges = 60, set()
+    while len(edges) < 240:
+        a, b = rng.randrange(n), rng.randrange(n)
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+    with open(tmp_path / "g.graph", "w") as f:
+        f.write("".join(f"v {v} 0\n" for v in range(n)))
+        f.write("".join(f"e {a} {b} 0\n" for a, b in sorted(edges)))
+    (tmp_path / "q.graph").write_text("v 0 0\nv 1 0\ne 0 1 0\n")
+    fresh = []
+    while len(fresh) < 10:
+        a, b = rng.randrange(n), rng.randrange(n)
+        e = (min(a, b), max(a, b))
+        if a != b and e not in edges and e not in fresh:
+            fresh.append(e)
+    gone = sorted(edges)[0]
+    src = [gone[0]] + [e[0] for e in fresh]
+    dst = [gone[1]] + [e[1] for e in fresh]
+    lab, op = [0] * 11, [1] + [0] * 10
+    with lib.Session(device=0, lanes=1, max_batch_edges=16) as s:
+        s.load_graph(str(tmp_path / "g.graph"))
+        q = s.add_query(str(tmp_path / "q.graph"))
+        s.build()
+        s.plan(q)
+        s.snapshot()
+        say("case failed batch: 8-word ring, 1 deletion + 10 insertions")
+        with pytest.raises(lib.CsmError) as e:
+            s.apply_edges(src, dst, lab, op)
+        assert e.value.status == 6 and "ring" in str(e.value) and "unusable" in str(e.value), str(e.value)
+        with pytest.raises(lib.CsmError) as e:
+            s.apply_edges(src[1:], dst[1:], lab[1:], op[1:])
+        assert e.value.status == 7, str(e.value)  # CSM_ERR_STATE
+        s.restore()
+        got, _ = s.apply_edges(src[:1], dst[:1], lab[:1], op[:1])  # one deletion fits the ring
+        assert got[0][1] == 2 and got[0][0] == 0  # the deleted edge's two embeddings, nothing new
+        assert sorted(s.adjacency(gone[0])).count(gone[1]) == 0
+
+
+def test_snapshot_restore_replays_stream(lib, tmp_path):
+    """csm_snapshot / csm_restore: after the whole stream, restoring the initial graph and applying
+    the stream again gives the same counts and hashes (the oracle's), and the adjacency after the
+    restore equals the initial graph."""
+    meta, queries = util.gen("c2", tmp_path, seed=9, scale=0.01)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = int(meta["batch_size"])
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=16)
+    initial = {}
+    for (x, y) in _edges_of(data):
+        initial.setdefault(x, set()).add(y)
+        initial.setdefault(y, set()).add(x)
+    with lib.Session(device=0) as s:
+        s.load_graph(str(data))
+        qids = [s.add_query(q) for q in queries]
+        nb = s.open_updates(str(upd), batch)
+        s.build()
+        for q in qids:
+            s.plan(q)
+        s.snapshot()
+        for rnd in range(3):
+            say(f"case snapshot replay round {rnd}")
+            for bi in range(nb):
+                got, _ = s.apply_batch(bi)
+                for k, r in enumerate(got):
+                    assert r == want[(bi, k)], f"round {rnd} batch {bi} query {k}"
+            s.restore()
+            if rnd == 0:
+                nv, _ = s.counts()
+                for v in range(0, nv, 7):
+                    assert sorted(s.adjacency(v)) == sorted(initial.get(v, ())), f"vertex {v}"
+
+
+@pytest.mark.parametrize("args", [["--algorithm", "1"], ["--algorithm", "2"], ["--algorithm", "4"], ["--algorithm", "5"],
+                                  ["--heavy_vertex_avoidance", "false", "--relation_switch", "false"],
+                                  ["--search_strategy", "3", "--cartesian_product", "False", "--local_index", "false"]])
+def test_csmg_end_to_end(lib, tmp_path, args):
+    """The csmg command line on the GPU: the algorithm presets and individual ablation flags print
+    the oracle's total positive count on "Num Positive Matches:" and a timing line the reference's
+    stat scripts parse (experiments/1_stat/0_stat_graph.py:24)."""
+    import re
+    import subprocess
+    meta, queries = util.gen("c1", tmp_path, seed=1)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    want, _ = util.oracle_stream(data, upd, 0xFFFFFFFF, queries, threads=8)
+    say(f"case csmg c1 {' '.join(args)}")
+    out = subprocess.run([util.CSMG, "--query", queries[0], "--data", str(data), "--update", str(upd)] + args,
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    m = re.search(r"Num Positive Matches: (\d+)", out.stdout)
+    assert m and int(m.group(1)) == sum(v[0] for v in want.values()) > 0
+    assert re.search(r"Incremental Matching, time \(ms\)\: \d*\(host\), (\d*)\(kernel\)", out.stdout)

@@ -14,7 +14,7 @@ BIN_DIR = os.path.join(HERE, "bin")
 EXPORTED = [
     "csm_session_create", "csm_session_destroy", "csm_last_error", "csm_load_graph_file",
     "csm_load_query_file", "csm_open_update_file", "csm_build_index", "csm_plan",
-    "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_apply_range", "csm_apply_edges", "csm_arena_usage",
+    "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_apply_range", "csm_apply_range_static", "csm_apply_edges", "csm_arena_usage",
     "csm_get_relation_edge_counts",
     "csm_get_counts", "csm_pool_usage", "csm_read_candidates", "csm_read_adjacency",
     "csm_synchronize", "csm_snapshot", "csm_restore", "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
@@ -77,6 +77,7 @@ def load():
         lib.csm_get_summary.argtypes = [vp, u32, P(u32), P(ctypes.c_float), u32, P(u32)]
         lib.csm_apply_batch.argtypes = [vp, u64, P(QueryResult), u32, P(BatchInfo)]
         lib.csm_apply_range.argtypes = [vp, u64, u64, P(QueryResult), u32, P(BatchInfo)]
+        lib.csm_apply_range_static.argtypes = [vp, u64, u64, P(QueryResult), u32, P(BatchInfo)]
         lib.csm_apply_edges.argtypes = [vp, P(u32), P(u32), P(u32), P(ctypes.c_uint8), u64, P(QueryResult), u32, P(BatchInfo)]
         lib.csm_arena_usage.argtypes = [vp, P(u64), P(u64)]
         lib.csm_get_relation_edge_counts.argtypes = [vp, u32, u64, P(u64), u32, P(u32)]
@@ -185,6 +186,14 @@ class Session:
         res = (QueryResult * max(1, nq))()
         info = BatchInfo()
         self._check(self.lib.csm_apply_range(self.h, first, n, res, nq, ctypes.byref(info)))
+        return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
+
+    def apply_range_static(self, first, n):
+        """csm_apply_range with the partial-result rings in the library's static device ring."""
+        nq = len(self.queries)
+        res = (QueryResult * max(1, nq))()
+        info = BatchInfo()
+        self._check(self.lib.csm_apply_range_static(self.h, first, n, res, nq, ctypes.byref(info)))
         return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
 
     def apply_edges(self, src, dst, elabel, op):

@@ -204,6 +204,17 @@ csm_status csm_apply_range(csm_session* s, uint64_t first, uint64_t n, csm_query
     return copy_results(out, bi, results, num_queries, info);
 }
 
+csm_status csm_apply_range_static(csm_session* s, uint64_t first, uint64_t n, csm_query_result* results, uint32_t num_queries,
+                                  csm_batch_info* info) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (num_queries && !results) return fail(s, CSM_ERR_ARGUMENT, "results array missing");
+    std::vector<QueryResult> out;
+    BatchInfo bi;
+    std::string err = s->engine.run_batch_static(first, n, out, bi);
+    if (!err.empty()) return classify(s, err);
+    return copy_results(out, bi, results, num_queries, info);
+}
+
 csm_status csm_apply_edges(csm_session* s, const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op,
                            uint64_t n, csm_query_result* results, uint32_t num_queries, csm_batch_info* info) {
     if (!s) return CSM_ERR_ARGUMENT;

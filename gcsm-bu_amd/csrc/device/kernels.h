@@ -240,4 +240,10 @@ void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // Plain copy of n words (calibration of the HBM traffic counters).
 void copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n, hipStream_t s);
 
+// Words of the library's static partial-result ring (scratch.hip): 256 MiB on every device used.
+constexpr uint64_t kStaticRingWords = 64ull << 20;
+// Device address of that ring on `device`, which must be the current device; resolved once per
+// device and cached.
+hipError_t static_ring(int device, uint32_t** ptr);
+
 }  // namespace csm

@@ -75,6 +75,7 @@ std::string Engine::init(const EngineOptions& o) {
     CSM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096, hipHostMallocDefault));
     CSM_HIP(hipEventCreateWithFlags(&phase_start_, hipEventDisableTiming));
+    CSM_HIP(static_ring(o.device, &static_ring_));
     if (const char* t = std::getenv("CSM_TRACE")) trace_ = std::fopen(t, "a");
     if (const char* t = std::getenv("CSM_LANES")) opt_.lanes = static_cast<uint32_t>(std::max(1, std::atoi(t)));
     // A/B switches of the scheduling and layout aids (all on by default)
@@ -999,6 +1000,38 @@ std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>&
     if (!built_) return tagged(Code::kState, "index not built");
     if (lo + n > stream_.size()) return tagged(Code::kArgument, "batch range beyond the update stream");
     return run_resident(lo, n, stream_.src.data() + lo, stream_.dst.data() + lo, stream_.op.data() + lo, out, info);
+}
+
+uint64_t Engine::static_ring_max_edges() const {
+    // a step's initial partial results take 2 words per anchor arc, at most 2 arcs per edge: half
+    // of a lane's slice holds them, the other half the levels (which split when they overflow)
+    const uint64_t lanes = std::max<uint64_t>(1, lanes_.empty() ? opt_.lanes : lanes_.size());
+    return (kStaticRingWords / lanes & ~63ull) / 8;
+}
+
+std::string Engine::run_batch_static(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info) {
+    if (!built_) return tagged(Code::kState, "index not built");
+    if (lo + n > stream_.size()) return tagged(Code::kArgument, "batch range beyond the update stream");
+    if (!static_ring_) return tagged(Code::kState, "static ring unavailable on this device");
+    const uint64_t max_n = static_ring_max_edges();
+    if (n > max_n)
+        return tagged(Code::kLimit, "batch of " + std::to_string(n) + " edges exceeds the static ring's limit of " + std::to_string(max_n) +
+                                        " edges with " + std::to_string(lanes_.size()) + " lanes");
+    const uint64_t slice = kStaticRingWords / lanes_.size() & ~63ull;
+    std::vector<std::pair<uint32_t*, uint64_t>> own;
+    for (size_t k = 0; k < lanes_.size(); k++) {
+        own.emplace_back(lanes_[k].ring, lanes_[k].ring_words);
+        lanes_[k].ring = static_ring_ + k * slice;
+        lanes_[k].ring_words = slice;
+    }
+    // run_resident returns with every lane's work finished (each phase joins the lanes, and its
+    // failure path synchronises them), so no kernel touches the static ring after this call
+    std::string err = run_resident(lo, n, stream_.src.data() + lo, stream_.dst.data() + lo, stream_.op.data() + lo, out, info);
+    for (size_t k = 0; k < lanes_.size(); k++) {
+        lanes_[k].ring = own[k].first;
+        lanes_[k].ring_words = own[k].second;
+    }
+    return err;
 }
 
 // Arena admission: the words the batch's relocated lists will take, replaying the device rule

@@ -105,6 +105,11 @@ class Engine {
     std::string build();                                  // upload + offline index + summaries
     std::string plan(uint32_t qid, const Technique& t);   // plans need the summaries
     std::string run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info);
+    // run_batch with every lane's partial-result ring taken from the library's static device ring
+    // (an equal slice per lane) instead of the session's pool; allocates nothing.  Batches over
+    // static_ring_max_edges() edges are refused before any device work.
+    std::string run_batch_static(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info);
+    uint64_t static_ring_max_edges() const;
     // In-memory batch (serving): the edges are copied to the device scratch region after the
     // resident stream and applied like a stream batch.  Rejected before any device work when the
     // adjacency arena cannot hold the lists it grows (admission).
@@ -254,6 +259,7 @@ class Engine {
     unsigned long long* results_ = nullptr;   // per query: pos, pos_hash, neg, neg_hash
     unsigned long long* summary_ = nullptr;   // [2]
     uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer (main stream)
+    uint32_t* static_ring_ = nullptr;         // this device's address of the static ring (scratch.hip)
     uint32_t last_err_ = 0;
     std::string failed_;  // set when a batch failed after changing the graph: every later batch is refused
     // snapshot(): one device allocation outside the pool holding the graph state

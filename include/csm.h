@@ -107,6 +107,17 @@ csm_status csm_apply_batch(csm_session* s, uint64_t batch_index, csm_query_resul
  * graph.cpp:511-516).  Same semantics and results as csm_apply_batch. */
 csm_status csm_apply_range(csm_session* s, uint64_t first, uint64_t n, csm_query_result* results, uint32_t num_queries,
                            csm_batch_info* info);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): csm_apply_range with every
+ * lane's partial-result ring taken from the library's static device ring (256 MiB per GPU, split
+ * equally over the session's lanes) instead of the session's pool; the call allocates nothing.
+ * Same results as csm_apply_range.  Limits: at most 2^23 / lanes edges per call (1,048,576 with
+ * the default 8 lanes), refused with CSM_ERR_LIMIT before any device work; a step whose first two
+ * matching-order vertices are not its anchor edge needs its initial partial results to fit one
+ * slice, else CSM_ERR_MEMORY.  All sessions on one GPU share the static ring: calls that overlap
+ * on one GPU race. */
+csm_status csm_apply_range_static(csm_session* s, uint64_t first, uint64_t n, csm_query_result* results, uint32_t num_queries,
+                                  csm_batch_info* info);
 /* Extension (serving; no counterpart in the reference, whose batches come only from the update
  * file, graph.cpp:482-563): one batch of n edges from host arrays; op[k] = 0 inserts, 1 deletes;
  * labels are raw edge labels already known to the session.  n <= csm_options.max_batch_edges (set

@@ -1,4 +1,5 @@
 #include "gen.h"
+#include "gen_count.h"
 
 #include <algorithm>
 #include <cmath>
@@ -9,6 +10,8 @@
 #include <string>
 #include <sys/stat.h>
 #include <atomic>
+#include <chrono>
+#include <dirent.h>
 #include <functional>
 #include <thread>
 #include <unordered_set>
@@ -81,22 +84,27 @@ struct Edge { uint32_t a, b, l; };
 
 // Chung-Lu: expected degree of vertex i proportional to (i + 1)^(-1/(gamma-1)), capped at dmax,
 // rescaled so that expected degrees sum to 2m; endpoints are drawn independently by weight and
-// self-loops / duplicates are discarded.  Vertex ids are shuffled so hubs are spread out.
-std::vector<uint64_t> chung_lu(uint32_t n, uint64_t m, double gamma, double dmax, Rng& rng) {
+// self-loops / duplicates are discarded.  With hubs > 0, the first `hubs` vertices instead get the
+// expected degree hub_deg and the power law covers the remaining 2m - hubs * hub_deg.  Vertex ids are
+// shuffled so hubs are spread out.
+std::vector<uint64_t> chung_lu(uint32_t n, uint64_t m, double gamma, double dmax, Rng& rng, uint32_t hubs = 0, double hub_deg = 0) {
     std::vector<double> w(n);
     const double ex = 1.0 / (gamma - 1.0);
-    for (uint32_t i = 0; i < n; i++) w[i] = std::pow(static_cast<double>(i) + 1.0, -ex);
+    for (uint32_t i = hubs; i < n; i++) w[i] = std::pow(static_cast<double>(i - hubs) + 1.0, -ex);
+    const double bulk = 2.0 * m - hubs * hub_deg;
     for (int it = 0; it < 60; it++) {
         double sum = 0;
-        for (double x : w) sum += x;
-        const double scale = (2.0 * m) / sum;
+        for (uint32_t i = hubs; i < n; i++) sum += w[i];
+        const double scale = bulk / sum;
         bool capped = false;
-        for (auto& x : w) {
+        for (uint32_t i = hubs; i < n; i++) {
+            double& x = w[i];
             x *= scale;
             if (x > dmax) { x = dmax; capped = true; }
         }
         if (!capped) break;
     }
+    for (uint32_t i = 0; i < hubs; i++) w[i] = hub_deg;
     std::vector<uint32_t> perm(n);
     for (uint32_t i = 0; i < n; i++) perm[i] = i;
     for (uint32_t i = n; i > 1; i--) std::swap(perm[i - 1], perm[rng.below(i)]);
@@ -192,11 +200,12 @@ struct Csr {  // undirected adjacency of the final graph, sorted lists (used for
     }
 };
 
-enum Shape { kTree, kSparse, kDense, kCycles, kC1 };
+enum Shape { kTree, kSparse, kDense, kCycles, kClosure, kC1 };
 
 struct QueryOut {
     std::vector<uint32_t> verts;  // data vertices, query vertex id = position
     std::vector<Edge> edges;      // query vertex ids
+    std::vector<Edge> spare;      // kCycles: the walk's other chords, in the order they may be added
 };
 
 // Random walk with restarts from visited vertices (so trees branch); returns the vertices and the
@@ -267,7 +276,63 @@ bool extract(const Csr& g, const std::vector<uint32_t>& vlab, uint32_t nq, Shape
             }
             return true;
         }
-        if (!walk(g, nq, edge_keys, rng, shape != kDense, vs, tree)) continue;
+        if (shape == kClosure) {
+            // From a random edge, each further vertex but the last 2-3 is a common neighbour of two
+            // query vertices and joins with both edges (a tree edge and a cycle edge); the last
+            // ones join as tree leaves of random query vertices.
+            const uint64_t key = edge_keys[rng.below(edge_keys.size())];
+            vs = {static_cast<uint32_t>(key >> 32), static_cast<uint32_t>(key)};
+            std::vector<std::pair<uint32_t, uint32_t>> es{{0, 1}};
+            const uint32_t leaves = 2 + static_cast<uint32_t>(rng.below(2));
+            bool ok = nq >= leaves + 3;
+            while (vs.size() < nq - leaves && ok) {
+                std::vector<std::pair<uint32_t, uint32_t>> hits;  // (data vertex, query position)
+                for (uint32_t i = 0; i < vs.size(); i++)
+                    for (uint64_t k = g.off[vs[i]]; k < g.off[vs[i] + 1]; k++)
+                        if (std::find(vs.begin(), vs.end(), g.nbr[k]) == vs.end()) hits.emplace_back(g.nbr[k], i);
+                std::sort(hits.begin(), hits.end());
+                std::vector<size_t> starts;  // runs of one data vertex adjacent to >= 2 query vertices
+                for (size_t k = 0; k < hits.size();) {
+                    size_t e = k;
+                    while (e < hits.size() && hits[e].first == hits[k].first) e++;
+                    if (e - k >= 2) starts.push_back(k);
+                    k = e;
+                }
+                if (starts.empty()) { ok = false; break; }
+                const size_t k = starts[rng.below(starts.size())];
+                size_t e = k;
+                while (e < hits.size() && hits[e].first == hits[k].first) e++;
+                const size_t a = k + rng.below(e - k);
+                size_t b = k + rng.below(e - k - 1);
+                if (b >= a) b++;
+                const uint32_t p = static_cast<uint32_t>(vs.size());
+                vs.push_back(hits[k].first);
+                es.emplace_back(hits[a].second, p);
+                es.emplace_back(hits[b].second, p);
+            }
+            for (int tries = 0; vs.size() < nq && ok && tries < 256; tries++) {
+                const uint32_t i = static_cast<uint32_t>(rng.below(vs.size()));
+                const uint64_t d = g.deg(vs[i]);
+                uint32_t x = vs[0];
+                for (int t = 0; t < 16 && d; t++) {
+                    x = g.nbr[g.off[vs[i]] + rng.below(d)];
+                    if (std::find(vs.begin(), vs.end(), x) == vs.end()) break;
+                }
+                if (std::find(vs.begin(), vs.end(), x) != vs.end()) continue;
+                es.emplace_back(i, static_cast<uint32_t>(vs.size()));
+                vs.push_back(x);
+            }
+            if (!ok || vs.size() != nq) continue;
+            q.verts = vs;
+            q.edges.clear();
+            q.spare.clear();
+            for (const auto& t : es) {
+                const int64_t pos = g.find(vs[t.first], vs[t.second]);
+                q.edges.push_back(Edge{t.first, t.second, g.lab[pos]});
+            }
+            return true;
+        }
+        if (!walk(g, nq, edge_keys, rng, shape != kDense && shape != kCycles, vs, tree)) continue;
         std::vector<std::pair<uint32_t, uint32_t>> extra;
         for (uint32_t i = 0; i < nq; i++)
             for (uint32_t j = i + 1; j < nq; j++) {
@@ -303,6 +368,13 @@ bool extract(const Csr& g, const std::vector<uint32_t>& vlab, uint32_t nq, Shape
         };
         for (auto& t : tree) add(t.first, t.second);
         for (size_t k = 0; k < take; k++) add(extra[k].first, extra[k].second);
+        q.spare.clear();
+        if (shape == kCycles) {
+            const size_t base = q.edges.size();
+            for (size_t k = take; k < extra.size(); k++) add(extra[k].first, extra[k].second);
+            q.spare.assign(q.edges.begin() + base, q.edges.end());
+            q.edges.resize(base);
+        }
         (void)vlab;
         return true;
     }
@@ -431,6 +503,12 @@ struct Spec {
     uint32_t batch;
     std::vector<std::pair<Shape, uint32_t>> queries;  // (shape, vertices)
     double emb_cap = 0;  // reject queries with more estimated embeddings in the final graph (0: none)
+    uint32_t hubs = 0;   // planted hub vertices of expected degree hub_deg (dmax then caps the rest)
+    double hub_deg = 0;
+    // exact selection: keep a query only if its exact number of embeddings in the union of all
+    // graphs the stream passes through is at most emb_cap, found within work_budget search steps
+    bool exact = false;
+    double work_budget = 0;
     // directed configurations: arcs a -> b, out-degree lognormal(mu, sigma), uniform edge labels, and
     // a batch-size file of lognormal sizes clipped to [batch_min, batch_max]
     bool directed = false;
@@ -456,6 +534,9 @@ Spec spec_of(const std::string& c) {
         if (c == "c2") {
             s.queries = {{kSparse, 6}, {kSparse, 7}, {kSparse, 8}, {kDense, 6}, {kDense, 7}, {kDense, 8}};
         } else {
+            s.exact = true;
+            s.emb_cap = 2e9;
+            s.work_budget = 4e8;
             for (int i = 0; i < 16; i++) s.queries.push_back({kTree, 4 + (i % 9)});
             for (int i = 0; i < 24; i++) s.queries.push_back({kSparse, 4 + (i % 9)});
             for (int i = 0; i < 24; i++) s.queries.push_back({kDense, 4 + (i % 9)});
@@ -472,27 +553,78 @@ Spec spec_of(const std::string& c) {
         // uniform random targets leave few cycles: sparse shapes fall back to trees when none is found
         for (int i = 0; i < 32; i++) s.queries.push_back({i % 2 == 0 ? kTree : kSparse, 4 + (i % 5)});
     } else if (c == "c3") {
-        s = Spec{"c3", 4000000, 35000000, 2.0, 20000, 16, false, 0.05, 0.05, 1048576, {}};
-        s.queries = {{kCycles, 8}, {kCycles, 9}, {kCycles, 10}, {kCycles, 8}, {kCycles, 9}, {kCycles, 10}};
+        s = Spec{"c3", 4000000, 35000000, 2.0, 2000, 16, false, 0.05, 0.05, 1048576, {}};
+        s.hubs = 8;
+        s.hub_deg = 20000;
+        // Trees with one or two added cycle edges have 10^12 or more embeddings on this graph (exact
+        // counts), beyond any enumerator; the queries keep 8-10 vertices and 2-3 tree leaves, and
+        // every other vertex closes a cycle.
+        s.queries = {{kClosure, 8}, {kClosure, 9}, {kClosure, 10}, {kClosure, 8}, {kClosure, 9}, {kClosure, 10}};
+        s.exact = true;
         s.emb_cap = 1e9;
+        s.work_budget = 5e8;
     }
     return s;
 }
 
 }  // namespace
 
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+unsigned worker_threads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+// Appends the decimal digits of x.
+inline void put_u32(std::string& out, uint32_t x) {
+    char buf[10];
+    int n = 0;
+    do { buf[n++] = static_cast<char>('0' + x % 10); x /= 10; } while (x);
+    while (n) out.push_back(buf[--n]);
+}
+
+// Writes `count` text lines produced by line(k, out) in order; chunks of lines are formatted in
+// parallel and written one round of chunks at a time.
+bool write_lines(const std::string& path, const char* head, size_t count, const std::function<void(size_t, std::string&)>& line) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    if (head) std::fputs(head, f);
+    const unsigned nth = worker_threads();
+    constexpr size_t kChunk = 1 << 18;
+    std::vector<std::string> buf(nth);
+    for (size_t base = 0; base < count; base += nth * kChunk) {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; t++)
+            th.emplace_back([&, t]() {
+                std::string& b = buf[t];
+                b.clear();
+                const size_t lo = base + t * kChunk, hi = std::min(count, lo + kChunk);
+                for (size_t k = lo; k < hi; k++) line(k, b);
+            });
+        for (auto& x : th) x.join();
+        for (auto& b : buf) std::fwrite(b.data(), 1, b.size(), f);
+    }
+    return std::fclose(f) == 0;
+}
+
+}  // namespace
+
 int gen_main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: gen <tiny|tinyins|c1|c2|c3|c4|c5> <out_dir> [--seed S] [--scale F]\n");
+        std::fprintf(stderr, "usage: gen <tiny|tinyins|c1|c2|c3|c4|c5> <out_dir> [--seed S] [--scale F] [--queries-from DIR]\n");
         return 2;
     }
     const std::string cfg = argv[1];
     const std::string out = argv[2];
     uint64_t seed = 1;
     double scale = 1.0;
+    std::string queries_from;
     for (int i = 3; i + 1 < argc; i += 2) {
         if (!std::strcmp(argv[i], "--seed")) seed = std::strtoull(argv[i + 1], nullptr, 10);
         else if (!std::strcmp(argv[i], "--scale")) scale = std::atof(argv[i + 1]);
+        else if (!std::strcmp(argv[i], "--queries-from")) queries_from = argv[i + 1];
     }
     Spec sp = spec_of(cfg);
     if (!sp.name) { std::fprintf(stderr, "unknown config %s\n", cfg.c_str()); return 2; }
@@ -500,19 +632,26 @@ int gen_main(int argc, char** argv) {
         sp.n = std::max<uint32_t>(64, static_cast<uint32_t>(sp.n * scale));
         sp.m = std::max<uint64_t>(256, static_cast<uint64_t>(sp.m * scale));
         sp.dmax = std::max(16.0, sp.dmax * std::sqrt(scale));
+        sp.hub_deg *= std::sqrt(scale);
         if (sp.batch != 0xFFFFFFFFu) sp.batch = std::max<uint32_t>(64, static_cast<uint32_t>(sp.batch * scale));
-        sp.batch_min = std::max<uint32_t>(16, static_cast<uint32_t>(sp.batch_min * scale));
-        sp.batch_max = std::max<uint32_t>(sp.batch_min, static_cast<uint32_t>(sp.batch_max * scale));
-        sp.emb_cap *= std::sqrt(scale);  // scaled-down graphs get lighter queries
+        if (sp.batch_max) {
+            sp.batch_min = std::max<uint32_t>(16, static_cast<uint32_t>(sp.batch_min * scale));
+            sp.batch_max = std::max<uint32_t>(sp.batch_min, static_cast<uint32_t>(sp.batch_max * scale));
+        }
+        sp.emb_cap *= sp.exact ? scale : std::sqrt(scale);  // scaled-down graphs get lighter queries
+        sp.work_budget *= std::max(scale, 0.1);
     }
     mkdir(out.c_str(), 0755);
+    const double t_start = now_s();
+    auto progress = [&](const char* what) { std::fprintf(stderr, "gen %s: %s (%.1fs)\n", sp.name, what, now_s() - t_start); };
     Rng rng(seed * 0x100000001B3ull + 7);
 
-    std::vector<uint64_t> keys = sp.directed ? lognormal_arcs(sp.n, sp.mu, sp.sigma, rng) : chung_lu(sp.n, sp.m, sp.gamma, sp.dmax, rng);
+    std::vector<uint64_t> keys = sp.directed ? lognormal_arcs(sp.n, sp.mu, sp.sigma, rng) : chung_lu(sp.n, sp.m, sp.gamma, sp.dmax, rng, sp.hubs, sp.hub_deg);
     std::vector<uint32_t> vlab = labels(sp.n, sp.nlabels, sp.zipf, rng);
     std::vector<uint32_t> elab(keys.size(), 0);  // edge label of keys[k]
     if (sp.nelabels > 1)
         for (auto& x : elab) x = static_cast<uint32_t>(rng.below(sp.nelabels));
+    progress("edges sampled");
     const size_t n_hold = static_cast<size_t>(keys.size() * sp.holdout);
     const size_t n_init = keys.size() - n_hold;
     // Stream: all held-out edges are inserted; deletions are distinct initial edges, so every
@@ -534,80 +673,151 @@ int gen_main(int argc, char** argv) {
 
     // data graph file
     {
-        FILE* f = std::fopen((out + "/data.graph").c_str(), "w");
-        if (!f) { std::fprintf(stderr, "cannot write %s\n", out.c_str()); return 1; }
-        std::fprintf(f, "t # 0\n");
-        for (uint32_t v = 0; v < sp.n; v++) std::fprintf(f, "v %u %u\n", v, vlab[v]);
-        for (size_t k = 0; k < n_init; k++)
-            std::fprintf(f, "e %u %u %u\n", static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), elab[k]);
-        std::fclose(f);
-    }
-    {
-        FILE* f = std::fopen((out + "/update.stream").c_str(), "w");
-        for (const auto& e : stream)
-            std::fprintf(f, "%se %u %u %u\n", e.second ? "-" : "", static_cast<uint32_t>(keys[e.first] >> 32), static_cast<uint32_t>(keys[e.first]),
-                         elab[e.first]);
-        std::fclose(f);
-    }
-    // queries are extracted from the final graph (initial minus deletions plus insertions)
-    std::vector<Edge> fin;
-    {
-        std::unordered_set<uint64_t> dels;
-        for (const auto& e : stream) if (e.second) dels.insert(keys[e.first]);
-        fin.reserve(keys.size());
-        for (size_t k = 0; k < keys.size(); k++)
-            if (k >= n_init || !dels.count(keys[k])) fin.push_back(Edge{static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), elab[k]});
-    }
-    Csr g;
-    g.build(sp.n, fin);
-    g.directed = sp.directed;
-    std::vector<uint64_t> fin_keys;
-    fin_keys.reserve(fin.size());
-    for (const auto& e : fin) fin_keys.push_back((static_cast<uint64_t>(e.a) << 32) | e.b);
-    uint32_t qk = 0;
-    for (const auto& qs : sp.queries) {
-        QueryOut q;
-        bool found = false;
-        // Candidates are drawn in fixed groups of 16 from the main generator; their estimates run
-        // in parallel with per-candidate seeds, and the first accepted candidate in group order is
-        // kept, so the outcome does not depend on the number of threads.
-        constexpr int kGroup = 16;
-        for (int group = 0; group < 256 && !found; group++) {
-            std::vector<QueryOut> cands;
-            std::vector<uint64_t> seeds;
-            for (int k = 0; k < kGroup; k++) {
-                QueryOut c;
-                if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, c)) break;
-                cands.push_back(c);
-                seeds.push_back(rng.next());
+        const bool ok = write_lines(out + "/data.graph", "t # 0\n", sp.n + n_init, [&](size_t k, std::string& o) {
+            if (k < sp.n) {
+                o += "v "; put_u32(o, static_cast<uint32_t>(k)); o += ' '; put_u32(o, vlab[k]); o += '\n';
+            } else {
+                const uint64_t e = keys[k - sp.n];
+                o += "e "; put_u32(o, static_cast<uint32_t>(e >> 32)); o += ' '; put_u32(o, static_cast<uint32_t>(e)); o += ' ';
+                put_u32(o, elab[k - sp.n]); o += '\n';
             }
-            if (cands.empty()) break;
-            if (sp.emb_cap <= 0) { q = cands[0]; found = true; break; }
-            std::vector<char> ok(cands.size(), 0);
-            std::vector<std::thread> th;
-            const unsigned nth = std::max(1u, std::min<unsigned>(kGroup, std::thread::hardware_concurrency()));
-            std::atomic<size_t> next{0};
-            for (unsigned t = 0; t < nth; t++)
-                th.emplace_back([&]() {
-                    for (size_t i = next++; i < cands.size(); i = next++) {
-                        Rng r(seeds[i]);
-                        double est = 0;
-                        ok[i] = estimate_embeddings(g, vlab, cands[i], r, 480, 200000, est) && est <= sp.emb_cap;
+        });
+        if (!ok) { std::fprintf(stderr, "cannot write %s\n", out.c_str()); return 1; }
+    }
+    {
+        const bool ok = write_lines(out + "/update.stream", nullptr, stream.size(), [&](size_t k, std::string& o) {
+            const auto& e = stream[k];
+            if (e.second) o += '-';
+            o += "e "; put_u32(o, static_cast<uint32_t>(keys[e.first] >> 32)); o += ' '; put_u32(o, static_cast<uint32_t>(keys[e.first]));
+            o += ' '; put_u32(o, elab[e.first]); o += '\n';
+        });
+        if (!ok) { std::fprintf(stderr, "cannot write %s\n", out.c_str()); return 1; }
+    }
+    progress("graph and stream written");
+    std::vector<uint64_t> qcounts;
+    uint32_t qk = 0;
+    if (!queries_from.empty()) {
+        // queries selected earlier for this configuration, seed and scale
+        for (; qk < sp.queries.size(); qk++) {
+            const std::string src = queries_from + "/q" + std::to_string(qk) + ".graph";
+            FILE* in = std::fopen(src.c_str(), "r");
+            if (!in) { std::fprintf(stderr, "missing %s\n", src.c_str()); return 1; }
+            FILE* o = std::fopen((out + "/q" + std::to_string(qk) + ".graph").c_str(), "w");
+            char b[4096];
+            for (size_t r; (r = std::fread(b, 1, sizeof b, in)) > 0;) std::fwrite(b, 1, r, o);
+            std::fclose(in);
+            std::fclose(o);
+        }
+    } else {
+        // queries are extracted from the final graph (initial minus deletions plus insertions)
+        std::vector<Edge> fin;
+        {
+            std::unordered_set<uint64_t> dels;
+            for (const auto& e : stream) if (e.second) dels.insert(keys[e.first]);
+            fin.reserve(keys.size());
+            for (size_t k = 0; k < keys.size(); k++)
+                if (k >= n_init || !dels.count(keys[k])) fin.push_back(Edge{static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), elab[k]});
+        }
+        Csr g;
+        g.build(sp.n, fin);
+        g.directed = sp.directed;
+        std::vector<uint64_t> fin_keys;
+        fin_keys.reserve(fin.size());
+        for (const auto& e : fin) fin_keys.push_back((static_cast<uint64_t>(e.a) << 32) | e.b);
+        CountGraph cg;
+        if (sp.exact) {  // every graph the stream passes through is a subgraph of initial + all insertions
+            std::vector<CountEdge> all(keys.size());
+            for (size_t k = 0; k < keys.size(); k++) all[k] = CountEdge{static_cast<uint32_t>(keys[k] >> 32), static_cast<uint32_t>(keys[k]), elab[k]};
+            cg.build(sp.n, vlab, std::max(1u, sp.nelabels), sp.directed, all, worker_threads());
+        }
+        progress("query search structures built");
+        double last_note = now_s();
+        const bool verbose = std::getenv("CSM_GEN_VERBOSE") != nullptr;
+        if (const char* x = std::getenv("CSM_GEN_CAP")) sp.emb_cap = std::atof(x);
+        if (const char* x = std::getenv("CSM_GEN_BUDGET")) sp.work_budget = std::atof(x);
+        for (const auto& qs : sp.queries) {
+            QueryOut q;
+            bool found = false;
+            uint64_t tried = 0, qcount = 0;
+            if (sp.exact) {
+                // Candidates in generator order; the first whose exact count stays within the cap and
+                // the work budget is kept (the outcome does not depend on the number of threads).
+                CountQuery cq;
+                for (int attempt = 0; attempt < 4096 && !found; attempt++) {
+                    QueryOut c;
+                    if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, c)) break;
+                    tried++;
+                    cq.n = static_cast<uint32_t>(c.verts.size());
+                    cq.vlab.assign(cq.n, 0);
+                    for (uint32_t i = 0; i < cq.n; i++) cq.vlab[i] = vlab[c.verts[i]];
+                    // the shape's edges first; then, while the count is over the cap, the walk's
+                    // further chords one at a time (each closes another cycle)
+                    for (size_t extra = 0;; extra++) {
+                        cq.edges.clear();
+                        for (const auto& e : c.edges) cq.edges.push_back(CountEdge{e.a, e.b, e.l});
+                        const CountResult r = count_embeddings(cg, cq, static_cast<uint64_t>(sp.emb_cap),
+                                                               static_cast<uint64_t>(sp.work_budget), worker_threads());
+                        if (verbose)
+                            std::fprintf(stderr, "  candidate %llu: %zu edges, within %d, count %llu, work %llu\n", (unsigned long long)tried,
+                                         c.edges.size(), r.within ? 1 : 0, (unsigned long long)r.count, (unsigned long long)r.work);
+                        if (r.within) { q = c; found = true; qcount = r.count; break; }
+                        if (extra >= c.spare.size()) break;
+                        c.edges.push_back(c.spare[extra]);
                     }
-                });
-            for (auto& x : th) x.join();
-            for (size_t i = 0; i < cands.size(); i++)
-                if (ok[i]) { q = cands[i]; found = true; break; }
+                    if (now_s() - last_note > 30) {
+                        std::fprintf(stderr, "gen %s: query %u, %llu candidates tried (%.1fs)\n", sp.name, qk, (unsigned long long)tried,
+                                     now_s() - t_start);
+                        last_note = now_s();
+                    }
+                }
+            } else {
+                // Candidates are drawn in fixed groups of 16 from the main generator; their estimates run
+                // in parallel with per-candidate seeds, and the first accepted candidate in group order is
+                // kept, so the outcome does not depend on the number of threads.
+                constexpr int kGroup = 16;
+                for (int group = 0; group < 256 && !found; group++) {
+                    std::vector<QueryOut> cands;
+                    std::vector<uint64_t> seeds;
+                    for (int k = 0; k < kGroup; k++) {
+                        QueryOut c;
+                        if (!extract(g, vlab, qs.second, qs.first, fin_keys, rng, c)) break;
+                        cands.push_back(c);
+                        seeds.push_back(rng.next());
+                    }
+                    if (cands.empty()) break;
+                    tried += cands.size();
+                    if (sp.emb_cap <= 0) { q = cands[0]; found = true; break; }
+                    std::vector<char> ok(cands.size(), 0);
+                    std::vector<std::thread> th;
+                    const unsigned nth = std::max(1u, std::min<unsigned>(kGroup, std::thread::hardware_concurrency()));
+                    std::atomic<size_t> next{0};
+                    for (unsigned t = 0; t < nth; t++)
+                        th.emplace_back([&]() {
+                            for (size_t i = next++; i < cands.size(); i = next++) {
+                                Rng r(seeds[i]);
+                                double est = 0;
+                                ok[i] = estimate_embeddings(g, vlab, cands[i], r, 480, 200000, est) && est <= sp.emb_cap;
+                            }
+                        });
+                    for (auto& x : th) x.join();
+                    for (size_t i = 0; i < cands.size(); i++)
+                        if (ok[i]) { q = cands[i]; found = true; break; }
+                }
+                if (!found && sp.directed && qs.first != kTree) {  // no cycle of this size: a tree instead
+                    for (int t = 0; t < 64 && !found; t++) found = extract(g, vlab, qs.second, kTree, fin_keys, rng, q);
+                }
+            }
+            if (!found) {
+                std::fprintf(stderr, "could not extract query %u (%llu candidates tried)\n", qk, (unsigned long long)tried);
+                return 1;
+            }
+            write_query(out + "/q" + std::to_string(qk) + ".graph", q, vlab);
+            qcounts.push_back(qcount);
+            if (sp.exact)
+                std::fprintf(stderr, "gen %s: query %u: %zu vertices, %zu edges, %llu embeddings in the union graph, %llu candidates tried (%.1fs)\n",
+                             sp.name, qk, q.verts.size(), q.edges.size(), (unsigned long long)qcount, (unsigned long long)tried, now_s() - t_start);
+            qk++;
         }
-        if (!found && sp.directed && qs.first != kTree) {  // no cycle of this size: a tree instead
-            for (int t = 0; t < 64 && !found; t++) found = extract(g, vlab, qs.second, kTree, fin_keys, rng, q);
-        }
-        if (!found) {
-            std::fprintf(stderr, "could not extract query %u\n", qk);
-            return 1;
-        }
-        write_query(out + "/q" + std::to_string(qk) + ".graph", q, vlab);
-        qk++;
     }
     if (sp.batch_max) {  // variable batches: lognormal sizes (median batch, sigma 1) clipped to [min, max]
         FILE* bf = std::fopen((out + "/batches.txt").c_str(), "w");
@@ -623,7 +833,12 @@ int gen_main(int argc, char** argv) {
     std::fprintf(f, "config %s\nseed %llu\nn %u\nm_initial %zu\nstream %zu\ninserts %zu\ndeletes %zu\nbatch_size %u\nqueries %u\n",
                  sp.name, (unsigned long long)seed, sp.n, n_init, stream.size(), n_hold, n_del, sp.batch, qk);
     std::fprintf(f, "directed %d\nbatch_max %u\n", sp.directed ? 1 : 0, sp.batch_max ? sp.batch_max : sp.batch);
+    if (sp.exact && queries_from.empty()) {
+        std::fprintf(f, "embedding_cap %.0f\n", sp.emb_cap);
+        for (size_t k = 0; k < qcounts.size(); k++) std::fprintf(f, "union_embeddings_q%zu %llu\n", k, (unsigned long long)qcounts[k]);
+    }
     std::fclose(f);
+    progress("done");
     return 0;
 }
 

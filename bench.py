@@ -1,10 +1,13 @@
 #!/usr/bin/env python3
 """Benchmark of the continuous-subgraph-matching path on one MI355X.
 
-A step is one update batch of the plain method on the one-GPU workload: a seeded Chung-Lu graph of
-400,000 vertices and ~3.4M edges (degree exponent 2.1, max degree ~3,000, Zipf(1.0) labels over 32),
-10% of the edges held out and streamed as 90% insertions / 10% deletions in batches of 65,536 edges,
-six standing queries of 6-8 vertices (three sparse, three dense) matched against every batch.
+A step is one update batch of the plain method.  The headline workload is the largest one-GPU
+configuration: a seeded Chung-Lu graph of 4M vertices and ~35M edges (degree exponent 2.0, eight
+planted hubs of expected degree 20,000, the other degrees capped at 2,000, 16 uniform vertex labels,
+one edge label), 5% of the edges held out and streamed as 95% insertions / 5% deletions in batches
+of 1,048,576 edges, with six standing queries of 8-10 vertices (2-3 tree leaves, every other vertex
+closing a cycle) matched against every batch.  The same measurement on the 400K-vertex / 3.4M-edge
+workload (batches of 65,536 edges, six 6-8-vertex queries) is reported beside it under "c2".
 Inputs are resident in HBM before the timed region.  Prints one JSON line.
 """
 import argparse
@@ -32,6 +35,9 @@ KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>",
              "level_write": "k_level_flat_chain<1>"}
 SERVING_METRIC = "serving: update edges/s and p50/p99 latency of in-memory batches (host arrays, copy included)"
 WORKLOADS = {
+    "c3": "plain method, large/skewed: Chung-Lu 4M vertices / ~35M edges, gamma 2.0, 8 hubs of expected degree "
+          "20,000 (others capped at 2,000), 16 uniform labels, 5% held out, 95/5 insert/delete stream, batches of "
+          "1,048,576 edges, 6 queries of 8-10 vertices (2-3 tree leaves, the other vertices each closing a cycle)",
     "c2": "plain method on one GPU: Chung-Lu 400K vertices / ~3.4M edges, gamma 2.1, max degree ~3000, "
           "Zipf(1.0) labels over 32, 10% held out, 90/10 insert/delete stream, batches of 65,536 edges, "
           "6 queries of 6-8 vertices (3 sparse, 3 dense)",
@@ -157,13 +163,17 @@ def traffic_of(kernel, config):
 
 def roofline_pass(ffi, data_dir, queries, batch, config):
     """The dominant enumeration kernel timed on one lane (one stream, no other stream's packets
-    between its HIP events), warm-up batch 0, then batches 1-2: algorithmic bytes / event time."""
+    between its HIP events): warm-up batch 0, back to the initial graph, then batches 0-1 timed;
+    algorithmic bytes / event time."""
     mask = sum(1 << FAMILIES.index(k) for k in BYTE_FAMILIES)
     s, nb = open_session(ffi, data_dir, queries, batch, lanes=1)
+    s.snapshot()
     s.profile_enable(mask)
     s.apply_batch(0)
+    s.synchronize()
+    s.restore()
     s.profile_enable(mask)  # counters from here on
-    for b in range(1, min(3, nb)):
+    for b in range(min(2, nb)):
         s.apply_batch(b)
     s.synchronize()
     stats = {k: s.profile_read(FAMILIES.index(k)) for k in BYTE_FAMILIES}
@@ -176,7 +186,7 @@ def roofline_pass(ffi, data_dir, queries, batch, config):
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
             "avg_launch_us": round(ms * 1e3 / launches, 3) if launches else None, "launches": launches,
             "algorithmic_bytes_per_launch": nbytes // launches if launches else 0,
-            "measured": "one-lane session, batches 1-2 after warm-up batch 0, HIP events on the lane's stream",
+            "measured": "one-lane session, batches 0-1 after a warm-up batch 0 and a restore, HIP events on the lane's stream",
             "traffic_source": f"profiles/{config}_kernel_traffic.md" if traffic is not None else None}
 
 
@@ -200,18 +210,79 @@ def cpu_baseline(data_dir, queries, batch, seconds):
                       f"{len(queries)} queries, stopped after ~{seconds:.0f}s: {anchors}, {matches} matches in {wall:.1f}s"}
 
 
+def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_mask=0, label=""):
+    """Warm-up batches untimed, then `steps` timed batches (wall clock around csm_apply_batch plus a
+    synchronisation), cycling through the stream; when it runs out, the session returns to the
+    initial graph (csm_restore, a device copy) outside the timed region."""
+    t = time.time()
+    s, nb = open_session(ffi, data_dir, queries, batch, **opts)
+    s.snapshot()  # the initial graph, restored whenever the stream runs out
+    log(f"{label}loaded + indexed + planned in {time.time() - t:.1f}s, {nb} batches, pool {s.pool_usage()[0] >> 20} MiB")
+    s.profile_enable(profile_mask)
+    nxt = 0
+
+    def rewind_if_done():
+        nonlocal nxt
+        if nxt >= nb:
+            s.restore()
+            nxt = 0
+
+    for w in range(warmup):
+        rewind_if_done()
+        t0 = time.perf_counter()
+        s.apply_batch(nxt)
+        s.synchronize()
+        log(f"{label}warmup {w} (batch {nxt}) done in {(time.perf_counter() - t0) * 1e3:.1f} ms")
+        nxt += 1
+    s.profile_enable(profile_mask)  # reset counters: only timed batches below
+    lat, matches, edges, per_query = [], 0, 0, [0] * len(queries)
+    s.synchronize()
+    for k in range(steps):
+        rewind_if_done()
+        b = nxt
+        t0 = time.perf_counter()
+        res, info = s.apply_batch(b)
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        nxt += 1
+        lat.append(dt)
+        for i, r in enumerate(res):
+            matches += r[0] + r[1]
+            per_query[i] += r[0] + r[1]
+        edges += info.edges
+        log(f"{label}step {k} (batch {b}): {dt * 1e3:.1f} ms, {sum(r[0] + r[1] for r in res)} matches, {info.edges} edges")
+    stats = {FAMILIES[f]: s.profile_read(f) for f in range(len(FAMILIES)) if profile_mask >> f & 1}
+    s.close()
+    total = sum(lat)
+    lat_ms = sorted(x * 1e3 for x in lat)
+    return {"value": matches / total if total > 0 else 0.0, "ms_per_step": total * 1e3 / max(1, steps),
+            "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": statistics.median(lat_ms),
+            "p99_batch_ms": lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))],
+            "matches": matches, "per_query_matches": per_query, "batches_in_stream": nb,
+            "kernel_ms": {k: round(v[1], 3) for k, v in stats.items()}}
+
+
+def config_block(config, meta, args, opts, scale):
+    return {"workload": WORKLOADS[config], "config": config, "seed": args.seed, "scale": scale,
+            "batch_edges": int(meta["batch_size"]), "queries": int(meta["queries"]),
+            "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP, heavy-vertex avoidance, relation switch)",
+            "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "8"))),
+            "semi_bfs_threshold": opts.get("semi_bfs_threshold") or header_default_threshold()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c2", "c4"],
-                    help="c2: the headline workload; c4: 64 standing queries on the same graph and stream, also "
-                         "timed with one lane (the steps one after another) for comparison")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c3", choices=["c3", "c2", "c4"],
+                    help="c3: the headline workload (c2 is measured beside it unless --no-c2); c2 alone; c4: 64 "
+                         "standing queries on the c2 graph and stream, also timed with one lane for comparison")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0,
-                    help="graph and stream scale of the c4 workload (c2 and serving always run at full size)")
+                    help="graph and stream scale of the c4 workload (c2, c3 and serving always run at full size)")
     ap.add_argument("--data-dir", default=None)
+    ap.add_argument("--no-c2", action="store_true", help="with --config c3: skip the c2 measurement beside it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-mask", type=int, default=0,
@@ -238,8 +309,14 @@ def main():
         args.config = "c5"
     if args.config != "c4":
         args.scale = 1.0
-    tag = "" if args.scale == 1.0 else f"_x{args.scale:g}"
-    data_dir = args.data_dir or os.path.join("/tmp", f"csm_bench_{args.config}_s{args.seed}{tag}")
+
+    def data_dir_of(config):
+        tag = "" if args.scale == 1.0 else f"_x{args.scale:g}"
+        if args.data_dir and config == args.config:
+            return args.data_dir
+        return os.path.join("/tmp", f"csm_bench_{config}_s{args.seed}{tag}")
+
+    data_dir = data_dir_of(args.config)
     meta, queries = workload(args.config, args.seed, data_dir, args.scale)
     batch = int(meta["batch_size"])
     if args.serving:
@@ -262,92 +339,48 @@ def main():
         opts = best.options()
         tuned = {"chosen": opts, "candidates": table, "batches": "1-2 after warm-up batch 0"}
         log(f"autotuned in {time.time() - t:.1f}s: {opts}")
-    t = time.time()
-    s, nb = open_session(ffi, data_dir, queries, batch, **opts)
-    s.snapshot()  # the initial graph, restored whenever the stream runs out
-    log(f"loaded + indexed + planned in {time.time() - t:.1f}s, {nb} batches, pool {s.pool_usage()[0] >> 20} MiB")
-    s.profile_enable(args.profile_mask)
-    nxt = 0
-
-    def rewind_if_done():
-        """Back to the initial graph once the stream is exhausted (device copy; never timed)."""
-        nonlocal nxt
-        if nxt >= nb:
-            s.restore()
-            nxt = 0
-
-    for w in range(args.warmup):
-        rewind_if_done()
-        t0 = time.perf_counter()
-        s.apply_batch(nxt)
-        s.synchronize()
-        log(f"warmup {w} (batch {nxt}) done in {(time.perf_counter() - t0) * 1e3:.1f} ms")
-        nxt += 1
-    s.profile_enable(args.profile_mask)  # reset counters: only timed batches below
-    lat, matches, edges, per_query = [], 0, 0, [0] * len(queries)
-    s.synchronize()
-    for k in range(args.steps):
-        rewind_if_done()
-        b = nxt
-        t0 = time.perf_counter()
-        res, info = s.apply_batch(b)
-        s.synchronize()
-        dt = time.perf_counter() - t0
-        nxt += 1
-        lat.append(dt)
-        for i, r in enumerate(res):
-            matches += r[0] + r[1]
-            per_query[i] += r[0] + r[1]
-        edges += info.edges
-        log(f"step {k} (batch {b}): {dt * 1e3:.1f} ms, {sum(r[0] + r[1] for r in res)} matches, {info.edges} edges")
-    total = sum(lat)
-    stats = {FAMILIES[f]: s.profile_read(f) for f in range(len(FAMILIES)) if args.profile_mask >> f & 1}
+    m = timed_batches(ffi, data_dir, queries, batch, args.steps, args.warmup, opts, args.profile_mask)
     calib = None
     if args.calibrate:
+        s, _ = open_session(ffi, data_dir, queries[:1], batch, lanes=1)
         words = 1 << 26
         ms = s.calibration_copy(words)
+        s.close()
         calib = {"kernel": "k_copy_u32", "bytes_read": 4 * words, "bytes_written": 4 * words, "ms": ms,
                  "GB/s": round(8 * words / (ms / 1e3) / 1e9, 1)}
-    s.close()
     one_lane = None
     if args.config == "c4":  # the same batches with one lane: every (query, edge) step in turn
-        s1, _ = open_session(ffi, data_dir, queries, batch, lanes=1)
-        for b in range(args.warmup):
-            s1.apply_batch(b)
-        t1 = []
-        for b in range(args.warmup, args.warmup + args.steps):
-            t0 = time.perf_counter()
-            s1.apply_batch(b % nb)
-            s1.synchronize()
-            t1.append(time.perf_counter() - t0)
-        s1.close()
-        one_lane = {"ms_per_step": sum(t1) * 1e3 / len(t1), "lanes": 1}
+        m1 = timed_batches(ffi, data_dir, queries, batch, args.steps, args.warmup, {"lanes": 1}, label="one lane: ")
+        one_lane = {"ms_per_step": m1["ms_per_step"], "p99_batch_ms": m1["p99_batch_ms"], "lanes": 1}
     roofline = roofline_pass(ffi, data_dir, queries, batch, args.config)
     cpu = None if args.no_cpu_baseline else cpu_baseline(data_dir, queries, batch, args.cpu_seconds)
-    lat_ms = sorted(x * 1e3 for x in lat)
-    p50 = statistics.median(lat_ms)
-    p99 = lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))]
     out = {
-        "metric": METRIC, "value": matches / total if total > 0 else 0.0, "unit": "matches/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": total * 1e3 / max(1, args.steps),
+        "metric": METRIC, "value": m["value"], "unit": "matches/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.config], "config": args.config, "seed": args.seed, "scale": args.scale,
-                   "batch_edges": batch,
-                   "queries": len(queries), "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP)",
-                   "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "8"))),
-                   "semi_bfs_threshold": opts.get("semi_bfs_threshold") or header_default_threshold()},
-        "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": p50, "p99_batch_ms": p99,
-        "matches": matches, "per_query_matches": per_query,
+        "config": config_block(args.config, meta, args, opts, args.scale),
+        "update_edges_per_s": m["update_edges_per_s"], "p50_batch_ms": m["p50_batch_ms"], "p99_batch_ms": m["p99_batch_ms"],
+        "matches": m["matches"], "per_query_matches": m["per_query_matches"],
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if one_lane:
         out["one_lane"] = one_lane
     if tuned:
         out["autotune"] = tuned
-    if stats:
-        out["kernel_ms"] = {k: round(v[1], 3) for k, v in stats.items()}
+    if m["kernel_ms"]:
+        out["kernel_ms"] = m["kernel_ms"]
     if calib:
         out["calibration_copy"] = calib
+    if args.config == "c3" and not args.no_c2:  # the 400K-vertex workload, same measurement
+        d2 = data_dir_of("c2")
+        meta2, queries2 = workload("c2", args.seed, d2)
+        b2 = int(meta2["batch_size"])
+        m2 = timed_batches(ffi, d2, queries2, b2, args.steps, args.warmup, {}, label="c2: ")
+        out["c2"] = {"value": m2["value"], "unit": "matches/s", "ms_per_step": m2["ms_per_step"],
+                     "update_edges_per_s": m2["update_edges_per_s"], "p50_batch_ms": m2["p50_batch_ms"],
+                     "p99_batch_ms": m2["p99_batch_ms"], "matches": m2["matches"],
+                     "config": config_block("c2", meta2, args, {}, 1.0),
+                     "roofline": roofline_pass(ffi, d2, queries2, b2, "c2")}
     print(json.dumps(out))
     return 0
 

@@ -274,11 +274,6 @@ __global__ void k_copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n) {
         dst[i] = src[i];
 }
 
-// Most blocks of a local-index kernel: their sizes are device-side, so the grid is fixed.  These
-// kernels run beside the other lanes' level kernels; on a 400K-vertex Chung-Lu graph with
-// 65,536-edge batches, 64-128 four-wave blocks measured faster than 32, 256, 512 or 1024.
-constexpr int kLocalGrid = 128;
-
 int grid_for(uint64_t n, int threads, int cap) {
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(static_cast<uint64_t>(cap), (n + threads - 1) / threads)));
 }
@@ -308,17 +303,17 @@ void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slo
 }
 
 void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
-                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, hipStream_t s) {
-    k_local_expand<<<grid_for(max_frontier, 4, kLocalGrid), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier,
+                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, int blocks, hipStream_t s) {
+    k_local_expand<<<grid_for(max_frontier, 4, blocks), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier,
                                                                      cand_u, out);
 }
 
 void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
-                 uint32_t* queue, uint32_t* queue_len, hipStream_t s) {
-    k_local_queue<<<grid_for(max_frontier, 4, kLocalGrid), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
-                                                                    out.bitmap, queue, queue_len);
-    k_local_check<<<grid_for(max_frontier, 4, kLocalGrid), 256, 0, s>>>(g, phase, queue, queue_len, g.dv, fl, out);
+                 uint32_t* queue, uint32_t* queue_len, int blocks, hipStream_t s) {
+    k_local_queue<<<grid_for(max_frontier, 4, blocks), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
+                                                                out.bitmap, queue, queue_len);
+    k_local_check<<<grid_for(max_frontier, 4, blocks), 256, 0, s>>>(g, phase, queue, queue_len, g.dv, fl, out);
 }
 
 void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s) {

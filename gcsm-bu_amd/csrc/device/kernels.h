@@ -98,9 +98,10 @@ void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slo
                    hipStream_t s);
 // L(u) seed: every vertex reached from the frontier through the relation view (b -> u) that is a
 // candidate of u; the output set's bitmap (cleared beforehand) deduplicates the list.
+// `blocks`: grid of the local-index kernels (their frontier sizes are device-side).
 void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                   const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
-                  const uint32_t* cand_u, VSet out, hipStream_t s);
+                  const uint32_t* cand_u, VSet out, int blocks, hipStream_t s);
 // Keeps the vertices y of `in` with a neighbour in L(b) along the view (u -> b).
 // Local index of one query vertex u in two launches: expand from the frontier along
 // (view, dir, slot), keep the candidates of cand_u that have a neighbour in every filter's bitmap.
@@ -115,7 +116,7 @@ struct LocalFilters {
 // queue (capacity dv) and *queue_len (zero on entry) hold the new candidates between the passes.
 void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
-                 uint32_t* queue, uint32_t* queue_len, hipStream_t s);
+                 uint32_t* queue, uint32_t* queue_len, int blocks, hipStream_t s);
 struct VSetTable {
     VSet s[kMaxQV];
 };

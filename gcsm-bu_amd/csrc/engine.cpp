@@ -179,6 +179,12 @@ std::string Engine::build() {
     max_batch_ = opt_.max_batch_edges ? opt_.max_batch_edges : std::max<uint32_t>(1, std::min<uint64_t>(batch_size_, stream_.size()));
     if (batch_size_ > 0 && batch_size_ < 0xFFFFFFFFu) max_batch_ = std::max<uint32_t>(max_batch_, std::min<uint64_t>(batch_size_, stream_.size()));
     max_batch_ = std::max<uint32_t>(max_batch_, 1);
+    // The local-index kernels run beside the other lanes' kernels on fixed grids (their frontier
+    // sizes are device-side).  128-256 four-wave blocks measured best both with 65,536-edge batches
+    // on a 400K-vertex graph and with 1,048,576-edge batches on a 4M-vertex graph (1024 blocks:
+    // 19% slower there, the waves contend with the other lanes' kernels).
+    local_grid_ = 128;
+    if (const char* t = std::getenv("CSM_LOCAL_GRID")) local_grid_ = std::max(1, std::atoi(t));
 
     // ---- host CSR of the initial graph ----
     std::vector<std::vector<uint32_t>> deg(ndirs_, std::vector<uint32_t>(dv_, 0));
@@ -597,7 +603,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
             prof_begin(w.prof, w.s, kKLocalIndex);
             if (bsv.size() == 1) {
                 local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u],
-                             w.s);
+                             local_grid_, w.s);
             } else {
                 // semi-joins with the other built neighbours, fused into the expansion
                 LocalFilters fl{};
@@ -607,7 +613,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
                     fl.f[fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[bsv[k]].bitmap};
                 }
                 local_build(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), fl,
-                            w.L[u], w.tmp_list, w.L[0].len + kQueueCounters + u, w.s);
+                            w.L[u], w.tmp_list, w.L[0].len + kQueueCounters + u, local_grid_, w.s);
             }
             w.L_used |= static_cast<uint16_t>(1u << u);
             prof_end(w.prof, w.s, kKLocalIndex);

@@ -260,6 +260,7 @@ class Engine {
     unsigned long long* summary_ = nullptr;   // [2]
     uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer (main stream)
     uint32_t* static_ring_ = nullptr;         // this device's address of the static ring (scratch.hip)
+    int local_grid_ = 128;                    // blocks of the local-index kernels (set at build)
     uint32_t last_err_ = 0;
     std::string failed_;  // set when a batch failed after changing the graph: every later batch is refused
     // snapshot(): one device allocation outside the pool holding the graph state

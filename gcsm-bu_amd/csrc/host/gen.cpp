@@ -537,7 +537,8 @@ Spec spec_of(const std::string& c) {
             s.exact = true;
             s.emb_cap = 2e9;
             s.work_budget = 4e8;
-            for (int i = 0; i < 16; i++) s.queries.push_back({kTree, 4 + (i % 9)});
+            // trees of 9 or more vertices exceed the cap on this graph (none of 4,096 candidates)
+            for (int i = 0; i < 16; i++) s.queries.push_back({kTree, 4 + (i % 5)});
             for (int i = 0; i < 24; i++) s.queries.push_back({kSparse, 4 + (i % 9)});
             for (int i = 0; i < 24; i++) s.queries.push_back({kDense, 4 + (i % 9)});
         }

@@ -297,6 +297,80 @@ def test_small_ring_and_record_chunks(lib, tmp_path, monkeypatch, chain):
             assert got == want
 
 
+C3_TECHS = [
+    # (order, strategy, local index, cp, heavy-vertex avoidance, relation switch)
+    (2, 2, True, True, True, True),      # GCSM-BU defaults
+    (2, 2, True, False, True, True),     # no Cartesian-product flattening
+    (2, 2, True, True, False, True),     # no heavy-vertex avoidance
+    (2, 2, True, True, True, False),     # no relation switch
+    (2, 0, True, True, True, True),      # DFS with stealing
+    (2, 1, True, True, True, True),      # DFS without stealing
+    (2, 3, True, True, True, True),      # BFS
+    (1, 0, False, True, False, False),   # QO-GAMMA switches (algorithm 2)
+]
+
+
+def run_profiled(ffi, data, update, batch, queries, tech, mask, **opts):
+    """run_library plus the launch counts of the kernel families in `mask`."""
+    with ffi.Session(device=0, **opts) as s:
+        s.load_graph(data)
+        qids = [s.add_query(q) for q in queries]
+        nb = s.open_updates(update, batch)
+        s.build()
+        for q in qids:
+            s.plan(q, tech)
+        s.profile_enable(mask)
+        out = {}
+        for b in range(nb):
+            res, _ = s.apply_batch(b)
+            for k, r in enumerate(res):
+                out[(b, k)] = r
+        launches = {f: s.profile_read(f)[0] for f in range(11) if mask >> f & 1}
+        return out, launches
+
+
+def test_c3_shape_scaled_stream(lib, tmp_path):
+    """The large/skewed workload (Chung-Lu with planted hubs, uniform labels, 95/5 insert/delete
+    stream, 8-10-vertex queries) at 2% scale: 80K vertices, hubs of expected degree ~2,800, batches
+    of 20,971 edges.  Every batch bit-exact with the oracle for every search strategy, with and
+    without Cartesian-product flattening, heavy-vertex avoidance and the relation switch; with a
+    semi-BFS threshold of 64 the Cartesian-product kernels must run."""
+    meta, queries = util.gen("c3", tmp_path, seed=1, scale=0.02)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = int(meta["batch_size"])
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=16)
+    assert len(want) == 2 * 6
+    assert sum(v[0] + v[1] for v in want.values()) > 10 ** 6
+    assert any(v[1] for v in want.values())
+    for t in C3_TECHS:
+        say(f"case c3 scale=0.02 seed=1 technique={t}")
+        got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(*t))
+        assert got == want, f"technique {t}"
+    cp_family = 1 << 2
+    say("case c3 scale=0.02 seed=1 semi_bfs_threshold=64 (Cartesian product forced)")
+    got, launches = run_profiled(lib, str(data), str(upd), batch, queries, lib.technique(), cp_family, semi_bfs_threshold=64)
+    assert got == want
+    assert launches[2] > 0
+
+
+def test_c3_full_size_strategy_invariance(lib, tmp_path):
+    """The full large/skewed workload (4M vertices, ~35M edges, 1,048,576-edge batches): no CPU
+    oracle finishes it, so every batch's counts and hashes must agree across search strategies and
+    the Cartesian-product, heavy-vertex and relation-switch ablations, and must be non-trivial."""
+    meta, queries = util.gen("c3", tmp_path, seed=1)
+    data, upd = str(tmp_path / "data.graph"), str(tmp_path / "update.stream")
+    batch = int(meta["batch_size"])
+    say("case c3 full size seed=1 defaults")
+    base = run_library(lib, data, upd, batch, queries, lib.technique())
+    assert len(base) == 2 * 6
+    assert all(v[0] > 0 for v in base.values())
+    assert any(v[1] > 0 for v in base.values())
+    for t in [(2, 2, True, False, True, True), (2, 3, True, True, True, True), (2, 2, True, True, False, False),
+              (2, 1, True, True, True, True)]:
+        say(f"case c3 full size seed=1 technique={t}")
+        assert run_library(lib, data, upd, batch, queries, lib.technique(*t)) == base, f"technique {t}"
+
+
 def test_c4_sixty_four_standing_queries(lib, tmp_path):
     """64 standing queries (trees, sparse and dense cycles of 4-12 vertices) on one session, all
     steps of a phase shared by the concurrent lanes: each query's counts and hashes equal the

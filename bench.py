@@ -74,7 +74,8 @@ def load_ffi():
 
 
 def ensure_built():
-    if not (os.path.exists(TOOL) and os.path.exists(ORACLE) and os.path.exists(os.path.join(PKG, "lib", "libcsm.so"))):
+    if not all(os.path.exists(p) for p in (TOOL, ORACLE, os.path.join(ROOT, "oracle", "bin", "baseline"),
+                                            os.path.join(PKG, "lib", "libcsm.so"))):
         spec = importlib.util.spec_from_file_location("graft_entry", os.path.join(ROOT, "__graft_entry__.py"))
         m = importlib.util.module_from_spec(spec)
         spec.loader.exec_module(m)
@@ -190,24 +191,27 @@ def roofline_pass(ffi, data_dir, queries, batch, config):
             "traffic_source": f"profiles/{config}_kernel_traffic.md" if traffic is not None else None}
 
 
+BASELINE = os.path.join(ROOT, "oracle", "bin", "baseline")
+
+
 def cpu_baseline(data_dir, queries, batch, seconds):
+    """The multithreaded CPU backtracking enumerator over the planner's matching orders (the
+    PrintOrders text each session wrote beside the queries), on a uniform random sample of batch
+    0's (phase, query, step, anchor) items: threads take items in a seeded random order and stop
+    taking new ones after `seconds`.  Host cores: the box's CPU share (at most 16 threads)."""
     threads = min(16, os.cpu_count() or 1)
-    cmd = [ORACLE, "stream", os.path.join(data_dir, "data.graph"), os.path.join(data_dir, "update.stream"), str(batch)]
-    cmd += queries + ["--threads", str(threads), "--max-batches", "1", "--time-limit-ms", str(int(seconds * 1000))]
-    t = time.perf_counter()
-    out = subprocess.run(cmd, capture_output=True, text=True, check=True).stdout
-    wall = time.perf_counter() - t
-    matches = 0
-    anchors = ""
-    for line in out.splitlines():
-        f = line.split()
-        if len(f) == 12 and f[2] == "query":
-            matches += int(f[5]) + int(f[9])
-        elif len(f) == 6 and f[2] == "anchors":
-            anchors = f"{f[3]} of {f[5]} anchor arcs"
-    return {"value": matches / wall if wall > 0 else None, "unit": "matches/s", "cores": threads, "kind": "port",
-            "sample": f"oracle anchored enumerator (std::thread x{threads}) on batch 0 of the same workload, all "
-                      f"{len(queries)} queries, stopped after ~{seconds:.0f}s: {anchors}, {matches} matches in {wall:.1f}s"}
+    cmd = [BASELINE, os.path.join(data_dir, "data.graph"), os.path.join(data_dir, "update.stream"), str(batch)]
+    for q in queries:
+        cmd += [q, q + ".orders"]
+    cmd += ["--threads", str(threads), "--max-batches", "1", "--time-limit-ms", str(int(seconds * 1000))]
+    out = subprocess.run(cmd, capture_output=True, text=True, check=True).stdout.split()
+    # batch <b> items <done> of <total> matches <count> ms <wall>
+    done, total, matches, ms = int(out[3]), int(out[5]), int(out[7]), float(out[9])
+    return {"value": matches / (ms / 1e3) if ms > 0 else None, "unit": "matches/s", "cores": threads, "kind": "port",
+            "sample": f"multithreaded backtracking (std::thread x{threads}) over the planner's matching orders, anchored "
+                      f"at each step's update edge, on batch 0 of the same workload: {done} of {total} (phase, query, "
+                      f"step, anchor) items taken in seeded random order until {seconds:.0f}s, {matches} matches in "
+                      f"{ms / 1e3:.1f}s"}
 
 
 def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_mask=0, label=""):
@@ -217,6 +221,9 @@ def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_ma
     t = time.time()
     s, nb = open_session(ffi, data_dir, queries, batch, **opts)
     s.snapshot()  # the initial graph, restored whenever the stream runs out
+    for q, path in zip(s.queries, queries):  # the plans, for the CPU baseline's matching orders
+        with open(path + ".orders", "w") as f:
+            f.write(s.plan_text(q))
     log(f"{label}loaded + indexed + planned in {time.time() - t:.1f}s, {nb} batches, pool {s.pool_usage()[0] >> 20} MiB")
     s.profile_enable(profile_mask)
     nxt = 0

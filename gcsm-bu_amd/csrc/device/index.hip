@@ -131,124 +131,195 @@ __global__ void k_local_anchors(DevGraph g, uint32_t phase, uint32_t dir, uint32
     }
 }
 
-// One wave per frontier vertex, lanes over its view run.
+// Work split of the local-index kernels.  The frontier (or queue) size is only known on the
+// device, so each launch picks it there: G items per wave, the smallest power of two that covers
+// the items with the grid's waves, each item scanned by a sub-group of W = 64 / G lanes.  Small
+// frontiers thus get a whole wave per item (short latency), large ones one lane per item (64
+// items' loads in flight per wave).  A run longer than kRoundsPerItem * W keys is left to the whole
+// wave afterwards, so a hub's run never serialises one sub-group.
+constexpr uint32_t kRoundsPerItem = 4;
+
+struct SubWave {
+    uint32_t G, W, sub, sl;  // items per wave, lanes per item, this lane's item slot and lane in it
+    uint64_t mask;           // lanes of this lane's sub-group
+};
+
+__device__ __forceinline__ SubWave sub_wave(uint32_t items) {
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    uint32_t G = 1;
+    while (G < 64 && static_cast<uint64_t>(G) * waves < items) G <<= 1;
+    SubWave w;
+    w.G = G;
+    w.W = 64 / G;
+    const uint32_t lane = lane_id();
+    w.sub = lane / w.W;
+    w.sl = lane % w.W;
+    w.mask = (w.W == 64 ? ~0ull : ((1ull << w.W) - 1)) << (w.sub * w.W);
+    return w;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) x = max(x, static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 64)));
+    return x;
+}
+
+__device__ __forceinline__ RunView shfl_run(const RunView& r, uint32_t src) {
+    return RunView{static_cast<uint32_t>(__shfl(static_cast<int>(r.lo), src, 64)), static_cast<uint32_t>(__shfl(static_cast<int>(r.hi), src, 64)),
+                   static_cast<uint32_t>(__shfl(static_cast<int>(r.dlo), src, 64)), static_cast<uint32_t>(__shfl(static_cast<int>(r.dhi), src, 64))};
+}
+
+// Expands the frontier along (view, dir, slot) into cand_u: every element y of a frontier vertex's
+// run that is a candidate of u and live in the view is offered to `bitmap` (atomic test-and-set),
+// and `emit(fresh, y)` runs once per round for the whole wave (fresh: this lane set y's bit first).
+template <class Emit>
+__device__ __forceinline__ void expand_frontier(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                                                const uint32_t* frontier, uint32_t F, const uint32_t* cand_u, uint32_t* bitmap, Emit emit) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const SubWave sw = sub_wave(F);
+    const uint32_t* dl = phase_delta(g, dir, phase);
+    auto offer = [&](uint32_t key, const RunView& r, bool& fresh, uint32_t& y) {
+        y = key & g.id_mask;
+        if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) {
+            const uint32_t bit = 1u << (y & 31);
+            fresh = !(atomicOr(&bitmap[y >> 5], bit) & bit);  // else offered by another visitor
+        }
+    };
+    for (uint32_t base = gw * sw.G; base < F; base += waves * sw.G) {
+        const uint32_t i = base + sw.sub;
+        RunView r{0, 0, 0, 0};
+        uint32_t nb = 0, tot = 0;
+        if (i < F) {
+            r = make_run(g, phase, view, dir, frontier[i], slot);
+            nb = run_base_len(r, view);
+            tot = run_seq_len(r, view);
+        }
+        const bool own = tot <= kRoundsPerItem * sw.W;
+        const uint32_t rounds = wave_max_u32(own ? (tot + sw.W - 1) / sw.W : 0u);
+        for (uint32_t t = 0; t < rounds; t++) {
+            bool fresh = false;
+            uint32_t y = 0;
+            const uint32_t e = t * sw.W + sw.sl;
+            if (own && e < tot) offer(run_elem(g, dl, r, nb, e), r, fresh, y);
+            emit(fresh, y);
+        }
+        for (uint64_t longs = __ballot(!own && sw.sl == 0); longs; longs &= longs - 1) {
+            const uint32_t src = static_cast<uint32_t>(__ffsll(static_cast<long long>(longs)) - 1);
+            const RunView lr = shfl_run(r, src);
+            const uint32_t lnb = static_cast<uint32_t>(__shfl(static_cast<int>(nb), src, 64));
+            const uint32_t ltot = static_cast<uint32_t>(__shfl(static_cast<int>(tot), src, 64));
+            for (uint32_t t0 = 0; t0 < ltot; t0 += 64) {  // wave-uniform trip count
+                bool fresh = false;
+                uint32_t y = 0;
+                if (t0 + lane < ltot) offer(run_elem(g, dl, lr, lnb, t0 + lane), lr, fresh, y);
+                emit(fresh, y);
+            }
+        }
+    }
+}
+
+// Appends the wave's flagged vertices to list[*len ...] with one reservation (entries past cap are
+// dropped; the caller sizes cap for every vertex).
+__device__ __forceinline__ void append_fresh(bool fresh, uint32_t y, uint32_t* list, uint32_t* len, uint32_t cap) {
+    const uint64_t m = __ballot(fresh);
+    if (!m) return;
+    uint32_t p = 0;
+    if (lane_id() == 0) p = atomicAdd(len, static_cast<uint32_t>(__popcll(m)));
+    p = static_cast<uint32_t>(__shfl(static_cast<int>(p), 0, 64)) + lanes_below(m);
+    if (fresh && p < cap) list[p] = y;
+}
+
 __global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                                                       const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
                                                       const uint32_t* cand_u, VSet out) {
-    const uint32_t lane = lane_id();
-    const uint32_t wpb = blockDim.x >> 6;
     const uint32_t F = min(*frontier_len, max_frontier);
-    const uint32_t* dl = phase_delta(g, dir, phase);
-    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < F; i += gridDim.x * wpb) {
-        const uint32_t x = frontier[i];
-        const RunView r = make_run(g, phase, view, dir, x, slot);
-        const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
-        for (uint32_t t0 = 0; t0 < tot; t0 += 64) {  // wave-uniform trip count: one list reservation per pass
-            const uint32_t t = t0 + lane;
-            bool fresh = false;
-            uint32_t y = 0;
-            if (t < tot) {
-                const uint32_t key = run_elem(g, dl, r, nb, t);
-                y = key & g.id_mask;
-                if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) {
-                    const uint32_t bit = 1u << (y & 31);
-                    fresh = !(atomicOr(&out.bitmap[y >> 5], bit) & bit);
-                }
-            }
-            const uint64_t m = __ballot(fresh);
-            if (!m) continue;
-            uint32_t p = 0;
-            if (lane == 0) p = atomicAdd(out.len, static_cast<uint32_t>(__popcll(m)));
-            p = __shfl(p, 0, 64) + lanes_below(m);
-            if (fresh && p < out.cap) out.list[p] = y;
-        }
-    }
+    expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, out.bitmap,
+                    [&](bool fresh, uint32_t y) { append_fresh(fresh, y, out.list, out.len, out.cap); });
 }
 
 // Expansion with the semi-joins in a second pass: k_local_queue expands the frontier along
 // (view, dir, slot) into cand_u and queues every new candidate y once (its first visitor sets its
-// bit in out.bitmap); k_local_check gives each queued y one wave, whose lanes scan y's run of every
-// filter relation 64 keys at a time for a member of the filter's bitmap.  Survivors join out.list;
-// the bits of the others are cleared.
+// bit in out.bitmap); k_local_check keeps the queued y that have a neighbour in every filter's
+// bitmap.  Survivors join out.list; the bits of the others are cleared.
 __global__ void __launch_bounds__(256) k_local_queue(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                                                      const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
                                                      const uint32_t* cand_u, uint32_t* bitmap, uint32_t* queue, uint32_t* queue_len) {
-    const uint32_t lane = lane_id();
-    const uint32_t wpb = blockDim.x >> 6;
     const uint32_t F = min(*frontier_len, max_frontier);
-    const uint32_t* dl = phase_delta(g, dir, phase);
-    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < F; i += gridDim.x * wpb) {
-        const uint32_t x = frontier[i];
-        const RunView r = make_run(g, phase, view, dir, x, slot);
-        const uint32_t nb = run_base_len(r, view), tot = run_seq_len(r, view);
-        for (uint32_t t0 = 0; t0 < tot; t0 += 64) {  // wave-uniform trip count: one queue reservation per pass
-            const uint32_t t = t0 + lane;
-            bool fresh = false;
-            uint32_t y = 0;
-            if (t < tot) {
-                const uint32_t key = run_elem(g, dl, r, nb, t);
-                y = key & g.id_mask;
-                if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) {
-                    const uint32_t bit = 1u << (y & 31);
-                    fresh = !(atomicOr(&bitmap[y >> 5], bit) & bit);  // else queued by another visitor
-                }
-            }
-            const uint64_t m = __ballot(fresh);
-            if (!m) continue;
-            uint32_t p = 0;
-            if (lane == 0) p = atomicAdd(queue_len, static_cast<uint32_t>(__popcll(m)));
-            p = __shfl(p, 0, 64);
-            if (fresh) queue[p + lanes_below(m)] = y;
-        }
-    }
+    expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, bitmap,
+                    [&](bool fresh, uint32_t y) { append_fresh(fresh, y, queue, queue_len, 0xFFFFFFFFu); });
 }
 
+// Each queued vertex gets a sub-group (the same split as the expansion), which scans its run of
+// every filter relation W keys a round for a member of the filter's bitmap, stopping at the first
+// hit; runs longer than kRoundsPerItem * W keys are scanned afterwards by the whole wave.
 __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase, const uint32_t* queue, const uint32_t* queue_len,
                                                      uint32_t max_len, LocalFilters fl, VSet out) {
-    // survivors are staged per wave and reserved 64 at a time (one atomic on the list length)
-    __shared__ uint32_t s_keep[4][64];
     const uint32_t lane = lane_id();
-    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t n = min(*queue_len, max_len);
-    uint32_t* keep = s_keep[threadIdx.x >> 6];
-    uint32_t kept = 0;
-    auto flush = [&]() {
-        wave_sync();
-        uint32_t p = 0;
-        if (lane == 0) p = atomicAdd(out.len, kept);
-        p = __shfl(p, 0, 64) + lane;
-        if (lane < kept && p < out.cap) out.list[p] = keep[lane];
-        kept = 0;
-        wave_sync();
-    };
-    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb) {
-        const uint32_t y = queue[i];
-        bool pass = true;
-        for (uint32_t k = 0; k < fl.n && pass; k++) {
+    const SubWave sw = sub_wave(n);
+    for (uint32_t base = gw * sw.G; base < n; base += waves * sw.G) {
+        const uint32_t i = base + sw.sub;
+        const bool active = i < n;
+        const uint32_t y = active ? queue[i] : 0u;
+        bool pass = active;
+        uint32_t pending = 0;  // filters whose runs are left to the whole wave
+        for (uint32_t k = 0; k < fl.n; k++) {
             const LocalFilter f = fl.f[k];
             const uint32_t* fdl = phase_delta(g, f.dir, phase);
-            const RunView fr = make_run(g, phase, f.view, f.dir, y, f.slot);
-            const uint32_t fnb = run_base_len(fr, f.view), ftot = run_seq_len(fr, f.view);
+            RunView fr{0, 0, 0, 0};
+            uint32_t fnb = 0, ftot = 0;
+            if (pass) {
+                fr = make_run(g, phase, f.view, f.dir, y, f.slot);
+                fnb = run_base_len(fr, f.view);
+                ftot = run_seq_len(fr, f.view);
+            }
+            const bool own = pass && ftot <= kRoundsPerItem * sw.W;
+            if (pass && !own) pending |= 1u << k;
+            const uint32_t rounds = wave_max_u32(own ? (ftot + sw.W - 1) / sw.W : 0u);
             bool hit = false;
-            for (uint32_t t0 = 0; t0 < ftot && !hit; t0 += 64) {
-                const uint32_t t = t0 + lane;
+            for (uint32_t t = 0; t < rounds; t++) {
+                const uint32_t e = t * sw.W + sw.sl;
                 bool h = false;
-                if (t < ftot) {
-                    const uint32_t fk = run_elem(g, fdl, fr, fnb, t);
+                if (own && !hit && e < ftot) {
+                    const uint32_t fk = run_elem(g, fdl, fr, fnb, e);
                     h = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
                 }
-                hit = __ballot(h) != 0;
+                const uint64_t m = __ballot(h);
+                hit = hit || (m & sw.mask) != 0;
+                if (__ballot(own && !hit && (t + 1) * sw.W < ftot) == 0) break;  // every sub-group decided
             }
-            pass = hit;
+            if (own) pass = hit;
         }
-        if (pass) {
-            if (lane == 0) keep[kept] = y;
-            if (++kept == 64) flush();
-        } else if (lane == 0) {
-            atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
+        for (uint64_t todo = __ballot(pass && pending && sw.sl == 0); todo; todo &= todo - 1) {
+            const uint32_t src = static_cast<uint32_t>(__ffsll(static_cast<long long>(todo)) - 1);
+            const uint32_t ys = static_cast<uint32_t>(__shfl(static_cast<int>(y), src, 64));
+            const uint32_t pm = static_cast<uint32_t>(__shfl(static_cast<int>(pending), src, 64));
+            bool ok = true;
+            for (uint32_t k = 0; k < fl.n && ok; k++) {
+                if (!(pm >> k & 1u)) continue;
+                const LocalFilter f = fl.f[k];
+                const uint32_t* fdl = phase_delta(g, f.dir, phase);
+                const RunView fr = make_run(g, phase, f.view, f.dir, ys, f.slot);
+                const uint32_t fnb = run_base_len(fr, f.view), ftot = run_seq_len(fr, f.view);
+                bool hit = false;
+                for (uint32_t t0 = 0; t0 < ftot && !hit; t0 += 64) {
+                    bool h = false;
+                    if (t0 + lane < ftot) {
+                        const uint32_t fk = run_elem(g, fdl, fr, fnb, t0 + lane);
+                        h = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
+                    }
+                    hit = __ballot(h) != 0;
+                }
+                ok = hit;
+            }
+            if ((sw.mask >> src) & 1ull) pass = ok;  // the whole sub-group of the pending vertex
         }
+        append_fresh(pass && sw.sl == 0, y, out.list, out.len, out.cap);
+        if (active && !pass && sw.sl == 0) atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
     }
-    if (kept) flush();
 }
 
 // Clears the bitmaps of every set in `mask` (whole words of its listed members).

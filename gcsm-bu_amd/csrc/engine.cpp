@@ -180,10 +180,10 @@ std::string Engine::build() {
     if (batch_size_ > 0 && batch_size_ < 0xFFFFFFFFu) max_batch_ = std::max<uint32_t>(max_batch_, std::min<uint64_t>(batch_size_, stream_.size()));
     max_batch_ = std::max<uint32_t>(max_batch_, 1);
     // The local-index kernels run beside the other lanes' kernels on fixed grids (their frontier
-    // sizes are device-side).  128-256 four-wave blocks measured best both with 65,536-edge batches
-    // on a 400K-vertex graph and with 1,048,576-edge batches on a 4M-vertex graph (1024 blocks:
-    // 19% slower there, the waves contend with the other lanes' kernels).
-    local_grid_ = 128;
+    // sizes are device-side; each launch splits the frontier over its waves itself).  Measured:
+    // 128 four-wave blocks best with 65,536-edge batches on a 400K-vertex graph (256: +0.5%), 256
+    // with 1,048,576-edge batches on a 4M-vertex graph (128: +6%, 512: +1%, 1024: +13%).
+    local_grid_ = max_batch_ >= (1u << 18) ? 256 : 128;
     if (const char* t = std::getenv("CSM_LOCAL_GRID")) local_grid_ = std::max(1, std::atoi(t));
 
     // ---- host CSR of the initial graph ----

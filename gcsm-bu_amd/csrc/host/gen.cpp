@@ -537,10 +537,11 @@ Spec spec_of(const std::string& c) {
             s.exact = true;
             s.emb_cap = 2e9;
             s.work_budget = 4e8;
-            // trees of 9 or more vertices exceed the cap on this graph (none of 4,096 candidates)
+            // Trees of 9 or more vertices and sparse cyclic queries of 10 or more exceed the cap on
+            // this graph (none of 4,096 candidates each), so the sizes stop at 8 and 9.
             for (int i = 0; i < 16; i++) s.queries.push_back({kTree, 4 + (i % 5)});
-            for (int i = 0; i < 24; i++) s.queries.push_back({kSparse, 4 + (i % 9)});
-            for (int i = 0; i < 24; i++) s.queries.push_back({kDense, 4 + (i % 9)});
+            for (int i = 0; i < 24; i++) s.queries.push_back({kSparse, 4 + (i % 6)});
+            for (int i = 0; i < 24; i++) s.queries.push_back({kDense, 4 + (i % 6)});
         }
     } else if (c == "c5") {
         s = Spec{"c5", 3000000, 0, 0, 0, 4, false, 0.10, 0.15, 16384, {}};

@@ -26,13 +26,16 @@ ORACLE = os.path.join(ROOT, "oracle", "bin", "oracle")
 METRIC = "incremental matches/s & update edges/s on 1 GPU; p50/p99 batch latency"
 HBM_PEAK_GBS = 8000.0
 FAMILIES = ["enumerate_count", "enumerate_write", "cartesian_product", "local_index", "batch_build", "apply_batch",
-            "cartesian_prepare", "anchors", "level_count", "level_write", "level_prepare"]
+            "cartesian_prepare", "anchors", "level_count", "level_write", "level_prepare", "local_check", "merge_insert"]
 # families whose kernels count algorithmic bytes on the device (the roofline candidates)
-BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_write"]
+BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_write", "local_index", "local_check",
+                 "merge_insert"]
 # (the level families' launches are the device-chained kernels; levels the host resumes after a
-# chain stop use k_level_flat<0>/<1>, the same kernel body, and are timed with them)
+# chain stop use k_level_flat<0>/<1>, the same kernel body, and are timed with them; the
+# local_index family is one k_local_queue or k_local_expand launch)
 KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat_chain<0>",
-             "level_write": "k_level_flat_chain<1>"}
+             "level_write": "k_level_flat_chain<1>", "local_index": "k_local_queue", "local_check": "k_local_check",
+             "merge_insert": "k_apply_insertions"}
 SERVING_METRIC = "serving: update edges/s and p50/p99 latency of in-memory batches (host arrays, copy included)"
 WORKLOADS = {
     "c3": "plain method, large/skewed: Chung-Lu 4M vertices / ~35M edges, gamma 2.0, 8 hubs of expected degree "
@@ -179,14 +182,21 @@ def roofline_pass(ffi, data_dir, queries, batch, config):
     s.synchronize()
     stats = {k: s.profile_read(FAMILIES.index(k)) for k in BYTE_FAMILIES}
     s.close()
+
+    def gbs(k):
+        _, ms, nbytes = stats[k]
+        return nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+
     dom = max(stats, key=lambda k: stats[k][1])
     launches, ms, nbytes = stats[dom]
-    achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    achieved = gbs(dom)
     traffic = traffic_of(KERNEL_OF[dom], config)
+    others = {KERNEL_OF[k]: {"ms": round(stats[k][1], 3), "launches": stats[k][0], "achieved_GBs": round(gbs(k), 1),
+                             "frac": round(gbs(k) / HBM_PEAK_GBS, 5)} for k in BYTE_FAMILIES if stats[k][0]}
     return {"bound": "hbm", "kernel": KERNEL_OF[dom], "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
             "avg_launch_us": round(ms * 1e3 / launches, 3) if launches else None, "launches": launches,
-            "algorithmic_bytes_per_launch": nbytes // launches if launches else 0,
+            "algorithmic_bytes_per_launch": nbytes // launches if launches else 0, "kernels": others,
             "measured": "one-lane session, batches 0-1 after a warm-up batch 0 and a restore, HIP events on the lane's stream",
             "traffic_source": f"profiles/{config}_kernel_traffic.md" if traffic is not None else None}
 

@@ -301,8 +301,9 @@ __global__ void __launch_bounds__(256) k_apply_deletions(DevGraph g, int dir, Ba
     }
 }
 
-__global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, BatchScratch w) {
+__global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, BatchScratch w, unsigned long long* nbytes) {
     const uint32_t lane = lane_id();
+    uint64_t bytes = 0;  // lane 0 of each wave: 8 * (old + inserted) + 16 per list, 8 per inserted key
     const uint32_t wpb = blockDim.x >> 6;
     const uint32_t T = *w.n_touched;
     DevAdj A = g.dir[dir];
@@ -314,6 +315,7 @@ __global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, B
         const uint2 a = A.adj[v];
         const uint32_t cap = A.cap[v];
         const uint32_t nd = a.y + s.y;
+        if (lane == 0) bytes += 8ull * nd + 16 + 8ull * s.y;
         const uint32_t* ins = D.ins + s.x;
         uint32_t dst = a.x;
         uint32_t newcap = cap;
@@ -370,6 +372,7 @@ __global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, B
             if (relocate) A.cap[v] = newcap;
         }
     }
+    if (nbytes && lane == 0 && bytes) atomicAdd(nbytes, static_cast<unsigned long long>(bytes));
 }
 
 // Admission of a built batch before anything changes, one wave per touched vertex: the list space
@@ -492,9 +495,10 @@ void batch_apply_deletions(const DevGraph& g, int dir, uint32_t max_touched, con
     k_apply_deletions<<<grid, 256, 0, s>>>(g, dir, w);
 }
 
-void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s) {
+void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, unsigned long long* bytes,
+                            hipStream_t s) {
     const int grid = static_cast<int>(std::min<uint32_t>(8192, (max_touched + 3) / 4 + 1));
-    k_apply_insertions<<<grid, 256, 0, s>>>(g, dir, w);
+    k_apply_insertions<<<grid, 256, 0, s>>>(g, dir, w, bytes);
 }
 
 void batch_admit(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, unsigned long long* adm, hipStream_t s) {

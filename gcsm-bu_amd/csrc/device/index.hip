@@ -171,9 +171,12 @@ __device__ __forceinline__ RunView shfl_run(const RunView& r, uint32_t src) {
 // Expands the frontier along (view, dir, slot) into cand_u: every element y of a frontier vertex's
 // run that is a candidate of u and live in the view is offered to `bitmap` (atomic test-and-set),
 // and `emit(fresh, y)` runs once per round for the whole wave (fresh: this lane set y's bit first).
+// Returns this lane's algorithmic bytes (items read, runs located, keys scanned).
 template <class Emit>
-__device__ __forceinline__ void expand_frontier(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
-                                                const uint32_t* frontier, uint32_t F, const uint32_t* cand_u, uint32_t* bitmap, Emit emit) {
+__device__ __forceinline__ uint64_t expand_frontier(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
+                                                    const uint32_t* frontier, uint32_t F, const uint32_t* cand_u, uint32_t* bitmap,
+                                                    Emit emit) {
+    uint64_t bytes = 0;
     const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
@@ -194,6 +197,7 @@ __device__ __forceinline__ void expand_frontier(const DevGraph& g, uint32_t phas
             r = make_run(g, phase, view, dir, frontier[i], slot);
             nb = run_base_len(r, view);
             tot = run_seq_len(r, view);
+            if (sw.sl == 0) bytes += 12 + 4ull * tot;
         }
         const bool own = tot <= kRoundsPerItem * sw.W;
         const uint32_t rounds = wave_max_u32(own ? (tot + sw.W - 1) / sw.W : 0u);
@@ -217,6 +221,14 @@ __device__ __forceinline__ void expand_frontier(const DevGraph& g, uint32_t phas
             }
         }
     }
+    return bytes;
+}
+
+// Adds the wave's byte counts to *out (once per wave).
+__device__ __forceinline__ void add_bytes(unsigned long long* out, uint64_t bytes) {
+    if (!out) return;
+    bytes = wave_sum_u64(bytes);
+    if (lane_id() == 0 && bytes) atomicAdd(out, static_cast<unsigned long long>(bytes));
 }
 
 // Appends the wave's flagged vertices to list[*len ...] with one reservation (entries past cap are
@@ -232,10 +244,14 @@ __device__ __forceinline__ void append_fresh(bool fresh, uint32_t y, uint32_t* l
 
 __global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                                                       const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
-                                                      const uint32_t* cand_u, VSet out) {
+                                                      const uint32_t* cand_u, VSet out, unsigned long long* nbytes) {
     const uint32_t F = min(*frontier_len, max_frontier);
-    expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, out.bitmap,
-                    [&](bool fresh, uint32_t y) { append_fresh(fresh, y, out.list, out.len, out.cap); });
+    uint64_t written = 0;
+    const uint64_t bytes = expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, out.bitmap, [&](bool fresh, uint32_t y) {
+        written += fresh ? 4u : 0u;
+        append_fresh(fresh, y, out.list, out.len, out.cap);
+    });
+    add_bytes(nbytes, bytes + written);
 }
 
 // Expansion with the semi-joins in a second pass: k_local_queue expands the frontier along
@@ -244,18 +260,24 @@ __global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase
 // bitmap.  Survivors join out.list; the bits of the others are cleared.
 __global__ void __launch_bounds__(256) k_local_queue(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                                                      const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
-                                                     const uint32_t* cand_u, uint32_t* bitmap, uint32_t* queue, uint32_t* queue_len) {
+                                                     const uint32_t* cand_u, uint32_t* bitmap, uint32_t* queue, uint32_t* queue_len,
+                                                     unsigned long long* nbytes) {
     const uint32_t F = min(*frontier_len, max_frontier);
-    expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, bitmap,
-                    [&](bool fresh, uint32_t y) { append_fresh(fresh, y, queue, queue_len, 0xFFFFFFFFu); });
+    uint64_t written = 0;
+    const uint64_t bytes = expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, bitmap, [&](bool fresh, uint32_t y) {
+        written += fresh ? 4u : 0u;
+        append_fresh(fresh, y, queue, queue_len, 0xFFFFFFFFu);
+    });
+    add_bytes(nbytes, bytes + written);
 }
 
 // Each queued vertex gets a sub-group (the same split as the expansion), which scans its run of
 // every filter relation W keys a round for a member of the filter's bitmap, stopping at the first
 // hit; runs longer than kRoundsPerItem * W keys are scanned afterwards by the whole wave.
 __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase, const uint32_t* queue, const uint32_t* queue_len,
-                                                     uint32_t max_len, LocalFilters fl, VSet out) {
+                                                     uint32_t max_len, LocalFilters fl, VSet out, unsigned long long* nbytes) {
     const uint32_t lane = lane_id();
+    uint64_t bytes = 0;  // queued vertex read, runs located, keys scanned, survivor written
     const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t n = min(*queue_len, max_len);
@@ -275,6 +297,7 @@ __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase,
                 fr = make_run(g, phase, f.view, f.dir, y, f.slot);
                 fnb = run_base_len(fr, f.view);
                 ftot = run_seq_len(fr, f.view);
+                if (sw.sl == 0) bytes += 8;
             }
             const bool own = pass && ftot <= kRoundsPerItem * sw.W;
             if (pass && !own) pending |= 1u << k;
@@ -286,6 +309,7 @@ __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase,
                 if (own && !hit && e < ftot) {
                     const uint32_t fk = run_elem(g, fdl, fr, fnb, e);
                     h = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
+                    bytes += 4;
                 }
                 const uint64_t m = __ballot(h);
                 hit = hit || (m & sw.mask) != 0;
@@ -310,6 +334,7 @@ __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase,
                     if (t0 + lane < ftot) {
                         const uint32_t fk = run_elem(g, fdl, fr, fnb, t0 + lane);
                         h = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
+                        bytes += 4;
                     }
                     hit = __ballot(h) != 0;
                 }
@@ -319,7 +344,9 @@ __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase,
         }
         append_fresh(pass && sw.sl == 0, y, out.list, out.len, out.cap);
         if (active && !pass && sw.sl == 0) atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
+        if (active && sw.sl == 0) bytes += pass ? 8 : 4;
     }
+    add_bytes(nbytes, bytes);
 }
 
 // Clears the bitmaps of every set in `mask` (whole words of its listed members).
@@ -374,17 +401,22 @@ void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slo
 }
 
 void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
-                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, int blocks, hipStream_t s) {
+                  const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, int blocks,
+                  unsigned long long* bytes, hipStream_t s) {
     k_local_expand<<<grid_for(max_frontier, 4, blocks), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier,
-                                                                     cand_u, out);
+                                                                     cand_u, out, bytes);
 }
 
-void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
-                 const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
-                 uint32_t* queue, uint32_t* queue_len, int blocks, hipStream_t s) {
+void local_queue(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
+                 const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, uint32_t* queue,
+                 uint32_t* queue_len, int blocks, unsigned long long* bytes, hipStream_t s) {
     k_local_queue<<<grid_for(max_frontier, 4, blocks), 256, 0, s>>>(g, phase, view, dir, slot, frontier, frontier_len, max_frontier, cand_u,
-                                                                out.bitmap, queue, queue_len);
-    k_local_check<<<grid_for(max_frontier, 4, blocks), 256, 0, s>>>(g, phase, queue, queue_len, g.dv, fl, out);
+                                                                out.bitmap, queue, queue_len, bytes);
+}
+
+void local_check(const DevGraph& g, uint32_t phase, const LocalFilters& fl, VSet out, const uint32_t* queue,
+                 const uint32_t* queue_len, int blocks, unsigned long long* bytes, hipStream_t s) {
+    k_local_check<<<blocks, 256, 0, s>>>(g, phase, queue, queue_len, g.dv, fl, out, bytes);
 }
 
 void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s) {

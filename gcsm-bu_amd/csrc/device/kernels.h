@@ -54,7 +54,10 @@ void batch_admit_finish(const DevGraph& g, unsigned long long* adm, hipStream_t 
 // the vertices list[0, min(*n_list, max_n)).
 void slot_dir_update(const DevGraph& g, int dir, const uint32_t* list, const uint32_t* n_list, uint32_t max_n, hipStream_t s);
 void batch_apply_deletions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
-void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
+// bytes (or null): += algorithmic bytes, 8 * (old + inserted keys) + 16 per touched list plus 8 per
+// inserted key (read old and delta, write merged; list header; batch arrays).
+void batch_apply_insertions(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, unsigned long long* bytes,
+                            hipStream_t s);
 void batch_cleanup(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
 
 // ---- global candidates (NLF), summary ---------------------------------------------------------
@@ -99,9 +102,11 @@ void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slo
 // L(u) seed: every vertex reached from the frontier through the relation view (b -> u) that is a
 // candidate of u; the output set's bitmap (cleared beforehand) deduplicates the list.
 // `blocks`: grid of the local-index kernels (their frontier sizes are device-side).
+// bytes (or null): += algorithmic bytes of the sparse sweep: 4 per item read, 8 per run located
+// (two directory words), 4 per key scanned, 4 per vertex written; bitmap probes count 0.
 void local_expand(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                   const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
-                  const uint32_t* cand_u, VSet out, int blocks, hipStream_t s);
+                  const uint32_t* cand_u, VSet out, int blocks, unsigned long long* bytes, hipStream_t s);
 // Keeps the vertices y of `in` with a neighbour in L(b) along the view (u -> b).
 // Local index of one query vertex u in two launches: expand from the frontier along
 // (view, dir, slot), keep the candidates of cand_u that have a neighbour in every filter's bitmap.
@@ -113,10 +118,13 @@ struct LocalFilters {
     uint32_t n;
     LocalFilter f[kMaxQV - 1];
 };
-// queue (capacity dv) and *queue_len (zero on entry) hold the new candidates between the passes.
-void local_build(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
-                 const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, const LocalFilters& fl, VSet out,
-                 uint32_t* queue, uint32_t* queue_len, int blocks, hipStream_t s);
+// queue (capacity dv) and *queue_len (zero on entry) hold the new candidates between the passes:
+// local_queue fills them, local_check filters them into out.
+void local_queue(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* frontier,
+                 const uint32_t* frontier_len, uint32_t max_frontier, const uint32_t* cand_u, VSet out, uint32_t* queue,
+                 uint32_t* queue_len, int blocks, unsigned long long* bytes, hipStream_t s);
+void local_check(const DevGraph& g, uint32_t phase, const LocalFilters& fl, VSet out, const uint32_t* queue,
+                 const uint32_t* queue_len, int blocks, unsigned long long* bytes, hipStream_t s);
 struct VSetTable {
     VSet s[kMaxQV];
 };

@@ -603,7 +603,8 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
             prof_begin(w.prof, w.s, kKLocalIndex);
             if (bsv.size() == 1) {
                 local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u],
-                             local_grid_, w.s);
+                             local_grid_, bytes_ptr(kKLocalIndex), w.s);
+                prof_end(w.prof, w.s, kKLocalIndex);
             } else {
                 // semi-joins with the other built neighbours, fused into the expansion
                 LocalFilters fl{};
@@ -612,11 +613,15 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
                     const uint32_t rub = q.rel(u, bsv[k]);
                     fl.f[fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[bsv[k]].bitmap};
                 }
-                local_build(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), fl,
-                            w.L[u], w.tmp_list, w.L[0].len + kQueueCounters + u, local_grid_, w.s);
+                uint32_t* queue_len = w.L[0].len + kQueueCounters + u;
+                local_queue(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u),
+                            w.L[u], w.tmp_list, queue_len, local_grid_, bytes_ptr(kKLocalIndex), w.s);
+                prof_end(w.prof, w.s, kKLocalIndex);
+                prof_begin(w.prof, w.s, kKLocalCheck);
+                local_check(g_, ph, fl, w.L[u], w.tmp_list, queue_len, local_grid_, bytes_ptr(kKLocalCheck), w.s);
+                prof_end(w.prof, w.s, kKLocalCheck);
             }
             w.L_used |= static_cast<uint16_t>(1u << u);
-            prof_end(w.prof, w.s, kKLocalIndex);
             built |= 1u << u;
         }
         CSM_HIP(hipMemcpyAsync(w.h_pinned, w.L[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, w.s));
@@ -1141,7 +1146,9 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
         match_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         prof_begin(prof_main_, s_, kKApply);
         for (int d = 0; d < ndirs_; d++) {
-            batch_apply_insertions(g_, d, max_arcs, bs_[d], s_);
+            prof_begin(prof_main_, s_, kKMergeInsert);
+            batch_apply_insertions(g_, d, max_arcs, bs_[d], bytes_ptr(kKMergeInsert), s_);
+            prof_end(prof_main_, s_, kKMergeInsert);
             slot_dir_update(g_, d, bs_[d].touched, bs_[d].n_touched, max_arcs, s_);
         }
         prof_end(prof_main_, s_, kKApply);

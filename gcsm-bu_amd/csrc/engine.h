@@ -145,7 +145,7 @@ class Engine {
 
     // Kernel families timed with HIP events on the engine's stream when profiling is enabled.
     enum KernelFamily : uint32_t { kKExtendCount = 0, kKExtendWrite, kKCpEnumerate, kKLocalIndex, kKBatchBuild, kKApply, kKCpPrepare,
-                                   kKAnchors, kKLevelCount, kKLevelWrite, kKLevelPrepare, kKFamilies };
+                                   kKAnchors, kKLevelCount, kKLevelWrite, kKLevelPrepare, kKLocalCheck, kKMergeInsert, kKFamilies };
     struct KernelStat {
         uint64_t launches = 0;
         double ms = 0;

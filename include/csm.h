@@ -154,7 +154,9 @@ csm_status csm_restore(csm_session* s);
  * utils/cuda_helpers.h:78-102).  Kernel families: 0 depth-first enumeration (count), 1 depth-first
  * enumeration (write partial results), 2 Cartesian product, 3 local index, 4 batch build, 5 apply
  * batch, 6 Cartesian-product preparation, 7 anchors, 8 one-level extension (count), 9 one-level
- * extension (write a level), 10 one-level preparation (run lengths and their scan).  The bytes of
+ * extension (write a level), 10 one-level preparation (run lengths and their scan), 11 local-index
+ * semi-join check, 12 merge of a batch's insertions into the adjacency lists (family 3 is the
+ * local-index expansion, family 5 the whole graph update).  The bytes of
  * families 8 and 9 include the runs their preparation sizes.  Bit f of mask times family f with HIP events on the session's stream and,
  * for the enumeration kernels, counts algorithmic bytes (every view run read once plus candidates
  * written). */

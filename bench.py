@@ -279,6 +279,45 @@ def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_ma
             "kernel_ms": {k: round(v[1], 3) for k, v in stats.items()}}
 
 
+def plain_path(ffi, data_dir, queries, batch):
+    """The standing queries matched the reference's way, one query per session (per process in the
+    reference: its descriptors are __constant__ globals, match_gpu.cu:20-24), against the batched
+    form: one session holding every query.  Both apply warm-up batch 0 and then batches 1.. of the
+    stream; a batch's plain latency is the sum of the single-query sessions' times for it.  Every
+    query's per-batch counts and hashes must be identical in the two forms."""
+    s, nb = open_session(ffi, data_dir, queries, batch)
+    s.apply_batch(0)
+    s.synchronize()
+    multi_ms, multi = [], []
+    for b in range(1, nb):
+        t0 = time.perf_counter()
+        res, _ = s.apply_batch(b)
+        s.synchronize()
+        multi_ms.append((time.perf_counter() - t0) * 1e3)
+        multi.append(res)
+    s.close()
+    plain_ms = [0.0] * (nb - 1)
+    for k, q in enumerate(queries):
+        s, _ = open_session(ffi, data_dir, [q], batch)
+        s.apply_batch(0)
+        s.synchronize()
+        for b in range(1, nb):
+            t0 = time.perf_counter()
+            res, _ = s.apply_batch(b)
+            s.synchronize()
+            plain_ms[b - 1] += (time.perf_counter() - t0) * 1e3
+            if res[0] != multi[b - 1][k]:
+                raise RuntimeError(f"query {k} batch {b}: one-query session {res[0]} != batched {multi[b - 1][k]}")
+        s.close()
+        log(f"plain path: query {k} done")
+    matches = sum(r[0] + r[1] for res in multi for r in res)
+    return {"batches": f"1-{nb - 1} after warm-up batch 0", "queries": len(queries),
+            "batched_ms_per_batch": sum(multi_ms) / len(multi_ms), "plain_ms_per_batch": sum(plain_ms) / len(plain_ms),
+            "batched_p99_ms": max(multi_ms), "plain_p99_ms": max(plain_ms),
+            "batched_matches_per_s": matches / (sum(multi_ms) / 1e3), "plain_matches_per_s": matches / (sum(plain_ms) / 1e3),
+            "speedup": sum(plain_ms) / sum(multi_ms), "results": "every query's per-batch counts and hashes identical"}
+
+
 def config_block(config, meta, args, opts, scale):
     return {"workload": WORKLOADS[config], "config": config, "seed": args.seed, "scale": scale,
             "batch_edges": int(meta["batch_size"]), "queries": int(meta["queries"]),
@@ -365,10 +404,11 @@ def main():
         s.close()
         calib = {"kernel": "k_copy_u32", "bytes_read": 4 * words, "bytes_written": 4 * words, "ms": ms,
                  "GB/s": round(8 * words / (ms / 1e3) / 1e9, 1)}
-    one_lane = None
+    one_lane = plain = None
     if args.config == "c4":  # the same batches with one lane: every (query, edge) step in turn
         m1 = timed_batches(ffi, data_dir, queries, batch, args.steps, args.warmup, {"lanes": 1}, label="one lane: ")
         one_lane = {"ms_per_step": m1["ms_per_step"], "p99_batch_ms": m1["p99_batch_ms"], "lanes": 1}
+        plain = plain_path(ffi, data_dir, queries, batch)
     roofline = roofline_pass(ffi, data_dir, queries, batch, args.config)
     cpu = None if args.no_cpu_baseline else cpu_baseline(data_dir, queries, batch, args.cpu_seconds)
     out = {
@@ -382,6 +422,8 @@ def main():
     }
     if one_lane:
         out["one_lane"] = one_lane
+    if plain:
+        out["plain_path"] = plain
     if tuned:
         out["autotune"] = tuned
     if m["kernel_ms"]:

@@ -14,7 +14,7 @@ BIN_DIR = os.path.join(HERE, "bin")
 EXPORTED = [
     "csm_session_create", "csm_session_destroy", "csm_last_error", "csm_load_graph_file",
     "csm_load_query_file", "csm_open_update_file", "csm_build_index", "csm_plan",
-    "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_apply_range", "csm_apply_range_static", "csm_apply_edges", "csm_arena_usage",
+    "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_apply_range", "csm_apply_range_static", "csm_apply_edges", "csm_serve_start", "csm_serve_submit", "csm_serve_wait", "csm_serve_stop", "csm_arena_usage",
     "csm_get_relation_edge_counts",
     "csm_get_counts", "csm_pool_usage", "csm_read_candidates", "csm_read_adjacency",
     "csm_synchronize", "csm_snapshot", "csm_restore", "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
@@ -79,6 +79,10 @@ def load():
         lib.csm_apply_range.argtypes = [vp, u64, u64, P(QueryResult), u32, P(BatchInfo)]
         lib.csm_apply_range_static.argtypes = [vp, u64, u64, P(QueryResult), u32, P(BatchInfo)]
         lib.csm_apply_edges.argtypes = [vp, P(u32), P(u32), P(u32), P(ctypes.c_uint8), u64, P(QueryResult), u32, P(BatchInfo)]
+        lib.csm_serve_start.argtypes = [vp]
+        lib.csm_serve_submit.argtypes = [vp, P(u32), P(u32), P(u32), P(ctypes.c_uint8), u64, P(u64)]
+        lib.csm_serve_wait.argtypes = [vp, u64, P(QueryResult), u32, P(BatchInfo), P(ctypes.c_double)]
+        lib.csm_serve_stop.argtypes = [vp]
         lib.csm_arena_usage.argtypes = [vp, P(u64), P(u64)]
         lib.csm_get_relation_edge_counts.argtypes = [vp, u32, u64, P(u64), u32, P(u32)]
         lib.csm_get_counts.argtypes = [vp, P(u32), P(u32)]
@@ -212,6 +216,38 @@ class Session:
         st = self.lib.csm_apply_edges(self.h, *args, n, res, nq, ctypes.byref(info))
         self._check(st)
         return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info
+
+    @staticmethod
+    def _edge_args(src, dst, elabel, op):
+        n = len(src)
+        if hasattr(src, "ctypes"):
+            P32, P8 = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8)
+            return (src.ctypes.data_as(P32), dst.ctypes.data_as(P32), elabel.ctypes.data_as(P32), op.ctypes.data_as(P8)), n
+        A = ctypes.c_uint32 * max(1, n)
+        return (A(*src), A(*dst), A(*elabel), (ctypes.c_uint8 * max(1, n))(*op)), n
+
+    def serve_start(self):
+        """Starts the request loop (csm_serve_start)."""
+        self._check(self.lib.csm_serve_start(self.h))
+
+    def serve_submit(self, src, dst, elabel, op):
+        """Submits one batch; returns its ticket."""
+        args, n = self._edge_args(src, dst, elabel, op)
+        t = ctypes.c_uint64()
+        self._check(self.lib.csm_serve_submit(self.h, *args, n, ctypes.byref(t)))
+        return t.value
+
+    def serve_wait(self, ticket):
+        """Waits for a request: (results, info, latency_ms).  Raises CsmError if it failed."""
+        nq = len(self.queries)
+        res = (QueryResult * max(1, nq))()
+        info = BatchInfo()
+        ms = ctypes.c_double()
+        self._check(self.lib.csm_serve_wait(self.h, ticket, res, nq, ctypes.byref(info), ctypes.byref(ms)))
+        return [(r.positive, r.negative, r.positive_hash, r.negative_hash) for r in res[:nq]], info, ms.value
+
+    def serve_stop(self):
+        self._check(self.lib.csm_serve_stop(self.h))
 
     def arena_usage(self):
         u, c = ctypes.c_uint64(), ctypes.c_uint64()

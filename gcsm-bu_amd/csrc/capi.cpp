@@ -1,4 +1,5 @@
 // C ABI over the engine (include/csm.h).
+#include <algorithm>
 #include "../../include/csm.h"
 
 #include <cstring>
@@ -227,9 +228,43 @@ csm_status csm_apply_edges(csm_session* s, const uint32_t* src, const uint32_t* 
     return copy_results(out, bi, results, num_queries, info);
 }
 
+csm_status csm_serve_start(csm_session* s) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.serve_start());
+}
+
+csm_status csm_serve_submit(csm_session* s, const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op,
+                            uint64_t n, uint64_t* ticket) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (!ticket) return fail(s, CSM_ERR_ARGUMENT, "ticket pointer missing");
+    if (n && (!src || !dst || !elabel || !op)) return fail(s, CSM_ERR_ARGUMENT, "edge arrays missing");
+    return classify(s, s->engine.serve_submit(src, dst, elabel, op, n, *ticket));
+}
+
+csm_status csm_serve_wait(csm_session* s, uint64_t ticket, csm_query_result* results, uint32_t num_queries, csm_batch_info* info,
+                          double* latency_ms) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    if (num_queries && !results) return fail(s, CSM_ERR_ARGUMENT, "results array missing");
+    std::vector<QueryResult> out;
+    BatchInfo bi;
+    double ms = 0;
+    std::string err = s->engine.serve_wait(ticket, out, bi, ms);
+    if (latency_ms) *latency_ms = ms;
+    if (!err.empty()) return classify(s, err);
+    return copy_results(out, bi, results, num_queries, info);
+}
+
+csm_status csm_serve_stop(csm_session* s) {
+    if (!s) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.serve_stop());
+}
+
 csm_status csm_arena_usage(csm_session* s, uint64_t* used_words, uint64_t* capacity_words) {
     if (!s) return CSM_ERR_ARGUMENT;
-    if (used_words) *used_words = s->engine.arena_used_words();
+    if (used_words) {
+        const uint64_t top = s->engine.arena_used_words();
+        *used_words = top - std::min(top, s->engine.arena_free_words());
+    }
     if (capacity_words) *capacity_words = s->engine.arena_words();
     return CSM_OK;
 }

@@ -56,7 +56,17 @@ struct DevGraph {
     // >= s, so the run of slot s is [row[s], row[s + 1]).
     uint32_t nslots;
     uint32_t* sdir[2];
+    // Free arena blocks by size class (a block of class c holds 2^c words): the blocks that
+    // relocated lists leave behind.  A relocation takes a block of its class from `blk` before it
+    // bumps arena_top; blocks freed during a batch wait in `pend` and join `blk` only after the
+    // batch's merge (a block being copied out of must not be handed out in the same launch).
+    uint32_t* fl_blk;        // class c entries at fl_blk[fl_cls[c].x ..], at most fl_cls[c].y of them
+    uint32_t* fl_pend;       // same layout
+    int* fl_top;             // [kFreeClasses] available entries (may dip below 0 during a batch)
+    uint32_t* fl_pend_n;     // [kFreeClasses] blocks freed in the current batch
+    const uint2* fl_cls;     // [kFreeClasses] {first entry, capacity}
 };
+constexpr uint32_t kFreeClasses = 32;
 
 // One backward neighbour of the vertex matched at some depth: the partner is at position `pos`
 // of the matching order, its list along the query relation is the run `slot` of its adjacency in

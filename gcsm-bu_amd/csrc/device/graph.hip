@@ -324,10 +324,18 @@ __global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, B
             uint32_t p2 = 1;
             while (p2 < nd) p2 <<= 1;
             newcap = max(kMinNbrSize, 2u * p2);
-            unsigned long long base = 0;
+            unsigned long long base = ~0ull;
             if (lane == 0) {
-                base = atomicAdd(g.arena_top, static_cast<unsigned long long>(newcap));
-                if (base + newcap > g.arena_words) { atomicOr(g.err, kErrArenaFull); base = ~0ull; }
+                const uint32_t c = 31u - __clz(newcap);
+                const uint2 cl = g.fl_cls[c];
+                if (cl.y) {  // a block freed by an earlier batch
+                    const int k = atomicSub(&g.fl_top[c], 1) - 1;
+                    if (k >= 0) base = g.fl_blk[cl.x + k];
+                }
+                if (base == ~0ull) {
+                    base = atomicAdd(g.arena_top, static_cast<unsigned long long>(newcap));
+                    if (base + newcap > g.arena_words) { atomicOr(g.err, kErrArenaFull); base = ~0ull; }
+                }
             }
             base = __shfl(base, 0, 64);
             if (base == ~0ull) continue;
@@ -369,10 +377,36 @@ __global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, B
         wave_sync();
         if (lane == 0) {
             A.adj[v] = make_uint2(dst, nd);
-            if (relocate) A.cap[v] = newcap;
+            if (relocate) {
+                A.cap[v] = newcap;
+                const uint32_t c = 31u - __clz(cap);  // the old block (capacities are powers of two)
+                const uint2 cl = g.fl_cls[c];
+                const uint32_t j = cl.y ? atomicAdd(&g.fl_pend_n[c], 1u) : 0xFFFFFFFFu;
+                if (j < cl.y) g.fl_pend[cl.x + j] = a.x;  // else the block stays unused
+            }
         }
     }
     if (nbytes && lane == 0 && bytes) atomicAdd(nbytes, static_cast<unsigned long long>(bytes));
+}
+
+// After a batch: the blocks its relocations freed become available to the next batches.
+__global__ void k_free_merge(DevGraph g) {
+    for (uint32_t c = blockIdx.y; c < kFreeClasses; c += gridDim.y) {
+        const uint2 cl = g.fl_cls[c];
+        if (!cl.y) continue;
+        const uint32_t t = static_cast<uint32_t>(max(g.fl_top[c], 0));
+        const uint32_t n = min(g.fl_pend_n[c], cl.y - t);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            g.fl_blk[cl.x + t + i] = g.fl_pend[cl.x + i];
+    }
+}
+__global__ void k_free_merge_counts(DevGraph g) {
+    const uint32_t c = threadIdx.x;
+    if (c >= kFreeClasses) return;
+    const uint2 cl = g.fl_cls[c];
+    const uint32_t t = static_cast<uint32_t>(max(g.fl_top[c], 0));
+    g.fl_top[c] = static_cast<int>(t + min(g.fl_pend_n[c], cl.y - t));
+    g.fl_pend_n[c] = 0;
 }
 
 // Admission of a built batch before anything changes, one wave per touched vertex: the list space
@@ -382,12 +416,14 @@ __global__ void __launch_bounds__(256) k_apply_insertions(DevGraph g, int dir, B
 // is not in the list, or the same arc twice in the batch (equal neighbouring keys in the sorted
 // segments: an undirected edge given as both (a, b) and (b, a) lands in a's segment twice).
 __global__ void __launch_bounds__(256) k_admit(DevGraph g, int dir, BatchScratch w, unsigned long long* adm) {
+    __shared__ uint32_t cls_n[kFreeClasses];  // relocations of this block by size class
+    if (threadIdx.x < kFreeClasses) cls_n[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t T = *w.n_touched;
     const DevAdj A = g.dir[dir];
     const DevDelta D = g.delta[dir];
     const uint32_t lane = lane_id();
     const uint32_t wpb = blockDim.x >> 6;
-    unsigned long long need = 0;
     bool bad = false;
     for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < T; i += gridDim.x * wpb) {
         const uint32_t v = w.touched[i];
@@ -408,22 +444,29 @@ __global__ void __launch_bounds__(256) k_admit(DevGraph g, int dir, BatchScratch
             if (s.y && nd > A.cap[v]) {
                 uint32_t p2 = 1;
                 while (p2 < nd) p2 <<= 1;
-                need += max(kMinNbrSize, 2u * p2);
+                atomicAdd(&cls_n[31u - __clz(max(kMinNbrSize, 2u * p2))], 1u);
             }
         }
     }
-    need = wave_sum_u64(need);
     const bool any_bad = __ballot(bad) != 0;
-    if (lane == 0) {
-        if (need) atomicAdd(&adm[0], need);
-        if (any_bad) atomicOr(reinterpret_cast<uint32_t*>(&adm[1]), 1u);
-    }
+    if (lane == 0 && any_bad) atomicOr(reinterpret_cast<uint32_t*>(&adm[1]), 1u);
+    __syncthreads();
+    if (threadIdx.x < kFreeClasses && cls_n[threadIdx.x]) atomicAdd(&adm[6 + threadIdx.x], static_cast<unsigned long long>(cls_n[threadIdx.x]));
 }
 
-// adm[2..5] = {need, bad, arena top, error bits} for the host; adm[0..1] cleared for the next batch.
+// adm[2..5] = {need, bad, arena top, error bits} for the host, need = the words of the relocations
+// that the free blocks of their classes cannot serve; adm[0..1] and the class counts adm[6..37]
+// cleared for the next batch.
 __global__ void k_admit_finish(DevGraph g, unsigned long long* adm) {
     if (threadIdx.x != 0) return;
-    adm[2] = adm[0];
+    unsigned long long need = 0;
+    for (uint32_t c = 0; c < kFreeClasses; c++) {
+        const unsigned long long n = adm[6 + c];
+        const unsigned long long avail = g.fl_cls[c].y ? static_cast<unsigned long long>(max(g.fl_top[c], 0)) : 0ull;
+        if (n > avail) need += (n - avail) << c;
+        adm[6 + c] = 0;
+    }
+    adm[2] = need;
     adm[3] = adm[1];
     adm[4] = *g.arena_top;
     adm[5] = *g.err;
@@ -507,6 +550,11 @@ void batch_admit(const DevGraph& g, int dir, uint32_t max_touched, const BatchSc
 }
 
 void batch_admit_finish(const DevGraph& g, unsigned long long* adm, hipStream_t s) { k_admit_finish<<<1, 64, 0, s>>>(g, adm); }
+
+void free_blocks_merge(const DevGraph& g, hipStream_t s) {
+    k_free_merge<<<dim3(64, kFreeClasses), 256, 0, s>>>(g);
+    k_free_merge_counts<<<1, 64, 0, s>>>(g);
+}
 
 void batch_cleanup(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, hipStream_t s) {
     const int grid = static_cast<int>(std::min<uint32_t>(4096, (max_touched + 255) / 256 + 1));

@@ -45,11 +45,14 @@ uint64_t scan_scratch_elems(uint32_t n_max);
 
 void batch_build(const DevGraph& g, int dir, int dirsel, const BatchInput& b, uint32_t lo, uint32_t n,
                  const BatchScratch& w, hipStream_t s);
-// Admission of the built batch (per direction, then finish once): adm[0] += list words the
-// insertions will allocate, adm[1] |= a vertex losing more arcs than it has; finish moves
-// {need, bad, arena top, error bits} to adm[2..5] and clears adm[0..1] (zero before the first batch).
+// Admission of the built batch (per direction, then finish once): adm[6 + c] += the insertions'
+// relocations into blocks of size class c, adm[1] |= a precondition violation; finish moves
+// {need, bad, arena top, error bits} to adm[2..5], need = words the free blocks cannot serve, and
+// clears adm[0..1] and adm[6..37] (all zero before the first batch).
 void batch_admit(const DevGraph& g, int dir, uint32_t max_touched, const BatchScratch& w, unsigned long long* adm, hipStream_t s);
 void batch_admit_finish(const DevGraph& g, unsigned long long* adm, hipStream_t s);
+// The blocks freed by a batch's relocations become available (after its insertions are applied).
+void free_blocks_merge(const DevGraph& g, hipStream_t s);
 // Recomputes label-slot directory rows of direction dir: every vertex when list is null, else
 // the vertices list[0, min(*n_list, max_n)).
 void slot_dir_update(const DevGraph& g, int dir, const uint32_t* list, const uint32_t* n_list, uint32_t max_n, hipStream_t s);

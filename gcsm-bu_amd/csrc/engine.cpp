@@ -48,6 +48,7 @@ void* DevicePool::take_bytes(uint64_t bytes) {
 
 // ---- engine -----------------------------------------------------------------------------------
 Engine::~Engine() {
+    (void)serve_stop();
     if (s_) (void)hipStreamSynchronize(s_);
     for (Lane& w : lanes_)
         if (w.s) (void)hipStreamSynchronize(w.s);
@@ -136,6 +137,8 @@ uint32_t cap_for(uint32_t deg) {  // max(8, 2^(ceil(log2 deg) + 1)), deg 0 -> 8 
 }
 }  // namespace
 
+bool Engine::knows_elabel(uint32_t raw) const { return rank_of(elabel_rank_, raw) != kNoVertex; }
+
 uint32_t Engine::key_of_edge(uint32_t nbr, uint32_t elabel_raw) const {
     const uint32_t er = rank_of(elabel_rank_, elabel_raw);
     const uint32_t slot = er * nvl_ + rank_of(vlabel_rank_, host_g_.vlabel[nbr]);
@@ -223,7 +226,7 @@ std::string Engine::build() {
         add(max_arcs * 4); add(16); add(4 * (max_arcs + 1) * 4); add(max_arcs * 4);
         add(scan_scratch_elems(static_cast<uint32_t>(max_arcs)) * 4); add(16); add(2 * kMaxBigSegments * 4);
     }
-    add(stream_.size() * 17ull + max_batch_ * 17ull);          // stream + live batch
+    add(stream_.size() * 17ull + 2 * max_batch_ * 17ull);      // stream + two live-batch slots
     const uint32_t n_lanes = std::max<uint32_t>(1, std::min<uint32_t>(opt_.lanes, kMaxLanes));
     for (uint32_t l = 0; l < n_lanes; l++) {  // per lane: mirrors the takes of alloc below
         add(kMaxQV * (words_ * 4ull + dv_ * 4ull + 512) + 128);  // local sets
@@ -241,7 +244,11 @@ std::string Engine::build() {
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
     add(qs_.size() * 32 + 64);
     add(kKFamilies * 8);
-    add(64);                                                    // batch admission words
+    add(64 * 8);                                                // batch admission words
+    // free arena blocks: per size class 2^c (c >= 3) at most min(2^20, arena / 2^c) entries, twice
+    uint64_t free_entries = 0;
+    for (uint32_t c = 3; c < kFreeClasses; c++) free_entries += std::min<uint64_t>(1u << 20, arena_words >> c);
+    add(free_entries * 4); add(free_entries * 4); add(kFreeClasses * 16);
     add(kMaxQV * qs_.size() * sizeof(NlfEntry) + kMaxRel * qs_.size() * sizeof(NlfGroup) + 512);
     const uint64_t sdir_bytes = (nslots + 1) * dv_ * 4ull;
     const bool use_sdir = opt_.slot_directory && sdir_bytes * ndirs_ <= kMaxSlotDirectoryBytes;
@@ -352,10 +359,10 @@ std::string Engine::build() {
     // ---- stream ----
     {
         stream_cap_ = stream_.size();
-        uint32_t* sb = pool_.take<uint32_t>(4 * (stream_cap_ + max_batch_));
-        uint8_t* ob = pool_.take<uint8_t>(stream_cap_ + max_batch_);
+        uint32_t* sb = pool_.take<uint32_t>(4 * (stream_cap_ + 2ull * max_batch_));
+        uint8_t* ob = pool_.take<uint8_t>(stream_cap_ + 2ull * max_batch_);
         if (!sb || !ob) return tagged(Code::kMemory, "device pool exhausted (stream)");
-        const uint64_t S = stream_cap_ + max_batch_;
+        const uint64_t S = stream_cap_ + 2ull * max_batch_;
         bin_.src = sb;
         bin_.dst = sb + S;
         bin_.key_fwd = sb + 2 * S;
@@ -425,9 +432,31 @@ std::string Engine::build() {
     prof_bytes_ = pool_.take<unsigned long long>(kKFamilies);
     CSM_HIP(hipMemsetAsync(prof_bytes_, 0, kKFamilies * 8, s_));
     summary_ = pool_.take<unsigned long long>(4);
-    adm_ = pool_.take<unsigned long long>(8);
+    adm_ = pool_.take<unsigned long long>(64);
     if (!adm_) return tagged(Code::kMemory, "device pool exhausted (admission words)");
-    CSM_HIP(hipMemsetAsync(adm_, 0, 64, s_));
+    CSM_HIP(hipMemsetAsync(adm_, 0, 64 * 8, s_));
+    {
+        std::vector<uint2> cls(kFreeClasses, make_uint2(0, 0));
+        uint64_t n = 0;
+        for (uint32_t c = 3; c < kFreeClasses; c++) {
+            const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(1u << 20, g_.arena_words >> c));
+            cls[c] = make_uint2(static_cast<uint32_t>(n), k);
+            n += k;
+        }
+        free_entries_ = n;
+        g_.fl_blk = pool_.take<uint32_t>(n);
+        g_.fl_pend = pool_.take<uint32_t>(n);
+        g_.fl_top = pool_.take<int>(kFreeClasses);
+        g_.fl_pend_n = pool_.take<uint32_t>(kFreeClasses);
+        uint2* d_cls = pool_.take<uint2>(kFreeClasses);
+        if (!g_.fl_blk || !g_.fl_pend || !d_cls) return tagged(Code::kMemory, "device pool exhausted (free arena blocks)");
+        g_.fl_cls = d_cls;
+        h_free_cls_ = cls;
+        CSM_HIP(hipMemcpyAsync(d_cls, cls.data(), kFreeClasses * sizeof(uint2), hipMemcpyHostToDevice, s_));
+        CSM_HIP(hipMemsetAsync(g_.fl_top, 0, kFreeClasses * 4, s_));
+        CSM_HIP(hipMemsetAsync(g_.fl_pend_n, 0, kFreeClasses * 4, s_));
+        CSM_HIP(hipStreamSynchronize(s_));
+    }
 
     // ---- queries: relation slots, NLF programs, candidate bitmaps ----
     for (auto& qsp : qs_) {
@@ -527,6 +556,19 @@ uint64_t Engine::arena_used_words() {
         return 0;
     std::memcpy(&top, h_pinned_ + 64, 8);
     return top;
+}
+
+uint64_t Engine::arena_free_words() {
+    if (!built_) return 0;
+    if (hipMemcpyAsync(h_pinned_ + 128, g_.fl_top, kFreeClasses * 4, hipMemcpyDeviceToHost, s_) != hipSuccess ||
+        hipStreamSynchronize(s_) != hipSuccess)
+        return 0;
+    uint64_t words = 0;
+    for (uint32_t c = 0; c < kFreeClasses; c++) {
+        const int t = static_cast<int>(h_pinned_[128 + c]);
+        if (t > 0) words += static_cast<uint64_t>(t) << c;
+    }
+    return words;
 }
 
 std::string Engine::check_err(const char* where) {
@@ -1008,6 +1050,7 @@ std::string Engine::run_phase(uint32_t ph) {
 }
 
 std::string Engine::run_batch(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info) {
+    if (srv_) return tagged(Code::kState, "the session is serving: batches go through the request loop");
     if (!built_) return tagged(Code::kState, "index not built");
     if (lo + n > stream_.size()) return tagged(Code::kArgument, "batch range beyond the update stream");
     return run_resident(lo, n, stream_.src.data() + lo, stream_.dst.data() + lo, stream_.op.data() + lo, out, info);
@@ -1021,6 +1064,7 @@ uint64_t Engine::static_ring_max_edges() const {
 }
 
 std::string Engine::run_batch_static(uint64_t lo, uint64_t n, std::vector<QueryResult>& out, BatchInfo& info) {
+    if (srv_) return tagged(Code::kState, "the session is serving: batches go through the request loop");
     if (!built_) return tagged(Code::kState, "index not built");
     if (lo + n > stream_.size()) return tagged(Code::kArgument, "batch range beyond the update stream");
     if (!static_ring_) return tagged(Code::kState, "static ring unavailable on this device");
@@ -1050,6 +1094,7 @@ std::string Engine::run_batch_static(uint64_t lo, uint64_t n, std::vector<QueryR
 // max(8, 2 * next_pow2(length))) on the host mirrors of degrees and capacities.
 std::string Engine::run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
                               std::vector<QueryResult>& out, BatchInfo& info) {
+    if (srv_) return tagged(Code::kState, "the session is serving: batches go through the request loop");
     if (!built_) return tagged(Code::kState, "index not built");
     if (n > max_batch_) return tagged(Code::kLimit, "batch larger than the scratch was sized for (max_batch_edges)");
     std::vector<uint32_t> kf(n), kb(n);
@@ -1151,6 +1196,7 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
             prof_end(prof_main_, s_, kKMergeInsert);
             slot_dir_update(g_, d, bs_[d].touched, bs_[d].n_touched, max_arcs, s_);
         }
+        free_blocks_merge(g_, s_);  // blocks the relocations left behind serve the next batches
         prof_end(prof_main_, s_, kKApply);
     }
     for (int d = 0; d < ndirs_; d++)
@@ -1286,6 +1332,7 @@ struct Region {
 }  // namespace
 
 std::string Engine::snapshot() {
+    if (srv_) return tagged(Code::kState, "the session is serving: batches go through the request loop");
     if (!built_) return tagged(Code::kState, "build the index before taking a snapshot");
     if (!failed_.empty()) return failed_;
     CSM_HIP(hipStreamSynchronize(s_));
@@ -1297,6 +1344,8 @@ std::string Engine::snapshot() {
         if (g_.sdir[d]) rs.push_back({g_.sdir[d], (g_.nslots + 1ull) * dv_ * 4});
     }
     for (auto& qs : qs_) rs.push_back({qs->cand, static_cast<uint64_t>(qs->q.n) * words_ * 4});
+    rs.push_back({g_.fl_top, kFreeClasses * 4ull});
+    rs.push_back({g_.fl_blk, free_entries_ * 4});
     uint64_t need = 0;
     for (const Region& r : rs) need += (r.bytes + 255) & ~255ull;
     // the arena part may grow between snapshots: keep room for the whole arena
@@ -1323,6 +1372,7 @@ std::string Engine::snapshot() {
 }
 
 std::string Engine::restore() {
+    if (srv_) return tagged(Code::kState, "the session is serving: batches go through the request loop");
     if (!snap_) return tagged(Code::kState, "no snapshot taken");
     CSM_HIP(hipStreamSynchronize(s_));
     for (Lane& w : lanes_) CSM_HIP(hipStreamSynchronize(w.s));
@@ -1333,6 +1383,8 @@ std::string Engine::restore() {
         if (g_.sdir[d]) rs.push_back({g_.sdir[d], (g_.nslots + 1ull) * dv_ * 4});
     }
     for (auto& qs : qs_) rs.push_back({qs->cand, static_cast<uint64_t>(qs->q.n) * words_ * 4});
+    rs.push_back({g_.fl_top, kFreeClasses * 4ull});
+    rs.push_back({g_.fl_blk, free_entries_ * 4});
     uint64_t off = 0;
     for (const Region& r : rs) {
         if (r.bytes) CSM_HIP(hipMemcpyAsync(r.p, snap_ + off, r.bytes, hipMemcpyDeviceToDevice, s_));

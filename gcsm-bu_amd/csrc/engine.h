@@ -8,11 +8,16 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <condition_variable>
 #include <cstdint>
 #include <cstdio>
+#include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/csm.h"
@@ -116,6 +121,14 @@ class Engine {
     std::string run_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
                           std::vector<QueryResult>& out, BatchInfo& info);
     std::string synchronize();
+    // Serving loop (continuous batching of in-memory batches): requests run in submission order;
+    // while one runs, the next ones are validated, keyed and copied to the device on a copy stream
+    // into the other of two staging slots.  Other batch calls are refused while it runs.
+    std::string serve_start();
+    std::string serve_submit(const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op, uint64_t n,
+                             uint64_t& ticket);
+    std::string serve_wait(uint64_t ticket, std::vector<QueryResult>& out, BatchInfo& info, double& latency_ms);
+    std::string serve_stop();
     // Device copy of the graph state (adjacency arena in use, list headers, label-slot directory,
     // candidate bitmaps, arena bump pointer) taken between batches, and its restoration: the session
     // then continues as if the batches after the snapshot had never been applied.
@@ -139,7 +152,8 @@ class Engine {
     std::string read_candidates(uint32_t qid, uint32_t u, std::vector<uint32_t>& bitmap);
     std::string read_adjacency(uint32_t v, uint32_t dir, std::vector<uint32_t>& ids, std::vector<uint32_t>& labels);
     uint32_t id_bits() const { return id_bits_; }
-    uint64_t arena_used_words();  // words of the adjacency arena in use (device bump pointer)
+    uint64_t arena_used_words();  // words of the adjacency arena below the device bump pointer
+    uint64_t arena_free_words();  // of those, the words in free blocks (reused by later relocations)
     uint64_t arena_words() const { return h_arena_words_; }
     uint32_t last_error_bits() const { return last_err_; }
 
@@ -260,9 +274,41 @@ class Engine {
     unsigned long long* summary_ = nullptr;   // [2]
     uint32_t* h_pinned_ = nullptr;            // small pinned read-back buffer (main stream)
     uint32_t* static_ring_ = nullptr;         // this device's address of the static ring (scratch.hip)
+    uint64_t free_entries_ = 0;               // entries of the free-block lists (all classes)
+    std::vector<uint2> h_free_cls_;           // {first entry, capacity} of each size class
     int local_grid_ = 128;                    // blocks of the local-index kernels (set at build)
     uint32_t last_err_ = 0;
     std::string failed_;  // set when a batch failed after changing the graph: every later batch is refused
+    // serving loop state (serve.cpp)
+    struct ServeReq {
+        uint64_t id = 0, n = 0;
+        std::vector<uint32_t> src, dst, elabel;
+        std::vector<uint8_t> op;
+        int slot = -1;
+        bool done = false;
+        std::string err;
+        std::vector<QueryResult> out;
+        BatchInfo info;
+        std::chrono::steady_clock::time_point t_submit, t_done;
+    };
+    struct Server {
+        std::mutex mu;
+        std::condition_variable cv;
+        std::deque<std::shared_ptr<ServeReq>> incoming, staged;
+        std::map<uint64_t, std::shared_ptr<ServeReq>> open;  // submitted and not yet waited for
+        bool slot_busy[2] = {false, false};
+        bool stop = false;
+        uint64_t next_id = 1;
+        hipStream_t copy = nullptr;
+        hipEvent_t ready[2] = {nullptr, nullptr};
+        uint8_t* pinned[2] = {nullptr, nullptr};  // staging: src, dst, key_fwd, key_bwd (u32 each), op
+        std::thread stager, runner;
+    };
+    std::unique_ptr<Server> srv_;
+    void serve_stage_loop();
+    void serve_run_loop();
+    std::string stage_keys(const ServeReq& r, uint32_t* kf, uint32_t* kb) const;
+    bool knows_elabel(uint32_t raw) const;
     // snapshot(): one device allocation outside the pool holding the graph state
     char* snap_ = nullptr;
     uint64_t snap_bytes_ = 0, snap_top_ = 0;

@@ -126,7 +126,27 @@ csm_status csm_apply_range_static(csm_session* s, uint64_t first, uint64_t n, cs
  * CSM_ERR_MEMORY before any device work, leaving the session as it was. */
 csm_status csm_apply_edges(csm_session* s, const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op,
                            uint64_t n, csm_query_result* results, uint32_t num_queries, csm_batch_info* info);
-/* Extension: adjacency arena words in use (initial lists plus every relocated list) and capacity. */
+/* Extension (serving; no counterpart in the reference, whose batches come only from the update
+ * file and run one after another, technique_permutation.h:127-215): continuous batching of
+ * in-memory batches.  csm_serve_start starts the session's request loop (the session is built and
+ * every query planned; set csm_options.max_batch_edges).  csm_serve_submit copies one batch (same
+ * arrays as csm_apply_edges; they may be reused on return) and returns its ticket.  Requests apply
+ * in submission order; while one runs on the GPU, the next is validated, keyed and copied to the
+ * device on a copy stream into the other of two staging slots, and a request enters a slot as soon
+ * as one is free; the adjacency arena then admits it (reusing the blocks earlier relocations
+ * freed) or refuses it with CSM_ERR_MEMORY, changing nothing.  csm_serve_wait blocks until the
+ * request is done and returns its results (equal to csm_apply_edges on the same sequence of
+ * batches) and its latency from submission to completion.  While serving, the other batch calls
+ * and snapshots return CSM_ERR_STATE.  csm_serve_stop finishes every submitted request and stops
+ * the loop (session destruction stops it too). */
+csm_status csm_serve_start(csm_session* s);
+csm_status csm_serve_submit(csm_session* s, const uint32_t* src, const uint32_t* dst, const uint32_t* elabel, const uint8_t* op,
+                            uint64_t n, uint64_t* ticket);
+csm_status csm_serve_wait(csm_session* s, uint64_t ticket, csm_query_result* results, uint32_t num_queries, csm_batch_info* info,
+                          double* latency_ms);
+csm_status csm_serve_stop(csm_session* s);
+/* Extension: adjacency arena words in use (the blocks of every current list; blocks that relocated
+ * lists left behind are reused by later batches and do not count) and capacity. */
 csm_status csm_arena_usage(csm_session* s, uint64_t* used_words, uint64_t* capacity_words);
 /* Per-relation edge counts of the reference's grouping (GraphFileIoManager::PrintNumDataEdges,
  * graph.cpp:470-480): batch_index = UINT64_MAX for the initial graph. */

@@ -539,6 +539,63 @@ def test_c5_directed_serving(lib, tmp_path):
             lo += x
 
 
+def test_serving_loop_solo_equals_mix(lib, tmp_path):
+    """Continuous batching: the directed serving workload's variable batches submitted to the
+    request loop all at once (each staged and copied while the one before it runs on the GPU) give
+    every request exactly the results it gets when the batches are applied alone, one after another
+    (csm_apply_edges), and those equal the oracle's.  A request that breaks a precondition is
+    refused in the mix and changes nothing: the requests after it still match."""
+    meta, queries = util.gen("c5", tmp_path, seed=2, scale=0.005)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    sizes = [int(x) for x in open(tmp_path / "batches.txt")]
+    res, _ = util.oracle_stream(data, upd, 1, queries, threads=16, directed=True,
+                                extra=["--batch-sizes", str(tmp_path / "batches.txt")])
+    src, dst, lab, op = _read_stream(upd)
+    batches, lo = [], 0
+    for x in sizes:
+        batches.append((src[lo:lo + x], dst[lo:lo + x], lab[lo:lo + x], op[lo:lo + x]))
+        lo += x
+    bad = ([src[0]], [dst[0]], [lab[0]], [op[0]])  # the first update again: inserts an edge twice / deletes one twice
+    say(f"case serving c5 scale=0.005 seed=2: {len(batches)} requests solo, then in the request loop")
+
+    def session():
+        s = lib.Session(device=0, directed=True, max_batch_edges=max(sizes))
+        s.load_graph(str(data))
+        for q in queries:
+            s.add_query(q)
+        s.build()
+        for q in s.queries:
+            s.plan(q)
+        return s
+
+    solo = []
+    with session() as s:
+        for b in batches:
+            solo.append(s.apply_edges(*b)[0])
+    for bi, r in enumerate(solo):
+        for k, x in enumerate(r):
+            assert x == res[(bi, k)], f"solo batch {bi} query {k}"
+    with session() as s:
+        s.serve_start()
+        tickets = [s.serve_submit(*b) for b in batches[:2]] + [s.serve_submit(*bad)] + [s.serve_submit(*b) for b in batches[2:]]
+        got, lat = [], []
+        for i, t in enumerate(tickets):
+            if i == 2:
+                with pytest.raises(lib.CsmError) as e:
+                    s.serve_wait(t)
+                assert e.value.status == 8  # PRECONDITION, refused at admission
+                continue
+            r, info, ms = s.serve_wait(t)
+            got.append(r)
+            lat.append(ms)
+        with pytest.raises(lib.CsmError) as e:
+            s.apply_edges(*batches[0])
+        assert e.value.status == 7  # other batch calls are refused while serving
+        s.serve_stop()
+    assert got == solo
+    assert all(x > 0 for x in lat)
+
+
 def _edges_of(path):
     es = {}
     for line in open(path):

@@ -131,104 +131,186 @@ __global__ void k_local_anchors(DevGraph g, uint32_t phase, uint32_t dir, uint32
     }
 }
 
-// Work split of the local-index kernels.  The frontier (or queue) size is only known on the
-// device, so each launch picks it there: G items per wave, the smallest power of two that covers
-// the items with the grid's waves, each item scanned by a sub-group of W = 64 / G lanes.  Small
-// frontiers thus get a whole wave per item (short latency), large ones one lane per item (64
-// items' loads in flight per wave).  A run longer than kRoundsPerItem * W keys is left to the whole
-// wave afterwards, so a hub's run never serialises one sub-group.
-constexpr uint32_t kRoundsPerItem = 4;
-
-struct SubWave {
-    uint32_t G, W, sub, sl;  // items per wave, lanes per item, this lane's item slot and lane in it
-    uint64_t mask;           // lanes of this lane's sub-group
+// Work split of the local-index kernels.  The frontier (or queue) size is only known on the device,
+// so each launch picks it there: G items per wave, the smallest power of two that covers the items
+// with the grid's waves (a whole wave per item for small frontiers, up to one lane per item for
+// large ones).  The runs of a wave's G items are then flattened: a wave-wide prefix of their lengths
+// lets each lane take one element of one run per round (64 elements a round, whatever the mix of
+// short runs and hub runs), finding its item by a binary search of the prefix in LDS.
+struct FlatLds {
+    uint32_t pre[4][65];   // exclusive prefix of the items' run lengths, [64] = total
+    RunView run[4][64];
+    uint32_t nb[4][64];    // base-run length of each item's run
+    uint32_t job[4][64];   // job of each item
+    uint32_t flag[4][64];  // check: the item's run holds a member of the filter bitmap
+    const uint32_t* bm[4][64];
+    uint32_t vd[4][64];    // check: view | dir << 8 of the item's filter relation
 };
-
-__device__ __forceinline__ SubWave sub_wave(uint32_t items) {
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    uint32_t G = 1;
-    while (G < 64 && static_cast<uint64_t>(G) * waves < items) G <<= 1;
-    SubWave w;
-    w.G = G;
-    w.W = 64 / G;
-    const uint32_t lane = lane_id();
-    w.sub = lane / w.W;
-    w.sl = lane % w.W;
-    w.mask = (w.W == 64 ? ~0ull : ((1ull << w.W) - 1)) << (w.sub * w.W);
-    return w;
-}
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     for (int o = 32; o > 0; o >>= 1) x = max(x, static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 64)));
     return x;
 }
 
-__device__ __forceinline__ RunView shfl_run(const RunView& r, uint32_t src) {
-    return RunView{static_cast<uint32_t>(__shfl(static_cast<int>(r.lo), src, 64)), static_cast<uint32_t>(__shfl(static_cast<int>(r.hi), src, 64)),
-                   static_cast<uint32_t>(__shfl(static_cast<int>(r.dlo), src, 64)), static_cast<uint32_t>(__shfl(static_cast<int>(r.dhi), src, 64))};
+__device__ __forceinline__ uint32_t items_per_wave(uint32_t items) {
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    uint32_t G = 1;
+    while (G < 64 && static_cast<uint64_t>(G) * waves < items) G <<= 1;
+    return G;
 }
 
-// Expands the frontier along (view, dir, slot) into cand_u: every element y of a frontier vertex's
-// run that is a candidate of u and live in the view is offered to `bitmap` (atomic test-and-set),
-// and `emit(fresh, y)` runs once per round for the whole wave (fresh: this lane set y's bit first).
-// Returns this lane's algorithmic bytes (items read, runs located, keys scanned).
-template <class Emit>
-__device__ __forceinline__ uint64_t expand_frontier(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
-                                                    const uint32_t* frontier, uint32_t F, const uint32_t* cand_u, uint32_t* bitmap,
-                                                    Emit emit) {
-    uint64_t bytes = 0;
+// Wave-wide exclusive prefix of x; returns it and sets total.
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t& total) {
     const uint32_t lane = lane_id();
-    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    const SubWave sw = sub_wave(F);
-    const uint32_t* dl = phase_delta(g, dir, phase);
-    auto offer = [&](uint32_t key, const RunView& r, bool& fresh, uint32_t& y) {
-        y = key & g.id_mask;
-        if (bit_test(cand_u, y) && run_elem_live(dl, r, view, key)) {
-            const uint32_t bit = 1u << (y & 31);
-            fresh = !(atomicOr(&bitmap[y >> 5], bit) & bit);  // else offered by another visitor
-        }
-    };
-    for (uint32_t base = gw * sw.G; base < F; base += waves * sw.G) {
-        const uint32_t i = base + sw.sub;
+    uint32_t inc = x;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+        if (lane >= o) inc += y;
+    }
+    total = static_cast<uint32_t>(__shfl(static_cast<int>(inc), 63, 64));
+    return inc - x;
+}
+
+// The item of the wave whose run holds flattened element e (largest k with pre[k] <= e).
+__device__ __forceinline__ uint32_t owner_of(const uint32_t* pre, uint32_t e) {
+    uint32_t lo = 0, hi = 63;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct Rel4 {
+    uint32_t phase, view, dir, slot;
+};
+
+// Expands `total` frontier items: item(i, j, x) gives item i's job and frontier vertex, rel(j) the
+// job's expansion relation, target(j, cand, bitmap) its candidate bitmap and output bitmap.  Every
+// element y of an item's run that is a candidate and live in the view is offered to the output
+// bitmap (atomic test-and-set); emit(fresh, y, j) is called by the whole wave once per round.
+// Returns this lane's algorithmic bytes.
+template <class Item, class Rel, class Target, class Emit>
+__device__ __forceinline__ uint64_t expand_flat(const DevGraph& g, FlatLds& lds, uint32_t total, Item item, Rel rel, Target target,
+                                                Emit emit) {
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wv, waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t G = items_per_wave(total);
+    uint64_t bytes = 0;
+    for (uint32_t base = gw * G; base < total; base += waves * G) {
+        const uint32_t i = base + lane;
+        uint32_t tot = 0, j = 0, nb = 0;
         RunView r{0, 0, 0, 0};
-        uint32_t nb = 0, tot = 0;
-        if (i < F) {
-            r = make_run(g, phase, view, dir, frontier[i], slot);
-            nb = run_base_len(r, view);
-            tot = run_seq_len(r, view);
-            if (sw.sl == 0) bytes += 12 + 4ull * tot;
+        if (lane < G && i < total) {
+            uint32_t x;
+            item(i, j, x);
+            const Rel4 R = rel(j);
+            r = make_run(g, R.phase, R.view, R.dir, x, R.slot);
+            nb = run_base_len(r, R.view);
+            tot = run_seq_len(r, R.view);
+            bytes += 12 + 4ull * tot;
         }
-        const bool own = tot <= kRoundsPerItem * sw.W;
-        const uint32_t rounds = wave_max_u32(own ? (tot + sw.W - 1) / sw.W : 0u);
-        for (uint32_t t = 0; t < rounds; t++) {
+        uint32_t T;
+        const uint32_t pre = wave_prefix(tot, T);
+        lds.pre[wv][lane] = pre;
+        lds.run[wv][lane] = r;
+        lds.nb[wv][lane] = nb;
+        lds.job[wv][lane] = j;
+        if (lane == 63) lds.pre[wv][64] = T;
+        wave_sync();
+        for (uint32_t e0 = 0; e0 < T; e0 += 64) {  // wave-uniform trip count
+            const uint32_t e = e0 + lane;
             bool fresh = false;
-            uint32_t y = 0;
-            const uint32_t e = t * sw.W + sw.sl;
-            if (own && e < tot) offer(run_elem(g, dl, r, nb, e), r, fresh, y);
-            emit(fresh, y);
-        }
-        for (uint64_t longs = __ballot(!own && sw.sl == 0); longs; longs &= longs - 1) {
-            const uint32_t src = static_cast<uint32_t>(__ffsll(static_cast<long long>(longs)) - 1);
-            const RunView lr = shfl_run(r, src);
-            const uint32_t lnb = static_cast<uint32_t>(__shfl(static_cast<int>(nb), src, 64));
-            const uint32_t ltot = static_cast<uint32_t>(__shfl(static_cast<int>(tot), src, 64));
-            for (uint32_t t0 = 0; t0 < ltot; t0 += 64) {  // wave-uniform trip count
-                bool fresh = false;
-                uint32_t y = 0;
-                if (t0 + lane < ltot) offer(run_elem(g, dl, lr, lnb, t0 + lane), lr, fresh, y);
-                emit(fresh, y);
+            uint32_t y = 0, jj = 0;
+            if (e < T) {
+                const uint32_t k = owner_of(lds.pre[wv], e);
+                jj = lds.job[wv][k];
+                const RunView rr = lds.run[wv][k];
+                const Rel4 R = rel(jj);
+                const uint32_t* dl = phase_delta(g, R.dir, R.phase);
+                const uint32_t key = run_elem(g, dl, rr, lds.nb[wv][k], e - lds.pre[wv][k]);
+                y = key & g.id_mask;
+                const uint32_t* cand_u;
+                uint32_t* bm;
+                target(jj, cand_u, bm);
+                if (bit_test(cand_u, y) && run_elem_live(dl, rr, R.view, key)) {
+                    const uint32_t bit = 1u << (y & 31);
+                    fresh = !(atomicOr(&bm[y >> 5], bit) & bit);  // else offered by another visitor
+                }
             }
+            emit(fresh, y, jj);
+            bytes += fresh ? 4u : 0u;
         }
+        wave_sync();  // the LDS rows are rewritten by the next items
     }
     return bytes;
 }
 
-// Adds the wave's byte counts to *out (once per wave).
-__device__ __forceinline__ void add_bytes(unsigned long long* out, uint64_t bytes) {
-    if (!out) return;
-    bytes = wave_sum_u64(bytes);
-    if (lane_id() == 0 && bytes) atomicAdd(out, static_cast<unsigned long long>(bytes));
+// Keeps the `total` queued items whose run of every filter relation holds a member of the
+// filter's bitmap: item(i, j, y, nf) gives item i's job, vertex and number of filters,
+// filter(j, k) job j's filter k, phase_of(j) its phase; keep(pass, y, j) is called by the whole wave
+// once per item group (pass only on the item's first lane).  The filters' runs are flattened like
+// the expansion's, one filter at a time; an item stops being scanned at its first hit.
+template <class Item, class Filter, class Phase, class Keep>
+__device__ __forceinline__ uint64_t check_flat(const DevGraph& g, FlatLds& lds, uint32_t total, Item item, Filter filter,
+                                               Phase phase_of, Keep keep) {
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wv, waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t G = items_per_wave(total);
+    uint64_t bytes = 0;
+    for (uint32_t base = gw * G; base < total; base += waves * G) {
+        const uint32_t i = base + lane;
+        const bool active = lane < G && i < total;
+        uint32_t j = 0, y = 0, nf = 0;
+        if (active) item(i, j, y, nf);
+        bool pass = active;
+        const uint32_t maxf = wave_max_u32(nf);
+        for (uint32_t k = 0; k < maxf; k++) {
+            const bool mine = pass && k < nf;
+            uint32_t tot = 0, nb = 0;
+            RunView r{0, 0, 0, 0};
+            LocalFilter f{0, 0, 0, nullptr};
+            if (mine) {
+                f = filter(j, k);
+                r = make_run(g, phase_of(j), f.view, f.dir, y, f.slot);
+                nb = run_base_len(r, f.view);
+                tot = run_seq_len(r, f.view);
+                bytes += 8;
+            }
+            uint32_t T;
+            const uint32_t pre = wave_prefix(tot, T);
+            lds.pre[wv][lane] = pre;
+            lds.run[wv][lane] = r;
+            lds.nb[wv][lane] = nb;
+            lds.job[wv][lane] = j;
+            lds.bm[wv][lane] = f.bitmap;
+            lds.vd[wv][lane] = f.view | (f.dir << 8);
+            lds.flag[wv][lane] = 0;
+            if (lane == 63) lds.pre[wv][64] = T;
+            wave_sync();
+            for (uint32_t e0 = 0; e0 < T; e0 += 64) {  // wave-uniform trip count
+                const uint32_t e = e0 + lane;
+                if (e < T) {
+                    const uint32_t o = owner_of(lds.pre[wv], e);
+                    if (!lds.flag[wv][o]) {  // an item is done at its first hit
+                        const uint32_t vd = lds.vd[wv][o];
+                        const uint32_t* fdl = phase_delta(g, vd >> 8, phase_of(lds.job[wv][o]));
+                        const RunView rr = lds.run[wv][o];
+                        const uint32_t fk = run_elem(g, fdl, rr, lds.nb[wv][o], e - lds.pre[wv][o]);
+                        bytes += 4;
+                        if (bit_test(lds.bm[wv][o], fk & g.id_mask) && run_elem_live(fdl, rr, vd & 0xFFu, fk)) lds.flag[wv][o] = 1u;
+                    }
+                }
+                wave_sync();
+            }
+            if (mine) pass = lds.flag[wv][lane] != 0;
+            wave_sync();
+        }
+        keep(active, pass, y, j);
+        if (active) bytes += pass ? 8 : 4;
+    }
+    return bytes;
 }
 
 // Appends the wave's flagged vertices to list[*len ...] with one reservation (entries past cap are
@@ -242,16 +324,24 @@ __device__ __forceinline__ void append_fresh(bool fresh, uint32_t y, uint32_t* l
     if (fresh && p < cap) list[p] = y;
 }
 
+// Adds the wave's byte counts to *out (once per wave).
+__device__ __forceinline__ void add_bytes(unsigned long long* out, uint64_t bytes) {
+    if (!out) return;
+    bytes = wave_sum_u64(bytes);
+    if (lane_id() == 0 && bytes) atomicAdd(out, static_cast<unsigned long long>(bytes));
+}
+
 __global__ void __launch_bounds__(256) k_local_expand(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                                                       const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
                                                       const uint32_t* cand_u, VSet out, unsigned long long* nbytes) {
+    __shared__ FlatLds lds;
     const uint32_t F = min(*frontier_len, max_frontier);
-    uint64_t written = 0;
-    const uint64_t bytes = expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, out.bitmap, [&](bool fresh, uint32_t y) {
-        written += fresh ? 4u : 0u;
-        append_fresh(fresh, y, out.list, out.len, out.cap);
-    });
-    add_bytes(nbytes, bytes + written);
+    const uint64_t bytes = expand_flat(
+        g, lds, F, [&](uint32_t i, uint32_t& j, uint32_t& x) { j = 0; x = frontier[i]; },
+        [&](uint32_t) { return Rel4{phase, view, dir, slot}; },
+        [&](uint32_t, const uint32_t*& c, uint32_t*& bm) { c = cand_u; bm = out.bitmap; },
+        [&](bool fresh, uint32_t y, uint32_t) { append_fresh(fresh, y, out.list, out.len, out.cap); });
+    add_bytes(nbytes, bytes);
 }
 
 // Expansion with the semi-joins in a second pass: k_local_queue expands the frontier along
@@ -262,91 +352,160 @@ __global__ void __launch_bounds__(256) k_local_queue(DevGraph g, uint32_t phase,
                                                      const uint32_t* frontier, const uint32_t* frontier_len, uint32_t max_frontier,
                                                      const uint32_t* cand_u, uint32_t* bitmap, uint32_t* queue, uint32_t* queue_len,
                                                      unsigned long long* nbytes) {
+    __shared__ FlatLds lds;
     const uint32_t F = min(*frontier_len, max_frontier);
-    uint64_t written = 0;
-    const uint64_t bytes = expand_frontier(g, phase, view, dir, slot, frontier, F, cand_u, bitmap, [&](bool fresh, uint32_t y) {
-        written += fresh ? 4u : 0u;
-        append_fresh(fresh, y, queue, queue_len, 0xFFFFFFFFu);
-    });
-    add_bytes(nbytes, bytes + written);
+    const uint64_t bytes = expand_flat(
+        g, lds, F, [&](uint32_t i, uint32_t& j, uint32_t& x) { j = 0; x = frontier[i]; },
+        [&](uint32_t) { return Rel4{phase, view, dir, slot}; },
+        [&](uint32_t, const uint32_t*& c, uint32_t*& bm) { c = cand_u; bm = bitmap; },
+        [&](bool fresh, uint32_t y, uint32_t) { append_fresh(fresh, y, queue, queue_len, 0xFFFFFFFFu); });
+    add_bytes(nbytes, bytes);
 }
 
-// Each queued vertex gets a sub-group (the same split as the expansion), which scans its run of
-// every filter relation W keys a round for a member of the filter's bitmap, stopping at the first
-// hit; runs longer than kRoundsPerItem * W keys are scanned afterwards by the whole wave.
 __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase, const uint32_t* queue, const uint32_t* queue_len,
                                                      uint32_t max_len, LocalFilters fl, VSet out, unsigned long long* nbytes) {
-    const uint32_t lane = lane_id();
-    uint64_t bytes = 0;  // queued vertex read, runs located, keys scanned, survivor written
-    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    __shared__ FlatLds lds;
     const uint32_t n = min(*queue_len, max_len);
-    const SubWave sw = sub_wave(n);
-    for (uint32_t base = gw * sw.G; base < n; base += waves * sw.G) {
-        const uint32_t i = base + sw.sub;
-        const bool active = i < n;
-        const uint32_t y = active ? queue[i] : 0u;
-        bool pass = active;
-        uint32_t pending = 0;  // filters whose runs are left to the whole wave
-        for (uint32_t k = 0; k < fl.n; k++) {
-            const LocalFilter f = fl.f[k];
-            const uint32_t* fdl = phase_delta(g, f.dir, phase);
-            RunView fr{0, 0, 0, 0};
-            uint32_t fnb = 0, ftot = 0;
-            if (pass) {
-                fr = make_run(g, phase, f.view, f.dir, y, f.slot);
-                fnb = run_base_len(fr, f.view);
-                ftot = run_seq_len(fr, f.view);
-                if (sw.sl == 0) bytes += 8;
-            }
-            const bool own = pass && ftot <= kRoundsPerItem * sw.W;
-            if (pass && !own) pending |= 1u << k;
-            const uint32_t rounds = wave_max_u32(own ? (ftot + sw.W - 1) / sw.W : 0u);
-            bool hit = false;
-            for (uint32_t t = 0; t < rounds; t++) {
-                const uint32_t e = t * sw.W + sw.sl;
-                bool h = false;
-                if (own && !hit && e < ftot) {
-                    const uint32_t fk = run_elem(g, fdl, fr, fnb, e);
-                    h = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
-                    bytes += 4;
-                }
-                const uint64_t m = __ballot(h);
-                hit = hit || (m & sw.mask) != 0;
-                if (__ballot(own && !hit && (t + 1) * sw.W < ftot) == 0) break;  // every sub-group decided
-            }
-            if (own) pass = hit;
-        }
-        for (uint64_t todo = __ballot(pass && pending && sw.sl == 0); todo; todo &= todo - 1) {
-            const uint32_t src = static_cast<uint32_t>(__ffsll(static_cast<long long>(todo)) - 1);
-            const uint32_t ys = static_cast<uint32_t>(__shfl(static_cast<int>(y), src, 64));
-            const uint32_t pm = static_cast<uint32_t>(__shfl(static_cast<int>(pending), src, 64));
-            bool ok = true;
-            for (uint32_t k = 0; k < fl.n && ok; k++) {
-                if (!(pm >> k & 1u)) continue;
-                const LocalFilter f = fl.f[k];
-                const uint32_t* fdl = phase_delta(g, f.dir, phase);
-                const RunView fr = make_run(g, phase, f.view, f.dir, ys, f.slot);
-                const uint32_t fnb = run_base_len(fr, f.view), ftot = run_seq_len(fr, f.view);
-                bool hit = false;
-                for (uint32_t t0 = 0; t0 < ftot && !hit; t0 += 64) {
-                    bool h = false;
-                    if (t0 + lane < ftot) {
-                        const uint32_t fk = run_elem(g, fdl, fr, fnb, t0 + lane);
-                        h = bit_test(f.bitmap, fk & g.id_mask) && run_elem_live(fdl, fr, f.view, fk);
-                        bytes += 4;
-                    }
-                    hit = __ballot(h) != 0;
-                }
-                ok = hit;
-            }
-            if ((sw.mask >> src) & 1ull) pass = ok;  // the whole sub-group of the pending vertex
-        }
-        append_fresh(pass && sw.sl == 0, y, out.list, out.len, out.cap);
-        if (active && !pass && sw.sl == 0) atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
-        if (active && sw.sl == 0) bytes += pass ? 8 : 4;
-    }
+    const uint64_t bytes = check_flat(
+        g, lds, n, [&](uint32_t i, uint32_t& j, uint32_t& y, uint32_t& nf) { j = 0; y = queue[i]; nf = fl.n; },
+        [&](uint32_t, uint32_t k) { return fl.f[k]; }, [&](uint32_t) { return phase; },
+        [&](bool active, bool pass, uint32_t y, uint32_t) {
+            append_fresh(active && pass, y, out.list, out.len, out.cap);
+            if (active && !pass) atomicAnd(&out.bitmap[y >> 5], ~(1u << (y & 31)));
+        });
     add_bytes(nbytes, bytes);
+}
+
+// ---- many steps at once --------------------------------------------------------------------
+
+// One block row per job; the body is k_local_anchors'.
+__global__ void k_local_anchors_multi(DevGraph g, const AnchorJob* jobs, const uint32_t* touched0, const uint32_t* n_touched0,
+                                      const uint32_t* touched1, const uint32_t* n_touched1) {
+    const AnchorJob J = jobs[blockIdx.y];
+    const uint32_t* touched = J.dir ? touched1 : touched0;
+    const uint32_t T = *(J.dir ? n_touched1 : n_touched0);
+    const uint32_t* dl = phase_delta(g, J.dir, J.phase);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        const uint32_t v = touched[i];
+        if (g.vrank[v] != J.vr0 || !bit_test(J.cand0, v)) continue;
+        const uint4 sg = g.delta[J.dir].seg[v];
+        const uint32_t off = J.phase ? sg.z : sg.x, len = J.phase ? sg.w : sg.y;
+        if (!len) continue;
+        uint32_t lo, hi;
+        slot_range(dl, off, off + len, J.slot, g.id_bits, lo, hi);
+        bool any = false;
+        for (uint32_t k = lo; k < hi; k++) {
+            const uint32_t x = dl[k] & g.id_mask;
+            if (!bit_test(J.cand1, x)) continue;
+            const uint32_t p = atomicAdd(J.n_pairs, 1u);
+            if (p < J.max_pairs) { J.pairs[2 * p] = v; J.pairs[2 * p + 1] = x; }
+            vset_add(J.L1, x);
+            any = true;
+        }
+        if (any) vset_add(J.L0, v);
+    }
+}
+
+// off[0..njobs]: exclusive prefix of the jobs' frontier (kind 0) or queue (kind 1) lengths.
+template <int kKind>
+__global__ void __launch_bounds__(kMaxLocalJobs) k_job_offsets(const LocalJob* jobs, uint32_t njobs, uint32_t max_len, uint32_t* off) {
+    __shared__ uint32_t s[kMaxLocalJobs];
+    const uint32_t t = threadIdx.x;
+    uint32_t v = 0;
+    if (t < njobs) v = min(kKind == 0 ? *jobs[t].frontier_len : *jobs[t].queue_len, max_len);
+    s[t] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < kMaxLocalJobs; o <<= 1) {
+        const uint32_t x = t >= o ? s[t - o] : 0u;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    if (t < njobs) off[t + 1] = s[t];
+    if (t == 0) off[0] = 0;
+}
+
+// The job whose items [off[j], off[j + 1]) hold item i (off in LDS, njobs >= 1).
+__device__ __forceinline__ uint32_t job_of(const uint32_t* off, uint32_t njobs, uint32_t i) {
+    uint32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Appends the flagged lanes' vertices to their jobs' queues (to_queue) or output lists: one
+// reservation when every flagged lane belongs to the same job, else one atomic per lane.
+__device__ __forceinline__ void append_jobs(bool flag, uint32_t y, uint32_t j, const LocalJob* jobs, bool to_queue) {
+    const uint64_t m = __ballot(flag);
+    if (!m) return;
+    const uint32_t first = static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+    const uint32_t j0 = static_cast<uint32_t>(__shfl(static_cast<int>(j), first, 64));
+    if (__ballot(flag && j != j0) == 0) {
+        const LocalJob& J = jobs[j0];
+        uint32_t* list = to_queue ? J.queue : J.out.list;
+        uint32_t* len = to_queue ? J.queue_len : J.out.len;
+        const uint32_t cap = to_queue ? 0xFFFFFFFFu : J.out.cap;
+        uint32_t p = 0;
+        if (lane_id() == first) p = atomicAdd(len, static_cast<uint32_t>(__popcll(m)));
+        p = static_cast<uint32_t>(__shfl(static_cast<int>(p), first, 64)) + lanes_below(m);
+        if (flag && p < cap) list[p] = y;
+    } else if (flag) {
+        const LocalJob& J = jobs[j];
+        if (to_queue) {
+            J.queue[atomicAdd(J.queue_len, 1u)] = y;
+        } else {
+            const uint32_t p = atomicAdd(J.out.len, 1u);
+            if (p < J.out.cap) J.out.list[p] = y;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_local_queue_multi(DevGraph g, const LocalJob* jobs, uint32_t njobs, const uint32_t* off,
+                                                           unsigned long long* nbytes) {
+    __shared__ FlatLds lds;
+    __shared__ uint32_t s_off[kMaxLocalJobs + 1];
+    for (uint32_t t = threadIdx.x; t <= njobs; t += blockDim.x) s_off[t] = off[t];
+    __syncthreads();
+    const uint64_t bytes = expand_flat(
+        g, lds, s_off[njobs],
+        [&](uint32_t i, uint32_t& j, uint32_t& x) {
+            j = job_of(s_off, njobs, i);
+            x = jobs[j].frontier[i - s_off[j]];
+        },
+        [&](uint32_t j) { return Rel4{jobs[j].phase, jobs[j].view, jobs[j].dir, jobs[j].slot}; },
+        [&](uint32_t j, const uint32_t*& c, uint32_t*& bm) { c = jobs[j].cand_u; bm = jobs[j].out.bitmap; },
+        [&](bool fresh, uint32_t y, uint32_t j) { append_jobs(fresh, y, j, jobs, true); });
+    add_bytes(nbytes, bytes);
+}
+
+__global__ void __launch_bounds__(256) k_local_check_multi(DevGraph g, const LocalJob* jobs, uint32_t njobs, const uint32_t* off,
+                                                           unsigned long long* nbytes) {
+    __shared__ FlatLds lds;
+    __shared__ uint32_t s_off[kMaxLocalJobs + 1];
+    for (uint32_t t = threadIdx.x; t <= njobs; t += blockDim.x) s_off[t] = off[t];
+    __syncthreads();
+    const uint64_t bytes = check_flat(
+        g, lds, s_off[njobs],
+        [&](uint32_t i, uint32_t& j, uint32_t& y, uint32_t& nf) {
+            j = job_of(s_off, njobs, i);
+            y = jobs[j].queue[i - s_off[j]];
+            nf = jobs[j].fl.n;
+        },
+        [&](uint32_t j, uint32_t k) { return jobs[j].fl.f[k]; }, [&](uint32_t j) { return jobs[j].phase; },
+        [&](bool active, bool pass, uint32_t y, uint32_t j) {
+            append_jobs(active && pass, y, j, jobs, false);
+            if (active && !pass) atomicAnd(&jobs[j].out.bitmap[y >> 5], ~(1u << (y & 31)));
+        });
+    add_bytes(nbytes, bytes);
+}
+
+__global__ void __launch_bounds__(256) k_clear_sets_multi(const VSet* sets, uint32_t max_len) {
+    const VSet v = sets[blockIdx.y];
+    const uint32_t n = min(*v.len, max_len);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v.bitmap[v.list[i] >> 5] = 0u;
 }
 
 // Clears the bitmaps of every set in `mask` (whole words of its listed members).
@@ -417,6 +576,31 @@ void local_queue(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir,
 void local_check(const DevGraph& g, uint32_t phase, const LocalFilters& fl, VSet out, const uint32_t* queue,
                  const uint32_t* queue_len, int blocks, unsigned long long* bytes, hipStream_t s) {
     k_local_check<<<blocks, 256, 0, s>>>(g, phase, queue, queue_len, g.dv, fl, out, bytes);
+}
+
+void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njobs, const uint32_t* touched0, const uint32_t* n_touched0,
+                         const uint32_t* touched1, const uint32_t* n_touched1, uint32_t max_touched, hipStream_t s) {
+    if (!njobs) return;
+    const dim3 grid(static_cast<uint32_t>(grid_for(max_touched, 256, static_cast<int>(std::max<uint32_t>(16, 8192 / njobs)))), njobs);
+    k_local_anchors_multi<<<grid, 256, 0, s>>>(g, jobs, touched0, n_touched0, touched1, n_touched1);
+}
+
+void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
+                       hipStream_t s) {
+    if (!njobs) return;
+    k_job_offsets<0><<<1, kMaxLocalJobs, 0, s>>>(jobs, njobs, g.dv, off);
+    k_local_queue_multi<<<blocks, 256, 0, s>>>(g, jobs, njobs, off, bytes);
+}
+
+void local_check_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
+                       hipStream_t s) {
+    if (!njobs) return;
+    k_job_offsets<1><<<1, kMaxLocalJobs, 0, s>>>(jobs, njobs, g.dv, off);
+    k_local_check_multi<<<blocks, 256, 0, s>>>(g, jobs, njobs, off, bytes);
+}
+
+void clear_sets_multi(const VSet* sets, uint32_t nsets, uint32_t max_len, hipStream_t s) {
+    if (nsets) k_clear_sets_multi<<<dim3(64, nsets), 256, 0, s>>>(sets, max_len);
 }
 
 void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s) {

@@ -131,6 +131,41 @@ void local_check(const DevGraph& g, uint32_t phase, const LocalFilters& fl, VSet
 struct VSetTable {
     VSet s[kMaxQV];
 };
+
+// ---- local index of many steps at once --------------------------------------------------------
+// The local index of every step of a group (query, query edge) is built in lockstep: one launch
+// computes the anchors of all of them, then, for each indexing-order position, one launch expands
+// and one checks the local sets of every step that builds a vertex at that position.  Each launch
+// treats its jobs' items as one space (device-side lengths, prefix-summed by the launch itself).
+struct AnchorJob {
+    uint32_t phase, dir, slot, vr0;
+    const uint32_t* cand0;
+    const uint32_t* cand1;
+    uint32_t* pairs;
+    uint32_t* n_pairs;
+    uint32_t max_pairs;
+    VSet L0, L1;
+};
+struct LocalJob {
+    uint32_t phase, view, dir, slot;   // expansion relation: the run of each frontier vertex
+    const uint32_t* frontier;          // L(b0) of an earlier position
+    const uint32_t* frontier_len;
+    const uint32_t* cand_u;            // global candidates of the vertex built
+    VSet out;                          // its local set (bitmap clear, length zero on entry)
+    uint32_t* queue;                   // dv words of queue space and its length (zero on entry)
+    uint32_t* queue_len;
+    LocalFilters fl;                   // the other built neighbours (none: every queued vertex is kept)
+};
+constexpr uint32_t kMaxLocalJobs = 256;
+void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njobs, const uint32_t* touched0, const uint32_t* n_touched0,
+                         const uint32_t* touched1, const uint32_t* n_touched1, uint32_t max_touched, hipStream_t s);
+// off: kMaxLocalJobs + 1 words of device scratch.  bytes: as local_queue / local_check.
+void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
+                       hipStream_t s);
+void local_check_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
+                       hipStream_t s);
+// Clears the bitmaps of sets[0, nsets) (whole words of their listed members).
+void clear_sets_multi(const VSet* sets, uint32_t nsets, uint32_t max_len, hipStream_t s);
 // Clears the bitmaps of the sets in mask (their lengths stay; the caller zeroes them).
 void clear_sets(const VSetTable& t, uint32_t mask, uint32_t max_len, hipStream_t s);
 // Stores w.v[0 .. w.n) at p (one tiny launch instead of several fills).

@@ -63,6 +63,7 @@ Engine::~Engine() {
     if (snap_) (void)hipFree(snap_);
     if (trace_) std::fclose(trace_);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
+    if (h_jobs_) (void)hipHostFree(h_jobs_);
     if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -240,6 +241,27 @@ std::string Engine::build() {
         add((cp_rec / kLevelTile + 2) * 8);                        // level tile totals
         add(cp_list_words * 4); add(64);
         add(64 << 10);                                             // alignment slack of the takes
+    }
+    // local-index slots: two groups of group_ steps (each: its sets, anchor pairs, queue, counters)
+    uint32_t max_qn = 2;
+    for (auto& qs : qs_) max_qn = std::max(max_qn, qs->q.n);
+    const uint64_t slot_bytes = max_qn * (words_ * 4ull + dv_ * 4ull + 512) + 2 * max_arcs * 4 + dv_ * 4ull + kSlotWords * 4 + (16 << 10);
+    {
+        // Off by default: with eight lanes, each step building its own local index measured as fast
+        // (4M-vertex graph, 1,048,576-edge batches: 60.2 against 61.4 ms per batch with groups of
+        // 15) or faster (400K-vertex graph, 65,536-edge batches: 21.6 against 25.0 ms, groups of 64).
+        group_ = 0;
+        if (const char* t = std::getenv("CSM_GROUP_STEPS")) {
+            uint64_t budget = kGroupBudgetBytes;
+            if (opt_.pool_bytes) budget = std::min<uint64_t>(budget, opt_.pool_bytes / 8);
+            const uint64_t fit = std::min<uint64_t>(kMaxGroup, budget / (2 * slot_bytes));
+            group_ = static_cast<uint32_t>(std::min<uint64_t>(fit, static_cast<uint64_t>(std::max(0L, std::atol(t)))));
+        }
+    }
+    if (group_) {
+        add(2ull * group_ * slot_bytes);
+        add(kMaxGroup * sizeof(AnchorJob)); add(kMaxQV * kMaxLocalJobs * sizeof(LocalJob));
+        add(2ull * kMaxGroup * kMaxQV * sizeof(VSet)); add(kMaxQV * (kMaxLocalJobs + 1) * 4ull);
     }
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
     add(qs_.size() * 32 + 64);
@@ -428,6 +450,36 @@ std::string Engine::build() {
                                tile_sum, cp_rec, kLevelRunsPerRecord * cp_rec};
     }
     opt_.cp_max_records = static_cast<uint32_t>(cp_rec);
+    slots_.assign(2ull * group_, Slot{});
+    slot_ctr_ = group_ ? pool_.take<uint32_t>(2ull * group_ * kSlotWords) : nullptr;
+    if (group_ && !slot_ctr_) return tagged(Code::kMemory, "device pool exhausted (local-index slots)");
+    if (group_) CSM_HIP(hipMemsetAsync(slot_ctr_, 0, 2ull * group_ * kSlotWords * 4, s_));
+    for (size_t k = 0; k < slots_.size(); k++) {
+        Slot& sl = slots_[k];
+        for (uint32_t u = 0; u < max_qn; u++) {
+            sl.L[u].bitmap = pool_.take<uint32_t>(words_);
+            sl.L[u].list = pool_.take<uint32_t>(dv_);
+            sl.L[u].cap = dv_;
+            if (!sl.L[u].bitmap || !sl.L[u].list) return tagged(Code::kMemory, "device pool exhausted (local-index slots)");
+            CSM_HIP(hipMemsetAsync(sl.L[u].bitmap, 0, words_ * 4ull, s_));
+        }
+        sl.counters = slot_ctr_ + k * kSlotWords;
+        sl.pairs = pool_.take<uint32_t>(2 * max_arcs);
+        sl.queue = pool_.take<uint32_t>(dv_);
+        if (!sl.pairs || !sl.queue) return tagged(Code::kMemory, "device pool exhausted (local-index slots)");
+        for (uint32_t u = 0; u < max_qn; u++) sl.L[u].len = sl.counters + u;
+        sl.n_pairs = sl.counters + kMaxQV;
+    }
+    if (group_) {
+        d_anchor_jobs_ = pool_.take<AnchorJob>(kMaxGroup);
+        d_local_jobs_ = pool_.take<LocalJob>(kMaxQV * kMaxLocalJobs);
+        d_clear_sets_ = pool_.take<VSet>(2ull * kMaxGroup * kMaxQV);
+        d_job_off_ = pool_.take<uint32_t>(kMaxQV * (kMaxLocalJobs + 1));
+        if (!d_anchor_jobs_ || !d_local_jobs_ || !d_clear_sets_ || !d_job_off_) return tagged(Code::kMemory, "device pool exhausted (local-index jobs)");
+        const uint64_t hbytes = kMaxGroup * sizeof(AnchorJob) + kMaxQV * kMaxLocalJobs * sizeof(LocalJob) +
+                                2ull * kMaxGroup * kMaxQV * sizeof(VSet) + 2ull * kMaxGroup * kSlotWords * 4 + 4096;
+        CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_jobs_), hbytes, hipHostMallocDefault));
+    }
     results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);
     prof_bytes_ = pool_.take<unsigned long long>(kKFamilies);
     CSM_HIP(hipMemsetAsync(prof_bytes_, 0, kKFamilies * 8, s_));
@@ -671,6 +723,20 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
         for (uint32_t u = 0; u < q.n; u++)
             if ((built >> u & 1u) && w.h_pinned[u] == 0) return "empty";
     }
+    return enumerate_step(w, qs, i, ph, w.L, built, w.pairs, w.n_pairs, np);
+}
+
+// The search of one step once its local index exists: L[u] for the query vertices in `built`,
+// the np anchor pairs of the step's query edge in pairs / *n_pairs.
+std::string Engine::enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
+                                   const uint32_t* n_pairs, uint32_t np) {
+    const Query& q = qs.q;
+    w.sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
+    const Plan& p = qs.plan;
+    const uint32_t u0 = q.qe_list[i].first, u1 = q.qe_list[i].second;
+    const uint32_t max_arcs = 2 * max_batch_;
+    auto cand = [&](uint32_t u) { return qs.cand + static_cast<uint64_t>(u) * words_; };
+    fill_u32(w.counters, 0u, kLaneWords - kStepCounters, w.s);
 
     // ---- step descriptor ----
     const OrderPerEdge& o = p.orders[i];
@@ -684,7 +750,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
         d.vrank[k] = qs.vrank[u];
         d.boff[k] = static_cast<uint8_t>(ne);
         d.pivot[k] = static_cast<uint8_t>(ne);
-        d.filter[k] = (built >> u & 1u) ? w.L[u].bitmap : cand(u);
+        d.filter[k] = (built >> u & 1u) ? L[u].bitmap : cand(u);
         if (k == 0) continue;
         bool have_delta = false;
         for (uint32_t kk = o.bni_offs[k]; kk < o.bni_offs[k + 1]; kk++) {
@@ -713,13 +779,13 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
     uint32_t count;
     if ((a == u0 && b == u1) || (a == u1 && b == u0)) {
         if (2ull * np > w.ring_words) return tagged(Code::kMemory, "partial-result ring too small for the batch's anchors");
-        init_from_pairs(w.pairs, w.n_pairs, max_arcs, a == u1, ring, w.counters, w.s);
+        init_from_pairs(pairs, n_pairs, max_arcs, a == u1, ring, w.counters, w.s);
         count = np;
     } else {
         const uint32_t rab = q.rel(a, b);
         const uint32_t* list;
         const uint32_t* len;
-        if (built >> a & 1u) { list = w.L[a].list; len = w.L[a].len; }
+        if (built >> a & 1u) { list = L[a].list; len = L[a].len; }
         else {
             fill_u32(w.tmp_len, 0u, 1, w.s);
             bitmap_to_list(cand(a), dv_, w.tmp_list, w.tmp_len, w.s);
@@ -982,6 +1048,131 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
     return "";
 }
 
+// The local index of a group of steps (each with its slot) in lockstep on the main stream: clear
+// the slots, compute every step's anchors in one launch, then for each indexing-order position one
+// expansion and one check launch over every step that builds a vertex there (the same rule as
+// run_step: heavy-vertex avoidance picks the expansion side, the other built neighbours filter).
+// Steps whose anchors or local sets come out empty get np = 0.
+std::string Engine::stage_group(uint32_t ph, std::vector<GroupStep>& steps) {
+    const uint32_t n = static_cast<uint32_t>(steps.size());
+    if (!n) return "";
+    const uint32_t max_arcs = 2 * max_batch_;
+    uint8_t* h_anchor = h_jobs_;
+    uint8_t* h_local = h_anchor + kMaxGroup * sizeof(AnchorJob);
+    uint8_t* h_clear = h_local + kMaxQV * kMaxLocalJobs * sizeof(LocalJob);
+    uint32_t* h_ctr = reinterpret_cast<uint32_t*>(h_clear + 2ull * kMaxGroup * kMaxQV * sizeof(VSet));
+    const int first_slot = steps[0].slot;
+    for (uint32_t k = 0; k < n; k++)
+        if (steps[k].slot != first_slot + static_cast<int>(k)) return tagged(Code::kState, "group slots must be consecutive");
+    uint32_t* ctr = slots_[first_slot].counters;
+
+    // clear what the slots' previous steps left, then zero their counters
+    VSet* cs = reinterpret_cast<VSet*>(h_clear);
+    uint32_t nclear = 0;
+    for (auto& st : steps) {
+        Slot& sl = slots_[st.slot];
+        for (uint32_t u = 0; u < kMaxQV; u++)
+            if (sl.used >> u & 1u) cs[nclear++] = sl.L[u];
+        sl.used = 0;
+    }
+    if (nclear) {
+        CSM_HIP(hipMemcpyAsync(d_clear_sets_, cs, nclear * sizeof(VSet), hipMemcpyHostToDevice, s_));
+        clear_sets_multi(d_clear_sets_, nclear, dv_, s_);
+    }
+    fill_u32(ctr, 0u, static_cast<uint64_t>(n) * kSlotWords, s_);
+
+    // anchors of every step
+    AnchorJob* aj = reinterpret_cast<AnchorJob*>(h_anchor);
+    for (uint32_t k = 0; k < n; k++) {
+        QState& qs = *qs_[steps[k].q];
+        const Query& q = qs.q;
+        const uint32_t i = steps[k].i;
+        const uint32_t u0 = q.qe_list[i].first, u1 = q.qe_list[i].second;
+        const RelInfo& R = qs.rel[q.rel(u0, u1)];
+        Slot& sl = slots_[steps[k].slot];
+        aj[k] = AnchorJob{ph, R.dir, R.slot, qs.vrank[u0], qs.cand + static_cast<uint64_t>(u0) * words_,
+                          qs.cand + static_cast<uint64_t>(u1) * words_, sl.pairs, sl.n_pairs, max_arcs, sl.L[u0], sl.L[u1]};
+        sl.used = static_cast<uint16_t>((1u << u0) | (1u << u1));
+        steps[k].built = (1u << u0) | (1u << u1);
+    }
+    CSM_HIP(hipMemcpyAsync(d_anchor_jobs_, aj, n * sizeof(AnchorJob), hipMemcpyHostToDevice, s_));
+    prof_begin(prof_main_, s_, kKAnchors);
+    local_anchors_multi(g_, d_anchor_jobs_, n, bs_[0].touched, bs_[0].n_touched, bs_[1].touched, bs_[1].n_touched, max_arcs, s_);
+    prof_end(prof_main_, s_, kKAnchors);
+    CSM_HIP(hipMemcpyAsync(h_ctr, ctr, static_cast<uint64_t>(n) * kSlotWords * 4, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    for (uint32_t k = 0; k < n; k++) steps[k].np = h_ctr[k * kSlotWords + kMaxQV];
+
+    // local sets, one indexing-order position at a time
+    for (uint32_t pos = 2; pos < kMaxQV; pos++) {
+        LocalJob* lj = reinterpret_cast<LocalJob*>(h_local) + static_cast<uint64_t>(pos) * kMaxLocalJobs;
+        uint32_t nj = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            GroupStep& st = steps[k];
+            QState& qs = *qs_[st.q];
+            const Query& q = qs.q;
+            if (st.np == 0 || pos >= q.n) continue;
+            const Plan& p = qs.plan;
+            const uint32_t i = st.i;
+            const OrderPerEdge& io = p.indexing[i];
+            const uint32_t u = io.vs[pos];
+            if (!(p.rebuild_B[i] >> u & 1u)) continue;
+            std::vector<uint32_t> bsv;
+            for (uint32_t kk = io.bni_offs[pos]; kk < io.bni_offs[pos + 1]; kk++) {
+                const uint32_t b = io.vs[io.bni[kk]];
+                if (st.built >> b & 1u) bsv.push_back(b);
+            }
+            if (bsv.empty()) continue;
+            size_t first = 0;
+            if (qs.tech.heavy_vertex_avoidance) {
+                for (size_t kk = 1; kk < bsv.size(); kk++)
+                    if (qs.avg[q.rel(bsv[kk], u)] < qs.avg[q.rel(bsv[first], u)]) first = kk;
+            }
+            const uint32_t b0 = bsv[first];
+            const uint32_t rb = q.rel(b0, u);
+            Slot& sl = slots_[st.slot];
+            LocalJob& J = lj[nj++];
+            J = LocalJob{};
+            J.phase = ph;
+            J.view = view_of(qs, rb, i, ph);
+            J.dir = qs.rel[rb].dir;
+            J.slot = qs.rel[rb].slot;
+            J.frontier = sl.L[b0].list;
+            J.frontier_len = sl.L[b0].len;
+            J.cand_u = qs.cand + static_cast<uint64_t>(u) * words_;
+            J.out = sl.L[u];
+            J.queue = sl.queue;
+            J.queue_len = sl.counters + kQueueCounters + u;
+            for (size_t kk = 0; kk < bsv.size(); kk++) {
+                if (kk == first) continue;
+                const uint32_t rub = q.rel(u, bsv[kk]);
+                J.fl.f[J.fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, sl.L[bsv[kk]].bitmap};
+            }
+            sl.used |= static_cast<uint16_t>(1u << u);
+            st.built |= 1u << u;
+        }
+        if (!nj) continue;
+        LocalJob* dj = d_local_jobs_ + static_cast<uint64_t>(pos) * kMaxLocalJobs;
+        uint32_t* off = d_job_off_ + static_cast<uint64_t>(pos) * (kMaxLocalJobs + 1);
+        CSM_HIP(hipMemcpyAsync(dj, lj, nj * sizeof(LocalJob), hipMemcpyHostToDevice, s_));
+        prof_begin(prof_main_, s_, kKLocalIndex);
+        local_queue_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalIndex), s_);
+        prof_end(prof_main_, s_, kKLocalIndex);
+        prof_begin(prof_main_, s_, kKLocalCheck);
+        local_check_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalCheck), s_);
+        prof_end(prof_main_, s_, kKLocalCheck);
+    }
+    CSM_HIP(hipMemcpyAsync(h_ctr, ctr, static_cast<uint64_t>(n) * kSlotWords * 4, hipMemcpyDeviceToHost, s_));
+    CSM_HIP(hipStreamSynchronize(s_));
+    for (uint32_t k = 0; k < n; k++) {
+        GroupStep& st = steps[k];
+        if (st.np == 0) continue;
+        for (uint32_t u = 0; u < qs_[st.q]->q.n; u++)
+            if ((st.built >> u & 1u) && h_ctr[k * kSlotWords + u] == 0) { st.np = 0; break; }
+    }
+    return check_err("local index");
+}
+
 std::string Engine::run_phase(uint32_t ph) {
     std::vector<std::pair<uint32_t, uint32_t>> steps;  // (query, edge)
     for (size_t qi = 0; qi < qs_.size(); qi++) {
@@ -1001,30 +1192,67 @@ std::string Engine::run_phase(uint32_t ph) {
     cost.resize(qs_.size() * kMaxQE, 0.0);
     CSM_HIP(hipEventRecord(phase_start_, s_));
     const auto phase_t0 = std::chrono::steady_clock::now();
-    std::atomic<size_t> next{0};
+    // Work items for the lanes: steps that build their own local index (slot -1), and steps
+    // whose local index a group stage built (their slot).  The main thread stages group g + 1
+    // while the lanes search group g; a group reuses the slots of group g - 1 once its steps are done.
+    struct Work {
+        std::mutex mu;
+        std::condition_variable cv;
+        std::deque<GroupStep> q;
+        bool closed = false;
+        uint32_t pending[2] = {0, 0};  // staged steps of each half of the slots not yet finished
+    } work;
     std::atomic<bool> failed{false};
     std::vector<std::string> errs(lanes_.size());
+    std::vector<GroupStep> grouped;
+    {
+        std::lock_guard<std::mutex> lk(work.mu);
+        for (const auto& st : steps) {
+            GroupStep g;
+            g.q = st.first;
+            g.i = st.second;
+            if (group_ && qs_[st.first]->tech.local_index) grouped.push_back(g);
+            else work.q.push_back(g);
+        }
+    }
     auto worker = [&](size_t li) {
         Lane& w = lanes_[li];
         auto run = [&]() -> std::string {
             CSM_HIP(hipSetDevice(opt_.device));
             CSM_HIP(hipStreamWaitEvent(w.s, phase_start_, 0));
-            while (!failed.load(std::memory_order_relaxed)) {
-                const size_t k = next.fetch_add(1);
-                if (k >= steps.size()) break;
-                QState& qs = *qs_[steps[k].first];
+            for (;;) {
+                GroupStep it;
+                {
+                    std::unique_lock<std::mutex> lk(work.mu);
+                    work.cv.wait(lk, [&]() { return failed.load() || work.closed || !work.q.empty(); });
+                    if (failed.load() || work.q.empty()) break;
+                    it = work.q.front();
+                    work.q.pop_front();
+                }
+                QState& qs = *qs_[it.q];
                 const auto t0 = std::chrono::steady_clock::now();
                 w.trace_line.clear();
-                std::string e = run_step(w, qs, steps[k].second, ph);
-                cost[static_cast<size_t>(steps[k].first) * kMaxQE + steps[k].second] =
-                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::string e;
+                if (it.slot < 0) {
+                    e = run_step(w, qs, it.i, ph);
+                } else {
+                    const Slot& sl = slots_[it.slot];
+                    e = enumerate_step(w, qs, it.i, ph, sl.L, it.built, sl.pairs, sl.n_pairs, it.np);
+                    // the slot is reused by a later group: its readers must be done
+                    const hipError_t se = hipStreamSynchronize(w.s);
+                    if (e.empty() && se != hipSuccess) e = std::string("HIP error: ") + hipGetErrorString(se) + " finishing a step";
+                    std::lock_guard<std::mutex> lk(work.mu);
+                    work.pending[static_cast<uint32_t>(it.slot) / std::max(1u, group_)]--;
+                    work.cv.notify_all();
+                }
+                cost[static_cast<size_t>(it.q) * kMaxQE + it.i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 if (trace_) {
                     (void)hipStreamSynchronize(w.s);
                     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                     std::lock_guard<std::mutex> lock(trace_mu_);
                     const double at = std::chrono::duration<double, std::milli>(t0 - phase_t0).count();
-                    std::fprintf(trace_, "{\"query\": %u, \"step\": %u, \"phase\": %u, \"lane\": %zu, \"at\": %.3f, \"ms\": %.3f, \"empty\": %d%s}\n",
-                                 steps[k].first, steps[k].second, ph, li, at, ms, e == "empty" ? 1 : 0, w.trace_line.c_str());
+                    std::fprintf(trace_, "{\"query\": %u, \"step\": %u, \"phase\": %u, \"lane\": %zu, \"slot\": %d, \"at\": %.3f, \"ms\": %.3f, \"empty\": %d%s}\n",
+                                 it.q, it.i, ph, li, it.slot, at, ms, e == "empty" ? 1 : 0, w.trace_line.c_str());
                 }
                 if (e == "empty") continue;
                 if (!e.empty()) return e;
@@ -1033,13 +1261,55 @@ std::string Engine::run_phase(uint32_t ph) {
             return "";
         };
         errs[li] = run();
-        if (!errs[li].empty()) failed.store(true);
+        if (!errs[li].empty()) {
+            failed.store(true);
+            std::lock_guard<std::mutex> lk(work.mu);
+            work.cv.notify_all();
+        }
     };
     std::vector<std::thread> th;
-    for (size_t li = 1; li < lanes_.size(); li++) th.emplace_back(worker, li);
-    worker(0);
+    for (size_t li = 0; li < lanes_.size(); li++) th.emplace_back(worker, li);
+    std::string stage_err;
+    for (size_t g0 = 0, gi = 0; g0 < grouped.size() && !failed.load(); g0 += group_, gi++) {
+        const uint32_t half = static_cast<uint32_t>(gi % 2);
+        {
+            std::unique_lock<std::mutex> lk(work.mu);
+            work.cv.wait(lk, [&]() { return failed.load() || work.pending[half] == 0; });
+        }
+        if (failed.load()) break;
+        std::vector<GroupStep> grp(grouped.begin() + g0, grouped.begin() + std::min(grouped.size(), g0 + group_));
+        for (size_t k = 0; k < grp.size(); k++) grp[k].slot = static_cast<int>(half * group_ + k);
+        const auto ts = std::chrono::steady_clock::now();
+        stage_err = stage_group(ph, grp);
+        if (trace_) {
+            std::lock_guard<std::mutex> lock(trace_mu_);
+            std::fprintf(trace_, "{\"stage\": %zu, \"phase\": %u, \"steps\": %zu, \"at\": %.3f, \"ms\": %.3f}\n", gi, ph, grp.size(),
+                         std::chrono::duration<double, std::milli>(ts - phase_t0).count(),
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count());
+        }
+        if (!stage_err.empty()) {
+            failed.store(true);
+            break;
+        }
+        std::lock_guard<std::mutex> lk(work.mu);
+        for (const GroupStep& st : grp) {
+            if (st.np == 0) continue;  // no anchors, or an empty local set: nothing to search
+            work.q.push_back(st);
+            work.pending[half]++;
+        }
+        work.cv.notify_all();
+    }
+    {
+        std::lock_guard<std::mutex> lk(work.mu);
+        work.closed = true;
+        work.cv.notify_all();
+    }
     for (auto& t : th) t.join();
     if (trace_) std::fflush(trace_);
+    if (!stage_err.empty()) {
+        for (Lane& w : lanes_) (void)hipStreamSynchronize(w.s);
+        return stage_err;
+    }
     for (size_t li = 0; li < lanes_.size(); li++)
         if (!errs[li].empty()) {
             for (Lane& w : lanes_) (void)hipStreamSynchronize(w.s);

@@ -66,6 +66,9 @@ constexpr uint64_t kLiveBatchesOfRoom = 16;  // arena room for in-memory batches
 constexpr uint32_t kQueueCounters = 18;
 constexpr uint32_t kStepCounters = 48;
 constexpr uint32_t kLaneWords = 64;
+constexpr uint32_t kSlotWords = 64;
+constexpr uint64_t kGroupBudgetBytes = 6ull << 30;  // device memory of the local-index slots
+constexpr uint32_t kMaxGroup = 64;
 
 struct QueryResult {
     uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;
@@ -219,10 +222,38 @@ class Engine {
         unsigned long long* work_counter() { return reinterpret_cast<unsigned long long*>(counters + 8); }
     };
 
+    // Local index of many steps at once: each slot holds one step's local sets, anchor pairs and
+    // expansion queue; slots_ holds two groups of group_ slots, so the main thread can build the
+    // next group's local index while the lanes search the steps of the current one.
+    struct Slot {
+        VSet L[kMaxQV]{};
+        uint32_t* pairs = nullptr;
+        uint32_t* n_pairs = nullptr;
+        uint32_t* queue = nullptr;
+        uint32_t* counters = nullptr;  // [kSlotWords]: set lengths [0, 16), pairs [16], queue lengths [kQueueCounters, +16)
+        uint16_t used = 0;             // sets holding data (cleared before the slot is reused)
+    };
+    struct GroupStep {
+        uint32_t q = 0, i = 0;  // query, query edge
+        int slot = -1;          // -1: the lane builds the local index itself (run_step)
+        uint32_t built = 0, np = 0;
+    };
+    std::string stage_group(uint32_t ph, std::vector<GroupStep>& steps);
+    std::vector<Slot> slots_;
+    uint32_t group_ = 0;  // steps per group; 0: every step builds its own local index on its lane
+    AnchorJob* d_anchor_jobs_ = nullptr;
+    LocalJob* d_local_jobs_ = nullptr;   // kMaxQV regions of kMaxLocalJobs jobs
+    VSet* d_clear_sets_ = nullptr;
+    uint32_t* d_job_off_ = nullptr;      // kMaxQV regions of kMaxLocalJobs + 1 words
+    uint8_t* h_jobs_ = nullptr;          // pinned staging of the job arrays, same layout, then counter read-back
+    uint32_t* slot_ctr_ = nullptr;       // every slot's counters, kSlotWords each, contiguous
+
     std::string run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, const uint32_t* hdst, const uint8_t* hop,
                              std::vector<QueryResult>& out, BatchInfo& info);
     std::string run_phase(uint32_t ph);
     std::string run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph);
+    std::string enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
+                               const uint32_t* n_pairs, uint32_t np);
     std::string search_chained(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot);
     std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,
                        uint64_t top, uint32_t target, uint32_t res_slot);

@@ -226,19 +226,22 @@ def test_adjacency_after_stream(lib, tmp_path):
             assert len(set(got)) == len(got)
 
 
-@pytest.mark.parametrize("split,sdir,lanes,chain", [(0, 0, 1, 0), (1, 0, 3, 1), (0, 1, 8, 0), (1, 1, 16, 1)])
-def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, lanes, chain):
-    """Root splitting, the label-slot directory, the number of concurrent step lanes and chaining
-    a step's levels on the device change scheduling and lookup, never results."""
+@pytest.mark.parametrize("split,sdir,lanes,chain,group", [(0, 0, 1, 0, 0), (1, 0, 3, 1, 1), (0, 1, 8, 0, 3), (1, 1, 16, 1, 64),
+                                                          (1, 1, 8, 1, 2)])
+def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, lanes, chain, group):
+    """Root splitting, the label-slot directory, the number of concurrent step lanes, chaining a
+    step's levels on the device and the number of steps whose local index is built together (0:
+    each step builds its own on its lane) change scheduling and lookup, never results."""
     monkeypatch.setenv("CSM_SPLIT_ROOTS", str(split))
     monkeypatch.setenv("CSM_SLOT_DIRECTORY", str(sdir))
     monkeypatch.setenv("CSM_LANES", str(lanes))
     monkeypatch.setenv("CSM_CHAIN_LEVELS", str(chain))
+    monkeypatch.setenv("CSM_GROUP_STEPS", str(group))
     meta, queries = util.gen("tinyins", tmp_path, seed=7)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
     want, _ = util.oracle_stream(data, upd, 64, queries)
     for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 2, True, False), (0, 1, True, True)]:
-        say(f"case split={split} sdir={sdir} lanes={lanes} chain={chain} order={mo} strategy={ss} local={li} cp={cp}")
+        say(f"case split={split} sdir={sdir} lanes={lanes} chain={chain} group={group} order={mo} strategy={ss} local={li} cp={cp}")
         got = run_library(lib, str(data), str(upd), 64, queries, lib.technique(mo, ss, li, cp))
         if got is not None:
             assert got == want
@@ -246,7 +249,7 @@ def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir,
     meta, queries = util.gen("c1", tmp_path / "c1", seed=3)
     data, upd = tmp_path / "c1" / "data.graph", tmp_path / "c1" / "update.stream"
     want, _ = util.oracle_stream(data, upd, 2000, queries, threads=8)
-    say(f"case split={split} sdir={sdir} lanes={lanes} chain={chain} c1 batch=2000")
+    say(f"case split={split} sdir={sdir} lanes={lanes} chain={chain} group={group} c1 batch=2000")
     assert run_library(lib, str(data), str(upd), 2000, queries, lib.technique()) == want
 
 

@@ -247,16 +247,17 @@ std::string Engine::build() {
     for (auto& qs : qs_) max_qn = std::max(max_qn, qs->q.n);
     const uint64_t slot_bytes = max_qn * (words_ * 4ull + dv_ * 4ull + 512) + 2 * max_arcs * 4 + dv_ * 4ull + kSlotWords * 4 + (16 << 10);
     {
-        // Off by default: with eight lanes, each step building its own local index measured as fast
-        // (4M-vertex graph, 1,048,576-edge batches: 60.2 against 61.4 ms per batch with groups of
-        // 15) or faster (400K-vertex graph, 65,536-edge batches: 21.6 against 25.0 ms, groups of 64).
-        group_ = 0;
-        if (const char* t = std::getenv("CSM_GROUP_STEPS")) {
-            uint64_t budget = kGroupBudgetBytes;
-            if (opt_.pool_bytes) budget = std::min<uint64_t>(budget, opt_.pool_bytes / 8);
-            const uint64_t fit = std::min<uint64_t>(kMaxGroup, budget / (2 * slot_bytes));
-            group_ = static_cast<uint32_t>(std::min<uint64_t>(fit, static_cast<uint64_t>(std::max(0L, std::atol(t)))));
-        }
+        // Measured with eight lanes: building the local index of many steps together pays when a
+        // phase has many steps (64 standing queries, ~500 steps: 107.9 against 120.6 ms per batch
+        // with groups of 64), not with six queries (4M-vertex graph: 61.4 against 60.2 ms with
+        // groups of 15; 400K-vertex graph: 25.0 against 21.6 ms with groups of 64).
+        uint32_t nsteps = 0;
+        for (auto& qs : qs_) nsteps += qs->q.m;
+        uint64_t want = nsteps >= kGroupMinSteps ? kMaxGroup : 0;
+        if (const char* t = std::getenv("CSM_GROUP_STEPS")) want = static_cast<uint64_t>(std::max(0L, std::atol(t)));
+        uint64_t budget = kGroupBudgetBytes;
+        if (opt_.pool_bytes) budget = std::min<uint64_t>(budget, opt_.pool_bytes / 8);
+        group_ = static_cast<uint32_t>(std::min<uint64_t>(want, std::min<uint64_t>(kMaxGroup, budget / (2 * slot_bytes))));
     }
     if (group_) {
         add(2ull * group_ * slot_bytes);

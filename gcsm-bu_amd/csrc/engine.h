@@ -69,6 +69,7 @@ constexpr uint32_t kLaneWords = 64;
 constexpr uint32_t kSlotWords = 64;
 constexpr uint64_t kGroupBudgetBytes = 6ull << 30;  // device memory of the local-index slots
 constexpr uint32_t kMaxGroup = 64;
+constexpr uint32_t kGroupMinSteps = 256;  // standing query edges from which steps' local indexes are built together
 
 struct QueryResult {
     uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;

@@ -36,7 +36,8 @@ BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_wri
 KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat_chain<0>",
              "level_write": "k_level_flat_chain<1>", "local_index": "k_local_queue", "local_check": "k_local_check",
              "merge_insert": "k_apply_insertions"}
-SERVING_METRIC = "serving: update edges/s and p50/p99 latency of in-memory batches (host arrays, copy included)"
+SERVING_METRIC = ("serving: update edges/s and p50/p99 arrival-to-completion latency of in-memory batches arriving as a "
+                  "Poisson process (host arrays, copy included)")
 WORKLOADS = {
     "c3": "plain method, large/skewed: Chung-Lu 4M vertices / ~35M edges, gamma 2.0, 8 hubs of expected degree "
           "20,000 (others capped at 2,000), 16 uniform labels, 5% held out, 95/5 insert/delete stream, batches of "
@@ -85,12 +86,20 @@ def ensure_built():
         m.build()
 
 
+# Queries the generator selected earlier for a configuration at full size (its exact, budgeted count
+# per candidate takes minutes there); the graph and stream are regenerated from the seed.
+SELECTED = {("c4", 1): os.path.join(PKG, "workloads", "c4_seed1")}
+
+
 def workload(config, seed, data_dir, scale=1.0):
     meta_path = os.path.join(data_dir, "meta.txt")
     if not os.path.exists(meta_path):
         log(f"generating workload {config} seed {seed} scale {scale} into {data_dir}")
         t = time.time()
-        subprocess.run([TOOL, "gen", config, data_dir, "--seed", str(seed), "--scale", str(scale)], check=True)
+        cmd = [TOOL, "gen", config, data_dir, "--seed", str(seed), "--scale", str(scale)]
+        if scale == 1.0 and (config, seed) in SELECTED:
+            cmd += ["--queries-from", SELECTED[(config, seed)]]
+        subprocess.run(cmd, check=True)
         log(f"generated in {time.time() - t:.1f}s")
     meta = dict(line.split() for line in open(meta_path))
     queries = [os.path.join(data_dir, f"q{k}.graph") for k in range(int(meta["queries"]))]
@@ -109,10 +118,18 @@ def open_session(ffi, data_dir, queries, batch, lanes=0, semi_bfs_threshold=0):
     return s, nb
 
 
-def serving(ffi, data_dir, queries, meta, max_batches):
-    """In-memory batches of the directed serving workload: lognormal batch sizes from batches.txt,
-    each handed over as host arrays (csm_apply_edges: host-to-device copy, arena admission, index
-    update, matching of every standing query).  Latency is wall clock per batch."""
+def serving(ffi, data_dir, queries, meta, max_batches, load=0.75, seed=7):
+    """The directed serving workload's in-memory batches (lognormal sizes from batches.txt) arriving
+    as a seeded Poisson process, served two ways on fresh sessions over the same graph:
+      - request loop (continuous batching, csm_serve_*): each batch is submitted at its arrival
+        time; the next request is validated, keyed and copied to the device while the current one
+        matches, and enters a staging slot as soon as one frees up;
+      - plain per-batch path: one csm_apply_edges call per batch, in arrival order, each starting
+        when it has arrived and the previous call has returned.
+    Latency is arrival to completion (queueing included).  The arrival rate is `load` times the
+    plain path's service rate, measured on the first batches beforehand.  Every request's results
+    must be identical in both runs.  Batch 0 warms each session up and is not timed."""
+    import threading
     import numpy as np
     cols = {"src": [], "dst": [], "lab": [], "op": []}
     for line in open(os.path.join(data_dir, "update.stream")):
@@ -123,32 +140,101 @@ def serving(ffi, data_dir, queries, meta, max_batches):
     src, dst = np.array(cols["src"], dtype=np.uint32), np.array(cols["dst"], dtype=np.uint32)
     lab, op = np.array(cols["lab"], dtype=np.uint32), np.array(cols["op"], dtype=np.uint8)
     sizes = [int(x) for x in open(os.path.join(data_dir, "batches.txt"))][:max_batches]
-    s = ffi.Session(device=0, directed=True, max_batch_edges=int(meta["batch_max"]))
-    s.load_graph(os.path.join(data_dir, "data.graph"))
-    for q in queries:
-        s.add_query(q)
-    s.build()
-    for q in s.queries:
-        s.plan(q)
-    lat, edges, matches, lo = [], 0, 0, 0
-    for b, x in enumerate(sizes):
-        t0 = time.perf_counter()
-        res, info = s.apply_edges(src[lo:lo + x], dst[lo:lo + x], lab[lo:lo + x], op[lo:lo + x])
-        s.synchronize()
-        if b > 0:  # batch 0 warms up
-            lat.append(time.perf_counter() - t0)
-            edges += x
-            matches += sum(r[0] + r[1] for r in res)
+    batches, lo = [], 0
+    for x in sizes:
+        batches.append((src[lo:lo + x], dst[lo:lo + x], lab[lo:lo + x], op[lo:lo + x]))
         lo += x
+
+    def session():
+        s = ffi.Session(device=0, directed=True, max_batch_edges=int(meta["batch_max"]))
+        s.load_graph(os.path.join(data_dir, "data.graph"))
+        for q in queries:
+            s.add_query(q)
+        s.build()
+        for q in s.queries:
+            s.plan(q)
+        s.apply_edges(*batches[0])  # warm-up
+        s.synchronize()
+        return s
+
+    work = batches[1:]
+    # service time of the plain path on a prefix (its own session, discarded)
+    s = session()
+    t0 = time.perf_counter()
+    k = min(8, len(work))
+    for b in work[:k]:
+        s.apply_edges(*b)
+    s.synchronize()
+    mean_service = (time.perf_counter() - t0) / k
+    s.close()
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(mean_service / load, size=len(work))
+    arrivals = np.cumsum(gaps) - gaps[0]  # seconds after the first arrival
+    log(f"serving: {len(work)} requests, mean plain service {mean_service * 1e3:.2f} ms, Poisson arrivals at load {load}")
+
+    # plain per-batch path
+    s = session()
+    plain_lat, plain_res = [], []
+    start = time.perf_counter()
+    for a, b in zip(arrivals, work):
+        now = time.perf_counter() - start
+        if now < a:
+            time.sleep(a - now)
+        res, _ = s.apply_edges(*b)
+        s.synchronize()
+        plain_lat.append((time.perf_counter() - start - a) * 1e3)
+        plain_res.append(res)
+    plain_wall = time.perf_counter() - start
+    s.close()
+    log(f"serving: plain path done in {plain_wall:.2f}s")
+
+    # request loop
+    s = session()
+    s.serve_start()
+    tickets = [None] * len(work)
+    submit_at = [0.0] * len(work)
+    start = time.perf_counter()
+
+    def submitter():
+        for i, (a, b) in enumerate(zip(arrivals, work)):
+            now = time.perf_counter() - start
+            if now < a:
+                time.sleep(a - now)
+            submit_at[i] = time.perf_counter() - start
+            tickets[i] = s.serve_submit(*b)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    loop_lat, loop_res, matches = [], [], 0
+    for i in range(len(work)):
+        while tickets[i] is None:
+            time.sleep(0.0002)
+        res, info, ms = s.serve_wait(tickets[i])
+        # latency from arrival: the library's submit-to-done time plus any submission lag
+        loop_lat.append(ms + (submit_at[i] - arrivals[i]) * 1e3)
+        loop_res.append(res)
+        matches += sum(r[0] + r[1] for r in res)
+    loop_wall = time.perf_counter() - start
+    th.join()
+    s.serve_stop()
     used, cap = s.arena_usage()
     s.close()
-    lat_ms = sorted(v * 1e3 for v in lat)
-    total = sum(lat)
-    return {"metric": SERVING_METRIC, "value": edges / total, "unit": "update edges/s", "batches": len(lat),
+    if loop_res != plain_res:
+        raise RuntimeError("request loop results differ from the plain per-batch path")
+    edges = sum(len(b[0]) for b in work)
+
+    def pct(v, q):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+    return {"metric": SERVING_METRIC, "value": edges / loop_wall, "unit": "update edges/s", "requests": len(work),
+            "arrivals": f"Poisson, rate {load} x the plain path's service rate ({1 / mean_service * load:.1f} batches/s), seed {seed}",
             "batch_sizes": {"min": min(sizes[1:]), "median": sorted(sizes[1:])[len(sizes[1:]) // 2], "max": max(sizes[1:])},
-            "p50_batch_ms": statistics.median(lat_ms),
-            "p99_batch_ms": lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))],
-            "matches_per_s": matches / total, "queries": len(queries), "arena_words_used": used, "arena_words": cap,
+            "p50_batch_ms": pct(loop_lat, 0.5), "p99_batch_ms": pct(loop_lat, 0.99), "matches_per_s": matches / loop_wall,
+            "plain_path": {"update_edges_per_s": edges / plain_wall, "p50_batch_ms": pct(plain_lat, 0.5),
+                           "p99_batch_ms": pct(plain_lat, 0.99), "matches_per_s": matches / plain_wall},
+            "results": "every request identical in both runs", "queries": len(queries),
+            "arena_words_used": used, "arena_words": cap,
             "config": {"workload": WORKLOADS[meta["config"]], "seed": int(meta["seed"])}}
 
 
@@ -348,7 +434,9 @@ def main():
                     help="also run a plain copy kernel of 2^26 words (HBM traffic counter calibration)")
     ap.add_argument("--serving", action="store_true",
                     help="run the in-memory serving workload (c5) instead and print its own JSON line")
-    ap.add_argument("--serving-batches", type=int, default=64)
+    ap.add_argument("--serving-batches", type=int, default=96)
+    ap.add_argument("--serving-load", type=float, default=0.75,
+                    help="arrival rate of the serving run as a fraction of the plain path's service rate")
     ap.add_argument("--autotune", action="store_true",
                     help="pick lanes and the semi-BFS threshold first: one fresh session per candidate on "
                          "batches 0-2 (batch 0 untimed), results checked equal; then measure with the fastest")
@@ -376,7 +464,7 @@ def main():
     meta, queries = workload(args.config, args.seed, data_dir, args.scale)
     batch = int(meta["batch_size"])
     if args.serving:
-        print(json.dumps(serving(ffi, data_dir, queries, meta, args.serving_batches + 1)))
+        print(json.dumps(serving(ffi, data_dir, queries, meta, args.serving_batches + 1, load=args.serving_load)))
         return 0
     if args.roofline_only:
         r = roofline_pass(ffi, data_dir, queries, batch, args.config)

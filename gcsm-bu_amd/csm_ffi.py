@@ -17,11 +17,12 @@ EXPORTED = [
     "csm_get_plan_text", "csm_get_summary", "csm_apply_batch", "csm_apply_range", "csm_apply_range_static", "csm_apply_edges", "csm_serve_start", "csm_serve_submit", "csm_serve_wait", "csm_serve_stop", "csm_arena_usage",
     "csm_get_relation_edge_counts",
     "csm_get_counts", "csm_pool_usage", "csm_read_candidates", "csm_read_adjacency",
-    "csm_synchronize", "csm_snapshot", "csm_restore", "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
+    "csm_synchronize", "csm_snapshot", "csm_restore", "csm_get_tunables", "csm_set_tunables", "csm_tune",
+    "csm_profile_enable", "csm_profile_read", "csm_calibration_copy",
 ]
 
 STATUS = {0: "OK", 1: "ARGUMENT", 2: "IO", 3: "FORMAT", 4: "LIMIT", 5: "DEVICE", 6: "MEMORY", 7: "STATE",
-          8: "PRECONDITION"}
+          8: "PRECONDITION", 9: "MISMATCH"}
 
 
 class Options(ctypes.Structure):
@@ -44,6 +45,16 @@ class QueryResult(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     _fields_ = [("edges", ctypes.c_uint64), ("inserts", ctypes.c_uint64), ("deletes", ctypes.c_uint64),
                 ("ms_total", ctypes.c_double), ("ms_index", ctypes.c_double), ("ms_match", ctypes.c_double)]
+
+
+TUNABLES = ("semi_bfs_threshold", "waves_per_anchor", "stack_depth", "dfs_blocks", "local_blocks")
+
+
+class Tunables(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint32) for name in TUNABLES]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name in TUNABLES}
 
 
 class CsmError(RuntimeError):
@@ -92,6 +103,9 @@ def load():
         lib.csm_synchronize.argtypes = [vp]
         lib.csm_snapshot.argtypes = [vp]
         lib.csm_restore.argtypes = [vp]
+        lib.csm_get_tunables.argtypes = [vp, P(Tunables)]
+        lib.csm_set_tunables.argtypes = [vp, P(Tunables)]
+        lib.csm_tune.argtypes = [vp, u64, u64, u32, u32, P(Tunables), u32, P(Tunables), P(ctypes.c_double), u32, P(u32), P(u32)]
         lib.csm_profile_enable.argtypes = [vp, u32]
         lib.csm_profile_read.argtypes = [vp, u32, P(u64), P(ctypes.c_double), P(u64)]
         lib.csm_calibration_copy.argtypes = [vp, u64, P(ctypes.c_double)]
@@ -283,6 +297,29 @@ class Session:
     def restore(self):
         """Back to the last snapshot's graph state (csm_restore)."""
         self._check(self.lib.csm_restore(self.h))
+
+    def tunables(self):
+        """The session's launch settings (csm_get_tunables) as a dict."""
+        t = Tunables()
+        self._check(self.lib.csm_get_tunables(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tunables(self, **kw):
+        """Change launch settings between batches (csm_set_tunables); omitted ones are kept."""
+        self._check(self.lib.csm_set_tunables(self.h, ctypes.byref(Tunables(**kw))))
+
+    def tune(self, first, batch_edges, warmup=1, batches=2, candidates=None, cap=256):
+        """The library's autotuner (csm_tune) over stream edges [first, first + (warmup + batches) *
+        batch_edges).  candidates: dicts of settings (omitted ones keep the session's), or None for
+        the coordinate search.  Returns (evaluations as [(settings dict, ms per batch)], index of the
+        fastest); the session keeps the fastest settings."""
+        cands = [Tunables(**c) for c in candidates or []]
+        arr = (Tunables * max(1, len(cands)))(*cands)
+        ev, ms = (Tunables * cap)(), (ctypes.c_double * cap)()
+        n, best = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self.lib.csm_tune(self.h, first, batch_edges, warmup, batches, arr, len(cands), ev, ms, cap,
+                                      ctypes.byref(n), ctypes.byref(best)))
+        return [(ev[k].as_dict(), ms[k]) for k in range(min(n.value, cap))], best.value
 
     def profile_enable(self, mask):
         self._check(self.lib.csm_profile_enable(self.h, mask))

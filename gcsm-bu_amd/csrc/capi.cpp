@@ -359,6 +359,41 @@ csm_status csm_restore(csm_session* s) {
     return classify(s, s->engine.restore());
 }
 
+csm_status csm_get_tunables(csm_session* s, csm_tunables* out) {
+    if (!s || !out) return CSM_ERR_ARGUMENT;
+    const Tunables t = s->engine.tunables();
+    *out = csm_tunables{t.semi_bfs_threshold, t.waves_per_anchor, t.stack_depth, t.dfs_blocks, t.local_blocks};
+    return CSM_OK;
+}
+
+csm_status csm_set_tunables(csm_session* s, const csm_tunables* t) {
+    if (!s || !t) return CSM_ERR_ARGUMENT;
+    return classify(s, s->engine.set_tunables(Tunables{t->semi_bfs_threshold, t->waves_per_anchor, t->stack_depth, t->dfs_blocks,
+                                                       t->local_blocks}));
+}
+
+csm_status csm_tune(csm_session* s, uint64_t first, uint64_t batch_edges, uint32_t warmup, uint32_t batches,
+                    const csm_tunables* candidates, uint32_t num_candidates, csm_tunables* evaluated, double* ms_per_batch,
+                    uint32_t cap, uint32_t* num_evaluated, uint32_t* best) {
+    if (!s || (num_candidates && !candidates)) return CSM_ERR_ARGUMENT;
+    std::vector<Tunables> cands;
+    for (uint32_t k = 0; k < num_candidates; k++) {
+        const csm_tunables& c = candidates[k];
+        cands.push_back(Tunables{c.semi_bfs_threshold, c.waves_per_anchor, c.stack_depth, c.dfs_blocks, c.local_blocks});
+    }
+    std::vector<Engine::TuneEval> trail;
+    uint32_t b = 0;
+    const csm_status st = classify(s, s->engine.tune(first, batch_edges, warmup, batches, cands, trail, b));
+    if (num_evaluated) *num_evaluated = static_cast<uint32_t>(trail.size());
+    if (best) *best = b;
+    for (size_t k = 0; k < trail.size() && k < cap; k++) {
+        const Tunables& t = trail[k].t;
+        if (evaluated) evaluated[k] = csm_tunables{t.semi_bfs_threshold, t.waves_per_anchor, t.stack_depth, t.dfs_blocks, t.local_blocks};
+        if (ms_per_batch) ms_per_batch[k] = trail[k].ms_per_batch;
+    }
+    return st;
+}
+
 csm_status csm_synchronize(csm_session* s) {
     if (!s) return CSM_ERR_ARGUMENT;
     return classify(s, s->engine.synchronize());

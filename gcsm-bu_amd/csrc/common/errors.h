@@ -17,6 +17,7 @@ enum class Code : char {
     kMemory = '6',        // CSM_ERR_MEMORY
     kState = '7',         // CSM_ERR_STATE
     kPrecondition = '8',  // CSM_ERR_PRECONDITION
+    kMismatch = '9',      // CSM_ERR_MISMATCH
 };
 
 inline std::string tagged(Code c, const std::string& msg) {

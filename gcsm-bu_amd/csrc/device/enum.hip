@@ -140,8 +140,10 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
         setup_frame(g, d, W, depth, lane, my_bytes, chunk == 0);
         if (split) {
             if (lane == 0) {
-                W.cur[d0] = chunk * 64u;
-                W.tot[d0] = min(W.tot[d0], chunk * 64u + 64u);
+                const uint32_t k = static_cast<uint32_t>(split[rec_idx + 1] - split[rec_idx]);  // chunks of the record
+                const uint32_t width = ((W.tot[d0] + k - 1) / k + 63u) & ~63u;
+                W.cur[d0] = chunk * width;
+                W.tot[d0] = min(W.tot[d0], chunk * width + width);
             }
             wave_sync();
         }
@@ -774,13 +776,13 @@ __global__ void __launch_bounds__(256) k_level_flat_chain(DevGraph g, StepDesc d
 }
 
 __global__ void __launch_bounds__(256) k_root_chunks(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
-                                                     uint32_t max_rec, unsigned long long* chunks) {
+                                                     uint32_t max_rec, uint32_t max_chunks, unsigned long long* chunks) {
     const uint32_t n = min(*n_rec, max_rec);
     const BackEntry be = d.be[d.pivot[d0]];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t v = rec[static_cast<uint64_t>(i) * d0 + be.pos];
         const RunView rv = make_run(g, d.phase, be.view, be.dir, v, be.slot);
-        chunks[i] = (run_seq_len(rv, be.view) + 63u) / 64u;
+        chunks[i] = min(max_chunks, (run_seq_len(rv, be.view) + 63u) / 64u);
     }
 }
 
@@ -1056,10 +1058,10 @@ void bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_
 }
 
 void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r, MatchOut out,
-            const unsigned long long* split, unsigned long long* work_counter, int schedule, hipStream_t s) {
-    // Persistent grid: enough waves to fill 256 CUs several times over, never more than records
-    // (a split launch has at least as many items as records with candidates).
-    const int grid = split ? 2048 : grid_for(r.in_max, kWaves, 2048);
+            const unsigned long long* split, unsigned long long* work_counter, int schedule, int blocks, hipStream_t s) {
+    // Persistent grid: `blocks` (2048 by default: enough waves to fill 256 CUs several times over),
+    // never more than records (a split launch has at least as many items as records with candidates).
+    const int grid = split ? blocks : grid_for(r.in_max, kWaves, blocks);
     const uint32_t rows = end_depth > d0 + 1 ? end_depth - 1 - d0 : 0u;
     const size_t lds = static_cast<size_t>(kWaves) * wave_lds_bytes(rows);
     if (mode == kCountAtEnd)
@@ -1117,8 +1119,8 @@ uint32_t level_max_records(const StepDesc& d, uint32_t d0, const LevelScratch& l
 }
 
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec,
-                 unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s) {
-    k_root_chunks<<<grid_for(max_rec, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n_rec, max_rec, chunks);
+                 uint32_t max_chunks, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s) {
+    k_root_chunks<<<grid_for(max_rec, 256, 4096), 256, 0, s>>>(g, d, d0, rec, n_rec, max_rec, max_chunks, chunks);
     scan_exclusive<unsigned long long>(chunks, chunks, n_rec, max_rec, true, scan_tmp, s);
 }
 

@@ -502,6 +502,22 @@ __global__ void __launch_bounds__(256) k_slot_dir(DevGraph g, int dir, const uin
     }
 }
 
+__global__ void k_bucket_count(DevGraph g, BatchScratch w) {
+    const uint32_t T = *w.n_touched;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        const uint32_t r = g.vrank[w.touched[i]];
+        if (r < g.nvl) atomicAdd(&w.label_cur[r], 1u);
+    }
+}
+__global__ void k_bucket_scatter(DevGraph g, BatchScratch w) {
+    const uint32_t T = *w.n_touched;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        const uint32_t v = w.touched[i];
+        const uint32_t r = g.vrank[v];
+        if (r < g.nvl) w.by_label[atomicAdd(&w.label_cur[r], 1u)] = v;
+    }
+}
+
 __global__ void k_reset_counters(BatchScratch w) {
     if (threadIdx.x == 0 && blockIdx.x == 0) { *w.n_touched = 0; *w.n_big = 0; }
 }
@@ -524,6 +540,15 @@ void batch_build(const DevGraph& g, int dir, int dirsel, const BatchInput& b, ui
     const int sgrid = static_cast<int>(std::min<uint32_t>(2048, (max_arcs + kSortWaves - 1) / kSortWaves + 1));
     k_batch_segsort<<<sgrid, kSortWaves * 64, 0, s>>>(w, g.delta[dir]);
     k_batch_segsort_big<<<64, 1024, 0, s>>>(w, g.delta[dir], g.err);
+}
+
+void batch_bucket(const DevGraph& g, uint32_t max_touched, const BatchScratch& w, hipStream_t s) {
+    const int grid = static_cast<int>(std::min<uint32_t>(2048, (max_touched + 255) / 256 + 1));
+    fill_u32(w.label_cur, 0u, g.nvl + 1, s);
+    k_bucket_count<<<grid, 256, 0, s>>>(g, w);
+    scan_exclusive<uint32_t>(w.label_cur, w.label_off, nullptr, g.nvl, true, w.label_tmp, s);
+    copy_u32(w.label_off, w.label_cur, g.nvl, s);  // the scatter's cursors
+    k_bucket_scatter<<<grid, 256, 0, s>>>(g, w);
 }
 
 void slot_dir_update(const DevGraph& g, int dir, const uint32_t* list, const uint32_t* n_list, uint32_t max_n, hipStream_t s) {

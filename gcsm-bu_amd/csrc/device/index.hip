@@ -106,13 +106,15 @@ __device__ __forceinline__ void vset_add(VSet s, uint32_t v) {
 }
 
 __global__ void k_local_anchors(DevGraph g, uint32_t phase, uint32_t dir, uint32_t slot, uint32_t vr0, const uint32_t* cand0,
-                                const uint32_t* cand1, const uint32_t* touched, const uint32_t* n_touched, uint32_t* pairs,
+                                const uint32_t* cand1, const uint32_t* by_label, const uint32_t* label_off, uint32_t* pairs,
                                 uint32_t* n_pairs, uint32_t max_pairs, VSet L0, VSet L1) {
-    const uint32_t T = *n_touched;
+    if (vr0 == kNoVertex) return;
+    const uint32_t first = label_off[vr0], T = label_off[vr0 + 1] - first;
+    const uint32_t* touched = by_label + first;
     const uint32_t* dl = phase_delta(g, dir, phase);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
         const uint32_t v = touched[i];
-        if (g.vrank[v] != vr0 || !bit_test(cand0, v)) continue;
+        if (!bit_test(cand0, v)) continue;
         const uint4 s = g.delta[dir].seg[v];
         const uint32_t off = phase ? s.z : s.x, len = phase ? s.w : s.y;
         if (!len) continue;
@@ -379,15 +381,17 @@ __global__ void __launch_bounds__(256) k_local_check(DevGraph g, uint32_t phase,
 // ---- many steps at once --------------------------------------------------------------------
 
 // One block row per job; the body is k_local_anchors'.
-__global__ void k_local_anchors_multi(DevGraph g, const AnchorJob* jobs, const uint32_t* touched0, const uint32_t* n_touched0,
-                                      const uint32_t* touched1, const uint32_t* n_touched1) {
+__global__ void k_local_anchors_multi(DevGraph g, const AnchorJob* jobs, const uint32_t* by_label0, const uint32_t* label_off0,
+                                      const uint32_t* by_label1, const uint32_t* label_off1) {
     const AnchorJob J = jobs[blockIdx.y];
-    const uint32_t* touched = J.dir ? touched1 : touched0;
-    const uint32_t T = *(J.dir ? n_touched1 : n_touched0);
+    if (J.vr0 == kNoVertex) return;
+    const uint32_t* label_off = J.dir ? label_off1 : label_off0;
+    const uint32_t first = label_off[J.vr0], T = label_off[J.vr0 + 1] - first;
+    const uint32_t* touched = (J.dir ? by_label1 : by_label0) + first;
     const uint32_t* dl = phase_delta(g, J.dir, J.phase);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
         const uint32_t v = touched[i];
-        if (g.vrank[v] != J.vr0 || !bit_test(J.cand0, v)) continue;
+        if (!bit_test(J.cand0, v)) continue;
         const uint4 sg = g.delta[J.dir].seg[v];
         const uint32_t off = J.phase ? sg.z : sg.x, len = J.phase ? sg.w : sg.y;
         if (!len) continue;
@@ -553,9 +557,9 @@ void relation_summary(const DevGraph& g, uint32_t dir, uint32_t slot, uint32_t v
 }
 
 void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slot, uint32_t vr0, const uint32_t* cand0,
-                   const uint32_t* cand1, const uint32_t* touched, const uint32_t* n_touched, uint32_t max_touched, uint32_t* pairs,
+                   const uint32_t* cand1, const uint32_t* by_label, const uint32_t* label_off, uint32_t max_touched, uint32_t* pairs,
                    uint32_t* n_pairs, uint32_t max_pairs, VSet L0, VSet L1, hipStream_t s) {
-    k_local_anchors<<<grid_for(max_touched, 256, 4096), 256, 0, s>>>(g, phase, dir, slot, vr0, cand0, cand1, touched, n_touched,
+    k_local_anchors<<<grid_for(max_touched, 256, 1024), 256, 0, s>>>(g, phase, dir, slot, vr0, cand0, cand1, by_label, label_off,
                                                                       pairs, n_pairs, max_pairs, L0, L1);
 }
 
@@ -578,11 +582,11 @@ void local_check(const DevGraph& g, uint32_t phase, const LocalFilters& fl, VSet
     k_local_check<<<blocks, 256, 0, s>>>(g, phase, queue, queue_len, g.dv, fl, out, bytes);
 }
 
-void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njobs, const uint32_t* touched0, const uint32_t* n_touched0,
-                         const uint32_t* touched1, const uint32_t* n_touched1, uint32_t max_touched, hipStream_t s) {
+void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njobs, const uint32_t* by_label0, const uint32_t* label_off0,
+                         const uint32_t* by_label1, const uint32_t* label_off1, uint32_t max_touched, hipStream_t s) {
     if (!njobs) return;
-    const dim3 grid(static_cast<uint32_t>(grid_for(max_touched, 256, static_cast<int>(std::max<uint32_t>(16, 8192 / njobs)))), njobs);
-    k_local_anchors_multi<<<grid, 256, 0, s>>>(g, jobs, touched0, n_touched0, touched1, n_touched1);
+    const dim3 grid(static_cast<uint32_t>(grid_for(max_touched, 256, static_cast<int>(std::max<uint32_t>(16, 4096 / njobs)))), njobs);
+    k_local_anchors_multi<<<grid, 256, 0, s>>>(g, jobs, by_label0, label_off0, by_label1, label_off1);
 }
 
 void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,

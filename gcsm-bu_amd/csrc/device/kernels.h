@@ -37,6 +37,12 @@ struct BatchScratch {
     uint32_t* n_big;     // [1]
     uint32_t* big_ptr;   // [kMaxBigSegments]
     uint32_t* big_len;
+    // the touched vertices grouped by label rank (batch_bucket): rank r's at by_label[label_off[r] ..
+    // label_off[r + 1])
+    uint32_t* by_label;  // [max_arcs]
+    uint32_t* label_off; // [nvl + 1]
+    uint32_t* label_cur; // [nvl + 1]
+    uint32_t* label_tmp; // scan_scratch_elems(nvl + 1)
 };
 
 template <typename T>
@@ -45,6 +51,8 @@ uint64_t scan_scratch_elems(uint32_t n_max);
 
 void batch_build(const DevGraph& g, int dir, int dirsel, const BatchInput& b, uint32_t lo, uint32_t n,
                  const BatchScratch& w, hipStream_t s);
+// Groups the built batch's touched vertices by label rank (w.by_label, w.label_off).
+void batch_bucket(const DevGraph& g, uint32_t max_touched, const BatchScratch& w, hipStream_t s);
 // Admission of the built batch (per direction, then finish once): adm[6 + c] += the insertions'
 // relocations into blocks of size class c, adm[1] |= a precondition violation; finish moves
 // {need, bad, arena top, error bits} to adm[2..5], need = words the free blocks cannot serve, and
@@ -98,8 +106,10 @@ struct VSet {
 
 // Anchors of the step's query edge u0->u1: batch arcs (v, x) of the phase with v in cand(u0) of
 // label rank vr0 and x in cand(u1).  Writes pairs (v, x), and fills L(u0) = {v}, L(u1) = {x}.
+// by_label / label_off: the touched vertices of direction dir grouped by label rank (batch_bucket);
+// only rank vr0's group is read.
 void local_anchors(const DevGraph& g, uint32_t phase, uint32_t dir, uint32_t slot, uint32_t vr0,
-                   const uint32_t* cand0, const uint32_t* cand1, const uint32_t* touched, const uint32_t* n_touched,
+                   const uint32_t* cand0, const uint32_t* cand1, const uint32_t* by_label, const uint32_t* label_off,
                    uint32_t max_touched, uint32_t* pairs, uint32_t* n_pairs, uint32_t max_pairs, VSet L0, VSet L1,
                    hipStream_t s);
 // L(u) seed: every vertex reached from the frontier through the relation view (b -> u) that is a
@@ -157,8 +167,8 @@ struct LocalJob {
     LocalFilters fl;                   // the other built neighbours (none: every queued vertex is kept)
 };
 constexpr uint32_t kMaxLocalJobs = 256;
-void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njobs, const uint32_t* touched0, const uint32_t* n_touched0,
-                         const uint32_t* touched1, const uint32_t* n_touched1, uint32_t max_touched, hipStream_t s);
+void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njobs, const uint32_t* by_label0, const uint32_t* label_off0,
+                         const uint32_t* by_label1, const uint32_t* label_off1, uint32_t max_touched, hipStream_t s);
 // off: kMaxLocalJobs + 1 words of device scratch.  bytes: as local_queue / local_check.
 void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
                        hipStream_t s);
@@ -209,10 +219,11 @@ enum ExtendMode : int { kCountAtEnd = 0, kWriteAtEnd = 1 };
 // end_depth == n, or writes the partial results of depth end_depth to the ring.
 // schedule: 0 static (item t by wave t mod W), 1 dynamic (global work counter, zeroed by the
 // caller).  Without `split` an item is one record.  With `split` (the exclusive scan of
-// root_chunks, total at [n]) an item is one 64-candidate chunk of a record's depth-d0 candidates,
-// so heavy records spread over many waves.
+// root_chunks, total at [n]) an item is one of a record's k chunks: its depth-d0 candidates cut
+// into k equal pieces of whole 64-candidate rounds, so a heavy record spreads over k waves.
+// blocks: the persistent grid (fewer when there are fewer items).
 void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r,
-            MatchOut out, const unsigned long long* split, unsigned long long* work_counter, int schedule, hipStream_t s);
+            MatchOut out, const unsigned long long* split, unsigned long long* work_counter, int schedule, int blocks, hipStream_t s);
 // One level from the n records of depth d0 at rec (n known on the host), flattened over all its
 // candidates: level_prepare writes, per record, the view runs of the depth's backward entries, the
 // entry with the shortest view sequence (it generates the candidates; the others are probed), the
@@ -261,8 +272,9 @@ void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain&
 void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,
                       MatchOut out, hipStream_t s);
 // chunks[i] = ceil(|pivot view run of record i at depth d0| / 64), then scanned in place (total at [n]).
+// Per record, its chunk count k = min(max_chunks, ceil(depth-d0 candidates / 64)), scanned.
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
-                 uint32_t max_rec, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s);
+                 uint32_t max_rec, uint32_t max_chunks, unsigned long long* chunks, unsigned long long* scan_tmp, hipStream_t s);
 
 // Cartesian-product flattening of trailing tree leaves (cartesian_product.cu:193-284).  A segment
 // is one (record, leaf) pair: segment i = record * leaves + leaf.

@@ -81,7 +81,7 @@ std::string Engine::init(const EngineOptions& o) {
     if (const char* t = std::getenv("CSM_TRACE")) trace_ = std::fopen(t, "a");
     if (const char* t = std::getenv("CSM_LANES")) opt_.lanes = static_cast<uint32_t>(std::max(1, std::atoi(t)));
     // A/B switches of the scheduling and layout aids (all on by default)
-    if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) opt_.split_roots = std::atoi(t) != 0;
+    if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) if (std::atoi(t) == 0) opt_.waves_per_anchor = 1;
     if (const char* t = std::getenv("CSM_SLOT_DIRECTORY")) opt_.slot_directory = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_CHAIN_LEVELS")) opt_.chain_levels = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_ARENA_SLACK_WORDS")) opt_.arena_slack_words = std::strtoull(t, nullptr, 10);
@@ -226,6 +226,7 @@ std::string Engine::build() {
         add(dv_ * 12ull);                                       // cnt_ins, cnt_del, tflag
         add(max_arcs * 4); add(16); add(4 * (max_arcs + 1) * 4); add(max_arcs * 4);
         add(scan_scratch_elems(static_cast<uint32_t>(max_arcs)) * 4); add(16); add(2 * kMaxBigSegments * 4);
+        add(max_arcs * 4); add(2 * (nvl_ + 1) * 4ull); add(scan_scratch_elems(nvl_ + 1) * 4);  // label buckets
     }
     add(stream_.size() * 17ull + 2 * max_batch_ * 17ull);      // stream + two live-batch slots
     const uint32_t n_lanes = std::max<uint32_t>(1, std::min<uint32_t>(opt_.lanes, kMaxLanes));
@@ -361,6 +362,11 @@ std::string Engine::build() {
             w.scan_tmp = pool_.take<uint32_t>(scan_scratch_elems(static_cast<uint32_t>(max_arcs)));
             w.big_ptr = pool_.take<uint32_t>(kMaxBigSegments);
             w.big_len = pool_.take<uint32_t>(kMaxBigSegments);
+            w.by_label = pool_.take<uint32_t>(max_arcs);
+            w.label_off = pool_.take<uint32_t>(nvl_ + 1);
+            w.label_cur = pool_.take<uint32_t>(nvl_ + 1);
+            w.label_tmp = pool_.take<uint32_t>(scan_scratch_elems(nvl_ + 1));
+            CSM_HIP(hipMemsetAsync(w.label_off, 0, (nvl_ + 1) * 4ull, s_));
         }
         if (use_sdir)
             for (int d = 0; d < ndirs_; d++) {
@@ -669,7 +675,7 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
 
     const RelInfo& R = qs.rel[r01];
     prof_begin(w.prof, w.s, kKAnchors);
-    local_anchors(g_, ph, R.dir, R.slot, qs.vrank[u0], cand(u0), cand(u1), bs_[R.dir].touched, bs_[R.dir].n_touched, max_arcs, w.pairs,
+    local_anchors(g_, ph, R.dir, R.slot, qs.vrank[u0], cand(u0), cand(u1), bs_[R.dir].by_label, bs_[R.dir].label_off, max_arcs, w.pairs,
                   w.n_pairs, max_arcs, w.L[u0], w.L[u1], w.s);
     prof_end(w.prof, w.s, kKAnchors);
     w.L_used |= static_cast<uint16_t>((1u << u0) | (1u << u1));
@@ -877,7 +883,7 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
         Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
         prof_begin(w.prof, w.s, kKExtendWrite);
         extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, nullptr, w.work_counter(), w.sched,
-               w.s);
+               static_cast<int>(opt_.dfs_blocks), w.s);
         prof_end(w.prof, w.s, kKExtendWrite);
         CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
         CSM_HIP(hipStreamSynchronize(w.s));
@@ -962,7 +968,7 @@ std::string Engine::search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, c
         std::string err = extend_to(w, d, rec, count, depth, depth + 1, out, (w.ring_words - top) / (depth + 1), ok, n_out);
         if (!err.empty()) return err;
         if (!ok) {
-            if (count == 1) return finish_records(w, qs, d, i, depth, count, rec, res_slot);
+            if (count == 1) return finish_records(w, qs, d, i, depth, count, rec, res_slot, true);
             const uint32_t h = count / 2;
             err = search(w, qs, d, i, rec, h, depth, top, target, res_slot);
             if (!err.empty()) return err;
@@ -979,7 +985,7 @@ std::string Engine::search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, c
 }
 
 std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, const uint32_t* rec,
-                                   uint32_t res_slot) {
+                                   uint32_t res_slot, bool any_depth) {
     const uint32_t ph = d.phase;
     MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr};
     uint32_t* rec_count = w.counters + 3;
@@ -1037,14 +1043,23 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
         }
         return "";
     }
+    // Stack depth: a counting DFS needs one LDS row per depth below its first; with more than the
+    // setting allows, breadth-first levels run first (records stay on the lane's ring: the words
+    // above `rec`'s records are free, as every caller leaves its records at the top).
+    const uint32_t rows = d.n - 1 - depth;
+    if (!any_depth && rows > opt_.stack_depth && rec >= w.ring && rec < w.ring + w.ring_words) {
+        const uint64_t top = static_cast<uint64_t>(rec - w.ring) + static_cast<uint64_t>(count) * depth;
+        return search(w, qs, d, i, rec, count, depth, top, d.n - 1 - opt_.stack_depth, res_slot);
+    }
     store_words(rec_count, Words{7, {count, 0, 0, 0, 0, 0, 0}}, w.s);  // [3]: count, [8..9]: work counter
     out.bytes = bytes_ptr(kKExtendCount);
     // Root splitting (dynamic schedule, few records: a handful of heavy records would otherwise
     // serialise on single waves).  The CP offsets buffer is free here.
-    const bool split = opt_.split_roots && w.sched == 1 && count <= kSplitMaxRecords;
+    const bool split = opt_.waves_per_anchor > 1 && w.sched == 1 && count <= kSplitMaxRecords;
     prof_begin(w.prof, w.s, kKExtendCount);
-    if (split) root_chunks(g_, d, depth, rec, rec_count, count, w.cp.prod, w.cp.scan_tmp, w.s);
-    extend(g_, d, depth, d.n, kCountAtEnd, r, out, split ? w.cp.prod : nullptr, w.work_counter(), w.sched, w.s);
+    if (split) root_chunks(g_, d, depth, rec, rec_count, count, opt_.waves_per_anchor, w.cp.prod, w.cp.scan_tmp, w.s);
+    extend(g_, d, depth, d.n, kCountAtEnd, r, out, split ? w.cp.prod : nullptr, w.work_counter(), w.sched,
+           static_cast<int>(opt_.dfs_blocks), w.s);
     prof_end(w.prof, w.s, kKExtendCount);
     return "";
 }
@@ -1098,7 +1113,7 @@ std::string Engine::stage_group(uint32_t ph, std::vector<GroupStep>& steps) {
     }
     CSM_HIP(hipMemcpyAsync(d_anchor_jobs_, aj, n * sizeof(AnchorJob), hipMemcpyHostToDevice, s_));
     prof_begin(prof_main_, s_, kKAnchors);
-    local_anchors_multi(g_, d_anchor_jobs_, n, bs_[0].touched, bs_[0].n_touched, bs_[1].touched, bs_[1].n_touched, max_arcs, s_);
+    local_anchors_multi(g_, d_anchor_jobs_, n, bs_[0].by_label, bs_[0].label_off, bs_[1].by_label, bs_[1].label_off, max_arcs, s_);
     prof_end(prof_main_, s_, kKAnchors);
     CSM_HIP(hipMemcpyAsync(h_ctr, ctr, static_cast<uint64_t>(n) * kSlotWords * 4, hipMemcpyDeviceToHost, s_));
     CSM_HIP(hipStreamSynchronize(s_));
@@ -1434,8 +1449,10 @@ std::string Engine::run_resident(uint64_t lo, uint64_t n, const uint32_t* hsrc, 
             return fail(tagged(Code::kPrecondition, "batch precondition violated: an inserted edge exists, a deleted edge is missing, or an edge occurs twice in the batch"));
         return fail(tagged(Code::kMemory, "adjacency arena budget exhausted: the batch needs " + std::to_string(a[0]) + " more words of list space"));
     }
-    for (int d = 0; d < ndirs_; d++)
+    for (int d = 0; d < ndirs_; d++) {
+        batch_bucket(g_, max_arcs, bs_[d], s_);
         nlf_touched(g_, d_entries_, n_entries_, d_groups_, bs_[d].touched, bs_[d].n_touched, max_arcs, 0, s_);
+    }
     fill_u32(reinterpret_cast<uint32_t*>(results_), 0u, qs_.size() * 8, s_);
     const auto t1 = std::chrono::steady_clock::now();
     double match_ms = 0;

@@ -47,7 +47,10 @@ struct EngineOptions {
     uint32_t max_batch_edges = 0;     // 0: the stream's batch size
     uint32_t semi_bfs_threshold = CSM_DEFAULT_SEMI_BFS_THRESHOLD;
     bool directed = false;
-    bool split_roots = true;          // spread heavy records of the counting DFS over many waves
+    // Launch settings that change scheduling only, never results (Tunables):
+    uint32_t waves_per_anchor = 1024; // most waves one record of the counting DFS is split over (1: no split)
+    uint32_t stack_depth = kMaxQV;    // most LDS stack rows of a counting DFS launch; deeper searches run levels first
+    uint32_t dfs_blocks = 2048;       // blocks of the persistent DFS grid
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
     bool chain_levels = true;         // semi-BFS levels of a step chained on the device (one host wait per step)
     uint32_t lanes = 8;               // concurrent matching steps (streams), each with its own scratch
@@ -70,6 +73,17 @@ constexpr uint32_t kSlotWords = 64;
 constexpr uint64_t kGroupBudgetBytes = 6ull << 30;  // device memory of the local-index slots
 constexpr uint32_t kMaxGroup = 64;
 constexpr uint32_t kGroupMinSteps = 256;  // standing query edges from which steps' local indexes are built together
+
+// The launch settings a session may change between batches; they never change results.  The
+// reference fixes all of them at compile time (config.h:47 MIN_NUM_RESULTS_TO_GPU, the block and
+// warp counts of match_gpu.cu's launches).
+struct Tunables {
+    uint32_t semi_bfs_threshold = 0;  // records at a Cartesian-product depth at which the levels stop expanding
+    uint32_t waves_per_anchor = 0;    // most waves one record's first-depth candidates are split over (1..65535)
+    uint32_t stack_depth = 0;         // most LDS stack rows (depths) of a counting DFS launch (1..kMaxQV)
+    uint32_t dfs_blocks = 0;          // blocks of the persistent DFS grid (64..16384)
+    uint32_t local_blocks = 0;        // blocks of the local-index kernels (16..4096)
+};
 
 struct QueryResult {
     uint64_t pos = 0, neg = 0, pos_hash = 0, neg_hash = 0;
@@ -138,6 +152,20 @@ class Engine {
     // then continues as if the batches after the snapshot had never been applied.
     std::string snapshot();
     std::string restore();
+    // Launch settings (fields of 0 keep their current value); tune() times candidate settings on
+    // the stream edges [first, first + (warmup + timed) * batch) replayed from a snapshot taken on
+    // entry (it replaces any earlier snapshot), checks that every candidate's results equal the
+    // first one's, leaves the graph as it found it and keeps the fastest settings.  Without
+    // candidates it runs a coordinate search from the current settings over built-in value lists
+    // (each setting in turn, the others held at the best so far); trail gets every evaluation.
+    Tunables tunables() const;
+    std::string set_tunables(const Tunables& t);
+    struct TuneEval {
+        Tunables t;
+        double ms_per_batch = 0;
+    };
+    std::string tune(uint64_t first, uint64_t batch, uint32_t warmup, uint32_t timed, const std::vector<Tunables>& cands,
+                     std::vector<TuneEval>& trail, uint32_t& best);
 
     uint32_t num_queries() const { return static_cast<uint32_t>(qs_.size()); }
     uint64_t num_batches() const;
@@ -260,8 +288,9 @@ class Engine {
                        uint64_t top, uint32_t target, uint32_t res_slot);
     std::string extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, uint32_t count, uint32_t depth, uint32_t end, uint32_t* out,
                           uint64_t cap, bool& ok, uint32_t& n_out);
+    // any_depth: count depth-first whatever the stack depth (a single record that no level holds)
     std::string finish_records(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t depth, uint32_t count, const uint32_t* rec,
-                               uint32_t res_slot);
+                               uint32_t res_slot, bool any_depth = false);
     uint32_t view_of(const QState& qs, uint32_t rel, uint32_t i, uint32_t ph) const;
     uint32_t key_of_edge(uint32_t nbr, uint32_t elabel_raw) const;
     std::string check_err(const char* where);

@@ -6,7 +6,10 @@
 // reference's formats: "<Phase>, time (ms): X(host), Y(kernel)" (cuda_helpers.h:98-102), the
 // "Orders:" block (plan.cpp:1200-1291) and "Num Positive Matches: N" (technique_permutation.h:224).
 // Extensions: --queries a,b,c (several standing queries), --directed, --pool_mb, --ring_mb, --lanes,
-// --semi_bfs_threshold, --json <path> (one JSON record per batch and query).
+// --semi_bfs_threshold, --json <path> (one JSON record per batch and query), the launch settings
+// --waves_per_anchor, --stack_depth, --dfs_blocks, --local_blocks, and --autotune true (the
+// library's autotuner, csm_tune, on the first --autotune_batches batches of the update stream
+// before the timed matching; it leaves the graph as it found it).
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -182,6 +185,34 @@ int main(int argc, char** argv) {
         csm_get_plan_text(s, id, &txt[0], need, &need);
         if (qids.size() > 1) std::cout << "query " << id << " (" << queries[id] << ")\n";
         std::cout << txt.c_str();
+    }
+    {
+        csm_tunables tn{};
+        tn.waves_per_anchor = p.get_u32("--waves_per_anchor", 0);
+        tn.stack_depth = p.get_u32("--stack_depth", 0);
+        tn.dfs_blocks = p.get_u32("--dfs_blocks", 0);
+        tn.local_blocks = p.get_u32("--local_blocks", 0);
+        if ((st = csm_set_tunables(s, &tn)) != CSM_OK) return die(s, st);
+    }
+    if (p.get_bool("--autotune", false) && nb > 0) {
+        const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(nb, std::max<uint32_t>(1, p.get_u32("--autotune_batches", 3))));
+        const uint32_t warm = k > 1 ? 1 : 0;
+        std::vector<csm_tunables> ev(64);
+        std::vector<double> ms(64);
+        uint32_t n = 0, best = 0;
+        t0 = std::chrono::steady_clock::now();
+        if ((st = csm_tune(s, 0, 0, warm, k - warm, nullptr, 0, ev.data(), ms.data(), 64, &n, &best)) != CSM_OK) return die(s, st);
+        for (uint32_t e = 0; e < n && e < 64; e++)
+            std::cout << "Autotune candidate " << e << ": semi_bfs_threshold " << ev[e].semi_bfs_threshold << ", waves_per_anchor "
+                      << ev[e].waves_per_anchor << ", stack_depth " << ev[e].stack_depth << ", dfs_blocks " << ev[e].dfs_blocks
+                      << ", local_blocks " << ev[e].local_blocks << ": " << ms[e] << " ms per batch" << (e == best ? " (best)" : "") << '\n';
+        time_line("Autotune", ms_since(t0), 0);
+    }
+    {
+        csm_tunables tn{};
+        csm_get_tunables(s, &tn);
+        std::cout << "Launch settings: semi_bfs_threshold " << tn.semi_bfs_threshold << ", waves_per_anchor " << tn.waves_per_anchor
+                  << ", stack_depth " << tn.stack_depth << ", dfs_blocks " << tn.dfs_blocks << ", local_blocks " << tn.local_blocks << '\n';
     }
     if (show_mem) {
         uint64_t used = 0, cap = 0;

@@ -35,7 +35,8 @@ typedef enum {
     CSM_ERR_DEVICE = 5,      /* HIP runtime failure (reference: cudaErrorCheck -> exit) */
     CSM_ERR_MEMORY = 6,      /* device pool budget or adjacency arena exhausted (match_gpu.cu:772-775) */
     CSM_ERR_STATE = 7,       /* call out of order (e.g. plan before build) */
-    CSM_ERR_PRECONDITION = 8 /* batch inserts an existing edge or deletes a missing one */
+    CSM_ERR_PRECONDITION = 8,/* batch inserts an existing edge or deletes a missing one */
+    CSM_ERR_MISMATCH = 9     /* csm_tune: two launch settings gave different results (a library fault) */
 } csm_status;
 
 typedef struct {
@@ -169,6 +170,40 @@ csm_status csm_synchronize(csm_session* s);
  * size), reused by later snapshots.  restore also clears a failed session's refusal state. */
 csm_status csm_snapshot(csm_session* s);
 csm_status csm_restore(csm_session* s);
+
+/* Launch settings that change scheduling only, never results.  The reference fixes them at compile
+ * time: the semi-BFS switch (config.h:47 MIN_NUM_RESULTS_TO_GPU, match_gpu.cu:1961) and the grid
+ * shapes of its enumeration launches (match_gpu.cu:1721-2294).  In csm_set_tunables a field of 0
+ * keeps the current value; settings change between batches (not while serving). */
+typedef struct {
+    uint32_t semi_bfs_threshold; /* records at a Cartesian-product depth at which the semi-BFS levels stop
+                                    expanding (csm_options.semi_bfs_threshold) */
+    uint32_t waves_per_anchor;   /* depth-first counting: most waves one record's first-depth candidates are
+                                    split over, in whole 64-candidate rounds (1 = one wave per record;
+                                    1..65535, default 1024) */
+    uint32_t stack_depth;        /* depth-first counting: most LDS stack rows (depths below the first) one
+                                    launch descends; deeper searches run breadth-first levels first
+                                    (1..16, default 16) */
+    uint32_t dfs_blocks;         /* blocks of the persistent depth-first grid (64..16384, default 2048) */
+    uint32_t local_blocks;       /* blocks of the local-index kernels (16..2048; default 256 for batches of
+                                    2^18 edges or more, else 128) */
+} csm_tunables;
+csm_status csm_get_tunables(csm_session* s, csm_tunables* out);
+csm_status csm_set_tunables(csm_session* s, const csm_tunables* t);
+/* Autotuner (no counterpart in the reference).  Times launch settings on the resident update
+ * stream's edges [first, first + (warmup + batches) * batch_edges), in batches of batch_edges (0:
+ * the update file's batch size; the last batch may end early at the end of the stream), replayed
+ * from a snapshot taken on entry (it replaces any earlier snapshot); the batches after the warm-up
+ * ones are timed.  With candidates (fields of 0 keep the session's setting) every one is
+ * evaluated; with num_candidates = 0 a coordinate search runs from the current settings (each
+ * setting in turn over built-in values, the others held at the best so far).  Every evaluation's
+ * per-batch results must equal the first one's, else CSM_ERR_MISMATCH.  On return the graph is as
+ * it was on entry and the session holds the fastest settings; evaluated[k] / ms_per_batch[k] (for
+ * k < cap, either may be NULL) list the evaluations in order, *num_evaluated their number and *best
+ * the index of the fastest. */
+csm_status csm_tune(csm_session* s, uint64_t first, uint64_t batch_edges, uint32_t warmup, uint32_t batches,
+                    const csm_tunables* candidates, uint32_t num_candidates, csm_tunables* evaluated, double* ms_per_batch,
+                    uint32_t cap, uint32_t* num_evaluated, uint32_t* best);
 
 /* Measurement (no counterpart in the reference, whose timing is the TIME_* macros of
  * utils/cuda_helpers.h:78-102).  Kernel families: 0 depth-first enumeration (count), 1 depth-first

@@ -162,6 +162,94 @@ def test_autotune_candidates_agree(lib, tmp_path):
     assert got == want
 
 
+def _run_with_settings(ffi, data, update, batch, queries, tech, settings):
+    """run_library with launch settings changed between batches: settings[b % len] before batch b."""
+    with ffi.Session(device=0) as s:
+        s.load_graph(data)
+        qids = [s.add_query(q) for q in queries]
+        nb = s.open_updates(update, batch)
+        s.build()
+        for q in qids:
+            try:
+                s.plan(q, tech)
+            except ffi.CsmError as e:
+                if "order generation" in str(e):
+                    return None
+                raise
+        out = {}
+        for b in range(nb):
+            s.set_tunables(**settings[b % len(settings)])
+            res, _ = s.apply_batch(b)
+            for k, r in enumerate(res):
+                out[(b, k)] = r
+        return out
+
+
+SETTINGS = [
+    {"waves_per_anchor": 1, "stack_depth": 1, "dfs_blocks": 64, "local_blocks": 16},
+    {"waves_per_anchor": 3, "stack_depth": 2, "dfs_blocks": 4096, "local_blocks": 512, "semi_bfs_threshold": 64},
+    {"waves_per_anchor": 65535, "stack_depth": 16, "dfs_blocks": 2048, "local_blocks": 128, "semi_bfs_threshold": 1 << 20},
+    {"waves_per_anchor": 16, "stack_depth": 4, "dfs_blocks": 333, "local_blocks": 2048, "semi_bfs_threshold": 4096},
+]
+
+
+def test_launch_settings_change_no_result(lib, tmp_path):
+    """csm_set_tunables between batches (waves per anchor, DFS stack depth, DFS and local-index
+    grids, semi-BFS threshold, each at its limits and in between) changes scheduling only: every
+    strategy's results equal the oracle's."""
+    for config, seed, batch in [("tinyins", 7, 64), ("c1", 2, 1000)]:
+        d = tmp_path / f"{config}{seed}"
+        d.mkdir()
+        meta, queries = util.gen(config, d, seed=seed)
+        data, upd = d / "data.graph", d / "update.stream"
+        want, _ = util.oracle_stream(data, upd, batch, queries, threads=8)
+        for (mo, ss, li, cp) in [(2, 0, True, True), (1, 1, False, True), (2, 2, True, True), (2, 3, True, False), (0, 0, True, False)]:
+            for rot in range(len(SETTINGS)):
+                order = SETTINGS[rot:] + SETTINGS[:rot]
+                say(f"case settings {config} seed={seed} batch={batch} order={mo} strategy={ss} local={li} cp={cp} rotation={rot}")
+                got = _run_with_settings(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp), order)
+                if got is None:
+                    break
+                assert got == want, f"{config} order={mo} strategy={ss} rotation={rot}"
+
+
+def test_library_autotuner(lib, tmp_path):
+    """csm_tune on a session: the coordinate search and an explicit candidate list replay the same
+    stream prefix from a snapshot (every evaluation's results must agree, else CSM_ERR_MISMATCH),
+    leave the graph as they found it and the fastest settings in place; the whole stream afterwards
+    equals the oracle's results."""
+    meta, queries = util.gen("c1", tmp_path, seed=2)
+    data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
+    batch = 1000
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=8)
+    for mode in ("coordinate", "candidates"):
+        with lib.Session(device=0) as s:
+            s.load_graph(str(data))
+            qids = [s.add_query(q) for q in queries]
+            nb = s.open_updates(str(upd), batch)
+            s.build()
+            for q in qids:
+                s.plan(q, lib.technique(2, 0, True, True))
+            say(f"case library autotuner c1 seed=2 batch={batch} mode={mode}")
+            if mode == "coordinate":
+                trail, best = s.tune(0, 0, warmup=1, batches=2)
+                assert len(trail) >= 10
+            else:
+                trail, best = s.tune(0, batch, warmup=0, batches=2, candidates=SETTINGS)
+                assert len(trail) == len(SETTINGS)
+                assert trail[0][0]["stack_depth"] == 1 and trail[1][0]["semi_bfs_threshold"] == 64
+            assert all(ms > 0 for _, ms in trail)
+            assert s.tunables() == trail[best][0]
+            assert trail[best][1] == min(ms for _, ms in trail) or mode == "coordinate"
+            for b in range(nb):
+                res, _ = s.apply_batch(b)
+                for k, r in enumerate(res):
+                    assert r == want[(b, k)], f"{mode} batch {b} query {k}"
+            with pytest.raises(lib.CsmError) as e:
+                s.tune(0, batch, warmup=nb, batches=1)  # beyond the stream
+            assert e.value.status == 1
+
+
 def _oracle_summary(data, query):
     out = util.run([util.ORACLE, "summary", str(data), str(query)])
     card, avg = [], []
@@ -730,7 +818,8 @@ def test_snapshot_restore_replays_stream(lib, tmp_path):
 
 @pytest.mark.parametrize("args", [["--algorithm", "1"], ["--algorithm", "2"], ["--algorithm", "4"], ["--algorithm", "5"],
                                   ["--heavy_vertex_avoidance", "false", "--relation_switch", "false"],
-                                  ["--search_strategy", "3", "--cartesian_product", "False", "--local_index", "false"]])
+                                  ["--search_strategy", "3", "--cartesian_product", "False", "--local_index", "false"],
+                                  ["--search_strategy", "0", "--autotune", "true", "--stack_depth", "2", "--waves_per_anchor", "8"]])
 def test_csmg_end_to_end(lib, tmp_path, args):
     """The csmg command line on the GPU: the algorithm presets and individual ablation flags print
     the oracle's total positive count on "Num Positive Matches:" and a timing line the reference's
@@ -747,3 +836,5 @@ def test_csmg_end_to_end(lib, tmp_path, args):
     m = re.search(r"Num Positive Matches: (\d+)", out.stdout)
     assert m and int(m.group(1)) == sum(v[0] for v in want.values()) > 0
     assert re.search(r"Incremental Matching, time \(ms\)\: \d*\(host\), (\d*)\(kernel\)", out.stdout)
+    if "--autotune" in args:
+        assert re.search(r"Autotune candidate \d+: .* \(best\)", out.stdout) and "Launch settings: " in out.stdout

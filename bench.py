@@ -310,13 +310,21 @@ def cpu_baseline(data_dir, queries, batch, seconds):
                       f"{ms / 1e3:.1f}s"}
 
 
-def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_mask=0, label=""):
+def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_mask=0, label="", tune=None):
     """Warm-up batches untimed, then `steps` timed batches (wall clock around csm_apply_batch plus a
     synchronisation), cycling through the stream; when it runs out, the session returns to the
-    initial graph (csm_restore, a device copy) outside the timed region."""
+    initial graph (csm_restore, a device copy) outside the timed region.  With `tune` (a dict), the
+    library's autotuner (csm_tune, coordinate search on batches 0-2, batch 0 untimed) first picks
+    the session's launch settings; its evaluations are stored in the dict."""
     t = time.time()
     s, nb = open_session(ffi, data_dir, queries, batch, **opts)
     s.snapshot()  # the initial graph, restored whenever the stream runs out
+    if tune is not None:
+        t1 = time.time()
+        trail, best = s.tune(0, 0, warmup=1, batches=min(2, nb - 1))  # leaves the initial graph, takes its snapshot
+        tune.update({"chosen": s.tunables(), "seconds": round(time.time() - t1, 1),
+                     "evaluations": [{**st, "ms_per_batch": round(ms, 3)} for st, ms in trail], "best": best})
+        log(f"{label}library autotuner: {len(trail)} evaluations in {time.time() - t1:.1f}s, chose {s.tunables()}")
     for q, path in zip(s.queries, queries):  # the plans, for the CPU baseline's matching orders
         with open(path + ".orders", "w") as f:
             f.write(s.plan_text(q))
@@ -438,8 +446,9 @@ def main():
     ap.add_argument("--serving-load", type=float, default=0.75,
                     help="arrival rate of the serving run as a fraction of the plain path's service rate")
     ap.add_argument("--autotune", action="store_true",
-                    help="pick lanes and the semi-BFS threshold first: one fresh session per candidate on "
-                         "batches 0-2 (batch 0 untimed), results checked equal; then measure with the fastest")
+                    help="pick lanes first (one fresh session per lane count on batches 0-2, batch 0 untimed, "
+                         "results checked equal), then the launch settings with the library's autotuner (csm_tune) "
+                         "in the measuring session; then measure")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the one-lane roofline pass (what the committed kernel profiles cover)")
     args = ap.parse_args()
@@ -475,15 +484,21 @@ def main():
         print(json.dumps({"roofline": r}))
         return 0
 
-    opts, tuned = {}, None
+    opts, tuned, lib_tune = {}, None, None
     if args.autotune:
         at = load_module("autotune")
         t = time.time()
-        best, table = at.tune(lambda o: open_session(ffi, data_dir, queries, batch, **o), warmup=1, batches=2)
-        opts = best.options()
-        tuned = {"chosen": opts, "candidates": table, "batches": "1-2 after warm-up batch 0"}
-        log(f"autotuned in {time.time() - t:.1f}s: {opts}")
-    m = timed_batches(ffi, data_dir, queries, batch, args.steps, args.warmup, opts, args.profile_mask)
+        thr = header_default_threshold()
+        best, table = at.tune(lambda o: open_session(ffi, data_dir, queries, batch, **o),
+                              [at.Candidate(lanes, thr) for lanes in (4, 8, 12)], warmup=1, batches=2)
+        opts = {"lanes": best.lanes}
+        lib_tune = {}
+        tuned = {"lanes": {"chosen": best.lanes, "candidates": table, "batches": "1-2 after warm-up batch 0"},
+                 "launch_settings": lib_tune}
+        log(f"lanes autotuned in {time.time() - t:.1f}s: {opts}")
+    m = timed_batches(ffi, data_dir, queries, batch, args.steps, args.warmup, opts, args.profile_mask, tune=lib_tune)
+    if lib_tune:
+        opts = {**opts, "semi_bfs_threshold": lib_tune["chosen"]["semi_bfs_threshold"]}
     calib = None
     if args.calibrate:
         s, _ = open_session(ffi, data_dir, queries[:1], batch, lanes=1)

@@ -99,9 +99,74 @@ __device__ __forceinline__ uint32_t find_record(const unsigned long long* off, u
     return uniform(lo);
 }
 
+// ---- Range donation (work stealing) of the depth-first kernel --------------------------------
+// Each wave owns one word per launch: the candidate range [cur, end) of its current record's first
+// depth still to be explored, packed as version (16 bits) | cur (24) | end (24), with the record's
+// index in a second array.  The owner takes its next 64-candidate round by moving cur up; a wave
+// out of work takes the upper half of another wave's range by moving that wave's end down to the
+// midpoint.  Both are compare-and-swaps on the one word, so the rounds taken and the halves
+// donated never overlap.  The version changes whenever the owner publishes a new record, so a
+// thief's swap fails if the record index it read belongs to another record.  Ranges only shrink,
+// so every wave's search ends.  (The reference's disabled design: gamma_enumeration.cu:209-357.)
+constexpr uint32_t kRangeBits = 24;
+constexpr uint32_t kRangeMask = (1u << kRangeBits) - 1;
+constexpr uint32_t kStealRounds = 8;  // victim probes of 64 waves each before a wave gives up
+
+__device__ __forceinline__ uint32_t range_cur(unsigned long long w) { return static_cast<uint32_t>(w >> kRangeBits) & kRangeMask; }
+__device__ __forceinline__ uint32_t range_end(unsigned long long w) { return static_cast<uint32_t>(w) & kRangeMask; }
+
+// Owner (one lane): the next round [t, t_end) of its published range, false when it is used up.
+__device__ __forceinline__ bool claim_round(unsigned long long* slot, uint32_t& t, uint32_t& t_end) {
+    unsigned long long w = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const uint32_t cur = range_cur(w), end = range_end(w);
+        if (cur >= end) return false;
+        const uint32_t nc = min(cur + 64u, end);
+        const unsigned long long nw = (w & ~(static_cast<unsigned long long>(kRangeMask) << kRangeBits)) |
+                                      (static_cast<unsigned long long>(nc) << kRangeBits);
+        const unsigned long long seen = atomicCAS(slot, w, nw);
+        if (seen == w) {
+            t = cur;
+            t_end = nc;
+            return true;
+        }
+        w = seen;
+    }
+}
+
+// Thief (whole wave): the upper half of some wave's range with at least two rounds left.
+__device__ __forceinline__ bool steal_range(unsigned long long* donate, const uint32_t* donate_rec, uint32_t total_waves, uint32_t gw,
+                                            uint32_t lane, uint32_t& rec, uint32_t& lo, uint32_t& hi) {
+    for (uint32_t a = 0; a < kStealRounds; a++) {
+        const uint32_t v = (gw + 1 + a * 64 + lane * (1 + a)) % total_waves;
+        const unsigned long long w = v != gw ? __hip_atomic_load(donate + v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        const uint32_t cur = range_cur(w), end = range_end(w);
+        const uint32_t rem = end > cur ? end - cur : 0u;
+        uint32_t key = rem << 6 | lane;  // rem < 2^24: fits
+        for (int o = 32; o > 0; o >>= 1) key = max(key, static_cast<uint32_t>(__shfl_xor(key, o, 64)));
+        if ((key >> 6) < 128u) continue;
+        const uint32_t src = key & 63u;
+        uint32_t got = 0, r = 0, mid = 0;
+        if (lane == src) {
+            mid = cur + (((rem >> 1) + 63u) & ~63u);  // cur + 64 <= mid < end
+            r = __hip_atomic_load(donate_rec + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long nw = (w & ~static_cast<unsigned long long>(kRangeMask)) | mid;
+            got = atomicCAS(donate + v, w, nw) == w ? 1u : 0u;
+        }
+        if (__shfl(got, src, 64)) {
+            rec = __shfl(r, src, 64);
+            lo = __shfl(mid, src, 64);
+            hi = __shfl(end, src, 64);
+            return true;
+        }
+    }
+    return false;
+}
+
 template <int kMode>
 __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, uint32_t d0, uint32_t end_depth, Ring r, MatchOut out,
-                                                         const unsigned long long* split, unsigned long long* work, int schedule) {
+                                                         const unsigned long long* split, unsigned long long* work, int schedule,
+                                                         unsigned long long* donate, uint32_t* donate_rec) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t rows = end_depth > d0 + 1 ? end_depth - 1 - d0 : 0u;
     unsigned char* wave_base = lds_raw + (threadIdx.x >> 6) * wave_lds_bytes(rows);
@@ -114,6 +179,8 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
     unsigned long long my_cnt = 0, my_hash = 0, my_bytes = 0;
 
+    const bool donating = schedule == 2;
+    uint32_t version = 0;
     unsigned long long item;
     if (schedule == 0) item = gw;
     else {
@@ -121,13 +188,22 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
         if (lane == 0) t = atomicAdd(work, 1ull);
         item = __shfl(t, 0, 64);
     }
-    while (item < nitems) {
-        if (kMode == kWriteAtEnd && uniform(__hip_atomic_load(r.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
-        uint32_t rec_idx = static_cast<uint32_t>(item), chunk = 0;
-        if (split) {
-            rec_idx = find_record(split, nrec, item, lane);
-            chunk = static_cast<uint32_t>(item - split[rec_idx]);
+    for (;;) {
+        // the next item of the launch, else (donating) a range taken from another wave
+        uint32_t rec_idx, chunk = 0, s_lo = 0, s_hi = 0;
+        bool stolen = false;
+        if (item < nitems) {
+            rec_idx = static_cast<uint32_t>(item);
+            if (split) {
+                rec_idx = find_record(split, nrec, item, lane);
+                chunk = static_cast<uint32_t>(item - split[rec_idx]);
+            }
+        } else if (donating && steal_range(donate, donate_rec, total_waves, gw, lane, rec_idx, s_lo, s_hi)) {
+            stolen = true;
+        } else {
+            break;
         }
+        if (kMode == kWriteAtEnd && uniform(__hip_atomic_load(r.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
         if (lane < d0) W.f[lane] = r.in[static_cast<uint64_t>(rec_idx) * d0 + lane];
         wave_sync();
         if (lane == 0) {
@@ -137,13 +213,38 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
         }
         wave_sync();
         uint32_t depth = d0;
-        setup_frame(g, d, W, depth, lane, my_bytes, chunk == 0);
-        if (split) {
+        setup_frame(g, d, W, depth, lane, my_bytes, !stolen && chunk == 0);
+        if (split && !stolen) {
             if (lane == 0) {
                 const uint32_t k = static_cast<uint32_t>(split[rec_idx + 1] - split[rec_idx]);  // chunks of the record
                 const uint32_t width = ((W.tot[d0] + k - 1) / k + 63u) & ~63u;
                 W.cur[d0] = chunk * width;
                 W.tot[d0] = min(W.tot[d0], chunk * width + width);
+            }
+            wave_sync();
+        }
+        // donating: publish the item's first-depth range and take its first round from the word
+        // (a range past 2^24 candidates stays the wave's own)
+        bool published = false;
+        if (donating) {
+            uint32_t lo = stolen ? s_lo : W.cur[d0], hi = stolen ? s_hi : W.tot[d0];
+            published = hi <= kRangeMask;
+            if (published) {
+                version = (version + 1) & 0xFFFFu;
+                uint32_t t = 0, t_end = 0, ok = 0;
+                if (lane == 0) {
+                    __hip_atomic_store(donate_rec + gw, rec_idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(donate + gw,
+                                       static_cast<unsigned long long>(version) << (2 * kRangeBits) |
+                                           static_cast<unsigned long long>(lo) << kRangeBits | hi,
+                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = claim_round(donate + gw, t, t_end) ? 1u : 0u;
+                    W.cur[d0] = ok ? t : 0u;
+                    W.tot[d0] = ok ? t_end : 0u;
+                }
+            } else if (stolen && lane == 0) {
+                W.cur[d0] = lo;
+                W.tot[d0] = hi;
             }
             wave_sync();
         }
@@ -162,7 +263,21 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
                 continue;
             }
             if (W.cur[depth] >= W.tot[depth]) {
-                if (depth == d0) break;
+                if (depth == d0) {
+                    if (!published) break;
+                    uint32_t ok = 0;
+                    if (lane == 0) {
+                        uint32_t t = 0, t_end = 0;
+                        ok = claim_round(donate + gw, t, t_end) ? 1u : 0u;
+                        if (ok) {
+                            W.cur[d0] = t;
+                            W.tot[d0] = t_end;
+                        }
+                    }
+                    if (!__shfl(ok, 0, 64)) break;
+                    wave_sync();
+                    continue;
+                }
                 depth -= 1;
                 continue;
             }
@@ -200,6 +315,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_extend(DevGraph g, StepDesc d, 
             }
             wave_sync();
         }
+        if (stolen) continue;  // the launch's items are used up: steal again
         if (schedule == 0) item += total_waves;
         else {
             unsigned long long t2 = 0;
@@ -775,6 +891,45 @@ __global__ void __launch_bounds__(256) k_level_flat_chain(DevGraph g, StepDesc d
     level_flat_body<kMode>(g, d, d0, c.ring + o, n, off, prun, runs, piv, ph, r, out);
 }
 
+// Level k of many steps' chains (grid.y = job): the kernels above with the job's descriptor, chain
+// and scratch read from the job table (uniform per block: scalar loads).
+__global__ void __launch_bounds__(kLenThreads) k_level_len_chain_multi(DevGraph g, const ChainJob* jobs, uint32_t k) {
+    const ChainJob& J = jobs[blockIdx.y];
+    if (k > J.L) return;
+    if (k == 0 && blockIdx.x == 0 && threadIdx.x < 2 * kMaxQV) J.c.ctr[threadIdx.x] = 0;
+    uint64_t off;
+    uint32_t n;
+    if (!chain_at(J.c, k, off, n)) return;
+    const bool counting = k == J.L;
+    level_len_tiles(g, J.d, J.c.depth0 + k, J.c.ring + off, n, J.ls.off, J.ls.prun, J.ls.runs, J.ls.piv, counting ? J.ls.ph : nullptr,
+                    J.ls.tile_sum, counting ? J.out.bytes : J.write_bytes);
+}
+
+__global__ void __launch_bounds__(kLenThreads) k_level_scan_chain_multi(const ChainJob* jobs, uint32_t k) {
+    const ChainJob& J = jobs[blockIdx.y];
+    if (k > J.L) return;
+    uint64_t o;
+    uint32_t n;
+    if (!chain_at(J.c, k, o, n)) return;
+    level_scan_tiles(J.ls.off, n, J.ls.tile_sum);
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(256) k_level_flat_chain_multi(DevGraph g, const ChainJob* jobs, uint32_t k) {
+    const ChainJob& J = jobs[blockIdx.y];
+    if (k > J.L || (k == J.L) != (kMode == kCountAtEnd)) return;
+    uint64_t o;
+    uint32_t n;
+    if (!chain_at(J.c, k, o, n)) return;
+    const uint32_t d0 = J.c.depth0 + k;
+    const uint64_t out_off = o + static_cast<uint64_t>(n) * d0;
+    const uint64_t room = out_off < J.c.ring_words ? (J.c.ring_words - out_off) / (d0 + 1) : 0ull;
+    const Ring r{J.c.ring + o, nullptr, n, J.c.ring + out_off, J.c.ctr + 2 * k, static_cast<uint32_t>(min(room, 0xFFFFFFFFull)),
+                 J.c.ctr + 2 * k + 1};
+    const MatchOut out = kMode == kCountAtEnd ? J.out : MatchOut{nullptr, nullptr, J.write_bytes};
+    level_flat_body<kMode>(g, J.d, d0, J.c.ring + o, n, J.ls.off, J.ls.prun, J.ls.runs, J.ls.piv, J.ls.ph, r, out);
+}
+
 __global__ void __launch_bounds__(256) k_root_chunks(DevGraph g, StepDesc d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,
                                                      uint32_t max_rec, uint32_t max_chunks, unsigned long long* chunks) {
     const uint32_t n = min(*n_rec, max_rec);
@@ -1058,16 +1213,19 @@ void bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_
 }
 
 void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r, MatchOut out,
-            const unsigned long long* split, unsigned long long* work_counter, int schedule, int blocks, hipStream_t s) {
+            const unsigned long long* split, unsigned long long* work_counter, int schedule, int blocks, Donation dn, hipStream_t s) {
     // Persistent grid: `blocks` (2048 by default: enough waves to fill 256 CUs several times over),
-    // never more than records (a split launch has at least as many items as records with candidates).
-    const int grid = split ? blocks : grid_for(r.in_max, kWaves, blocks);
+    // never more than records (a split launch has at least as many items as records with candidates;
+    // a donating launch keeps the full grid, its waves share the heavy records' ranges).
+    blocks = std::min<int>(blocks, static_cast<int>(kMaxDfsBlocks));
+    const int grid = split || schedule == 2 ? blocks : grid_for(r.in_max, kWaves, blocks);
     const uint32_t rows = end_depth > d0 + 1 ? end_depth - 1 - d0 : 0u;
     const size_t lds = static_cast<size_t>(kWaves) * wave_lds_bytes(rows);
+    if (schedule == 2) fill_u32(reinterpret_cast<uint32_t*>(dn.range), 0u, 2ull * grid * kWaves, s);  // every word: nothing to take
     if (mode == kCountAtEnd)
-        k_extend<kCountAtEnd><<<grid, kWaves * 64, lds, s>>>(g, d, d0, end_depth, r, out, split, work_counter, schedule);
+        k_extend<kCountAtEnd><<<grid, kWaves * 64, lds, s>>>(g, d, d0, end_depth, r, out, split, work_counter, schedule, dn.range, dn.rec);
     else
-        k_extend<kWriteAtEnd><<<grid, kWaves * 64, lds, s>>>(g, d, d0, end_depth, r, out, split, work_counter, schedule);
+        k_extend<kWriteAtEnd><<<grid, kWaves * 64, lds, s>>>(g, d, d0, end_depth, r, out, split, work_counter, schedule, dn.range, dn.rec);
 }
 
 void level_prepare(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, uint32_t n, const LevelScratch& ls,
@@ -1111,6 +1269,18 @@ void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c,
         k_level_flat_chain<kCountAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
     else
         k_level_flat_chain<kWriteAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
+}
+
+void level_prepare_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, hipStream_t s) {
+    if (!njobs) return;
+    k_level_len_chain_multi<<<dim3(kChainLenGrid, njobs), kLenThreads, 0, s>>>(g, jobs, k);
+    k_level_scan_chain_multi<<<dim3(kChainLenGrid, njobs), kLenThreads, 0, s>>>(jobs, k);
+}
+
+void level_flat_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int mode, hipStream_t s) {
+    if (!njobs) return;
+    if (mode == kCountAtEnd) k_level_flat_chain_multi<kCountAtEnd><<<dim3(kChainFlatGrid, njobs), 256, 0, s>>>(g, jobs, k);
+    else k_level_flat_chain_multi<kWriteAtEnd><<<dim3(kChainFlatGrid, njobs), 256, 0, s>>>(g, jobs, k);
 }
 
 uint32_t level_max_records(const StepDesc& d, uint32_t d0, const LevelScratch& ls) {

@@ -215,15 +215,24 @@ void count_records(const StepDesc& d, const uint32_t* rec, const uint32_t* n_rec
 void bitmap_to_list(const uint32_t* bitmap, uint32_t dv, uint32_t* list, uint32_t* len, hipStream_t s);
 
 enum ExtendMode : int { kCountAtEnd = 0, kWriteAtEnd = 1 };
+// Range donation of the depth-first kernel (schedule 2): one range word and one record index per
+// wave of the launch (kMaxDfsBlocks * 4 waves at most); extend clears the words before the launch.
+constexpr uint32_t kMaxDfsBlocks = 16384;
+struct Donation {
+    unsigned long long* range;  // [kMaxDfsBlocks * 4]
+    uint32_t* rec;              // [kMaxDfsBlocks * 4]
+};
 // DFS from the records at depth d0 down to end_depth: counts/hashes complete embeddings when
 // end_depth == n, or writes the partial results of depth end_depth to the ring.
 // schedule: 0 static (item t by wave t mod W), 1 dynamic (global work counter, zeroed by the
-// caller).  Without `split` an item is one record.  With `split` (the exclusive scan of
+// caller), 2 dynamic with range donation: a wave out of items takes the upper half of the
+// first-depth candidate range another wave is still exploring (dn).  Without `split` an item is one record.  With `split` (the exclusive scan of
 // root_chunks, total at [n]) an item is one of a record's k chunks: its depth-d0 candidates cut
 // into k equal pieces of whole 64-candidate rounds, so a heavy record spreads over k waves.
 // blocks: the persistent grid (fewer when there are fewer items).
 void extend(const DevGraph& g, const StepDesc& d, uint32_t d0, uint32_t end_depth, int mode, Ring r,
-            MatchOut out, const unsigned long long* split, unsigned long long* work_counter, int schedule, int blocks, hipStream_t s);
+            MatchOut out, const unsigned long long* split, unsigned long long* work_counter, int schedule, int blocks, Donation dn,
+            hipStream_t s);
 // One level from the n records of depth d0 at rec (n known on the host), flattened over all its
 // candidates: level_prepare writes, per record, the view runs of the depth's backward entries, the
 // entry with the shortest view sequence (it generates the candidates; the others are probed), the
@@ -271,6 +280,23 @@ void level_prepare_chain(const DevGraph& g, const StepDesc& d, const LevelChain&
                          unsigned long long* bytes, hipStream_t s);
 void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c, uint32_t k, int mode, const LevelScratch& ls,
                       MatchOut out, hipStream_t s);
+// The level chains of several steps run together: level k of every job in one launch per kernel
+// (grid.y = job), each job with its own descriptor, ring, counters and level scratch.  Job j runs
+// levels 0 .. L (writing levels 0 .. L - 1, counting level L); at launch k it does nothing when
+// k > L or when chain_at says its chain stopped.
+struct ChainJob {
+    StepDesc d;
+    LevelChain c;
+    LevelScratch ls;
+    MatchOut out;                    // the counting level's count, hash and bytes
+    unsigned long long* write_bytes; // the writing levels' algorithmic bytes (nullable)
+    uint32_t L;
+    uint32_t pad;
+};
+// Level k of the njobs jobs at `jobs` (device memory): preparation (run lengths and scan), then
+// the flattened candidate kernel of the jobs that write at k (mode kWriteAtEnd) or count at k.
+void level_prepare_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, hipStream_t s);
+void level_flat_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int mode, hipStream_t s);
 // chunks[i] = ceil(|pivot view run of record i at depth d0| / 64), then scanned in place (total at [n]).
 // Per record, its chunk count k = min(max_chunks, ceil(depth-d0 candidates / 64)), scanned.
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,

@@ -64,6 +64,7 @@ Engine::~Engine() {
     if (trace_) std::fclose(trace_);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
     if (h_jobs_) (void)hipHostFree(h_jobs_);
+    if (h_chain_jobs_) (void)hipHostFree(h_chain_jobs_);
     if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -80,6 +81,7 @@ std::string Engine::init(const EngineOptions& o) {
     CSM_HIP(static_ring(o.device, &static_ring_));
     if (const char* t = std::getenv("CSM_TRACE")) trace_ = std::fopen(t, "a");
     if (const char* t = std::getenv("CSM_LANES")) opt_.lanes = static_cast<uint32_t>(std::max(1, std::atoi(t)));
+    if (const char* t = std::getenv("CSM_TEAM")) opt_.team = static_cast<uint32_t>(std::max(1, std::atoi(t)));
     // A/B switches of the scheduling and layout aids (all on by default)
     if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) if (std::atoi(t) == 0) opt_.waves_per_anchor = 1;
     if (const char* t = std::getenv("CSM_SLOT_DIRECTORY")) opt_.slot_directory = std::atoi(t) != 0;
@@ -242,6 +244,8 @@ std::string Engine::build() {
         add((cp_rec / kLevelTile + 2) * 8);                        // level tile totals
         add(cp_list_words * 4); add(64);
         add(64 << 10);                                             // alignment slack of the takes
+        add(sizeof(ChainJob) + 256);                               // the lane's level-chain job
+        add(kMaxDfsBlocks * 4ull * 12 + 512);                      // range donation words
     }
     // local-index slots: two groups of group_ steps (each: its sets, anchor pairs, queue, counters)
     uint32_t max_qn = 2;
@@ -256,6 +260,7 @@ std::string Engine::build() {
         for (auto& qs : qs_) nsteps += qs->q.m;
         uint64_t want = nsteps >= kGroupMinSteps ? kMaxGroup : 0;
         if (const char* t = std::getenv("CSM_GROUP_STEPS")) want = static_cast<uint64_t>(std::max(0L, std::atol(t)));
+        if (opt_.team > 1) want = 0;  // teams build their steps' local indexes themselves
         uint64_t budget = kGroupBudgetBytes;
         if (opt_.pool_bytes) budget = std::min<uint64_t>(budget, opt_.pool_bytes / 8);
         group_ = static_cast<uint32_t>(std::min<uint64_t>(want, std::min<uint64_t>(kMaxGroup, budget / (2 * slot_bytes))));
@@ -415,6 +420,7 @@ std::string Engine::build() {
 
     // ---- runtime buffers: one set per lane ----
     lanes_.assign(n_lanes, Lane{});
+    team_ = std::max<uint32_t>(1, std::min<uint32_t>(opt_.team, n_lanes));
     for (Lane& w : lanes_) {
         CSM_HIP(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking));
         CSM_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
@@ -452,6 +458,9 @@ std::string Engine::build() {
         uint4* level_runs = pool_.take<uint4>(kLevelRunsPerRecord * cp_rec);
         unsigned long long* tile_sum = pool_.take<unsigned long long>(cp_rec / kLevelTile + 2);
         w.chain_ctr = pool_.take<uint32_t>(2 * kMaxQV);
+        w.dn.range = pool_.take<unsigned long long>(kMaxDfsBlocks * 4ull);
+        w.dn.rec = pool_.take<uint32_t>(kMaxDfsBlocks * 4ull);
+        if (!w.dn.range || !w.dn.rec) return tagged(Code::kMemory, "device pool exhausted (lane scratch)");
         if (!level_runs || !tile_sum || !w.chain_ctr) return tagged(Code::kMemory, "device pool exhausted (lane scratch)");
         w.level = LevelScratch{w.cp.prod, w.cp.seg_run, level_runs, reinterpret_cast<uint8_t*>(w.cp.seg_cnt), w.cp.rec_hash,
                                tile_sum, cp_rec, kLevelRunsPerRecord * cp_rec};
@@ -487,6 +496,9 @@ std::string Engine::build() {
                                 2ull * kMaxGroup * kMaxQV * sizeof(VSet) + 2ull * kMaxGroup * kSlotWords * 4 + 4096;
         CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_jobs_), hbytes, hipHostMallocDefault));
     }
+    d_chain_jobs_ = pool_.take<ChainJob>(lanes_.size());
+    if (!d_chain_jobs_) return tagged(Code::kMemory, "device pool exhausted (level-chain jobs)");
+    CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_chain_jobs_), lanes_.size() * sizeof(ChainJob), hipHostMallocDefault));
     results_ = pool_.take<unsigned long long>(qs_.size() * 4 + 4);
     prof_bytes_ = pool_.take<unsigned long long>(kKFamilies);
     CSM_HIP(hipMemsetAsync(prof_bytes_, 0, kKFamilies * 8, s_));
@@ -655,16 +667,13 @@ uint32_t Engine::view_of(const QState& qs, uint32_t rel, uint32_t i, uint32_t ph
     return kViewBase;
 }
 
-std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
+// First stage of a step: clear the lane's local sets of its previous step, zero their lengths and
+// the step counters, and enqueue the anchor pairs of query edge i (their count at *w.n_pairs).
+void Engine::step_anchors(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
     const Query& q = qs.q;
-    w.sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
-    const Plan& p = qs.plan;
     const uint32_t u0 = q.qe_list[i].first, u1 = q.qe_list[i].second;
-    const uint32_t r01 = q.rel(u0, u1);
     const uint32_t max_arcs = 2 * max_batch_;
     auto cand = [&](uint32_t u) { return qs.cand + static_cast<uint64_t>(u) * words_; };
-
-    // clear the local sets of the previous step, then zero their lengths and the step counters
     if (w.L_used) {
         VSetTable t;
         for (uint32_t u = 0; u < kMaxQV; u++) t.s[u] = w.L[u];
@@ -672,62 +681,76 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
     }
     w.L_used = 0;
     fill_u32(w.L[0].len, 0u, kLaneWords, w.s);
-
-    const RelInfo& R = qs.rel[r01];
+    const RelInfo& R = qs.rel[q.rel(u0, u1)];
     prof_begin(w.prof, w.s, kKAnchors);
     local_anchors(g_, ph, R.dir, R.slot, qs.vrank[u0], cand(u0), cand(u1), bs_[R.dir].by_label, bs_[R.dir].label_off, max_arcs, w.pairs,
                   w.n_pairs, max_arcs, w.L[u0], w.L[u1], w.s);
     prof_end(w.prof, w.s, kKAnchors);
     w.L_used |= static_cast<uint16_t>((1u << u0) | (1u << u1));
+}
+
+// Second stage (local index on): enqueue the local sets of the step's indexing order; returns the
+// query vertices whose set is built (their lengths at w.L[u].len).
+uint32_t Engine::step_local(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
+    const Query& q = qs.q;
+    const Plan& p = qs.plan;
+    const uint32_t u0 = q.qe_list[i].first, u1 = q.qe_list[i].second;
+    auto cand = [&](uint32_t u) { return qs.cand + static_cast<uint64_t>(u) * words_; };
+    uint32_t built = (1u << u0) | (1u << u1);
+    if (!qs.tech.local_index) return built;
+    const OrderPerEdge& io = p.indexing[i];
+    for (uint32_t pos = 2; pos < q.n; pos++) {
+        const uint32_t u = io.vs[pos];
+        if (!(p.rebuild_B[i] >> u & 1u)) continue;
+        std::vector<uint32_t> bsv;
+        for (uint32_t k = io.bni_offs[pos]; k < io.bni_offs[pos + 1]; k++) {
+            const uint32_t b = io.vs[io.bni[k]];
+            if (built >> b & 1u) bsv.push_back(b);
+        }
+        if (bsv.empty()) continue;
+        size_t first = 0;
+        if (qs.tech.heavy_vertex_avoidance) {
+            for (size_t k = 1; k < bsv.size(); k++)
+                if (qs.avg[q.rel(bsv[k], u)] < qs.avg[q.rel(bsv[first], u)]) first = k;
+        }
+        const uint32_t b0 = bsv[first];
+        const uint32_t rb = q.rel(b0, u);
+        prof_begin(w.prof, w.s, kKLocalIndex);
+        if (bsv.size() == 1) {
+            local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u],
+                         local_grid_, bytes_ptr(kKLocalIndex), w.s);
+            prof_end(w.prof, w.s, kKLocalIndex);
+        } else {
+            // semi-joins with the other built neighbours, fused into the expansion
+            LocalFilters fl{};
+            for (size_t k = 0; k < bsv.size(); k++) {
+                if (k == first) continue;
+                const uint32_t rub = q.rel(u, bsv[k]);
+                fl.f[fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[bsv[k]].bitmap};
+            }
+            uint32_t* queue_len = w.L[0].len + kQueueCounters + u;
+            local_queue(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u],
+                        w.tmp_list, queue_len, local_grid_, bytes_ptr(kKLocalIndex), w.s);
+            prof_end(w.prof, w.s, kKLocalIndex);
+            prof_begin(w.prof, w.s, kKLocalCheck);
+            local_check(g_, ph, fl, w.L[u], w.tmp_list, queue_len, local_grid_, bytes_ptr(kKLocalCheck), w.s);
+            prof_end(w.prof, w.s, kKLocalCheck);
+        }
+        w.L_used |= static_cast<uint16_t>(1u << u);
+        built |= 1u << u;
+    }
+    return built;
+}
+
+std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
+    step_anchors(w, qs, i, ph);
     const uint32_t np = read_u32(w.s, w.h_pinned, w.n_pairs);
     if (np == 0) return "empty";
-    uint32_t built = (1u << u0) | (1u << u1);
-
+    const uint32_t built = step_local(w, qs, i, ph);
     if (qs.tech.local_index) {
-        const OrderPerEdge& io = p.indexing[i];
-        for (uint32_t pos = 2; pos < q.n; pos++) {
-            const uint32_t u = io.vs[pos];
-            if (!(p.rebuild_B[i] >> u & 1u)) continue;
-            std::vector<uint32_t> bsv;
-            for (uint32_t k = io.bni_offs[pos]; k < io.bni_offs[pos + 1]; k++) {
-                const uint32_t b = io.vs[io.bni[k]];
-                if (built >> b & 1u) bsv.push_back(b);
-            }
-            if (bsv.empty()) continue;
-            size_t first = 0;
-            if (qs.tech.heavy_vertex_avoidance) {
-                for (size_t k = 1; k < bsv.size(); k++)
-                    if (qs.avg[q.rel(bsv[k], u)] < qs.avg[q.rel(bsv[first], u)]) first = k;
-            }
-            const uint32_t b0 = bsv[first];
-            const uint32_t rb = q.rel(b0, u);
-            prof_begin(w.prof, w.s, kKLocalIndex);
-            if (bsv.size() == 1) {
-                local_expand(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u), w.L[u],
-                             local_grid_, bytes_ptr(kKLocalIndex), w.s);
-                prof_end(w.prof, w.s, kKLocalIndex);
-            } else {
-                // semi-joins with the other built neighbours, fused into the expansion
-                LocalFilters fl{};
-                for (size_t k = 0; k < bsv.size(); k++) {
-                    if (k == first) continue;
-                    const uint32_t rub = q.rel(u, bsv[k]);
-                    fl.f[fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[bsv[k]].bitmap};
-                }
-                uint32_t* queue_len = w.L[0].len + kQueueCounters + u;
-                local_queue(g_, ph, view_of(qs, rb, i, ph), qs.rel[rb].dir, qs.rel[rb].slot, w.L[b0].list, w.L[b0].len, dv_, cand(u),
-                            w.L[u], w.tmp_list, queue_len, local_grid_, bytes_ptr(kKLocalIndex), w.s);
-                prof_end(w.prof, w.s, kKLocalIndex);
-                prof_begin(w.prof, w.s, kKLocalCheck);
-                local_check(g_, ph, fl, w.L[u], w.tmp_list, queue_len, local_grid_, bytes_ptr(kKLocalCheck), w.s);
-                prof_end(w.prof, w.s, kKLocalCheck);
-            }
-            w.L_used |= static_cast<uint16_t>(1u << u);
-            built |= 1u << u;
-        }
         CSM_HIP(hipMemcpyAsync(w.h_pinned, w.L[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, w.s));
         CSM_HIP(hipStreamSynchronize(w.s));
-        for (uint32_t u = 0; u < q.n; u++)
+        for (uint32_t u = 0; u < qs.q.n; u++)
             if ((built >> u & 1u) && w.h_pinned[u] == 0) return "empty";
     }
     return enumerate_step(w, qs, i, ph, w.L, built, w.pairs, w.n_pairs, np);
@@ -737,8 +760,34 @@ std::string Engine::run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph) {
 // the np anchor pairs of the step's query edge in pairs / *n_pairs.
 std::string Engine::enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
                                    const uint32_t* n_pairs, uint32_t np) {
+    StepDesc d{};
+    bool pending = false;
+    uint32_t count = 0;
+    std::string err = enumerate_begin(w, qs, i, ph, L, built, pairs, n_pairs, np, d, pending, count);
+    if (!err.empty()) return err;
+    if (pending) {
+        CSM_HIP(hipStreamSynchronize(w.s));
+        err = initial_count(w, count);
+        if (!err.empty()) return err;
+    }
+    return enumerate_rest(w, qs, i, ph, d, count);
+}
+
+// The initial partial results of a step started by init_from_set: their count (or the overflow)
+// from the lane's pinned buffer, once the stream has passed the copy enumerate_begin enqueued.
+std::string Engine::initial_count(Lane& w, uint32_t& count) {
+    if (w.h_pinned[2]) return tagged(Code::kMemory, "partial-result ring too small for the initial partial results");
+    count = w.h_pinned[0];
+    return "";
+}
+
+// Step descriptor and initial partial results (depth 2, at the bottom of the lane's ring).  With
+// the anchor pairs as the first two vertices their count is np; otherwise init_from_set runs and
+// `pending` says that its count arrives in the lane's pinned buffer (initial_count, after a wait).
+std::string Engine::enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
+                                    const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count) {
     const Query& q = qs.q;
-    w.sched = qs.tech.strategy == SearchStrategy::kDfsWithoutWorkStealing ? 0 : 1;
+    w.sched = sched_of(qs.tech.strategy);
     const Plan& p = qs.plan;
     const uint32_t u0 = q.qe_list[i].first, u1 = q.qe_list[i].second;
     const uint32_t max_arcs = 2 * max_batch_;
@@ -747,7 +796,9 @@ std::string Engine::enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph,
 
     // ---- step descriptor ----
     const OrderPerEdge& o = p.orders[i];
-    StepDesc d{};
+    d = StepDesc{};
+    pending = false;
+    count = 0;
     d.n = q.n;
     d.phase = ph;
     uint32_t ne = 0;
@@ -783,7 +834,6 @@ std::string Engine::enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph,
     // ---- initial partial results (depth 2) at the bottom of the lane's ring ----
     uint32_t* ring = w.ring;
     const uint32_t a = o.vs[0], b = o.vs[1];
-    uint32_t count;
     if ((a == u0 && b == u1) || (a == u1 && b == u0)) {
         if (2ull * np > w.ring_words) return tagged(Code::kMemory, "partial-result ring too small for the batch's anchors");
         init_from_pairs(pairs, n_pairs, max_arcs, a == u1, ring, w.counters, w.s);
@@ -803,13 +853,31 @@ std::string Engine::enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph,
                w.counters + 2};
         init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, w.s);
         CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
-        CSM_HIP(hipStreamSynchronize(w.s));
-        if (w.h_pinned[2]) return tagged(Code::kMemory, "partial-result ring too small for the initial partial results");
-        count = w.h_pinned[0];
+        pending = true;
     }
-    if (trace_) w.trace_line += ", \"anchors\": " + std::to_string(np) + ", \"initial\": " + std::to_string(count);
+    if (trace_) w.trace_line += ", \"anchors\": " + std::to_string(np);
+    return "";
+}
+
+// The depth-first kernel's schedule of a strategy: static for DFS without work stealing (match_gpu.cu
+// 1721-1914 with a fixed partition), range donation for DFS with work stealing, dynamic otherwise.
+int Engine::sched_of(SearchStrategy st) {
+    return st == SearchStrategy::kDfsWithoutWorkStealing ? 0 : st == SearchStrategy::kDfsWithWorkStealing ? 2 : 1;
+}
+
+uint32_t Engine::slot_of(const QState& qs) const {
+    return static_cast<uint32_t>(std::find_if(qs_.begin(), qs_.end(), [&](const std::unique_ptr<QState>& x) { return x.get() == &qs; }) -
+                                 qs_.begin());
+}
+
+// The search of a step from its `count` initial partial results (enumerate_begin).
+std::string Engine::enumerate_rest(Lane& w, QState& qs, uint32_t i, uint32_t ph, const StepDesc& d, uint32_t count) {
+    const Query& q = qs.q;
+    const Plan& p = qs.plan;
+    uint32_t* ring = w.ring;
+    if (trace_) w.trace_line += ", \"initial\": " + std::to_string(count);
     if (count == 0) return "";
-    const uint32_t res_slot = static_cast<uint32_t>(std::find_if(qs_.begin(), qs_.end(), [&](const std::unique_ptr<QState>& x) { return x.get() == &qs; }) - qs_.begin());
+    const uint32_t res_slot = slot_of(qs);
     if (q.n == 2) {
         count_records(d, ring, w.counters, count, MatchOut{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, nullptr}, w.s);
         return "";
@@ -883,7 +951,7 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
         Ring r{rec, w.counters, count, out, w.counters + 1, static_cast<uint32_t>(cap), w.counters + 2};
         prof_begin(w.prof, w.s, kKExtendWrite);
         extend(g_, d, depth, end, kWriteAtEnd, r, MatchOut{nullptr, nullptr, bytes_ptr(kKExtendWrite)}, nullptr, w.work_counter(), w.sched,
-               static_cast<int>(opt_.dfs_blocks), w.s);
+               static_cast<int>(opt_.dfs_blocks), w.dn, w.s);
         prof_end(w.prof, w.s, kKExtendWrite);
         CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
         CSM_HIP(hipStreamSynchronize(w.s));
@@ -902,9 +970,34 @@ std::string Engine::extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, u
 // semi-BFS threshold, a Cartesian-product leaf, a ring overflow); at the first level that does,
 // the chain stops and `search` goes on from that level's records.
 std::string Engine::search_chained(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot) {
-    const uint32_t depth0 = 2, L = d.n - 1 - depth0;  // writing levels; level L counts the leaf
-    const uint8_t* cpi = qs.plan.cp_info[i];
     LevelChain c{};
+    const uint32_t L = chain_build(w, qs, d, i, count, c);
+    for (uint32_t k = 0; k < L; k++) {
+        prof_begin(w.prof, w.s, kKLevelPrepare);
+        level_prepare_chain(g_, d, c, k, w.level, false, bytes_ptr(kKLevelWrite), w.s);
+        prof_end(w.prof, w.s, kKLevelPrepare);
+        prof_begin(w.prof, w.s, kKLevelWrite);
+        level_flat_chain(g_, d, c, k, kWriteAtEnd, w.level, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
+        prof_end(w.prof, w.s, kKLevelWrite);
+    }
+    const MatchOut out = chain_out(d, res_slot);
+    prof_begin(w.prof, w.s, kKLevelPrepare);
+    level_prepare_chain(g_, d, c, L, w.level, true, out.bytes, w.s);
+    prof_end(w.prof, w.s, kKLevelPrepare);
+    prof_begin(w.prof, w.s, kKLevelCount);
+    level_flat_chain(g_, d, c, L, kCountAtEnd, w.level, out, w.s);
+    prof_end(w.prof, w.s, kKLevelCount);
+    if (L) CSM_HIP(hipMemcpyAsync(w.h_pinned, w.chain_ctr, 8ull * L, hipMemcpyDeviceToHost, w.s));
+    CSM_HIP(hipStreamSynchronize(w.s));
+    return chain_finish(w, qs, d, i, count, res_slot, c, L);
+}
+
+// The chain of a step from its `count` initial records (depth 2 at the bottom of the lane's ring):
+// writing levels 0 .. L - 1, counting level L.
+uint32_t Engine::chain_build(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, LevelChain& c) {
+    const uint32_t depth0 = 2, L = d.n - 1 - depth0;
+    const uint8_t* cpi = qs.plan.cp_info[i];
+    c = LevelChain{};
     c.ring = w.ring;
     c.ring_words = w.ring_words;
     c.ctr = w.chain_ctr;
@@ -917,25 +1010,19 @@ std::string Engine::search_chained(Lane& w, QState& qs, const StepDesc& d, uint3
         c.cap[k] = level_max_records(d, depth, w.level);
         c.stop_at[k] = !cp_depth ? 0xFFFFFFFFu : k < L ? opt_.semi_bfs_threshold : 0u;
     }
-    for (uint32_t k = 0; k < L; k++) {
-        prof_begin(w.prof, w.s, kKLevelPrepare);
-        level_prepare_chain(g_, d, c, k, w.level, false, bytes_ptr(kKLevelWrite), w.s);
-        prof_end(w.prof, w.s, kKLevelPrepare);
-        prof_begin(w.prof, w.s, kKLevelWrite);
-        level_flat_chain(g_, d, c, k, kWriteAtEnd, w.level, MatchOut{nullptr, nullptr, bytes_ptr(kKLevelWrite)}, w.s);
-        prof_end(w.prof, w.s, kKLevelWrite);
-    }
-    const uint32_t ph = d.phase;
-    const MatchOut out{results_ + 4 * res_slot + 2 * ph, results_ + 4 * res_slot + 2 * ph + 1, bytes_ptr(kKLevelCount)};
-    prof_begin(w.prof, w.s, kKLevelPrepare);
-    level_prepare_chain(g_, d, c, L, w.level, true, out.bytes, w.s);
-    prof_end(w.prof, w.s, kKLevelPrepare);
-    prof_begin(w.prof, w.s, kKLevelCount);
-    level_flat_chain(g_, d, c, L, kCountAtEnd, w.level, out, w.s);
-    prof_end(w.prof, w.s, kKLevelCount);
-    if (L) CSM_HIP(hipMemcpyAsync(w.h_pinned, w.chain_ctr, 8ull * L, hipMemcpyDeviceToHost, w.s));
-    CSM_HIP(hipStreamSynchronize(w.s));
-    // replay the chain's decisions (chain_at) to find where it stopped
+    return L;
+}
+
+// The counting level's outputs: the step's query's count and hash of this phase.
+MatchOut Engine::chain_out(const StepDesc& d, uint32_t res_slot) {
+    return MatchOut{results_ + 4 * res_slot + 2 * d.phase, results_ + 4 * res_slot + 2 * d.phase + 1, bytes_ptr(kKLevelCount)};
+}
+
+// After the chain ran and its per-level {count, overflow} words reached w.h_pinned: replay the
+// chain's decisions (chain_at) to find where it stopped, and go on from there with `search`.
+std::string Engine::chain_finish(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot, const LevelChain& c,
+                                 uint32_t L) {
+    const uint32_t depth0 = c.depth0;
     uint64_t off = 0;
     uint32_t n = count;
     for (uint32_t k = 0;; k++) {
@@ -1021,7 +1108,7 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
     }
     Ring r{rec, rec_count, count, nullptr, nullptr, 0, nullptr};
     if (trace_) w.trace_line += ", \"count_from\": " + std::to_string(depth) + ", \"records\": " + std::to_string(count);
-    if (depth + 1 == d.n && w.sched == 1) {
+    if (depth + 1 == d.n && w.sched != 0) {
         out.bytes = bytes_ptr(kKLevelCount);
         const uint32_t chunk = level_max_records(d, depth, w.level);
         for (uint32_t s0 = 0; s0 < count; s0 += chunk) {
@@ -1055,11 +1142,11 @@ std::string Engine::finish_records(Lane& w, QState& qs, const StepDesc& d, uint3
     out.bytes = bytes_ptr(kKExtendCount);
     // Root splitting (dynamic schedule, few records: a handful of heavy records would otherwise
     // serialise on single waves).  The CP offsets buffer is free here.
-    const bool split = opt_.waves_per_anchor > 1 && w.sched == 1 && count <= kSplitMaxRecords;
+    const bool split = opt_.waves_per_anchor > 1 && w.sched != 0 && count <= kSplitMaxRecords;
     prof_begin(w.prof, w.s, kKExtendCount);
     if (split) root_chunks(g_, d, depth, rec, rec_count, count, opt_.waves_per_anchor, w.cp.prod, w.cp.scan_tmp, w.s);
     extend(g_, d, depth, d.n, kCountAtEnd, r, out, split ? w.cp.prod : nullptr, w.work_counter(), w.sched,
-           static_cast<int>(opt_.dfs_blocks), w.s);
+           static_cast<int>(opt_.dfs_blocks), w.dn, w.s);
     prof_end(w.prof, w.s, kKExtendCount);
     return "";
 }
@@ -1189,6 +1276,123 @@ std::string Engine::stage_group(uint32_t ph, std::vector<GroupStep>& steps) {
     return check_err("local index");
 }
 
+std::string Engine::run_team_steps(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph) {
+    const size_t n = items.size();
+    Lane& lead = lanes_[first_lane];
+    hipStream_t s = lead.s;  // every member's stream while the team runs
+    auto lane = [&](size_t k) -> Lane& { return lanes_[first_lane + k]; };
+    std::vector<uint32_t> np(n, 0), built(n, 0), count(n, 0);
+    // anchors of every step, one wait
+    for (size_t k = 0; k < n; k++) {
+        Lane& w = lane(k);
+        QState& qs = *qs_[items[k].q];
+        w.sched = sched_of(qs.tech.strategy);
+        step_anchors(w, qs, items[k].i, ph);
+        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.n_pairs, 4, hipMemcpyDeviceToHost, s));
+    }
+    CSM_HIP(hipStreamSynchronize(s));
+    // local indexes, one wait
+    bool wait = false;
+    for (size_t k = 0; k < n; k++) {
+        Lane& w = lane(k);
+        QState& qs = *qs_[items[k].q];
+        np[k] = w.h_pinned[0];
+        if (!np[k]) continue;
+        built[k] = step_local(w, qs, items[k].i, ph);
+        if (qs.tech.local_index) {
+            CSM_HIP(hipMemcpyAsync(w.h_pinned, w.L[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, s));
+            wait = true;
+        }
+    }
+    if (wait) CSM_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < n; k++) {
+        QState& qs = *qs_[items[k].q];
+        if (!np[k] || !qs.tech.local_index) continue;
+        for (uint32_t u = 0; u < qs.q.n; u++)
+            if ((built[k] >> u & 1u) && lane(k).h_pinned[u] == 0) { np[k] = 0; break; }
+    }
+    // initial partial results, one wait
+    std::vector<StepDesc> d(n);
+    std::vector<char> pend(n, 0);
+    wait = false;
+    for (size_t k = 0; k < n; k++) {
+        if (!np[k]) continue;
+        Lane& w = lane(k);
+        bool p = false;
+        std::string err = enumerate_begin(w, *qs_[items[k].q], items[k].i, ph, w.L, built[k], w.pairs, w.n_pairs, np[k], d[k], p, count[k]);
+        if (!err.empty()) return err;
+        pend[k] = p;
+        wait = wait || p;
+    }
+    if (wait) CSM_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < n; k++)
+        if (np[k] && pend[k]) {
+            std::string err = initial_count(lane(k), count[k]);
+            if (!err.empty()) return err;
+        }
+    // the level chains of the semi-BFS steps as one chain of multi-job launches, one wait
+    ChainJob* hj = h_chain_jobs_ + first_lane;
+    std::vector<size_t> job_step;
+    uint32_t max_l = 0;
+    std::vector<char> chained(n, 0);
+    for (size_t k = 0; k < n; k++) {
+        QState& qs = *qs_[items[k].q];
+        if (!np[k] || !count[k] || qs.q.n <= 2 || qs.tech.strategy != SearchStrategy::kDynamicBfsDfs || !opt_.chain_levels || trace_) continue;
+        Lane& w = lane(k);
+        ChainJob& J = hj[job_step.size()];
+        J = ChainJob{};
+        J.d = d[k];
+        J.L = chain_build(w, qs, d[k], items[k].i, count[k], J.c);
+        J.ls = w.level;
+        J.out = chain_out(d[k], slot_of(qs));
+        J.write_bytes = bytes_ptr(kKLevelWrite);
+        max_l = std::max(max_l, J.L);
+        job_step.push_back(k);
+        chained[k] = 1;
+    }
+    const uint32_t nj = static_cast<uint32_t>(job_step.size());
+    if (nj) {
+        ChainJob* dj = d_chain_jobs_ + first_lane;
+        CSM_HIP(hipMemcpyAsync(dj, hj, nj * sizeof(ChainJob), hipMemcpyHostToDevice, s));
+        for (uint32_t lv = 0; lv <= max_l; lv++) {
+            bool any_write = false, any_count = false;
+            for (uint32_t j = 0; j < nj; j++) {
+                any_write = any_write || lv < hj[j].L;
+                any_count = any_count || lv == hj[j].L;
+            }
+            prof_begin(lead.prof, s, kKLevelPrepare);
+            level_prepare_chain_multi(g_, dj, nj, lv, s);
+            prof_end(lead.prof, s, kKLevelPrepare);
+            if (any_write) {
+                prof_begin(lead.prof, s, kKLevelWrite);
+                level_flat_chain_multi(g_, dj, nj, lv, kWriteAtEnd, s);
+                prof_end(lead.prof, s, kKLevelWrite);
+            }
+            if (any_count) {
+                prof_begin(lead.prof, s, kKLevelCount);
+                level_flat_chain_multi(g_, dj, nj, lv, kCountAtEnd, s);
+                prof_end(lead.prof, s, kKLevelCount);
+            }
+        }
+        for (uint32_t j = 0; j < nj; j++)
+            if (hj[j].L) CSM_HIP(hipMemcpyAsync(lane(job_step[j]).h_pinned, hj[j].c.ctr, 8ull * hj[j].L, hipMemcpyDeviceToHost, s));
+        CSM_HIP(hipStreamSynchronize(s));
+        for (uint32_t j = 0; j < nj; j++) {
+            const size_t k = job_step[j];
+            QState& qs = *qs_[items[k].q];
+            std::string err = chain_finish(lane(k), qs, d[k], items[k].i, count[k], slot_of(qs), hj[j].c, hj[j].L);
+            if (!err.empty()) return err;
+        }
+    }
+    // every other step on its own (other strategies, Cartesian-product-free two-vertex queries, ...)
+    for (size_t k = 0; k < n; k++) {
+        if (!np[k] || chained[k]) continue;
+        std::string err = enumerate_rest(lane(k), *qs_[items[k].q], items[k].i, ph, d[k], count[k]);
+        if (!err.empty()) return err;
+    }
+    return "";
+}
+
 std::string Engine::run_phase(uint32_t ph) {
     std::vector<std::pair<uint32_t, uint32_t>> steps;  // (query, edge)
     for (size_t qi = 0; qi < qs_.size(); qi++) {
@@ -1227,10 +1431,54 @@ std::string Engine::run_phase(uint32_t ph) {
             GroupStep g;
             g.q = st.first;
             g.i = st.second;
-            if (group_ && qs_[st.first]->tech.local_index) grouped.push_back(g);
+            if (group_ && team_ == 1 && qs_[st.first]->tech.local_index) grouped.push_back(g);
             else work.q.push_back(g);
         }
     }
+    // A team (team_ > 1) of lanes [li, li + team_): up to team_ steps at a time on the stream of
+    // lane li / team_.  The runtime hands hardware queues to streams round-robin in creation order,
+    // so the first lanes' streams (created one after another) sit on different queues, while
+    // every team_-th lane's stream could share one.
+    std::vector<hipStream_t> team_stream;
+    for (size_t li = 0; li < lanes_.size(); li += team_) team_stream.push_back(lanes_[li / team_].s);
+    auto team_worker = [&](size_t li) {
+        Lane& lead = lanes_[li];
+        const size_t size = std::min<size_t>(team_, lanes_.size() - li);
+        std::vector<hipStream_t> own;
+        for (size_t k = 0; k < size; k++) own.push_back(lanes_[li + k].s);
+        for (size_t k = 0; k < size; k++) lanes_[li + k].s = team_stream[li / team_];
+        auto run = [&]() -> std::string {
+            CSM_HIP(hipSetDevice(opt_.device));
+            CSM_HIP(hipStreamWaitEvent(lead.s, phase_start_, 0));
+            for (;;) {
+                std::vector<GroupStep> items;
+                {
+                    std::unique_lock<std::mutex> lk(work.mu);
+                    work.cv.wait(lk, [&]() { return failed.load() || work.closed || !work.q.empty(); });
+                    if (failed.load() || work.q.empty()) break;
+                    while (items.size() < size && !work.q.empty()) {
+                        items.push_back(work.q.front());
+                        work.q.pop_front();
+                    }
+                }
+                const auto t0 = std::chrono::steady_clock::now();
+                for (size_t k = 0; k < size; k++) lanes_[li + k].trace_line.clear();
+                const std::string e = run_team_steps(li, items, ph);
+                const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                for (const GroupStep& it : items) cost[static_cast<size_t>(it.q) * kMaxQE + it.i] = sec;
+                if (!e.empty() && e != "empty") return e;
+            }
+            for (size_t k = 0; k < size; k++) CSM_HIP(hipEventRecord(lanes_[li + k].done, lead.s));
+            return "";
+        };
+        errs[li] = run();
+        for (size_t k = 0; k < size; k++) lanes_[li + k].s = own[k];
+        if (!errs[li].empty()) {
+            failed.store(true);
+            std::lock_guard<std::mutex> lk(work.mu);
+            work.cv.notify_all();
+        }
+    };
     auto worker = [&](size_t li) {
         Lane& w = lanes_[li];
         auto run = [&]() -> std::string {
@@ -1284,7 +1532,10 @@ std::string Engine::run_phase(uint32_t ph) {
         }
     };
     std::vector<std::thread> th;
-    for (size_t li = 0; li < lanes_.size(); li++) th.emplace_back(worker, li);
+    if (team_ > 1)
+        for (size_t li = 0; li < lanes_.size(); li += team_) th.emplace_back(team_worker, li);
+    else
+        for (size_t li = 0; li < lanes_.size(); li++) th.emplace_back(worker, li);
     std::string stage_err;
     for (size_t g0 = 0, gi = 0; g0 < grouped.size() && !failed.load(); g0 += group_, gi++) {
         const uint32_t half = static_cast<uint32_t>(gi % 2);

@@ -54,6 +54,7 @@ struct EngineOptions {
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
     bool chain_levels = true;         // semi-BFS levels of a step chained on the device (one host wait per step)
     uint32_t lanes = 8;               // concurrent matching steps (streams), each with its own scratch
+    uint32_t team = 1;                // lanes per team: a team's steps share a stream, host waits and level launches
     uint64_t arena_slack_words = 1u << 20;  // adjacency arena room beyond the growth rule
 };
 
@@ -82,7 +83,8 @@ struct Tunables {
     uint32_t waves_per_anchor = 0;    // most waves one record's first-depth candidates are split over (1..65535)
     uint32_t stack_depth = 0;         // most LDS stack rows (depths) of a counting DFS launch (1..kMaxQV)
     uint32_t dfs_blocks = 0;          // blocks of the persistent DFS grid (64..16384)
-    uint32_t local_blocks = 0;        // blocks of the local-index kernels (16..4096)
+    uint32_t local_blocks = 0;        // blocks of the local-index kernels (16..2048)
+    uint32_t team = 0;                // lanes per team (1: every lane runs its steps alone; up to the lanes)
 };
 
 struct QueryResult {
@@ -247,7 +249,8 @@ class Engine {
         hipEvent_t done = nullptr;     // end of the lane's work in a phase
         Prof prof;
         std::string trace_line;
-        int sched = 1;                 // DFS schedule of the current step: 0 static, 1 dynamic
+        int sched = 1;                 // DFS schedule of the current step: 0 static, 1 dynamic, 2 dynamic + range donation
+        Donation dn{};                 // range words of the depth-first kernel's waves (schedule 2)
         unsigned long long* work_counter() { return reinterpret_cast<unsigned long long*>(counters + 8); }
     };
 
@@ -281,9 +284,29 @@ class Engine {
                              std::vector<QueryResult>& out, BatchInfo& info);
     std::string run_phase(uint32_t ph);
     std::string run_step(Lane& w, QState& qs, uint32_t i, uint32_t ph);
+    // the stages of run_step, so that a team can run each stage for all its steps before one wait
+    void step_anchors(Lane& w, QState& qs, uint32_t i, uint32_t ph);
+    uint32_t step_local(Lane& w, QState& qs, uint32_t i, uint32_t ph);
     std::string enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
                                const uint32_t* n_pairs, uint32_t np);
+    std::string enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
+                                const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count);
+    std::string initial_count(Lane& w, uint32_t& count);
+    std::string enumerate_rest(Lane& w, QState& qs, uint32_t i, uint32_t ph, const StepDesc& d, uint32_t count);
     std::string search_chained(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot);
+    uint32_t chain_build(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, LevelChain& c);
+    MatchOut chain_out(const StepDesc& d, uint32_t res_slot);
+    std::string chain_finish(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot, const LevelChain& c,
+                             uint32_t L);
+    uint32_t slot_of(const QState& qs) const;
+    static int sched_of(SearchStrategy st);
+    // Teams: lanes [t * team_, (t + 1) * team_) run up to team_ steps at a time on the first lane's
+    // stream from one host thread, each stage for all of them before one wait, and their level
+    // chains as one chain of multi-job launches (ChainJob tables at d_chain_jobs_ + first lane).
+    std::string run_team_steps(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph);
+    uint32_t team_ = 1;
+    ChainJob* d_chain_jobs_ = nullptr;   // one per lane
+    ChainJob* h_chain_jobs_ = nullptr;   // pinned staging, one per lane
     std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,
                        uint64_t top, uint32_t target, uint32_t res_slot);
     std::string extend_to(Lane& w, const StepDesc& d, const uint32_t* rec, uint32_t count, uint32_t depth, uint32_t end, uint32_t* out,

@@ -187,6 +187,9 @@ typedef struct {
     uint32_t dfs_blocks;         /* blocks of the persistent depth-first grid (64..16384, default 2048) */
     uint32_t local_blocks;       /* blocks of the local-index kernels (16..2048; default 256 for batches of
                                     2^18 edges or more, else 128) */
+    uint32_t team;               /* lanes per team (1..16, default 1): a team runs that many steps at a time on
+                                    one stream from one host thread, each stage for all of them before one
+                                    wait, and their semi-BFS level chains as one chain of multi-step launches */
 } csm_tunables;
 csm_status csm_get_tunables(csm_session* s, csm_tunables* out);
 csm_status csm_set_tunables(csm_session* s, const csm_tunables* t);

@@ -186,17 +186,17 @@ def _run_with_settings(ffi, data, update, batch, queries, tech, settings):
 
 
 SETTINGS = [
-    {"waves_per_anchor": 1, "stack_depth": 1, "dfs_blocks": 64, "local_blocks": 16},
-    {"waves_per_anchor": 3, "stack_depth": 2, "dfs_blocks": 4096, "local_blocks": 512, "semi_bfs_threshold": 64},
-    {"waves_per_anchor": 65535, "stack_depth": 16, "dfs_blocks": 2048, "local_blocks": 128, "semi_bfs_threshold": 1 << 20},
-    {"waves_per_anchor": 16, "stack_depth": 4, "dfs_blocks": 333, "local_blocks": 2048, "semi_bfs_threshold": 4096},
+    {"waves_per_anchor": 1, "stack_depth": 1, "dfs_blocks": 64, "local_blocks": 16, "team": 3},
+    {"waves_per_anchor": 3, "stack_depth": 2, "dfs_blocks": 4096, "local_blocks": 512, "semi_bfs_threshold": 64, "team": 1},
+    {"waves_per_anchor": 65535, "stack_depth": 16, "dfs_blocks": 2048, "local_blocks": 128, "semi_bfs_threshold": 1 << 20, "team": 8},
+    {"waves_per_anchor": 16, "stack_depth": 4, "dfs_blocks": 333, "local_blocks": 2048, "semi_bfs_threshold": 4096, "team": 2},
 ]
 
 
 def test_launch_settings_change_no_result(lib, tmp_path):
     """csm_set_tunables between batches (waves per anchor, DFS stack depth, DFS and local-index
-    grids, semi-BFS threshold, each at its limits and in between) changes scheduling only: every
-    strategy's results equal the oracle's."""
+    grids, semi-BFS threshold, lanes per team, each at its limits and in between) changes
+    scheduling only: every strategy's results equal the oracle's."""
     for config, seed, batch in [("tinyins", 7, 64), ("c1", 2, 1000)]:
         d = tmp_path / f"{config}{seed}"
         d.mkdir()
@@ -339,6 +339,37 @@ def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir,
     want, _ = util.oracle_stream(data, upd, 2000, queries, threads=8)
     say(f"case split={split} sdir={sdir} lanes={lanes} chain={chain} group={group} c1 batch=2000")
     assert run_library(lib, str(data), str(upd), 2000, queries, lib.technique()) == want
+
+
+@pytest.mark.parametrize("team,lanes", [(2, 8), (3, 8), (8, 8), (4, 16)])
+def test_teams_change_no_result(lib, tmp_path, monkeypatch, team, lanes):
+    """Teams of lanes (steps run a stage at a time, their level chains as multi-job launches; the
+    last team smaller when the lanes do not divide) change scheduling only: tinyins and c1 for
+    several strategies, and the large/skewed workload at 2% scale, equal the oracle's results."""
+    monkeypatch.setenv("CSM_TEAM", str(team))
+    monkeypatch.setenv("CSM_LANES", str(lanes))
+    for config, seed, batch in [("tinyins", 7, 64), ("c1", 3, 2000)]:
+        d = tmp_path / f"{config}{seed}"
+        d.mkdir()
+        meta, queries = util.gen(config, d, seed=seed)
+        data, upd = d / "data.graph", d / "update.stream"
+        want, _ = util.oracle_stream(data, upd, batch, queries, threads=8)
+        for (mo, ss, li, cp) in [(2, 2, True, True), (1, 0, False, True), (2, 2, True, False), (0, 1, True, True), (2, 3, True, True)]:
+            say(f"case team={team} lanes={lanes} {config} order={mo} strategy={ss} local={li} cp={cp}")
+            got = run_library(lib, str(data), str(upd), batch, queries, lib.technique(mo, ss, li, cp))
+            if got is not None:
+                assert got == want, f"team={team} {config} order={mo} strategy={ss}"
+    d = tmp_path / "c3"
+    d.mkdir()
+    meta, queries = util.gen("c3", d, seed=1, scale=0.02)
+    data, upd = d / "data.graph", d / "update.stream"
+    batch = int(meta["batch_size"])
+    want, _ = util.oracle_stream(data, upd, batch, queries, threads=16)
+    for t in [(2, 2, True, True), (2, 2, True, False, False, False)]:
+        say(f"case team={team} lanes={lanes} c3 scale=0.02 technique={t}")
+        assert run_library(lib, str(data), str(upd), batch, queries, lib.technique(*t)) == want
+    say(f"case team={team} lanes={lanes} c3 scale=0.02 semi_bfs_threshold=64")
+    assert run_library(lib, str(data), str(upd), batch, queries, lib.technique(), semi_bfs_threshold=64) == want
 
 
 def test_c2_shape_scaled_stream(lib, tmp_path):

@@ -416,7 +416,7 @@ def config_block(config, meta, args, opts, scale):
     return {"workload": WORKLOADS[config], "config": config, "seed": args.seed, "scale": scale,
             "batch_edges": int(meta["batch_size"]), "queries": int(meta["queries"]),
             "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP, heavy-vertex avoidance, relation switch)",
-            "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "8"))),
+            "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "16"))),
             "semi_bfs_threshold": opts.get("semi_bfs_threshold") or header_default_threshold()}
 
 
@@ -490,7 +490,7 @@ def main():
         t = time.time()
         thr = header_default_threshold()
         best, table = at.tune(lambda o: open_session(ffi, data_dir, queries, batch, **o),
-                              [at.Candidate(lanes, thr) for lanes in (4, 8, 12)], warmup=1, batches=2)
+                              [at.Candidate(lanes, thr) for lanes in (8, 12, 16)], warmup=1, batches=2)
         opts = {"lanes": best.lanes}
         lib_tune = {}
         tuned = {"lanes": {"chosen": best.lanes, "candidates": table, "batches": "1-2 after warm-up batch 0"},

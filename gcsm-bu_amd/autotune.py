@@ -27,7 +27,7 @@ class Candidate:
         return {"lanes": self.lanes, "semi_bfs_threshold": self.semi_bfs_threshold}
 
 
-DEFAULT_CANDIDATES = tuple(Candidate(lanes, t) for lanes in (4, 8, 12) for t in (1 << 14, 1 << 16, 1 << 18, 1 << 20))
+DEFAULT_CANDIDATES = tuple(Candidate(lanes, t) for lanes in (8, 12, 16) for t in (1 << 14, 1 << 16, 1 << 18, 1 << 20))
 
 
 class TuneMismatch(RuntimeError):

@@ -57,14 +57,15 @@ Engine::~Engine() {
         for (hipEvent_t e : w.prof.pool) (void)hipEventDestroy(e);
         if (w.done) (void)hipEventDestroy(w.done);
         if (w.h_pinned) (void)hipHostFree(w.h_pinned);
-        if (w.s) (void)hipStreamDestroy(w.s);
     }
+    for (hipStream_t st : lane_streams_) (void)hipStreamDestroy(st);
     if (phase_start_) (void)hipEventDestroy(phase_start_);
     if (snap_) (void)hipFree(snap_);
     if (trace_) std::fclose(trace_);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
     if (h_jobs_) (void)hipHostFree(h_jobs_);
     if (h_chain_jobs_) (void)hipHostFree(h_chain_jobs_);
+    if (h_tajobs_) (void)hipHostFree(h_tajobs_);
     if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -75,12 +76,17 @@ std::string Engine::init(const EngineOptions& o) {
     if (n == 0) return tagged(Code::kDevice, "no HIP device available");
     if (o.device < 0 || o.device >= n) return tagged(Code::kArgument, "device id out of range");
     CSM_HIP(hipSetDevice(o.device));
+    if (const char* t = std::getenv("CSM_LANES")) opt_.lanes = static_cast<uint32_t>(std::max(1, std::atoi(t)));
+    // The lanes' streams first, then the main stream: the runtime gives streams hardware queues in
+    // creation order (new queues while it has fewer than its maximum, then the least used one), so
+    // the first lanes (the teams' streams) get queues of their own also in a fresh process.
+    lane_streams_.resize(std::max<uint32_t>(1, std::min<uint32_t>(opt_.lanes, kMaxLanes)));
+    for (hipStream_t& st : lane_streams_) CSM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     CSM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096, hipHostMallocDefault));
     CSM_HIP(hipEventCreateWithFlags(&phase_start_, hipEventDisableTiming));
     CSM_HIP(static_ring(o.device, &static_ring_));
     if (const char* t = std::getenv("CSM_TRACE")) trace_ = std::fopen(t, "a");
-    if (const char* t = std::getenv("CSM_LANES")) opt_.lanes = static_cast<uint32_t>(std::max(1, std::atoi(t)));
     if (const char* t = std::getenv("CSM_TEAM")) opt_.team = static_cast<uint32_t>(std::max(1, std::atoi(t)));
     // A/B switches of the scheduling and layout aids (all on by default)
     if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) if (std::atoi(t) == 0) opt_.waves_per_anchor = 1;
@@ -246,6 +252,7 @@ std::string Engine::build() {
         add(64 << 10);                                             // alignment slack of the takes
         add(sizeof(ChainJob) + 256);                               // the lane's level-chain job
         add(kMaxDfsBlocks * 4ull * 12 + 512);                      // range donation words
+        add(kLaneWords * 4 + sizeof(AnchorJob) + kMaxQV * (sizeof(LocalJob) + sizeof(VSet) + (kMaxLanes + 1) * 4ull) + 1024);  // team jobs
     }
     // local-index slots: two groups of group_ steps (each: its sets, anchor pairs, queue, counters)
     uint32_t max_qn = 2;
@@ -421,8 +428,23 @@ std::string Engine::build() {
     // ---- runtime buffers: one set per lane ----
     lanes_.assign(n_lanes, Lane{});
     team_ = std::max<uint32_t>(1, std::min<uint32_t>(opt_.team, n_lanes));
+    lens_all_ = pool_.take<uint32_t>(static_cast<uint64_t>(kLaneWords) * n_lanes);
+    d_tajobs_ = pool_.take<AnchorJob>(n_lanes);
+    d_tljobs_ = pool_.take<LocalJob>(static_cast<uint64_t>(kMaxQV) * n_lanes);
+    d_tclear_ = pool_.take<VSet>(static_cast<uint64_t>(kMaxQV) * n_lanes);
+    d_toff_ = pool_.take<uint32_t>(static_cast<uint64_t>(n_lanes) * kMaxQV * (kMaxLanes + 1));
+    if (!lens_all_ || !d_tajobs_ || !d_tljobs_ || !d_tclear_ || !d_toff_) return tagged(Code::kMemory, "device pool exhausted (lane counters)");
+    {
+        const uint64_t hb = n_lanes * (sizeof(AnchorJob) + kMaxQV * sizeof(LocalJob) + kMaxQV * sizeof(VSet) + kLaneWords * 4ull) + 1024;
+        uint8_t* h = nullptr;
+        CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h), hb, hipHostMallocDefault));
+        h_tajobs_ = reinterpret_cast<AnchorJob*>(h);
+        h_tljobs_ = reinterpret_cast<LocalJob*>(h_tajobs_ + n_lanes);
+        h_tclear_ = reinterpret_cast<VSet*>(h_tljobs_ + static_cast<uint64_t>(kMaxQV) * n_lanes);
+        h_tlens_ = reinterpret_cast<uint32_t*>(h_tclear_ + static_cast<uint64_t>(kMaxQV) * n_lanes);
+    }
     for (Lane& w : lanes_) {
-        CSM_HIP(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking));
+        w.s = lane_streams_[static_cast<size_t>(&w - lanes_.data())];
         CSM_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
         CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&w.h_pinned), 4096, hipHostMallocDefault));
         for (uint32_t u = 0; u < kMaxQV; u++) {
@@ -432,8 +454,7 @@ std::string Engine::build() {
             if (!w.L[u].bitmap || !w.L[u].list) return tagged(Code::kMemory, "device pool exhausted (local sets)");
             CSM_HIP(hipMemsetAsync(w.L[u].bitmap, 0, words_ * 4ull, s_));
         }
-        uint32_t* lens = pool_.take<uint32_t>(kLaneWords);  // set lengths, list counters, step counters
-        if (!lens) return tagged(Code::kMemory, "device pool exhausted (local sets)");
+        uint32_t* lens = lens_all_ + static_cast<uint64_t>(&w - lanes_.data()) * kLaneWords;  // set lengths, list and step counters
         for (uint32_t u = 0; u < kMaxQV; u++) w.L[u].len = lens + u;
         w.tmp_len = lens + kMaxQV;
         w.n_pairs = lens + kMaxQV + 1;
@@ -1276,45 +1297,139 @@ std::string Engine::stage_group(uint32_t ph, std::vector<GroupStep>& steps) {
     return check_err("local index");
 }
 
+// The local indexes of a team's steps (each on its lane's sets) with the multi-step kernels: the
+// anchors of every step in one launch, then per indexing-order position one expansion and one
+// check launch over every step that builds a vertex there (the rule of step_local); one wait after
+// the anchors and one after the sets.  np[k] = 0 for a step without anchors or with an empty set.
+std::string Engine::team_local(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph, std::vector<uint32_t>& np,
+                               std::vector<uint32_t>& built) {
+    const size_t n = items.size();
+    Lane& lead = lanes_[first_lane];
+    hipStream_t s = lead.s;
+    const uint32_t max_arcs = 2 * max_batch_;
+    auto lane = [&](size_t k) -> Lane& { return lanes_[first_lane + k]; };
+    // clear the sets the lanes' previous steps left, then zero the lanes' counters
+    VSet* hc = h_tclear_ + first_lane * kMaxQV;
+    VSet* dc = d_tclear_ + first_lane * kMaxQV;
+    uint32_t nclear = 0;
+    for (size_t k = 0; k < n; k++) {
+        Lane& w = lane(k);
+        for (uint32_t u = 0; u < kMaxQV; u++)
+            if (w.L_used >> u & 1u) hc[nclear++] = w.L[u];
+        w.L_used = 0;
+    }
+    if (nclear) {
+        CSM_HIP(hipMemcpyAsync(dc, hc, nclear * sizeof(VSet), hipMemcpyHostToDevice, s));
+        clear_sets_multi(dc, nclear, dv_, s);
+    }
+    uint32_t* lens = lens_all_ + first_lane * kLaneWords;
+    uint32_t* hl = h_tlens_ + first_lane * kLaneWords;
+    fill_u32(lens, 0u, n * kLaneWords, s);
+    // anchors of every step
+    AnchorJob* ha = h_tajobs_ + first_lane;
+    for (size_t k = 0; k < n; k++) {
+        Lane& w = lane(k);
+        QState& qs = *qs_[items[k].q];
+        const Query& q = qs.q;
+        const uint32_t u0 = q.qe_list[items[k].i].first, u1 = q.qe_list[items[k].i].second;
+        const RelInfo& R = qs.rel[q.rel(u0, u1)];
+        ha[k] = AnchorJob{ph, R.dir, R.slot, qs.vrank[u0], qs.cand + static_cast<uint64_t>(u0) * words_,
+                          qs.cand + static_cast<uint64_t>(u1) * words_, w.pairs, w.n_pairs, max_arcs, w.L[u0], w.L[u1]};
+        w.L_used = static_cast<uint16_t>((1u << u0) | (1u << u1));
+        built[k] = (1u << u0) | (1u << u1);
+    }
+    CSM_HIP(hipMemcpyAsync(d_tajobs_ + first_lane, ha, n * sizeof(AnchorJob), hipMemcpyHostToDevice, s));
+    prof_begin(lead.prof, s, kKAnchors);
+    local_anchors_multi(g_, d_tajobs_ + first_lane, static_cast<uint32_t>(n), bs_[0].by_label, bs_[0].label_off, bs_[1].by_label,
+                        bs_[1].label_off, max_arcs, s);
+    prof_end(lead.prof, s, kKAnchors);
+    CSM_HIP(hipMemcpyAsync(hl, lens, n * kLaneWords * 4ull, hipMemcpyDeviceToHost, s));
+    CSM_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < n; k++) np[k] = hl[k * kLaneWords + kMaxQV + 1];  // Lane::n_pairs
+    // local sets, one indexing-order position at a time
+    bool any = false;
+    for (uint32_t pos = 2; pos < kMaxQV; pos++) {
+        LocalJob* lj = h_tljobs_ + first_lane * kMaxQV + pos * n;
+        uint32_t nj = 0;
+        for (size_t k = 0; k < n; k++) {
+            QState& qs = *qs_[items[k].q];
+            const Query& q = qs.q;
+            if (!np[k] || !qs.tech.local_index || pos >= q.n) continue;
+            const Plan& p = qs.plan;
+            const uint32_t i = items[k].i;
+            const OrderPerEdge& io = p.indexing[i];
+            const uint32_t u = io.vs[pos];
+            if (!(p.rebuild_B[i] >> u & 1u)) continue;
+            std::vector<uint32_t> bsv;
+            for (uint32_t kk = io.bni_offs[pos]; kk < io.bni_offs[pos + 1]; kk++) {
+                const uint32_t b = io.vs[io.bni[kk]];
+                if (built[k] >> b & 1u) bsv.push_back(b);
+            }
+            if (bsv.empty()) continue;
+            size_t first = 0;
+            if (qs.tech.heavy_vertex_avoidance) {
+                for (size_t kk = 1; kk < bsv.size(); kk++)
+                    if (qs.avg[q.rel(bsv[kk], u)] < qs.avg[q.rel(bsv[first], u)]) first = kk;
+            }
+            const uint32_t b0 = bsv[first];
+            const uint32_t rb = q.rel(b0, u);
+            Lane& w = lane(k);
+            LocalJob& J = lj[nj++];
+            J = LocalJob{};
+            J.phase = ph;
+            J.view = view_of(qs, rb, i, ph);
+            J.dir = qs.rel[rb].dir;
+            J.slot = qs.rel[rb].slot;
+            J.frontier = w.L[b0].list;
+            J.frontier_len = w.L[b0].len;
+            J.cand_u = qs.cand + static_cast<uint64_t>(u) * words_;
+            J.out = w.L[u];
+            J.queue = w.tmp_list;
+            J.queue_len = w.L[0].len + kQueueCounters + u;
+            for (size_t kk = 0; kk < bsv.size(); kk++) {
+                if (kk == first) continue;
+                const uint32_t rub = q.rel(u, bsv[kk]);
+                J.fl.f[J.fl.n++] = LocalFilter{view_of(qs, rub, i, ph), qs.rel[rub].dir, qs.rel[rub].slot, w.L[bsv[kk]].bitmap};
+            }
+            w.L_used |= static_cast<uint16_t>(1u << u);
+            built[k] |= 1u << u;
+        }
+        if (!nj) continue;
+        any = true;
+        LocalJob* dj = d_tljobs_ + first_lane * kMaxQV + pos * n;
+        uint32_t* off = d_toff_ + (first_lane * kMaxQV + pos) * (kMaxLanes + 1);
+        CSM_HIP(hipMemcpyAsync(dj, lj, nj * sizeof(LocalJob), hipMemcpyHostToDevice, s));
+        prof_begin(lead.prof, s, kKLocalIndex);
+        local_queue_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalIndex), s);
+        prof_end(lead.prof, s, kKLocalIndex);
+        prof_begin(lead.prof, s, kKLocalCheck);
+        local_check_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalCheck), s);
+        prof_end(lead.prof, s, kKLocalCheck);
+    }
+    if (!any) return "";
+    CSM_HIP(hipMemcpyAsync(hl, lens, n * kLaneWords * 4ull, hipMemcpyDeviceToHost, s));
+    CSM_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < n; k++) {
+        if (!np[k] || !qs_[items[k].q]->tech.local_index) continue;
+        for (uint32_t u = 0; u < qs_[items[k].q]->q.n; u++)
+            if ((built[k] >> u & 1u) && hl[k * kLaneWords + u] == 0) { np[k] = 0; break; }
+    }
+    return "";
+}
+
 std::string Engine::run_team_steps(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph) {
     const size_t n = items.size();
     Lane& lead = lanes_[first_lane];
     hipStream_t s = lead.s;  // every member's stream while the team runs
     auto lane = [&](size_t k) -> Lane& { return lanes_[first_lane + k]; };
     std::vector<uint32_t> np(n, 0), built(n, 0), count(n, 0);
-    // anchors of every step, one wait
-    for (size_t k = 0; k < n; k++) {
-        Lane& w = lane(k);
-        QState& qs = *qs_[items[k].q];
-        w.sched = sched_of(qs.tech.strategy);
-        step_anchors(w, qs, items[k].i, ph);
-        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.n_pairs, 4, hipMemcpyDeviceToHost, s));
-    }
-    CSM_HIP(hipStreamSynchronize(s));
-    // local indexes, one wait
-    bool wait = false;
-    for (size_t k = 0; k < n; k++) {
-        Lane& w = lane(k);
-        QState& qs = *qs_[items[k].q];
-        np[k] = w.h_pinned[0];
-        if (!np[k]) continue;
-        built[k] = step_local(w, qs, items[k].i, ph);
-        if (qs.tech.local_index) {
-            CSM_HIP(hipMemcpyAsync(w.h_pinned, w.L[0].len, kMaxQV * 4, hipMemcpyDeviceToHost, s));
-            wait = true;
-        }
-    }
-    if (wait) CSM_HIP(hipStreamSynchronize(s));
-    for (size_t k = 0; k < n; k++) {
-        QState& qs = *qs_[items[k].q];
-        if (!np[k] || !qs.tech.local_index) continue;
-        for (uint32_t u = 0; u < qs.q.n; u++)
-            if ((built[k] >> u & 1u) && lane(k).h_pinned[u] == 0) { np[k] = 0; break; }
-    }
+    for (size_t k = 0; k < n; k++) lane(k).sched = sched_of(qs_[items[k].q]->tech.strategy);
+    std::string lerr = team_local(first_lane, items, ph, np, built);
+    if (!lerr.empty()) return lerr;
     // initial partial results, one wait
     std::vector<StepDesc> d(n);
     std::vector<char> pend(n, 0);
-    wait = false;
+    bool wait = false;
     for (size_t k = 0; k < n; k++) {
         if (!np[k]) continue;
         Lane& w = lane(k);

@@ -53,8 +53,8 @@ struct EngineOptions {
     uint32_t dfs_blocks = 2048;       // blocks of the persistent DFS grid
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
     bool chain_levels = true;         // semi-BFS levels of a step chained on the device (one host wait per step)
-    uint32_t lanes = 8;               // concurrent matching steps (streams), each with its own scratch
-    uint32_t team = 1;                // lanes per team: a team's steps share a stream, host waits and level launches
+    uint32_t lanes = 16;              // concurrent matching steps, each with its own scratch
+    uint32_t team = 4;                // lanes per team: a team's steps share a stream, host waits and multi-step launches
     uint64_t arena_slack_words = 1u << 20;  // adjacency arena room beyond the growth rule
 };
 
@@ -304,7 +304,21 @@ class Engine {
     // stream from one host thread, each stage for all of them before one wait, and their level
     // chains as one chain of multi-job launches (ChainJob tables at d_chain_jobs_ + first lane).
     std::string run_team_steps(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph);
+    std::string team_local(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph, std::vector<uint32_t>& np,
+                           std::vector<uint32_t>& built);
     uint32_t team_ = 1;
+    // The teams' multi-step local-index jobs: per lane one anchor job, kMaxQV local jobs and
+    // kMaxQV sets to clear, and per lane and indexing position kMaxLanes + 1 job offsets; a team
+    // uses the regions of its lanes (device arrays and their pinned staging, same layout).
+    uint32_t* lens_all_ = nullptr;       // every lane's kLaneWords counters, lane after lane
+    AnchorJob* d_tajobs_ = nullptr;
+    LocalJob* d_tljobs_ = nullptr;
+    VSet* d_tclear_ = nullptr;
+    uint32_t* d_toff_ = nullptr;
+    AnchorJob* h_tajobs_ = nullptr;
+    LocalJob* h_tljobs_ = nullptr;
+    VSet* h_tclear_ = nullptr;
+    uint32_t* h_tlens_ = nullptr;
     ChainJob* d_chain_jobs_ = nullptr;   // one per lane
     ChainJob* h_chain_jobs_ = nullptr;   // pinned staging, one per lane
     std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,
@@ -347,6 +361,7 @@ class Engine {
     std::vector<NlfEntry> h_entries_;
     std::vector<NlfGroup> h_groups_;
     std::vector<Lane> lanes_;
+    std::vector<hipStream_t> lane_streams_;   // created at init, before the main stream
     std::vector<double> step_cost_[2];        // per phase, per (query, edge): host seconds in the last batch
     // host mirrors of list degrees and capacities per direction, and of the arena bump pointer
     uint64_t h_arena_words_ = 0;

@@ -47,8 +47,8 @@ typedef struct {
     uint64_t cp_list_bytes;  /* Cartesian-product leaf lists of each lane (config.h:43 SIZE_SPACE), 0 = 512 MiB */
     uint32_t max_batch_edges;/* largest batch the scratch is sized for, 0 = the stream's batch */
     uint32_t semi_bfs_threshold; /* config.h:47 MIN_NUM_RESULTS_TO_GPU, 0 = CSM_DEFAULT_SEMI_BFS_THRESHOLD */
-    uint32_t lanes;          /* extension: concurrent step lanes (streams + host threads, each with its own
-                                ring and scratch), 0 = 8, at most 16 */
+    uint32_t lanes;          /* extension: concurrent step lanes (each with its own ring and scratch; grouped
+                                in teams, csm_tunables.team), 0 = 16, at most 16 */
 } csm_options;
 
 /* TechniqueOption (utils/technique_option.h:37-56); values as the csmg flags. */
@@ -112,8 +112,8 @@ csm_status csm_apply_range(csm_session* s, uint64_t first, uint64_t n, csm_query
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc): csm_apply_range with every
  * lane's partial-result ring taken from the library's static device ring (256 MiB per GPU, split
  * equally over the session's lanes) instead of the session's pool; the call allocates nothing.
- * Same results as csm_apply_range.  Limits: at most 2^23 / lanes edges per call (1,048,576 with
- * the default 8 lanes), refused with CSM_ERR_LIMIT before any device work; a step whose first two
+ * Same results as csm_apply_range.  Limits: at most 2^23 / lanes edges per call (524,288 with
+ * the default 16 lanes), refused with CSM_ERR_LIMIT before any device work; a step whose first two
  * matching-order vertices are not its anchor edge needs its initial partial results to fit one
  * slice, else CSM_ERR_MEMORY.  All sessions on one GPU share the static ring: calls that overlap
  * on one GPU race. */
@@ -187,7 +187,7 @@ typedef struct {
     uint32_t dfs_blocks;         /* blocks of the persistent depth-first grid (64..16384, default 2048) */
     uint32_t local_blocks;       /* blocks of the local-index kernels (16..2048; default 256 for batches of
                                     2^18 edges or more, else 128) */
-    uint32_t team;               /* lanes per team (1..16, default 1): a team runs that many steps at a time on
+    uint32_t team;               /* lanes per team (1..16, default 4): a team runs that many steps at a time on
                                     one stream from one host thread, each stage for all of them before one
                                     wait, and their semi-BFS level chains as one chain of multi-step launches */
 } csm_tunables;

@@ -47,7 +47,7 @@ class BatchInfo(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_index", ctypes.c_double), ("ms_match", ctypes.c_double)]
 
 
-TUNABLES = ("semi_bfs_threshold", "waves_per_anchor", "stack_depth", "dfs_blocks", "local_blocks", "team")
+TUNABLES = ("semi_bfs_threshold", "waves_per_anchor", "stack_depth", "dfs_blocks", "local_blocks", "team", "level_blocks")
 
 
 class Tunables(ctypes.Structure):

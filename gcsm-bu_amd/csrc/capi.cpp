@@ -362,14 +362,14 @@ csm_status csm_restore(csm_session* s) {
 csm_status csm_get_tunables(csm_session* s, csm_tunables* out) {
     if (!s || !out) return CSM_ERR_ARGUMENT;
     const Tunables t = s->engine.tunables();
-    *out = csm_tunables{t.semi_bfs_threshold, t.waves_per_anchor, t.stack_depth, t.dfs_blocks, t.local_blocks, t.team};
+    *out = csm_tunables{t.semi_bfs_threshold, t.waves_per_anchor, t.stack_depth, t.dfs_blocks, t.local_blocks, t.team, t.level_blocks};
     return CSM_OK;
 }
 
 csm_status csm_set_tunables(csm_session* s, const csm_tunables* t) {
     if (!s || !t) return CSM_ERR_ARGUMENT;
     return classify(s, s->engine.set_tunables(Tunables{t->semi_bfs_threshold, t->waves_per_anchor, t->stack_depth, t->dfs_blocks,
-                                                       t->local_blocks, t->team}));
+                                                       t->local_blocks, t->team, t->level_blocks}));
 }
 
 csm_status csm_tune(csm_session* s, uint64_t first, uint64_t batch_edges, uint32_t warmup, uint32_t batches,
@@ -379,7 +379,7 @@ csm_status csm_tune(csm_session* s, uint64_t first, uint64_t batch_edges, uint32
     std::vector<Tunables> cands;
     for (uint32_t k = 0; k < num_candidates; k++) {
         const csm_tunables& c = candidates[k];
-        cands.push_back(Tunables{c.semi_bfs_threshold, c.waves_per_anchor, c.stack_depth, c.dfs_blocks, c.local_blocks, c.team});
+        cands.push_back(Tunables{c.semi_bfs_threshold, c.waves_per_anchor, c.stack_depth, c.dfs_blocks, c.local_blocks, c.team, c.level_blocks});
     }
     std::vector<Engine::TuneEval> trail;
     uint32_t b = 0;
@@ -388,7 +388,7 @@ csm_status csm_tune(csm_session* s, uint64_t first, uint64_t batch_edges, uint32
     if (best) *best = b;
     for (size_t k = 0; k < trail.size() && k < cap; k++) {
         const Tunables& t = trail[k].t;
-        if (evaluated) evaluated[k] = csm_tunables{t.semi_bfs_threshold, t.waves_per_anchor, t.stack_depth, t.dfs_blocks, t.local_blocks, t.team};
+        if (evaluated) evaluated[k] = csm_tunables{t.semi_bfs_threshold, t.waves_per_anchor, t.stack_depth, t.dfs_blocks, t.local_blocks, t.team, t.level_blocks};
         if (ms_per_batch) ms_per_batch[k] = trail[k].ms_per_batch;
     }
     return st;

@@ -1271,16 +1271,18 @@ void level_flat_chain(const DevGraph& g, const StepDesc& d, const LevelChain& c,
         k_level_flat_chain<kWriteAtEnd><<<kChainFlatGrid, 256, 0, s>>>(g, d, c, k, ls.off, ls.prun, ls.runs, ls.piv, ls.ph, out);
 }
 
-void level_prepare_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, hipStream_t s) {
+void level_prepare_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int blocks, hipStream_t s) {
     if (!njobs) return;
-    k_level_len_chain_multi<<<dim3(kChainLenGrid, njobs), kLenThreads, 0, s>>>(g, jobs, k);
-    k_level_scan_chain_multi<<<dim3(kChainLenGrid, njobs), kLenThreads, 0, s>>>(jobs, k);
+    const dim3 grid(static_cast<uint32_t>(std::max(16, blocks / 2)), njobs);
+    k_level_len_chain_multi<<<grid, kLenThreads, 0, s>>>(g, jobs, k);
+    k_level_scan_chain_multi<<<grid, kLenThreads, 0, s>>>(jobs, k);
 }
 
-void level_flat_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int mode, hipStream_t s) {
+void level_flat_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int mode, int blocks, hipStream_t s) {
     if (!njobs) return;
-    if (mode == kCountAtEnd) k_level_flat_chain_multi<kCountAtEnd><<<dim3(kChainFlatGrid, njobs), 256, 0, s>>>(g, jobs, k);
-    else k_level_flat_chain_multi<kWriteAtEnd><<<dim3(kChainFlatGrid, njobs), 256, 0, s>>>(g, jobs, k);
+    const dim3 grid(static_cast<uint32_t>(std::max(16, blocks)), njobs);
+    if (mode == kCountAtEnd) k_level_flat_chain_multi<kCountAtEnd><<<grid, 256, 0, s>>>(g, jobs, k);
+    else k_level_flat_chain_multi<kWriteAtEnd><<<grid, 256, 0, s>>>(g, jobs, k);
 }
 
 uint32_t level_max_records(const StepDesc& d, uint32_t d0, const LevelScratch& ls) {

@@ -295,8 +295,9 @@ struct ChainJob {
 };
 // Level k of the njobs jobs at `jobs` (device memory): preparation (run lengths and scan), then
 // the flattened candidate kernel of the jobs that write at k (mode kWriteAtEnd) or count at k.
-void level_prepare_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, hipStream_t s);
-void level_flat_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int mode, hipStream_t s);
+// blocks: of the candidate kernel per job (the preparation takes half as many).
+void level_prepare_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int blocks, hipStream_t s);
+void level_flat_chain_multi(const DevGraph& g, const ChainJob* jobs, uint32_t njobs, uint32_t k, int mode, int blocks, hipStream_t s);
 // chunks[i] = ceil(|pivot view run of record i at depth d0| / 64), then scanned in place (total at [n]).
 // Per record, its chunk count k = min(max_chunks, ceil(depth-d0 candidates / 64)), scanned.
 void root_chunks(const DevGraph& g, const StepDesc& d, uint32_t d0, const uint32_t* rec, const uint32_t* n_rec,

@@ -191,11 +191,12 @@ std::string Engine::build() {
     max_batch_ = opt_.max_batch_edges ? opt_.max_batch_edges : std::max<uint32_t>(1, std::min<uint64_t>(batch_size_, stream_.size()));
     if (batch_size_ > 0 && batch_size_ < 0xFFFFFFFFu) max_batch_ = std::max<uint32_t>(max_batch_, std::min<uint64_t>(batch_size_, stream_.size()));
     max_batch_ = std::max<uint32_t>(max_batch_, 1);
-    // The local-index kernels run beside the other lanes' kernels on fixed grids (their frontier
-    // sizes are device-side; each launch splits the frontier over its waves itself).  Measured:
-    // 128 four-wave blocks best with 65,536-edge batches on a 400K-vertex graph (256: +0.5%), 256
-    // with 1,048,576-edge batches on a 4M-vertex graph (128: +6%, 512: +1%, 1024: +13%).
-    local_grid_ = max_batch_ >= (1u << 18) ? 256 : 128;
+    // The local-index kernels run beside the other teams' kernels on fixed grids (their frontier
+    // sizes are device-side; each launch splits the frontier over its waves itself).  Measured with
+    // 16 lanes in teams of 4: 128 four-wave blocks with 65,536-edge batches on a 400K-vertex graph
+    // (256 within 1%), 512 with 1,048,576-edge batches on a 4M-vertex graph (45.7 ms per batch;
+    // 256: 47.4, 384: 46.2, 1024: +1.5%).
+    local_grid_ = max_batch_ >= (1u << 18) ? 512 : 128;
     if (const char* t = std::getenv("CSM_LOCAL_GRID")) local_grid_ = std::max(1, std::atoi(t));
 
     // ---- host CSR of the initial graph ----
@@ -1476,16 +1477,16 @@ std::string Engine::run_team_steps(size_t first_lane, std::vector<GroupStep>& it
                 any_count = any_count || lv == hj[j].L;
             }
             prof_begin(lead.prof, s, kKLevelPrepare);
-            level_prepare_chain_multi(g_, dj, nj, lv, s);
+            level_prepare_chain_multi(g_, dj, nj, lv, static_cast<int>(opt_.level_blocks), s);
             prof_end(lead.prof, s, kKLevelPrepare);
             if (any_write) {
                 prof_begin(lead.prof, s, kKLevelWrite);
-                level_flat_chain_multi(g_, dj, nj, lv, kWriteAtEnd, s);
+                level_flat_chain_multi(g_, dj, nj, lv, kWriteAtEnd, static_cast<int>(opt_.level_blocks), s);
                 prof_end(lead.prof, s, kKLevelWrite);
             }
             if (any_count) {
                 prof_begin(lead.prof, s, kKLevelCount);
-                level_flat_chain_multi(g_, dj, nj, lv, kCountAtEnd, s);
+                level_flat_chain_multi(g_, dj, nj, lv, kCountAtEnd, static_cast<int>(opt_.level_blocks), s);
                 prof_end(lead.prof, s, kKLevelCount);
             }
         }

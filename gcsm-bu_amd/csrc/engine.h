@@ -51,6 +51,7 @@ struct EngineOptions {
     uint32_t waves_per_anchor = 1024; // most waves one record of the counting DFS is split over (1: no split)
     uint32_t stack_depth = kMaxQV;    // most LDS stack rows of a counting DFS launch; deeper searches run levels first
     uint32_t dfs_blocks = 2048;       // blocks of the persistent DFS grid
+    uint32_t level_blocks = 512;      // blocks per step of a team's multi-step level kernels
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
     bool chain_levels = true;         // semi-BFS levels of a step chained on the device (one host wait per step)
     uint32_t lanes = 16;              // concurrent matching steps, each with its own scratch
@@ -85,6 +86,7 @@ struct Tunables {
     uint32_t dfs_blocks = 0;          // blocks of the persistent DFS grid (64..16384)
     uint32_t local_blocks = 0;        // blocks of the local-index kernels (16..2048)
     uint32_t team = 0;                // lanes per team (1: every lane runs its steps alone; up to the lanes)
+    uint32_t level_blocks = 0;        // blocks per step of a team's level kernels (64..4096)
 };
 
 struct QueryResult {

@@ -7,7 +7,7 @@
 // "Orders:" block (plan.cpp:1200-1291) and "Num Positive Matches: N" (technique_permutation.h:224).
 // Extensions: --queries a,b,c (several standing queries), --directed, --pool_mb, --ring_mb, --lanes,
 // --semi_bfs_threshold, --json <path> (one JSON record per batch and query), the launch settings
-// --waves_per_anchor, --stack_depth, --dfs_blocks, --local_blocks, --team, and --autotune true (the
+// --waves_per_anchor, --stack_depth, --dfs_blocks, --local_blocks, --team, --level_blocks, and --autotune true (the
 // library's autotuner, csm_tune, on the first --autotune_batches batches of the update stream
 // before the timed matching; it leaves the graph as it found it).
 #include <algorithm>
@@ -193,6 +193,7 @@ int main(int argc, char** argv) {
         tn.dfs_blocks = p.get_u32("--dfs_blocks", 0);
         tn.local_blocks = p.get_u32("--local_blocks", 0);
         tn.team = p.get_u32("--team", 0);
+        tn.level_blocks = p.get_u32("--level_blocks", 0);
         if ((st = csm_set_tunables(s, &tn)) != CSM_OK) return die(s, st);
     }
     if (p.get_bool("--autotune", false) && nb > 0) {
@@ -206,7 +207,7 @@ int main(int argc, char** argv) {
         for (uint32_t e = 0; e < n && e < 64; e++)
             std::cout << "Autotune candidate " << e << ": semi_bfs_threshold " << ev[e].semi_bfs_threshold << ", waves_per_anchor "
                       << ev[e].waves_per_anchor << ", stack_depth " << ev[e].stack_depth << ", dfs_blocks " << ev[e].dfs_blocks
-                      << ", local_blocks " << ev[e].local_blocks << ", team " << ev[e].team << ": " << ms[e] << " ms per batch"
+                      << ", local_blocks " << ev[e].local_blocks << ", team " << ev[e].team << ", level_blocks " << ev[e].level_blocks << ": " << ms[e] << " ms per batch"
                       << (e == best ? " (best)" : "") << '\n';
         time_line("Autotune", ms_since(t0), 0);
     }
@@ -215,7 +216,7 @@ int main(int argc, char** argv) {
         csm_get_tunables(s, &tn);
         std::cout << "Launch settings: semi_bfs_threshold " << tn.semi_bfs_threshold << ", waves_per_anchor " << tn.waves_per_anchor
                   << ", stack_depth " << tn.stack_depth << ", dfs_blocks " << tn.dfs_blocks << ", local_blocks " << tn.local_blocks
-                  << ", team " << tn.team << '\n';
+                  << ", team " << tn.team << ", level_blocks " << tn.level_blocks << '\n';
     }
     if (show_mem) {
         uint64_t used = 0, cap = 0;

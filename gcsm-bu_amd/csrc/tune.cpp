@@ -22,6 +22,7 @@ Tunables Engine::tunables() const {
     t.dfs_blocks = opt_.dfs_blocks;
     t.local_blocks = static_cast<uint32_t>(local_grid_);
     t.team = team_;
+    t.level_blocks = opt_.level_blocks;
     return t;
 }
 
@@ -32,12 +33,14 @@ std::string Engine::set_tunables(const Tunables& t) {
     if (t.dfs_blocks && (t.dfs_blocks < 64 || t.dfs_blocks > 16384)) return tagged(Code::kArgument, "dfs_blocks outside 64..16384");
     if (t.local_blocks && (t.local_blocks < 16 || t.local_blocks > 2048)) return tagged(Code::kArgument, "local_blocks outside 16..2048");
     if (t.team > kMaxLanes) return tagged(Code::kArgument, "team outside 1..16");
+    if (t.level_blocks && (t.level_blocks < 64 || t.level_blocks > 4096)) return tagged(Code::kArgument, "level_blocks outside 64..4096");
     if (t.semi_bfs_threshold) opt_.semi_bfs_threshold = t.semi_bfs_threshold;
     if (t.waves_per_anchor) opt_.waves_per_anchor = t.waves_per_anchor;
     if (t.stack_depth) opt_.stack_depth = t.stack_depth;
     if (t.dfs_blocks) opt_.dfs_blocks = t.dfs_blocks;
     if (t.local_blocks) local_grid_ = static_cast<int>(t.local_blocks);
     if (t.team) team_ = std::min<uint32_t>(t.team, std::max<size_t>(1, lanes_.size()));
+    if (t.level_blocks) opt_.level_blocks = t.level_blocks;
     return "";
 }
 
@@ -57,6 +60,7 @@ constexpr uint32_t kWavesPerAnchor[] = {1, 16, 256, 1024};
 constexpr uint32_t kStackDepth[] = {2, 4, kMaxQV};
 constexpr uint32_t kDfsBlocks[] = {1024, 2048, 4096};
 constexpr uint32_t kTeam[] = {1, 2, 4};
+constexpr uint32_t kLevelBlocks[] = {256, 512, 1024};
 // A candidate replaces the best so far only when it is faster by this fraction (timing noise).
 constexpr double kMinGain = 0.01;
 
@@ -116,6 +120,7 @@ std::string Engine::tune(uint64_t first, uint64_t batch, uint32_t warmup, uint32
             if (c.dfs_blocks) t.dfs_blocks = c.dfs_blocks;
             if (c.local_blocks) t.local_blocks = c.local_blocks;
             if (c.team) t.team = c.team;
+            if (c.level_blocks) t.level_blocks = c.level_blocks;
             if (!(err = eval(t)).empty()) break;
             if (trail.back().ms_per_batch < trail[best].ms_per_batch) best = static_cast<uint32_t>(trail.size() - 1);
         }
@@ -138,6 +143,7 @@ std::string Engine::tune(uint64_t first, uint64_t batch, uint32_t warmup, uint32
         if (err.empty()) coord(&Tunables::stack_depth, kStackDepth, std::size(kStackDepth));
         if (err.empty()) coord(&Tunables::dfs_blocks, kDfsBlocks, std::size(kDfsBlocks));
         if (err.empty() && lanes_.size() > 1) coord(&Tunables::team, kTeam, std::size(kTeam));
+        if (err.empty() && trail[best].t.team > 1) coord(&Tunables::level_blocks, kLevelBlocks, std::size(kLevelBlocks));
     }
     // leave the graph as it was, with the best settings (the starting ones after a failure)
     const std::string back = set_tunables(err.empty() ? trail[best].t : start);

@@ -185,11 +185,12 @@ typedef struct {
                                     launch descends; deeper searches run breadth-first levels first
                                     (1..16, default 16) */
     uint32_t dfs_blocks;         /* blocks of the persistent depth-first grid (64..16384, default 2048) */
-    uint32_t local_blocks;       /* blocks of the local-index kernels (16..2048; default 256 for batches of
+    uint32_t local_blocks;       /* blocks of the local-index kernels (16..2048; default 512 for batches of
                                     2^18 edges or more, else 128) */
     uint32_t team;               /* lanes per team (1..16, default 4): a team runs that many steps at a time on
                                     one stream from one host thread, each stage for all of them before one
                                     wait, and their semi-BFS level chains as one chain of multi-step launches */
+    uint32_t level_blocks;       /* blocks per step of a team's level kernels (64..4096, default 512) */
 } csm_tunables;
 csm_status csm_get_tunables(csm_session* s, csm_tunables* out);
 csm_status csm_set_tunables(csm_session* s, const csm_tunables* t);

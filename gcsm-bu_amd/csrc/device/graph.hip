@@ -502,19 +502,60 @@ __global__ void __launch_bounds__(256) k_slot_dir(DevGraph g, int dir, const uin
     }
 }
 
-__global__ void k_bucket_count(DevGraph g, BatchScratch w) {
+// Label buckets of the touched vertices: a histogram, its scan, then a scatter.  With few labels
+// every block counts in LDS and reserves its run of each bucket with one device atomic per label
+// (a device atomic per vertex would serialise on the few bucket counters); with more labels than
+// the LDS table holds, per-vertex device atomics.
+constexpr uint32_t kBucketLds = 4096;
+
+__global__ void __launch_bounds__(256) k_bucket_count(DevGraph g, BatchScratch w) {
+    __shared__ uint32_t h[kBucketLds];
     const uint32_t T = *w.n_touched;
+    const bool lds = g.nvl <= kBucketLds;
+    if (lds) {
+        for (uint32_t r = threadIdx.x; r < g.nvl; r += blockDim.x) h[r] = 0;
+        __syncthreads();
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
         const uint32_t r = g.vrank[w.touched[i]];
-        if (r < g.nvl) atomicAdd(&w.label_cur[r], 1u);
+        if (r >= g.nvl) continue;
+        if (lds) atomicAdd(&h[r], 1u);
+        else atomicAdd(&w.label_cur[r], 1u);
+    }
+    if (lds) {
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r < g.nvl; r += blockDim.x)
+            if (h[r]) atomicAdd(&w.label_cur[r], h[r]);
     }
 }
-__global__ void k_bucket_scatter(DevGraph g, BatchScratch w) {
+
+// Same walk over the touched vertices as k_bucket_count (same grid): each block counts again in
+// LDS, reserves its runs from the bucket cursors, then places its vertices in them.
+__global__ void __launch_bounds__(256) k_bucket_scatter(DevGraph g, BatchScratch w) {
+    __shared__ uint32_t h[kBucketLds];
     const uint32_t T = *w.n_touched;
+    if (g.nvl > kBucketLds) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+            const uint32_t v = w.touched[i];
+            const uint32_t r = g.vrank[v];
+            if (r < g.nvl) w.by_label[atomicAdd(&w.label_cur[r], 1u)] = v;
+        }
+        return;
+    }
+    for (uint32_t r = threadIdx.x; r < g.nvl; r += blockDim.x) h[r] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+        const uint32_t r = g.vrank[w.touched[i]];
+        if (r < g.nvl) atomicAdd(&h[r], 1u);
+    }
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < g.nvl; r += blockDim.x)
+        if (h[r]) h[r] = atomicAdd(&w.label_cur[r], h[r]);  // the block's first position in bucket r
+    __syncthreads();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
         const uint32_t v = w.touched[i];
         const uint32_t r = g.vrank[v];
-        if (r < g.nvl) w.by_label[atomicAdd(&w.label_cur[r], 1u)] = v;
+        if (r < g.nvl) w.by_label[atomicAdd(&h[r], 1u)] = v;
     }
 }
 

@@ -430,19 +430,21 @@ std::string Engine::build() {
     lanes_.assign(n_lanes, Lane{});
     team_ = std::max<uint32_t>(1, std::min<uint32_t>(opt_.team, n_lanes));
     lens_all_ = pool_.take<uint32_t>(static_cast<uint64_t>(kLaneWords) * n_lanes);
+    chain_ctr_all_ = pool_.take<uint32_t>(2ull * kMaxQV * n_lanes);
     d_tajobs_ = pool_.take<AnchorJob>(n_lanes);
     d_tljobs_ = pool_.take<LocalJob>(static_cast<uint64_t>(kMaxQV) * n_lanes);
     d_tclear_ = pool_.take<VSet>(static_cast<uint64_t>(kMaxQV) * n_lanes);
     d_toff_ = pool_.take<uint32_t>(static_cast<uint64_t>(n_lanes) * kMaxQV * (kMaxLanes + 1));
-    if (!lens_all_ || !d_tajobs_ || !d_tljobs_ || !d_tclear_ || !d_toff_) return tagged(Code::kMemory, "device pool exhausted (lane counters)");
+    if (!lens_all_ || !chain_ctr_all_ || !d_tajobs_ || !d_tljobs_ || !d_tclear_ || !d_toff_) return tagged(Code::kMemory, "device pool exhausted (lane counters)");
     {
-        const uint64_t hb = n_lanes * (sizeof(AnchorJob) + kMaxQV * sizeof(LocalJob) + kMaxQV * sizeof(VSet) + kLaneWords * 4ull) + 1024;
+        const uint64_t hb = n_lanes * (sizeof(AnchorJob) + kMaxQV * sizeof(LocalJob) + kMaxQV * sizeof(VSet) + kLaneWords * 4ull + 2 * kMaxQV * 4ull) + 1024;
         uint8_t* h = nullptr;
         CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h), hb, hipHostMallocDefault));
         h_tajobs_ = reinterpret_cast<AnchorJob*>(h);
         h_tljobs_ = reinterpret_cast<LocalJob*>(h_tajobs_ + n_lanes);
         h_tclear_ = reinterpret_cast<VSet*>(h_tljobs_ + static_cast<uint64_t>(kMaxQV) * n_lanes);
         h_tlens_ = reinterpret_cast<uint32_t*>(h_tclear_ + static_cast<uint64_t>(kMaxQV) * n_lanes);
+        h_tctr_ = h_tlens_ + static_cast<uint64_t>(kLaneWords) * n_lanes;
     }
     for (Lane& w : lanes_) {
         w.s = lane_streams_[static_cast<size_t>(&w - lanes_.data())];
@@ -479,7 +481,7 @@ std::string Engine::build() {
         // level scratch: offsets, generating runs, entry tags and hashes shared with the CP scratch
         uint4* level_runs = pool_.take<uint4>(kLevelRunsPerRecord * cp_rec);
         unsigned long long* tile_sum = pool_.take<unsigned long long>(cp_rec / kLevelTile + 2);
-        w.chain_ctr = pool_.take<uint32_t>(2 * kMaxQV);
+        w.chain_ctr = chain_ctr_all_ + static_cast<uint64_t>(&w - lanes_.data()) * 2 * kMaxQV;
         w.dn.range = pool_.take<unsigned long long>(kMaxDfsBlocks * 4ull);
         w.dn.rec = pool_.take<uint32_t>(kMaxDfsBlocks * 4ull);
         if (!w.dn.range || !w.dn.rec) return tagged(Code::kMemory, "device pool exhausted (lane scratch)");
@@ -789,17 +791,17 @@ std::string Engine::enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph,
     if (!err.empty()) return err;
     if (pending) {
         CSM_HIP(hipStreamSynchronize(w.s));
-        err = initial_count(w, count);
+        err = initial_count(w.h_pinned, count);
         if (!err.empty()) return err;
     }
     return enumerate_rest(w, qs, i, ph, d, count);
 }
 
 // The initial partial results of a step started by init_from_set: their count (or the overflow)
-// from the lane's pinned buffer, once the stream has passed the copy enumerate_begin enqueued.
-std::string Engine::initial_count(Lane& w, uint32_t& count) {
-    if (w.h_pinned[2]) return tagged(Code::kMemory, "partial-result ring too small for the initial partial results");
-    count = w.h_pinned[0];
+// from a host copy of the lane's step counters, once the stream has passed the copy.
+std::string Engine::initial_count(const uint32_t* counters, uint32_t& count) {
+    if (counters[2]) return tagged(Code::kMemory, "partial-result ring too small for the initial partial results");
+    count = counters[0];
     return "";
 }
 
@@ -807,7 +809,7 @@ std::string Engine::initial_count(Lane& w, uint32_t& count) {
 // the anchor pairs as the first two vertices their count is np; otherwise init_from_set runs and
 // `pending` says that its count arrives in the lane's pinned buffer (initial_count, after a wait).
 std::string Engine::enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
-                                    const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count) {
+                                    const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count, bool copy_count) {
     const Query& q = qs.q;
     w.sched = sched_of(qs.tech.strategy);
     const Plan& p = qs.plan;
@@ -874,7 +876,7 @@ std::string Engine::enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph
         Ring r{nullptr, nullptr, 0, ring, w.counters, static_cast<uint32_t>(std::min<uint64_t>(w.ring_words / 2, 0xFFFFFFFFull)),
                w.counters + 2};
         init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, w.s);
-        CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
+        if (copy_count) CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
         pending = true;
     }
     if (trace_) w.trace_line += ", \"anchors\": " + std::to_string(np);
@@ -1011,7 +1013,7 @@ std::string Engine::search_chained(Lane& w, QState& qs, const StepDesc& d, uint3
     prof_end(w.prof, w.s, kKLevelCount);
     if (L) CSM_HIP(hipMemcpyAsync(w.h_pinned, w.chain_ctr, 8ull * L, hipMemcpyDeviceToHost, w.s));
     CSM_HIP(hipStreamSynchronize(w.s));
-    return chain_finish(w, qs, d, i, count, res_slot, c, L);
+    return chain_finish(w, qs, d, i, count, res_slot, c, L, w.h_pinned);
 }
 
 // The chain of a step from its `count` initial records (depth 2 at the bottom of the lane's ring):
@@ -1040,21 +1042,22 @@ MatchOut Engine::chain_out(const StepDesc& d, uint32_t res_slot) {
     return MatchOut{results_ + 4 * res_slot + 2 * d.phase, results_ + 4 * res_slot + 2 * d.phase + 1, bytes_ptr(kKLevelCount)};
 }
 
-// After the chain ran and its per-level {count, overflow} words reached w.h_pinned: replay the
-// chain's decisions (chain_at) to find where it stopped, and go on from there with `search`.
+// After the chain ran and its per-level {count, overflow} words reached host memory at `ctr`:
+// replay the chain's decisions (chain_at) to find where it stopped, and go on from there with
+// `search`.
 std::string Engine::chain_finish(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot, const LevelChain& c,
-                                 uint32_t L) {
+                                 uint32_t L, const uint32_t* ctr) {
     const uint32_t depth0 = c.depth0;
     uint64_t off = 0;
     uint32_t n = count;
     for (uint32_t k = 0;; k++) {
         const uint32_t depth = depth0 + k;
         if (n == 0) return "";
-        if (n > c.cap[k] || n >= c.stop_at[k] || (k < L && w.h_pinned[2 * k + 1]))
+        if (n > c.cap[k] || n >= c.stop_at[k] || (k < L && ctr[2 * k + 1]))
             return search(w, qs, d, i, w.ring + off, n, depth, off + static_cast<uint64_t>(n) * depth, d.n, res_slot);
         if (k == L) return "";  // the leaf level counted
         off += static_cast<uint64_t>(n) * depth;
-        n = w.h_pinned[2 * k];
+        n = ctr[2 * k];
     }
 }
 
@@ -1347,11 +1350,13 @@ std::string Engine::team_local(size_t first_lane, std::vector<GroupStep>& items,
     CSM_HIP(hipMemcpyAsync(hl, lens, n * kLaneWords * 4ull, hipMemcpyDeviceToHost, s));
     CSM_HIP(hipStreamSynchronize(s));
     for (size_t k = 0; k < n; k++) np[k] = hl[k * kLaneWords + kMaxQV + 1];  // Lane::n_pairs
-    // local sets, one indexing-order position at a time
-    bool any = false;
+    // local sets, one indexing-order position at a time: every position's jobs first (they depend
+    // only on the plans), copied to the device at once, then the launches
+    uint32_t njobs[kMaxQV] = {};
+    uint32_t last = 0;
     for (uint32_t pos = 2; pos < kMaxQV; pos++) {
         LocalJob* lj = h_tljobs_ + first_lane * kMaxQV + pos * n;
-        uint32_t nj = 0;
+        uint32_t& nj = njobs[pos];
         for (size_t k = 0; k < n; k++) {
             QState& qs = *qs_[items[k].q];
             const Query& q = qs.q;
@@ -1395,11 +1400,16 @@ std::string Engine::team_local(size_t first_lane, std::vector<GroupStep>& items,
             w.L_used |= static_cast<uint16_t>(1u << u);
             built[k] |= 1u << u;
         }
+        if (nj) last = pos;
+    }
+    if (!last) return "";
+    CSM_HIP(hipMemcpyAsync(d_tljobs_ + first_lane * kMaxQV + 2 * n, h_tljobs_ + first_lane * kMaxQV + 2 * n,
+                           (last - 1) * n * sizeof(LocalJob), hipMemcpyHostToDevice, s));
+    for (uint32_t pos = 2; pos <= last; pos++) {
+        const uint32_t nj = njobs[pos];
         if (!nj) continue;
-        any = true;
         LocalJob* dj = d_tljobs_ + first_lane * kMaxQV + pos * n;
         uint32_t* off = d_toff_ + (first_lane * kMaxQV + pos) * (kMaxLanes + 1);
-        CSM_HIP(hipMemcpyAsync(dj, lj, nj * sizeof(LocalJob), hipMemcpyHostToDevice, s));
         prof_begin(lead.prof, s, kKLocalIndex);
         local_queue_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalIndex), s);
         prof_end(lead.prof, s, kKLocalIndex);
@@ -1407,7 +1417,6 @@ std::string Engine::team_local(size_t first_lane, std::vector<GroupStep>& items,
         local_check_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalCheck), s);
         prof_end(lead.prof, s, kKLocalCheck);
     }
-    if (!any) return "";
     CSM_HIP(hipMemcpyAsync(hl, lens, n * kLaneWords * 4ull, hipMemcpyDeviceToHost, s));
     CSM_HIP(hipStreamSynchronize(s));
     for (size_t k = 0; k < n; k++) {
@@ -1435,15 +1444,20 @@ std::string Engine::run_team_steps(size_t first_lane, std::vector<GroupStep>& it
         if (!np[k]) continue;
         Lane& w = lane(k);
         bool p = false;
-        std::string err = enumerate_begin(w, *qs_[items[k].q], items[k].i, ph, w.L, built[k], w.pairs, w.n_pairs, np[k], d[k], p, count[k]);
+        std::string err =
+            enumerate_begin(w, *qs_[items[k].q], items[k].i, ph, w.L, built[k], w.pairs, w.n_pairs, np[k], d[k], p, count[k], false);
         if (!err.empty()) return err;
         pend[k] = p;
         wait = wait || p;
     }
-    if (wait) CSM_HIP(hipStreamSynchronize(s));
+    uint32_t* hl = h_tlens_ + first_lane * kLaneWords;
+    if (wait) {  // every lane's counters in one copy (they are adjacent)
+        CSM_HIP(hipMemcpyAsync(hl, lens_all_ + first_lane * kLaneWords, n * kLaneWords * 4ull, hipMemcpyDeviceToHost, s));
+        CSM_HIP(hipStreamSynchronize(s));
+    }
     for (size_t k = 0; k < n; k++)
         if (np[k] && pend[k]) {
-            std::string err = initial_count(lane(k), count[k]);
+            std::string err = initial_count(hl + k * kLaneWords + kStepCounters, count[k]);
             if (!err.empty()) return err;
         }
     // the level chains of the semi-BFS steps as one chain of multi-job launches, one wait
@@ -1490,13 +1504,14 @@ std::string Engine::run_team_steps(size_t first_lane, std::vector<GroupStep>& it
                 prof_end(lead.prof, s, kKLevelCount);
             }
         }
-        for (uint32_t j = 0; j < nj; j++)
-            if (hj[j].L) CSM_HIP(hipMemcpyAsync(lane(job_step[j]).h_pinned, hj[j].c.ctr, 8ull * hj[j].L, hipMemcpyDeviceToHost, s));
+        // every chain's counters in one copy (the lanes' counters are adjacent)
+        uint32_t* hc = h_tctr_ + first_lane * 2 * kMaxQV;
+        CSM_HIP(hipMemcpyAsync(hc, chain_ctr_all_ + first_lane * 2 * kMaxQV, n * 2 * kMaxQV * 4ull, hipMemcpyDeviceToHost, s));
         CSM_HIP(hipStreamSynchronize(s));
         for (uint32_t j = 0; j < nj; j++) {
             const size_t k = job_step[j];
             QState& qs = *qs_[items[k].q];
-            std::string err = chain_finish(lane(k), qs, d[k], items[k].i, count[k], slot_of(qs), hj[j].c, hj[j].L);
+            std::string err = chain_finish(lane(k), qs, d[k], items[k].i, count[k], slot_of(qs), hj[j].c, hj[j].L, hc + k * 2 * kMaxQV);
             if (!err.empty()) return err;
         }
     }

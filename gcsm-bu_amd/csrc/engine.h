@@ -292,14 +292,14 @@ class Engine {
     std::string enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
                                const uint32_t* n_pairs, uint32_t np);
     std::string enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
-                                const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count);
-    std::string initial_count(Lane& w, uint32_t& count);
+                                const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count, bool copy_count = true);
+    static std::string initial_count(const uint32_t* counters, uint32_t& count);
     std::string enumerate_rest(Lane& w, QState& qs, uint32_t i, uint32_t ph, const StepDesc& d, uint32_t count);
     std::string search_chained(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot);
     uint32_t chain_build(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, LevelChain& c);
     MatchOut chain_out(const StepDesc& d, uint32_t res_slot);
     std::string chain_finish(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot, const LevelChain& c,
-                             uint32_t L);
+                             uint32_t L, const uint32_t* ctr);
     uint32_t slot_of(const QState& qs) const;
     static int sched_of(SearchStrategy st);
     // Teams: lanes [t * team_, (t + 1) * team_) run up to team_ steps at a time on the first lane's
@@ -321,6 +321,8 @@ class Engine {
     LocalJob* h_tljobs_ = nullptr;
     VSet* h_tclear_ = nullptr;
     uint32_t* h_tlens_ = nullptr;
+    uint32_t* chain_ctr_all_ = nullptr;  // every lane's 2 * kMaxQV chain counters, lane after lane
+    uint32_t* h_tctr_ = nullptr;         // their pinned read-back, same layout
     ChainJob* d_chain_jobs_ = nullptr;   // one per lane
     ChainJob* h_chain_jobs_ = nullptr;   // pinned staging, one per lane
     std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,

@@ -108,6 +108,38 @@ __device__ __forceinline__ void wave_sync() {
 }
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Flattened item processing (the local-index kernels, the initial partial results): G items per
+// wave, the smallest power of two that covers `items` with the grid's waves.
+__device__ __forceinline__ uint32_t items_per_wave(uint32_t items) {
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    uint32_t G = 1;
+    while (G < 64 && static_cast<uint64_t>(G) * waves < items) G <<= 1;
+    return G;
+}
+
+// Wave-wide exclusive prefix of x; returns it and sets total.
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t& total) {
+    const uint32_t lane = lane_id();
+    uint32_t inc = x;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+        if (lane >= o) inc += y;
+    }
+    total = static_cast<uint32_t>(__shfl(static_cast<int>(inc), 63, 64));
+    return inc - x;
+}
+
+// The item of the wave whose run holds flattened element e (largest k with pre[k] <= e).
+__device__ __forceinline__ uint32_t owner_of(const uint32_t* pre, uint32_t e) {
+    uint32_t lo = 0, hi = 63;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
     return x;

@@ -952,39 +952,69 @@ __global__ void k_init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs
     if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = n;
 }
 
-// One wave per start vertex: pairs (v, x) with x in the view run of v along a->b and x in filter(b).
-__global__ void __launch_bounds__(256) k_init_from_set(DevGraph g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
-                                                       const uint32_t* list, const uint32_t* len, uint32_t max_len,
-                                                       const uint32_t* filter_b, Ring r) {
-    const uint32_t lane = lane_id();
-    const uint32_t wpb = blockDim.x >> 6;
-    const uint32_t n = min(*len, max_len);
-    const uint32_t* dl = phase_delta(g, dir, phase);
-    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb) {
-        const uint32_t v = list[i];
-        const RunView rv = make_run(g, phase, view, dir, v, slot);
-        const uint32_t nb = run_base_len(rv, view), tot = run_seq_len(rv, view);
-        for (uint32_t t0 = 0; t0 < tot; t0 += 64) {
-            const uint32_t t = t0 + lane;
-            const bool valid = t < tot;
-            const uint32_t key = valid ? run_elem(g, dl, rv, nb, t) : 0u;
-            const uint32_t x = key & g.id_mask;
-            const bool ok = valid && x != v && bit_test(filter_b, x) && run_elem_live(dl, rv, view, key);
+// Pairs (v, x) with v in the start list, x in the view run of v along a->b and x in filter(b).
+// The runs of a wave's G start vertices are flattened (a wave prefix of their lengths, one element
+// per lane per round, its vertex found by a search of the prefix in LDS), and the wave reserves
+// its survivors' output once per round: most start vertices have a run of a few elements.
+__device__ __forceinline__ void init_set_body(const DevGraph& g, const InitJob& J) {
+    __shared__ uint32_t s_pre[4][65];
+    __shared__ RunView s_run[4][64];
+    __shared__ uint32_t s_nb[4][64], s_v[4][64];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wv, waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t n = min(*J.len, J.max_len);
+    const uint32_t* dl = phase_delta(g, J.dir, J.phase);
+    const uint32_t G = items_per_wave(n);
+    for (uint32_t base = gw * G; base < n; base += waves * G) {
+        const uint32_t i = base + lane;
+        uint32_t tot = 0, nb = 0, v = 0;
+        RunView r{0, 0, 0, 0};
+        if (lane < G && i < n) {
+            v = J.list[i];
+            r = make_run(g, J.phase, J.view, J.dir, v, J.slot);
+            nb = run_base_len(r, J.view);
+            tot = run_seq_len(r, J.view);
+        }
+        uint32_t T;
+        const uint32_t pre = wave_prefix(tot, T);
+        s_pre[wv][lane] = pre;
+        s_run[wv][lane] = r;
+        s_nb[wv][lane] = nb;
+        s_v[wv][lane] = v;
+        if (lane == 63) s_pre[wv][64] = T;
+        wave_sync();
+        for (uint32_t e0 = 0; e0 < T; e0 += 64) {  // wave-uniform trip count
+            const uint32_t e = e0 + lane;
+            bool ok = false;
+            uint32_t sv = 0, x = 0;
+            if (e < T) {
+                const uint32_t k = owner_of(s_pre[wv], e);
+                const RunView rr = s_run[wv][k];
+                sv = s_v[wv][k];
+                const uint32_t key = run_elem(g, dl, rr, s_nb[wv][k], e - s_pre[wv][k]);
+                x = key & g.id_mask;
+                ok = x != sv && bit_test(J.filter_b, x) && run_elem_live(dl, rr, J.view, key);
+            }
             const uint64_t m = __ballot(ok);
             if (!m) continue;
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(r.out_count, static_cast<uint32_t>(__popcll(m)));
-            base = __shfl(base, 0, 64);
-            if (static_cast<uint64_t>(base) + __popcll(m) > r.out_cap) {
-                if (lane == 0) atomicOr(r.overflow, 1u);
+            uint32_t o = 0;
+            if (lane == 0) o = atomicAdd(J.out_count, static_cast<uint32_t>(__popcll(m)));
+            o = __shfl(o, 0, 64);
+            if (static_cast<uint64_t>(o) + __popcll(m) > J.out_cap) {
+                if (lane == 0) atomicOr(J.overflow, 1u);
             } else if (ok) {
-                const uint32_t o = base + lanes_below(m);
-                r.out[2 * static_cast<uint64_t>(o)] = v;
-                r.out[2 * static_cast<uint64_t>(o) + 1] = x;
+                const uint64_t p = o + lanes_below(m);
+                J.out[2 * p] = sv;
+                J.out[2 * p + 1] = x;
             }
         }
+        wave_sync();  // the LDS rows are rewritten by the next items
     }
 }
+
+__global__ void __launch_bounds__(256) k_init_from_set(DevGraph g, InitJob J) { init_set_body(g, J); }
+
+__global__ void __launch_bounds__(256) k_init_from_set_multi(DevGraph g, const InitJob* jobs) { init_set_body(g, jobs[blockIdx.y]); }
 
 // ---- Cartesian product -------------------------------------------------------------------------
 // The trailing leaves d0 .. n-1 of a record each hang off one matched vertex.  A segment is one
@@ -1201,7 +1231,12 @@ void init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t ma
 
 void init_from_set(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot, const uint32_t* list,
                    const uint32_t* len, uint32_t max_len, const uint32_t* filter_b, Ring r, hipStream_t s) {
-    k_init_from_set<<<grid_for(max_len, 4, 1024), 256, 0, s>>>(g, phase, view, dir, slot, list, len, max_len, filter_b, r);
+    const InitJob J{phase, view, dir, slot, list, len, filter_b, r.out, r.out_count, r.overflow, r.out_cap, max_len};
+    k_init_from_set<<<grid_for(max_len, 4 * 64, 512), 256, 0, s>>>(g, J);
+}
+
+void init_from_set_multi(const DevGraph& g, const InitJob* jobs, uint32_t njobs, hipStream_t s) {
+    if (njobs) k_init_from_set_multi<<<dim3(256, njobs), 256, 0, s>>>(g, jobs);
 }
 
 void count_records(const StepDesc& d, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec, MatchOut out, hipStream_t s) {

@@ -154,35 +154,6 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     return x;
 }
 
-__device__ __forceinline__ uint32_t items_per_wave(uint32_t items) {
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    uint32_t G = 1;
-    while (G < 64 && static_cast<uint64_t>(G) * waves < items) G <<= 1;
-    return G;
-}
-
-// Wave-wide exclusive prefix of x; returns it and sets total.
-__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t& total) {
-    const uint32_t lane = lane_id();
-    uint32_t inc = x;
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
-        if (lane >= o) inc += y;
-    }
-    total = static_cast<uint32_t>(__shfl(static_cast<int>(inc), 63, 64));
-    return inc - x;
-}
-
-// The item of the wave whose run holds flattened element e (largest k with pre[k] <= e).
-__device__ __forceinline__ uint32_t owner_of(const uint32_t* pre, uint32_t e) {
-    uint32_t lo = 0, hi = 63;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 struct Rel4 {
     uint32_t phase, view, dir, slot;

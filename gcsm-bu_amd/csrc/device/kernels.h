@@ -205,10 +205,25 @@ struct Ring {
 // Initial partial results of depth 2 from the anchor pairs (optionally swapped).
 void init_from_pairs(const uint32_t* pairs, const uint32_t* n_pairs, uint32_t max_pairs, bool swap,
                      uint32_t* out, uint32_t* out_count, hipStream_t s);
-// Initial partial results of depth 2 from a start vertex set: (v, x) with x in the view a->b of v.
+// Initial partial results of depth 2 from a start vertex set: (v, x) with x in the view a->b of v
+// and in filter_b, written as pairs at out[0 ..] (count at *out_count, *overflow set when more
+// than out_cap do not fit).
+struct InitJob {
+    uint32_t phase, view, dir, slot;
+    const uint32_t* list;
+    const uint32_t* len;
+    const uint32_t* filter_b;
+    uint32_t* out;
+    uint32_t* out_count;
+    uint32_t* overflow;
+    uint32_t out_cap;
+    uint32_t max_len;
+};
 void init_from_set(const DevGraph& g, uint32_t phase, uint32_t view, uint32_t dir, uint32_t slot,
                    const uint32_t* list, const uint32_t* len, uint32_t max_len, const uint32_t* filter_b,
                    Ring r, hipStream_t s);
+// The same for njobs steps at once (jobs in device memory, grid.y = job).
+void init_from_set_multi(const DevGraph& g, const InitJob* jobs, uint32_t njobs, hipStream_t s);
 // Counts and hashes records that are already complete embeddings (two-vertex queries).
 void count_records(const StepDesc& d, const uint32_t* rec, const uint32_t* n_rec, uint32_t max_rec, MatchOut out, hipStream_t s);
 // Compacts the set bits of a bitmap over [0, dv) into a list.

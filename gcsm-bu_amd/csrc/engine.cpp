@@ -253,7 +253,8 @@ std::string Engine::build() {
         add(64 << 10);                                             // alignment slack of the takes
         add(sizeof(ChainJob) + 256);                               // the lane's level-chain job
         add(kMaxDfsBlocks * 4ull * 12 + 512);                      // range donation words
-        add(kLaneWords * 4 + sizeof(AnchorJob) + kMaxQV * (sizeof(LocalJob) + sizeof(VSet) + (kMaxLanes + 1) * 4ull) + 1024);  // team jobs
+        add(kLaneWords * 4 + sizeof(AnchorJob) + sizeof(InitJob) + kMaxQV * (sizeof(LocalJob) + sizeof(VSet) + (kMaxLanes + 1) * 4ull) +
+            2 * kMaxQV * 4 + 1024);  // team jobs
     }
     // local-index slots: two groups of group_ steps (each: its sets, anchor pairs, queue, counters)
     uint32_t max_qn = 2;
@@ -432,12 +433,15 @@ std::string Engine::build() {
     lens_all_ = pool_.take<uint32_t>(static_cast<uint64_t>(kLaneWords) * n_lanes);
     chain_ctr_all_ = pool_.take<uint32_t>(2ull * kMaxQV * n_lanes);
     d_tajobs_ = pool_.take<AnchorJob>(n_lanes);
+    d_tinit_ = pool_.take<InitJob>(n_lanes);
     d_tljobs_ = pool_.take<LocalJob>(static_cast<uint64_t>(kMaxQV) * n_lanes);
     d_tclear_ = pool_.take<VSet>(static_cast<uint64_t>(kMaxQV) * n_lanes);
     d_toff_ = pool_.take<uint32_t>(static_cast<uint64_t>(n_lanes) * kMaxQV * (kMaxLanes + 1));
-    if (!lens_all_ || !chain_ctr_all_ || !d_tajobs_ || !d_tljobs_ || !d_tclear_ || !d_toff_) return tagged(Code::kMemory, "device pool exhausted (lane counters)");
+    if (!lens_all_ || !chain_ctr_all_ || !d_tajobs_ || !d_tinit_ || !d_tljobs_ || !d_tclear_ || !d_toff_) return tagged(Code::kMemory, "device pool exhausted (lane counters)");
     {
-        const uint64_t hb = n_lanes * (sizeof(AnchorJob) + kMaxQV * sizeof(LocalJob) + kMaxQV * sizeof(VSet) + kLaneWords * 4ull + 2 * kMaxQV * 4ull) + 1024;
+        const uint64_t hb =
+            n_lanes * (sizeof(AnchorJob) + kMaxQV * sizeof(LocalJob) + kMaxQV * sizeof(VSet) + kLaneWords * 4ull + 2 * kMaxQV * 4ull + sizeof(InitJob)) +
+            1024;
         uint8_t* h = nullptr;
         CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h), hb, hipHostMallocDefault));
         h_tajobs_ = reinterpret_cast<AnchorJob*>(h);
@@ -445,6 +449,7 @@ std::string Engine::build() {
         h_tclear_ = reinterpret_cast<VSet*>(h_tljobs_ + static_cast<uint64_t>(kMaxQV) * n_lanes);
         h_tlens_ = reinterpret_cast<uint32_t*>(h_tclear_ + static_cast<uint64_t>(kMaxQV) * n_lanes);
         h_tctr_ = h_tlens_ + static_cast<uint64_t>(kLaneWords) * n_lanes;
+        h_tinit_ = reinterpret_cast<InitJob*>(h_tctr_ + 2ull * kMaxQV * n_lanes);
     }
     for (Lane& w : lanes_) {
         w.s = lane_streams_[static_cast<size_t>(&w - lanes_.data())];
@@ -809,7 +814,7 @@ std::string Engine::initial_count(const uint32_t* counters, uint32_t& count) {
 // the anchor pairs as the first two vertices their count is np; otherwise init_from_set runs and
 // `pending` says that its count arrives in the lane's pinned buffer (initial_count, after a wait).
 std::string Engine::enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
-                                    const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count, bool copy_count) {
+                                    const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count, InitJob* defer) {
     const Query& q = qs.q;
     w.sched = sched_of(qs.tech.strategy);
     const Plan& p = qs.plan;
@@ -873,10 +878,15 @@ std::string Engine::enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph
             list = w.tmp_list;
             len = w.tmp_len;
         }
-        Ring r{nullptr, nullptr, 0, ring, w.counters, static_cast<uint32_t>(std::min<uint64_t>(w.ring_words / 2, 0xFFFFFFFFull)),
-               w.counters + 2};
-        init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, w.s);
-        if (copy_count) CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
+        const uint32_t cap = static_cast<uint32_t>(std::min<uint64_t>(w.ring_words / 2, 0xFFFFFFFFull));
+        if (defer) {  // the caller launches it with other steps' (init_from_set_multi) and reads the count
+            *defer = InitJob{ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, d.filter[1], ring, w.counters,
+                             w.counters + 2, cap, dv_};
+        } else {
+            Ring r{nullptr, nullptr, 0, ring, w.counters, cap, w.counters + 2};
+            init_from_set(g_, ph, view_of(qs, rab, i, ph), qs.rel[rab].dir, qs.rel[rab].slot, list, len, dv_, d.filter[1], r, w.s);
+            CSM_HIP(hipMemcpyAsync(w.h_pinned, w.counters, 12, hipMemcpyDeviceToHost, w.s));
+        }
         pending = true;
     }
     if (trace_) w.trace_line += ", \"anchors\": " + std::to_string(np);
@@ -1436,19 +1446,26 @@ std::string Engine::run_team_steps(size_t first_lane, std::vector<GroupStep>& it
     for (size_t k = 0; k < n; k++) lane(k).sched = sched_of(qs_[items[k].q]->tech.strategy);
     std::string lerr = team_local(first_lane, items, ph, np, built);
     if (!lerr.empty()) return lerr;
-    // initial partial results, one wait
+    // initial partial results (the steps that start from a vertex set in one launch), one wait
     std::vector<StepDesc> d(n);
     std::vector<char> pend(n, 0);
     bool wait = false;
+    InitJob* hi = h_tinit_ + first_lane;
+    uint32_t ni = 0;
     for (size_t k = 0; k < n; k++) {
         if (!np[k]) continue;
         Lane& w = lane(k);
         bool p = false;
         std::string err =
-            enumerate_begin(w, *qs_[items[k].q], items[k].i, ph, w.L, built[k], w.pairs, w.n_pairs, np[k], d[k], p, count[k], false);
+            enumerate_begin(w, *qs_[items[k].q], items[k].i, ph, w.L, built[k], w.pairs, w.n_pairs, np[k], d[k], p, count[k], hi + ni);
         if (!err.empty()) return err;
         pend[k] = p;
+        if (p) ni++;
         wait = wait || p;
+    }
+    if (ni) {
+        CSM_HIP(hipMemcpyAsync(d_tinit_ + first_lane, hi, ni * sizeof(InitJob), hipMemcpyHostToDevice, s));
+        init_from_set_multi(g_, d_tinit_ + first_lane, ni, s);
     }
     uint32_t* hl = h_tlens_ + first_lane * kLaneWords;
     if (wait) {  // every lane's counters in one copy (they are adjacent)

@@ -292,7 +292,7 @@ class Engine {
     std::string enumerate_step(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
                                const uint32_t* n_pairs, uint32_t np);
     std::string enumerate_begin(Lane& w, QState& qs, uint32_t i, uint32_t ph, const VSet* L, uint32_t built, const uint32_t* pairs,
-                                const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count, bool copy_count = true);
+                                const uint32_t* n_pairs, uint32_t np, StepDesc& d, bool& pending, uint32_t& count, InitJob* defer = nullptr);
     static std::string initial_count(const uint32_t* counters, uint32_t& count);
     std::string enumerate_rest(Lane& w, QState& qs, uint32_t i, uint32_t ph, const StepDesc& d, uint32_t count);
     std::string search_chained(Lane& w, QState& qs, const StepDesc& d, uint32_t i, uint32_t count, uint32_t res_slot);
@@ -323,6 +323,8 @@ class Engine {
     uint32_t* h_tlens_ = nullptr;
     uint32_t* chain_ctr_all_ = nullptr;  // every lane's 2 * kMaxQV chain counters, lane after lane
     uint32_t* h_tctr_ = nullptr;         // their pinned read-back, same layout
+    InitJob* d_tinit_ = nullptr;         // one initial-results job per lane, and its pinned staging
+    InitJob* h_tinit_ = nullptr;
     ChainJob* d_chain_jobs_ = nullptr;   // one per lane
     ChainJob* h_chain_jobs_ = nullptr;   // pinned staging, one per lane
     std::string search(Lane& w, QState& qs, const StepDesc& d, uint32_t i, const uint32_t* rec, uint32_t count, uint32_t depth,

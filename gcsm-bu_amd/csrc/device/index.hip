@@ -381,23 +381,27 @@ __global__ void k_local_anchors_multi(DevGraph g, const AnchorJob* jobs, const u
     }
 }
 
-// off[0..njobs]: exclusive prefix of the jobs' frontier (kind 0) or queue (kind 1) lengths.
+// Every block of a multi-job launch computes off[0..njobs] in LDS, the exclusive prefix of the jobs'
+// frontier (kind 0) or queue (kind 1) lengths: the lengths are independent loads, and wave 0 scans
+// them 64 at a time.
 template <int kKind>
-__global__ void __launch_bounds__(kMaxLocalJobs) k_job_offsets(const LocalJob* jobs, uint32_t njobs, uint32_t max_len, uint32_t* off) {
-    __shared__ uint32_t s[kMaxLocalJobs];
-    const uint32_t t = threadIdx.x;
-    uint32_t v = 0;
-    if (t < njobs) v = min(kKind == 0 ? *jobs[t].frontier_len : *jobs[t].queue_len, max_len);
-    s[t] = v;
+__device__ __forceinline__ void job_offsets(const LocalJob* jobs, uint32_t njobs, uint32_t max_len, uint32_t* off) {
+    for (uint32_t t = threadIdx.x; t < njobs; t += blockDim.x)
+        off[t + 1] = min(kKind == 0 ? *jobs[t].frontier_len : *jobs[t].queue_len, max_len);
     __syncthreads();
-    for (uint32_t o = 1; o < kMaxLocalJobs; o <<= 1) {
-        const uint32_t x = t >= o ? s[t - o] : 0u;
-        __syncthreads();
-        s[t] += x;
-        __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t carry = 0;
+        for (uint32_t c = 0; c < njobs; c += 64) {
+            const uint32_t t = c + threadIdx.x;
+            uint32_t total;
+            const uint32_t x = t < njobs ? off[t + 1] : 0u;
+            const uint32_t ex = wave_prefix(x, total);
+            if (t < njobs) off[t + 1] = carry + ex + x;
+            carry += total;
+        }
+        if (threadIdx.x == 0) off[0] = 0;
     }
-    if (t < njobs) off[t + 1] = s[t];
-    if (t == 0) off[0] = 0;
+    __syncthreads();
 }
 
 // The job whose items [off[j], off[j + 1]) hold item i (off in LDS, njobs >= 1).
@@ -438,12 +442,10 @@ __device__ __forceinline__ void append_jobs(bool flag, uint32_t y, uint32_t j, c
     }
 }
 
-__global__ void __launch_bounds__(256) k_local_queue_multi(DevGraph g, const LocalJob* jobs, uint32_t njobs, const uint32_t* off,
-                                                           unsigned long long* nbytes) {
+__global__ void __launch_bounds__(256) k_local_queue_multi(DevGraph g, const LocalJob* jobs, uint32_t njobs, unsigned long long* nbytes) {
     __shared__ FlatLds lds;
     __shared__ uint32_t s_off[kMaxLocalJobs + 1];
-    for (uint32_t t = threadIdx.x; t <= njobs; t += blockDim.x) s_off[t] = off[t];
-    __syncthreads();
+    job_offsets<0>(jobs, njobs, g.dv, s_off);
     const uint64_t bytes = expand_flat(
         g, lds, s_off[njobs],
         [&](uint32_t i, uint32_t& j, uint32_t& x) {
@@ -456,12 +458,10 @@ __global__ void __launch_bounds__(256) k_local_queue_multi(DevGraph g, const Loc
     add_bytes(nbytes, bytes);
 }
 
-__global__ void __launch_bounds__(256) k_local_check_multi(DevGraph g, const LocalJob* jobs, uint32_t njobs, const uint32_t* off,
-                                                           unsigned long long* nbytes) {
+__global__ void __launch_bounds__(256) k_local_check_multi(DevGraph g, const LocalJob* jobs, uint32_t njobs, unsigned long long* nbytes) {
     __shared__ FlatLds lds;
     __shared__ uint32_t s_off[kMaxLocalJobs + 1];
-    for (uint32_t t = threadIdx.x; t <= njobs; t += blockDim.x) s_off[t] = off[t];
-    __syncthreads();
+    job_offsets<1>(jobs, njobs, g.dv, s_off);
     const uint64_t bytes = check_flat(
         g, lds, s_off[njobs],
         [&](uint32_t i, uint32_t& j, uint32_t& y, uint32_t& nf) {
@@ -560,18 +560,14 @@ void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njob
     k_local_anchors_multi<<<grid, 256, 0, s>>>(g, jobs, by_label0, label_off0, by_label1, label_off1);
 }
 
-void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
-                       hipStream_t s) {
+void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, int blocks, unsigned long long* bytes, hipStream_t s) {
     if (!njobs) return;
-    k_job_offsets<0><<<1, kMaxLocalJobs, 0, s>>>(jobs, njobs, g.dv, off);
-    k_local_queue_multi<<<blocks, 256, 0, s>>>(g, jobs, njobs, off, bytes);
+    k_local_queue_multi<<<blocks, 256, 0, s>>>(g, jobs, njobs, bytes);
 }
 
-void local_check_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
-                       hipStream_t s) {
+void local_check_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, int blocks, unsigned long long* bytes, hipStream_t s) {
     if (!njobs) return;
-    k_job_offsets<1><<<1, kMaxLocalJobs, 0, s>>>(jobs, njobs, g.dv, off);
-    k_local_check_multi<<<blocks, 256, 0, s>>>(g, jobs, njobs, off, bytes);
+    k_local_check_multi<<<blocks, 256, 0, s>>>(g, jobs, njobs, bytes);
 }
 
 void clear_sets_multi(const VSet* sets, uint32_t nsets, uint32_t max_len, hipStream_t s) {

@@ -169,11 +169,9 @@ struct LocalJob {
 constexpr uint32_t kMaxLocalJobs = 256;
 void local_anchors_multi(const DevGraph& g, const AnchorJob* jobs, uint32_t njobs, const uint32_t* by_label0, const uint32_t* label_off0,
                          const uint32_t* by_label1, const uint32_t* label_off1, uint32_t max_touched, hipStream_t s);
-// off: kMaxLocalJobs + 1 words of device scratch.  bytes: as local_queue / local_check.
-void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
-                       hipStream_t s);
-void local_check_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, uint32_t* off, int blocks, unsigned long long* bytes,
-                       hipStream_t s);
+// bytes: as local_queue / local_check.
+void local_queue_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, int blocks, unsigned long long* bytes, hipStream_t s);
+void local_check_multi(const DevGraph& g, const LocalJob* jobs, uint32_t njobs, int blocks, unsigned long long* bytes, hipStream_t s);
 // Clears the bitmaps of sets[0, nsets) (whole words of their listed members).
 void clear_sets_multi(const VSet* sets, uint32_t nsets, uint32_t max_len, hipStream_t s);
 // Clears the bitmaps of the sets in mask (their lengths stay; the caller zeroes them).

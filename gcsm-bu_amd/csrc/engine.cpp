@@ -253,8 +253,7 @@ std::string Engine::build() {
         add(64 << 10);                                             // alignment slack of the takes
         add(sizeof(ChainJob) + 256);                               // the lane's level-chain job
         add(kMaxDfsBlocks * 4ull * 12 + 512);                      // range donation words
-        add(kLaneWords * 4 + sizeof(AnchorJob) + sizeof(InitJob) + kMaxQV * (sizeof(LocalJob) + sizeof(VSet) + (kMaxLanes + 1) * 4ull) +
-            2 * kMaxQV * 4 + 1024);  // team jobs
+        add(kLaneWords * 4 + sizeof(AnchorJob) + sizeof(InitJob) + kMaxQV * (sizeof(LocalJob) + sizeof(VSet)) + 2 * kMaxQV * 4 + 1024);  // team jobs
     }
     // local-index slots: two groups of group_ steps (each: its sets, anchor pairs, queue, counters)
     uint32_t max_qn = 2;
@@ -277,7 +276,7 @@ std::string Engine::build() {
     if (group_) {
         add(2ull * group_ * slot_bytes);
         add(kMaxGroup * sizeof(AnchorJob)); add(kMaxQV * kMaxLocalJobs * sizeof(LocalJob));
-        add(2ull * kMaxGroup * kMaxQV * sizeof(VSet)); add(kMaxQV * (kMaxLocalJobs + 1) * 4ull);
+        add(2ull * kMaxGroup * kMaxQV * sizeof(VSet));
     }
     for (auto& qs : qs_) add(static_cast<uint64_t>(qs->q.n) * words_ * 4);
     add(qs_.size() * 32 + 64);
@@ -436,8 +435,7 @@ std::string Engine::build() {
     d_tinit_ = pool_.take<InitJob>(n_lanes);
     d_tljobs_ = pool_.take<LocalJob>(static_cast<uint64_t>(kMaxQV) * n_lanes);
     d_tclear_ = pool_.take<VSet>(static_cast<uint64_t>(kMaxQV) * n_lanes);
-    d_toff_ = pool_.take<uint32_t>(static_cast<uint64_t>(n_lanes) * kMaxQV * (kMaxLanes + 1));
-    if (!lens_all_ || !chain_ctr_all_ || !d_tajobs_ || !d_tinit_ || !d_tljobs_ || !d_tclear_ || !d_toff_) return tagged(Code::kMemory, "device pool exhausted (lane counters)");
+    if (!lens_all_ || !chain_ctr_all_ || !d_tajobs_ || !d_tinit_ || !d_tljobs_ || !d_tclear_) return tagged(Code::kMemory, "device pool exhausted (lane counters)");
     {
         const uint64_t hb =
             n_lanes * (sizeof(AnchorJob) + kMaxQV * sizeof(LocalJob) + kMaxQV * sizeof(VSet) + kLaneWords * 4ull + 2 * kMaxQV * 4ull + sizeof(InitJob)) +
@@ -519,8 +517,7 @@ std::string Engine::build() {
         d_anchor_jobs_ = pool_.take<AnchorJob>(kMaxGroup);
         d_local_jobs_ = pool_.take<LocalJob>(kMaxQV * kMaxLocalJobs);
         d_clear_sets_ = pool_.take<VSet>(2ull * kMaxGroup * kMaxQV);
-        d_job_off_ = pool_.take<uint32_t>(kMaxQV * (kMaxLocalJobs + 1));
-        if (!d_anchor_jobs_ || !d_local_jobs_ || !d_clear_sets_ || !d_job_off_) return tagged(Code::kMemory, "device pool exhausted (local-index jobs)");
+        if (!d_anchor_jobs_ || !d_local_jobs_ || !d_clear_sets_) return tagged(Code::kMemory, "device pool exhausted (local-index jobs)");
         const uint64_t hbytes = kMaxGroup * sizeof(AnchorJob) + kMaxQV * kMaxLocalJobs * sizeof(LocalJob) +
                                 2ull * kMaxGroup * kMaxQV * sizeof(VSet) + 2ull * kMaxGroup * kSlotWords * 4 + 4096;
         CSM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_jobs_), hbytes, hipHostMallocDefault));
@@ -1291,13 +1288,12 @@ std::string Engine::stage_group(uint32_t ph, std::vector<GroupStep>& steps) {
         }
         if (!nj) continue;
         LocalJob* dj = d_local_jobs_ + static_cast<uint64_t>(pos) * kMaxLocalJobs;
-        uint32_t* off = d_job_off_ + static_cast<uint64_t>(pos) * (kMaxLocalJobs + 1);
         CSM_HIP(hipMemcpyAsync(dj, lj, nj * sizeof(LocalJob), hipMemcpyHostToDevice, s_));
         prof_begin(prof_main_, s_, kKLocalIndex);
-        local_queue_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalIndex), s_);
+        local_queue_multi(g_, dj, nj, local_grid_, bytes_ptr(kKLocalIndex), s_);
         prof_end(prof_main_, s_, kKLocalIndex);
         prof_begin(prof_main_, s_, kKLocalCheck);
-        local_check_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalCheck), s_);
+        local_check_multi(g_, dj, nj, local_grid_, bytes_ptr(kKLocalCheck), s_);
         prof_end(prof_main_, s_, kKLocalCheck);
     }
     CSM_HIP(hipMemcpyAsync(h_ctr, ctr, static_cast<uint64_t>(n) * kSlotWords * 4, hipMemcpyDeviceToHost, s_));
@@ -1419,12 +1415,11 @@ std::string Engine::team_local(size_t first_lane, std::vector<GroupStep>& items,
         const uint32_t nj = njobs[pos];
         if (!nj) continue;
         LocalJob* dj = d_tljobs_ + first_lane * kMaxQV + pos * n;
-        uint32_t* off = d_toff_ + (first_lane * kMaxQV + pos) * (kMaxLanes + 1);
         prof_begin(lead.prof, s, kKLocalIndex);
-        local_queue_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalIndex), s);
+        local_queue_multi(g_, dj, nj, local_grid_, bytes_ptr(kKLocalIndex), s);
         prof_end(lead.prof, s, kKLocalIndex);
         prof_begin(lead.prof, s, kKLocalCheck);
-        local_check_multi(g_, dj, nj, off, local_grid_, bytes_ptr(kKLocalCheck), s);
+        local_check_multi(g_, dj, nj, local_grid_, bytes_ptr(kKLocalCheck), s);
         prof_end(lead.prof, s, kKLocalCheck);
     }
     CSM_HIP(hipMemcpyAsync(hl, lens, n * kLaneWords * 4ull, hipMemcpyDeviceToHost, s));

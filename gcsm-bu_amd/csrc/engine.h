@@ -278,7 +278,6 @@ class Engine {
     AnchorJob* d_anchor_jobs_ = nullptr;
     LocalJob* d_local_jobs_ = nullptr;   // kMaxQV regions of kMaxLocalJobs jobs
     VSet* d_clear_sets_ = nullptr;
-    uint32_t* d_job_off_ = nullptr;      // kMaxQV regions of kMaxLocalJobs + 1 words
     uint8_t* h_jobs_ = nullptr;          // pinned staging of the job arrays, same layout, then counter read-back
     uint32_t* slot_ctr_ = nullptr;       // every slot's counters, kSlotWords each, contiguous
 
@@ -316,7 +315,6 @@ class Engine {
     AnchorJob* d_tajobs_ = nullptr;
     LocalJob* d_tljobs_ = nullptr;
     VSet* d_tclear_ = nullptr;
-    uint32_t* d_toff_ = nullptr;
     AnchorJob* h_tajobs_ = nullptr;
     LocalJob* h_tljobs_ = nullptr;
     VSet* h_tclear_ = nullptr;

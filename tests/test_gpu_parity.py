@@ -318,13 +318,15 @@ def test_adjacency_after_stream(lib, tmp_path):
                                                           (1, 1, 8, 1, 2)])
 def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir, lanes, chain, group):
     """Root splitting, the label-slot directory, the number of concurrent step lanes, chaining a
-    step's levels on the device and the number of steps whose local index is built together (0:
-    each step builds its own on its lane) change scheduling and lookup, never results."""
+    step's levels on the device and the number of steps whose local index is built together on the
+    main stream (0: teams of four lanes build their own) change scheduling and lookup, never
+    results."""
     monkeypatch.setenv("CSM_SPLIT_ROOTS", str(split))
     monkeypatch.setenv("CSM_SLOT_DIRECTORY", str(sdir))
     monkeypatch.setenv("CSM_LANES", str(lanes))
     monkeypatch.setenv("CSM_CHAIN_LEVELS", str(chain))
     monkeypatch.setenv("CSM_GROUP_STEPS", str(group))
+    monkeypatch.setenv("CSM_TEAM", "1" if group else "4")  # teams build their local indexes themselves
     meta, queries = util.gen("tinyins", tmp_path, seed=7)
     data, upd = tmp_path / "data.graph", tmp_path / "update.stream"
     want, _ = util.oracle_stream(data, upd, 64, queries)

@@ -51,17 +51,17 @@ struct EngineOptions {
     uint32_t waves_per_anchor = 1024; // most waves one record of the counting DFS is split over (1: no split)
     uint32_t stack_depth = kMaxQV;    // most LDS stack rows of a counting DFS launch; deeper searches run levels first
     uint32_t dfs_blocks = 2048;       // blocks of the persistent DFS grid
-    uint32_t level_blocks = 512;      // blocks per step of a team's multi-step level kernels
+    uint32_t level_blocks = 384;      // blocks per step of a team's multi-step level kernels
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
     bool chain_levels = true;         // semi-BFS levels of a step chained on the device (one host wait per step)
-    uint32_t lanes = 16;              // concurrent matching steps, each with its own scratch
-    uint32_t team = 4;                // lanes per team: a team's steps share a stream, host waits and multi-step launches
+    uint32_t lanes = 32;              // concurrent matching steps, each with its own scratch
+    uint32_t team = 8;                // lanes per team: a team's steps share a stream, host waits and multi-step launches
     uint64_t arena_slack_words = 1u << 20;  // adjacency arena room beyond the growth rule
 };
 
 constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
 constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
-constexpr uint32_t kMaxLanes = 16;
+constexpr uint32_t kMaxLanes = 32;
 // Per-entry view runs a lane's level scratch holds, in records' worth of cp_max_records: levels
 // whose depth has more backward entries than this take fewer records per call.
 constexpr uint64_t kLevelRunsPerRecord = 3;
@@ -308,9 +308,9 @@ class Engine {
     std::string team_local(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph, std::vector<uint32_t>& np,
                            std::vector<uint32_t>& built);
     uint32_t team_ = 1;
-    // The teams' multi-step local-index jobs: per lane one anchor job, kMaxQV local jobs and
-    // kMaxQV sets to clear, and per lane and indexing position kMaxLanes + 1 job offsets; a team
-    // uses the regions of its lanes (device arrays and their pinned staging, same layout).
+    // The teams' multi-step jobs: per lane one anchor job, one initial-results job, kMaxQV local
+    // jobs and kMaxQV sets to clear; a team uses the regions of its lanes (device arrays and their
+    // pinned staging, same layout).
     uint32_t* lens_all_ = nullptr;       // every lane's kLaneWords counters, lane after lane
     AnchorJob* d_tajobs_ = nullptr;
     LocalJob* d_tljobs_ = nullptr;

@@ -32,7 +32,7 @@ std::string Engine::set_tunables(const Tunables& t) {
     if (t.stack_depth > kMaxQV) return tagged(Code::kArgument, "stack_depth outside 1..16");
     if (t.dfs_blocks && (t.dfs_blocks < 64 || t.dfs_blocks > 16384)) return tagged(Code::kArgument, "dfs_blocks outside 64..16384");
     if (t.local_blocks && (t.local_blocks < 16 || t.local_blocks > 2048)) return tagged(Code::kArgument, "local_blocks outside 16..2048");
-    if (t.team > kMaxLanes) return tagged(Code::kArgument, "team outside 1..16");
+    if (t.team > kMaxLanes) return tagged(Code::kArgument, "team outside 1..32");
     if (t.level_blocks && (t.level_blocks < 64 || t.level_blocks > 4096)) return tagged(Code::kArgument, "level_blocks outside 64..4096");
     if (t.semi_bfs_threshold) opt_.semi_bfs_threshold = t.semi_bfs_threshold;
     if (t.waves_per_anchor) opt_.waves_per_anchor = t.waves_per_anchor;
@@ -59,8 +59,8 @@ constexpr uint32_t kLocalBlocks[] = {64, 128, 256, 512};
 constexpr uint32_t kWavesPerAnchor[] = {1, 16, 256, 1024};
 constexpr uint32_t kStackDepth[] = {2, 4, kMaxQV};
 constexpr uint32_t kDfsBlocks[] = {1024, 2048, 4096};
-constexpr uint32_t kTeam[] = {1, 2, 4};
-constexpr uint32_t kLevelBlocks[] = {256, 512, 1024};
+constexpr uint32_t kTeam[] = {1, 4, 8};
+constexpr uint32_t kLevelBlocks[] = {256, 384, 512, 1024};
 // A candidate replaces the best so far only when it is faster by this fraction (timing noise).
 constexpr double kMinGain = 0.01;
 

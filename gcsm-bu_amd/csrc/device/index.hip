@@ -145,9 +145,14 @@ struct FlatLds {
     uint32_t nb[4][64];    // base-run length of each item's run
     uint32_t job[4][64];   // job of each item
     uint32_t flag[4][64];  // check: the item's run holds a member of the filter bitmap
-    const uint32_t* bm[4][64];
-    uint32_t vd[4][64];    // check: view | dir << 8 of the item's filter relation
+    const uint32_t* bm[4][64];    // expansion: the job's output bitmap; check: the filter's bitmap
+    const uint32_t* cand[4][64];  // expansion: the job's candidate bitmap
+    uint32_t vd[4][64];    // view | dir << 8 | phase << 16 of the item's relation
 };
+
+// Run elements a lane tests per round in the flattened loops: their loads are issued together, so
+// a wave holding a long (hub) run makes four times the progress per round trip.
+constexpr uint32_t kFlatIlp = 4;
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     for (int o = 32; o > 0; o >>= 1) x = max(x, static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 64)));
@@ -162,8 +167,8 @@ struct Rel4 {
 // Expands `total` frontier items: item(i, j, x) gives item i's job and frontier vertex, rel(j) the
 // job's expansion relation, target(j, cand, bitmap) its candidate bitmap and output bitmap.  Every
 // element y of an item's run that is a candidate and live in the view is offered to the output
-// bitmap (atomic test-and-set); emit(fresh, y, j) is called by the whole wave once per round.
-// Returns this lane's algorithmic bytes.
+// bitmap (atomic test-and-set); emit(fresh, y, j) is called by the whole wave once per element
+// group.  Returns this lane's algorithmic bytes.
 template <class Item, class Rel, class Target, class Emit>
 __device__ __forceinline__ uint64_t expand_flat(const DevGraph& g, FlatLds& lds, uint32_t total, Item item, Rel rel, Target target,
                                                 Emit emit) {
@@ -173,7 +178,9 @@ __device__ __forceinline__ uint64_t expand_flat(const DevGraph& g, FlatLds& lds,
     uint64_t bytes = 0;
     for (uint32_t base = gw * G; base < total; base += waves * G) {
         const uint32_t i = base + lane;
-        uint32_t tot = 0, j = 0, nb = 0;
+        uint32_t tot = 0, j = 0, nb = 0, vd = 0;
+        const uint32_t* cand = nullptr;
+        uint32_t* bm = nullptr;
         RunView r{0, 0, 0, 0};
         if (lane < G && i < total) {
             uint32_t x;
@@ -182,6 +189,8 @@ __device__ __forceinline__ uint64_t expand_flat(const DevGraph& g, FlatLds& lds,
             r = make_run(g, R.phase, R.view, R.dir, x, R.slot);
             nb = run_base_len(r, R.view);
             tot = run_seq_len(r, R.view);
+            vd = R.view | R.dir << 8 | R.phase << 16;
+            target(j, cand, bm);
             bytes += 12 + 4ull * tot;
         }
         uint32_t T;
@@ -190,30 +199,46 @@ __device__ __forceinline__ uint64_t expand_flat(const DevGraph& g, FlatLds& lds,
         lds.run[wv][lane] = r;
         lds.nb[wv][lane] = nb;
         lds.job[wv][lane] = j;
+        lds.vd[wv][lane] = vd;
+        lds.cand[wv][lane] = cand;
+        lds.bm[wv][lane] = bm;
         if (lane == 63) lds.pre[wv][64] = T;
         wave_sync();
-        for (uint32_t e0 = 0; e0 < T; e0 += 64) {  // wave-uniform trip count
-            const uint32_t e = e0 + lane;
-            bool fresh = false;
-            uint32_t y = 0, jj = 0;
-            if (e < T) {
-                const uint32_t k = owner_of(lds.pre[wv], e);
-                jj = lds.job[wv][k];
-                const RunView rr = lds.run[wv][k];
-                const Rel4 R = rel(jj);
-                const uint32_t* dl = phase_delta(g, R.dir, R.phase);
-                const uint32_t key = run_elem(g, dl, rr, lds.nb[wv][k], e - lds.pre[wv][k]);
-                y = key & g.id_mask;
-                const uint32_t* cand_u;
-                uint32_t* bm;
-                target(jj, cand_u, bm);
-                if (bit_test(cand_u, y) && run_elem_live(dl, rr, R.view, key)) {
-                    const uint32_t bit = 1u << (y & 31);
-                    fresh = !(atomicOr(&bm[y >> 5], bit) & bit);  // else offered by another visitor
-                }
+        for (uint32_t e0 = 0; e0 < T; e0 += 64 * kFlatIlp) {  // wave-uniform trip count
+            uint32_t y[kFlatIlp], jj[kFlatIlp], key[kFlatIlp], kk[kFlatIlp];
+            bool ok[kFlatIlp];
+#pragma unroll
+            for (uint32_t u = 0; u < kFlatIlp; u++) {  // the run elements: independent loads
+                const uint32_t e = e0 + 64 * u + lane;
+                ok[u] = e < T;
+                kk[u] = ok[u] ? owner_of(lds.pre[wv], e) : 0u;
+                jj[u] = lds.job[wv][kk[u]];
+                const uint32_t v = lds.vd[wv][kk[u]];
+                key[u] = ok[u] ? run_elem(g, phase_delta(g, (v >> 8) & 0xFFu, v >> 16), lds.run[wv][kk[u]], lds.nb[wv][kk[u]],
+                                          e - lds.pre[wv][kk[u]])
+                               : 0u;
+                y[u] = key[u] & g.id_mask;
             }
-            emit(fresh, y, jj);
-            bytes += fresh ? 4u : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < kFlatIlp; u++) ok[u] = ok[u] && bit_test(lds.cand[wv][kk[u]], y[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < kFlatIlp; u++) {
+                const uint32_t v = lds.vd[wv][kk[u]];
+                ok[u] = ok[u] && run_elem_live(phase_delta(g, (v >> 8) & 0xFFu, v >> 16), lds.run[wv][kk[u]], v & 0xFFu, key[u]);
+            }
+            bool fresh[kFlatIlp];
+#pragma unroll
+            for (uint32_t u = 0; u < kFlatIlp; u++) {
+                const uint32_t bit = 1u << (y[u] & 31);
+                // else offered by another visitor
+                fresh[u] = ok[u] && !(atomicOr(const_cast<uint32_t*>(lds.bm[wv][kk[u]]) + (y[u] >> 5), bit) & bit);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kFlatIlp; u++) {
+                if (e0 + 64 * u >= T) break;  // wave-uniform
+                emit(fresh[u], y[u], jj[u]);
+                bytes += fresh[u] ? 4u : 0u;
+            }
         }
         wave_sync();  // the LDS rows are rewritten by the next items
     }
@@ -241,14 +266,16 @@ __device__ __forceinline__ uint64_t check_flat(const DevGraph& g, FlatLds& lds, 
         const uint32_t maxf = wave_max_u32(nf);
         for (uint32_t k = 0; k < maxf; k++) {
             const bool mine = pass && k < nf;
-            uint32_t tot = 0, nb = 0;
+            uint32_t tot = 0, nb = 0, vd = 0;
             RunView r{0, 0, 0, 0};
             LocalFilter f{0, 0, 0, nullptr};
             if (mine) {
                 f = filter(j, k);
-                r = make_run(g, phase_of(j), f.view, f.dir, y, f.slot);
+                const uint32_t ph = phase_of(j);
+                r = make_run(g, ph, f.view, f.dir, y, f.slot);
                 nb = run_base_len(r, f.view);
                 tot = run_seq_len(r, f.view);
+                vd = f.view | f.dir << 8 | ph << 16;
                 bytes += 8;
             }
             uint32_t T;
@@ -256,24 +283,33 @@ __device__ __forceinline__ uint64_t check_flat(const DevGraph& g, FlatLds& lds, 
             lds.pre[wv][lane] = pre;
             lds.run[wv][lane] = r;
             lds.nb[wv][lane] = nb;
-            lds.job[wv][lane] = j;
             lds.bm[wv][lane] = f.bitmap;
-            lds.vd[wv][lane] = f.view | (f.dir << 8);
+            lds.vd[wv][lane] = vd;
             lds.flag[wv][lane] = 0;
             if (lane == 63) lds.pre[wv][64] = T;
             wave_sync();
-            for (uint32_t e0 = 0; e0 < T; e0 += 64) {  // wave-uniform trip count
-                const uint32_t e = e0 + lane;
-                if (e < T) {
-                    const uint32_t o = owner_of(lds.pre[wv], e);
-                    if (!lds.flag[wv][o]) {  // an item is done at its first hit
-                        const uint32_t vd = lds.vd[wv][o];
-                        const uint32_t* fdl = phase_delta(g, vd >> 8, phase_of(lds.job[wv][o]));
-                        const RunView rr = lds.run[wv][o];
-                        const uint32_t fk = run_elem(g, fdl, rr, lds.nb[wv][o], e - lds.pre[wv][o]);
-                        bytes += 4;
-                        if (bit_test(lds.bm[wv][o], fk & g.id_mask) && run_elem_live(fdl, rr, vd & 0xFFu, fk)) lds.flag[wv][o] = 1u;
-                    }
+            for (uint32_t e0 = 0; e0 < T; e0 += 64 * kFlatIlp) {  // wave-uniform trip count
+                uint32_t o[kFlatIlp], fk[kFlatIlp];
+                bool act[kFlatIlp];
+#pragma unroll
+                for (uint32_t u = 0; u < kFlatIlp; u++) {  // the run elements: independent loads
+                    const uint32_t e = e0 + 64 * u + lane;
+                    o[u] = e < T ? owner_of(lds.pre[wv], e) : 0u;
+                    act[u] = e < T && !lds.flag[wv][o[u]];  // an item is done at its first hit
+                    const uint32_t v = lds.vd[wv][o[u]];
+                    fk[u] = act[u] ? run_elem(g, phase_delta(g, (v >> 8) & 0xFFu, v >> 16), lds.run[wv][o[u]], lds.nb[wv][o[u]],
+                                              e - lds.pre[wv][o[u]])
+                                   : 0u;
+                    bytes += act[u] ? 4u : 0u;
+                }
+                bool hit[kFlatIlp];
+#pragma unroll
+                for (uint32_t u = 0; u < kFlatIlp; u++) hit[u] = act[u] && bit_test(lds.bm[wv][o[u]], fk[u] & g.id_mask);
+#pragma unroll
+                for (uint32_t u = 0; u < kFlatIlp; u++) {
+                    const uint32_t v = lds.vd[wv][o[u]];
+                    if (hit[u] && run_elem_live(phase_delta(g, (v >> 8) & 0xFFu, v >> 16), lds.run[wv][o[u]], v & 0xFFu, fk[u]))
+                        lds.flag[wv][o[u]] = 1u;
                 }
                 wave_sync();
             }

@@ -30,12 +30,13 @@ FAMILIES = ["enumerate_count", "enumerate_write", "cartesian_product", "local_in
 # families whose kernels count algorithmic bytes on the device (the roofline candidates)
 BYTE_FAMILIES = ["enumerate_count", "enumerate_write", "level_count", "level_write", "local_index", "local_check",
                  "merge_insert"]
-# (the level families' launches are the device-chained kernels; levels the host resumes after a
-# chain stop use k_level_flat<0>/<1>, the same kernel body, and are timed with them; the
-# local_index family is one k_local_queue or k_local_expand launch)
-KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat_chain<0>",
-             "level_write": "k_level_flat_chain<1>", "local_index": "k_local_queue", "local_check": "k_local_check",
-             "merge_insert": "k_apply_insertions"}
+# (the level families' launches are a team's device-chained multi-step kernels; levels the host
+# resumes after a chain stop use k_level_flat<0>/<1>, the same kernel body, and are timed with
+# them; the local_index family is one k_local_queue_multi launch per indexing position)
+KERNEL_OF = {"enumerate_count": "k_extend<0>", "enumerate_write": "k_extend<1>", "level_count": "k_level_flat_chain_multi<0>",
+             "level_write": "k_level_flat_chain_multi<1>", "local_index": "k_local_queue_multi",
+             "local_check": "k_local_check_multi", "merge_insert": "k_apply_insertions"}
+ROOFLINE_LANES = 8  # one team of the default size: one stream, so HIP events bracket one kernel
 SERVING_METRIC = ("serving: update edges/s and p50/p99 arrival-to-completion latency of in-memory batches arriving as a "
                   "Poisson process (host arrays, copy included)")
 WORKLOADS = {
@@ -252,11 +253,13 @@ def traffic_of(kernel, config):
 
 
 def roofline_pass(ffi, data_dir, queries, batch, config):
-    """The dominant enumeration kernel timed on one lane (one stream, no other stream's packets
-    between its HIP events): warm-up batch 0, back to the initial graph, then batches 0-1 timed;
-    algorithmic bytes / event time."""
+    """The dominant byte-counting kernel timed in a session of one team (eight lanes on one stream:
+    the multi-step kernels of the timed run, with no other stream's packets between their HIP
+    events): warm-up batch 0, back to the initial graph, then batches 0-1 timed; algorithmic bytes
+    / event time."""
     mask = sum(1 << FAMILIES.index(k) for k in BYTE_FAMILIES)
-    s, nb = open_session(ffi, data_dir, queries, batch, lanes=1)
+    s, nb = open_session(ffi, data_dir, queries, batch, lanes=ROOFLINE_LANES)
+    assert s.tunables()["team"] == ROOFLINE_LANES, "the roofline session must be one team"
     s.snapshot()
     s.profile_enable(mask)
     s.apply_batch(0)
@@ -283,7 +286,8 @@ def roofline_pass(ffi, data_dir, queries, batch, config):
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
             "avg_launch_us": round(ms * 1e3 / launches, 3) if launches else None, "launches": launches,
             "algorithmic_bytes_per_launch": nbytes // launches if launches else 0, "kernels": others,
-            "measured": "one-lane session, batches 0-1 after a warm-up batch 0 and a restore, HIP events on the lane's stream",
+            "measured": "session of one team of eight lanes (one stream), batches 0-1 after a warm-up batch 0 and a restore, "
+                        "HIP events on the team's stream",
             "traffic_source": f"profiles/{config}_kernel_traffic.md" if traffic is not None else None}
 
 
@@ -363,13 +367,14 @@ def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_ma
         edges += info.edges
         log(f"{label}step {k} (batch {b}): {dt * 1e3:.1f} ms, {sum(r[0] + r[1] for r in res)} matches, {info.edges} edges")
     stats = {FAMILIES[f]: s.profile_read(f) for f in range(len(FAMILIES)) if profile_mask >> f & 1}
+    settings = s.tunables()
     s.close()
     total = sum(lat)
     lat_ms = sorted(x * 1e3 for x in lat)
     return {"value": matches / total if total > 0 else 0.0, "ms_per_step": total * 1e3 / max(1, steps),
             "update_edges_per_s": edges / total if total > 0 else 0.0, "p50_batch_ms": statistics.median(lat_ms),
             "p99_batch_ms": lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))],
-            "matches": matches, "per_query_matches": per_query, "batches_in_stream": nb,
+            "matches": matches, "per_query_matches": per_query, "batches_in_stream": nb, "launch_settings": settings,
             "kernel_ms": {k: round(v[1], 3) for k, v in stats.items()}}
 
 
@@ -490,7 +495,7 @@ def main():
         t = time.time()
         thr = header_default_threshold()
         best, table = at.tune(lambda o: open_session(ffi, data_dir, queries, batch, **o),
-                              [at.Candidate(lanes, thr) for lanes in (8, 12, 16)], warmup=1, batches=2)
+                              [at.Candidate(lanes, thr) for lanes in (16, 24, 32)], warmup=1, batches=2)
         opts = {"lanes": best.lanes}
         lib_tune = {}
         tuned = {"lanes": {"chosen": best.lanes, "candidates": table, "batches": "1-2 after warm-up batch 0"},
@@ -518,7 +523,7 @@ def main():
         "metric": METRIC, "value": m["value"], "unit": "matches/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": config_block(args.config, meta, args, opts, args.scale),
+        "config": {**config_block(args.config, meta, args, opts, args.scale), "launch_settings": m["launch_settings"]},
         "update_edges_per_s": m["update_edges_per_s"], "p50_batch_ms": m["p50_batch_ms"], "p99_batch_ms": m["p99_batch_ms"],
         "matches": m["matches"], "per_query_matches": m["per_query_matches"],
         "roofline": roofline, "cpu_baseline": cpu,

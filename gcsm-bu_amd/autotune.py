@@ -1,12 +1,14 @@
-"""Choice of a session's launch settings for one workload, by measurement.
+"""Choice of a session's lane count for one workload, by measurement.
 
 The reference fixes its tunables at compile time. The semi-BFS levels stop expanding at
 MIN_NUM_RESULTS_TO_GPU = 65,536 partial results (config.h:47, used at match_gpu.cu:1961). One CUDA
 stream runs the steps of a batch one after another. Here both are session options
-(`csm_options.semi_bfs_threshold`, `csm_options.lanes`), and `tune` picks them for a given graph,
-update stream and query set. It opens one fresh session per candidate and applies the same prefix of
-the stream: warm-up batches untimed, then timed batches. It keeps the candidate with the least time
-per timed batch.
+(`csm_options.semi_bfs_threshold`, `csm_options.lanes`). The lane count sizes the session's scratch,
+so `tune` opens one fresh session per candidate and applies the same prefix of the stream: warm-up
+batches untimed, then timed batches. It keeps the candidate with the least time per timed batch.
+The settings a session can change between batches (semi-BFS threshold, waves per anchor, DFS stack
+depth, grids, lanes per team) are tuned inside one session by the library itself (`csm_tune`,
+`Session.tune`).
 
 The settings change scheduling only, never results. Every candidate's per-batch results (counts and
 hashes of every query) must equal the first candidate's, and a difference raises `TuneMismatch`.
@@ -27,7 +29,7 @@ class Candidate:
         return {"lanes": self.lanes, "semi_bfs_threshold": self.semi_bfs_threshold}
 
 
-DEFAULT_CANDIDATES = tuple(Candidate(lanes, t) for lanes in (8, 12, 16) for t in (1 << 14, 1 << 16, 1 << 18, 1 << 20))
+DEFAULT_CANDIDATES = tuple(Candidate(lanes, t) for lanes in (16, 24, 32) for t in (1 << 14, 1 << 16, 1 << 18, 1 << 20))
 
 
 class TuneMismatch(RuntimeError):

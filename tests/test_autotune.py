@@ -96,7 +96,7 @@ def test_short_stream_and_bad_candidates_rejected():
 
 
 def test_default_candidates_cover_the_library_defaults():
-    assert at.Candidate(16, 1 << 16) in at.DEFAULT_CANDIDATES  # the library's defaults (16 lanes, 65,536)
+    assert at.Candidate(32, 1 << 16) in at.DEFAULT_CANDIDATES  # the library's defaults (32 lanes, 65,536)
     assert all(1 <= c.lanes <= at.MAX_LANES for c in at.DEFAULT_CANDIDATES)
 
 

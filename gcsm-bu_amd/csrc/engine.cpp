@@ -58,6 +58,7 @@ Engine::~Engine() {
         if (w.done) (void)hipEventDestroy(w.done);
         if (w.h_pinned) (void)hipHostFree(w.h_pinned);
     }
+    for (auto& kv : chain_graphs_) (void)hipGraphExecDestroy(kv.second);
     for (hipStream_t st : lane_streams_) (void)hipStreamDestroy(st);
     if (phase_start_) (void)hipEventDestroy(phase_start_);
     if (snap_) (void)hipFree(snap_);
@@ -92,6 +93,7 @@ std::string Engine::init(const EngineOptions& o) {
     if (const char* t = std::getenv("CSM_SPLIT_ROOTS")) if (std::atoi(t) == 0) opt_.waves_per_anchor = 1;
     if (const char* t = std::getenv("CSM_SLOT_DIRECTORY")) opt_.slot_directory = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_CHAIN_LEVELS")) opt_.chain_levels = std::atoi(t) != 0;
+    if (const char* t = std::getenv("CSM_GRAPH_CHAINS")) opt_.graph_chains = std::atoi(t) != 0;
     if (const char* t = std::getenv("CSM_ARENA_SLACK_WORDS")) opt_.arena_slack_words = std::strtoull(t, nullptr, 10);
     if (const char* t = std::getenv("CSM_CP_MAX_RECORDS")) opt_.cp_max_records = static_cast<uint32_t>(std::max(1024L, std::atol(t)));
     // test hook: an exact (possibly tiny) partial-result ring per lane, in words
@@ -1496,25 +1498,54 @@ std::string Engine::run_team_steps(size_t first_lane, std::vector<GroupStep>& it
     if (nj) {
         ChainJob* dj = d_chain_jobs_ + first_lane;
         CSM_HIP(hipMemcpyAsync(dj, hj, nj * sizeof(ChainJob), hipMemcpyHostToDevice, s));
-        for (uint32_t lv = 0; lv <= max_l; lv++) {
-            bool any_write = false, any_count = false;
-            for (uint32_t j = 0; j < nj; j++) {
-                any_write = any_write || lv < hj[j].L;
-                any_count = any_count || lv == hj[j].L;
+        // which levels have jobs that write, and which have jobs that count
+        uint32_t wmask = 0, cmask = 0;
+        for (uint32_t j = 0; j < nj; j++) {
+            wmask |= (1u << hj[j].L) - 1u;
+            cmask |= 1u << hj[j].L;
+        }
+        auto enqueue = [&](bool timed) {
+            for (uint32_t lv = 0; lv <= max_l; lv++) {
+                if (timed) prof_begin(lead.prof, s, kKLevelPrepare);
+                level_prepare_chain_multi(g_, dj, nj, lv, static_cast<int>(opt_.level_blocks), s);
+                if (timed) prof_end(lead.prof, s, kKLevelPrepare);
+                if (wmask >> lv & 1u) {
+                    if (timed) prof_begin(lead.prof, s, kKLevelWrite);
+                    level_flat_chain_multi(g_, dj, nj, lv, kWriteAtEnd, static_cast<int>(opt_.level_blocks), s);
+                    if (timed) prof_end(lead.prof, s, kKLevelWrite);
+                }
+                if (cmask >> lv & 1u) {
+                    if (timed) prof_begin(lead.prof, s, kKLevelCount);
+                    level_flat_chain_multi(g_, dj, nj, lv, kCountAtEnd, static_cast<int>(opt_.level_blocks), s);
+                    if (timed) prof_end(lead.prof, s, kKLevelCount);
+                }
             }
-            prof_begin(lead.prof, s, kKLevelPrepare);
-            level_prepare_chain_multi(g_, dj, nj, lv, static_cast<int>(opt_.level_blocks), s);
-            prof_end(lead.prof, s, kKLevelPrepare);
-            if (any_write) {
-                prof_begin(lead.prof, s, kKLevelWrite);
-                level_flat_chain_multi(g_, dj, nj, lv, kWriteAtEnd, static_cast<int>(opt_.level_blocks), s);
-                prof_end(lead.prof, s, kKLevelWrite);
+        };
+        if (prof_mask_ || !opt_.graph_chains) {
+            enqueue(prof_mask_ != 0);
+        } else {
+            // The launches depend only on the team, the job count, the write / count levels and the
+            // grid (the jobs themselves are read from device memory): one captured graph per such
+            // shape, replayed.  The graph holds kernels only (no memset or copy nodes).
+            const ChainGraphKey key{first_lane, nj, wmask, cmask, opt_.level_blocks};
+            hipGraphExec_t exec = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(chain_graphs_mu_);
+                auto it = chain_graphs_.find(key);
+                if (it != chain_graphs_.end()) exec = it->second;
             }
-            if (any_count) {
-                prof_begin(lead.prof, s, kKLevelCount);
-                level_flat_chain_multi(g_, dj, nj, lv, kCountAtEnd, static_cast<int>(opt_.level_blocks), s);
-                prof_end(lead.prof, s, kKLevelCount);
+            if (!exec) {
+                hipGraph_t graph = nullptr;
+                CSM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                enqueue(false);
+                CSM_HIP(hipStreamEndCapture(s, &graph));
+                const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) return std::string("HIP error: ") + hipGetErrorString(ie) + " instantiating a level-chain graph";
+                std::lock_guard<std::mutex> lk(chain_graphs_mu_);
+                chain_graphs_.emplace(key, exec);
             }
+            CSM_HIP(hipGraphLaunch(exec, s));
         }
         // every chain's counters in one copy (the lanes' counters are adjacent)
         uint32_t* hc = h_tctr_ + first_lane * 2 * kMaxQV;

@@ -18,6 +18,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/csm.h"
@@ -54,6 +55,10 @@ struct EngineOptions {
     uint32_t level_blocks = 384;      // blocks per step of a team's multi-step level kernels
     bool slot_directory = true;       // per-vertex label-slot offsets (when they fit kMaxSlotDirectoryBytes)
     bool chain_levels = true;         // semi-BFS levels of a step chained on the device (one host wait per step)
+    // A team's chained levels launched as a captured, replayed HIP graph.  Off by default: measured
+    // slower than direct launches on every workload (C3 34.4-37.3 against 33.7-34.1 ms per batch,
+    // C4 85-91 against 81-82 ms, C2 19.7-20.8 against 18.8-18.9 ms).
+    bool graph_chains = false;
     uint32_t lanes = 32;              // concurrent matching steps, each with its own scratch
     uint32_t team = 8;                // lanes per team: a team's steps share a stream, host waits and multi-step launches
     uint64_t arena_slack_words = 1u << 20;  // adjacency arena room beyond the growth rule
@@ -308,6 +313,16 @@ class Engine {
     std::string team_local(size_t first_lane, std::vector<GroupStep>& items, uint32_t ph, std::vector<uint32_t>& np,
                            std::vector<uint32_t>& built);
     uint32_t team_ = 1;
+    // Captured level chains of the teams (graph_chains), by launch shape.
+    struct ChainGraphKey {
+        size_t first_lane;
+        uint32_t njobs, wmask, cmask, blocks;
+        bool operator<(const ChainGraphKey& o) const {
+            return std::tie(first_lane, njobs, wmask, cmask, blocks) < std::tie(o.first_lane, o.njobs, o.wmask, o.cmask, o.blocks);
+        }
+    };
+    std::map<ChainGraphKey, hipGraphExec_t> chain_graphs_;
+    std::mutex chain_graphs_mu_;
     // The teams' multi-step jobs: per lane one anchor job, one initial-results job, kMaxQV local
     // jobs and kMaxQV sets to clear; a team uses the regions of its lanes (device arrays and their
     // pinned staging, same layout).

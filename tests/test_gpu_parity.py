@@ -345,11 +345,14 @@ def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir,
 
 @pytest.mark.parametrize("team,lanes", [(2, 8), (3, 8), (8, 8), (4, 16)])
 def test_teams_change_no_result(lib, tmp_path, monkeypatch, team, lanes):
-    """Teams of lanes (steps run a stage at a time, their level chains as multi-job launches; the
-    last team smaller when the lanes do not divide) change scheduling only: tinyins and c1 for
-    several strategies, and the large/skewed workload at 2% scale, equal the oracle's results."""
+    """Teams of lanes (steps run a stage at a time, their level chains as multi-job launches,
+    captured once per launch shape and replayed as HIP graphs, or launched directly; the last team
+    smaller when the lanes do not divide) change scheduling only: tinyins and c1 for several
+    strategies, and the large/skewed workload at 2% scale, equal the oracle's results batch after
+    batch."""
     monkeypatch.setenv("CSM_TEAM", str(team))
     monkeypatch.setenv("CSM_LANES", str(lanes))
+    monkeypatch.setenv("CSM_GRAPH_CHAINS", "0" if team == 3 else "1")  # chains launched directly / as replayed graphs
     for config, seed, batch in [("tinyins", 7, 64), ("c1", 3, 2000)]:
         d = tmp_path / f"{config}{seed}"
         d.mkdir()

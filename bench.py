@@ -259,7 +259,7 @@ def roofline_pass(ffi, data_dir, queries, batch, config):
     / event time."""
     mask = sum(1 << FAMILIES.index(k) for k in BYTE_FAMILIES)
     s, nb = open_session(ffi, data_dir, queries, batch, lanes=ROOFLINE_LANES)
-    assert s.tunables()["team"] == ROOFLINE_LANES, "the roofline session must be one team"
+    s.set_tunables(team=ROOFLINE_LANES)  # one team: the size of a team of the default 32 lanes
     s.snapshot()
     s.profile_enable(mask)
     s.apply_batch(0)
@@ -378,13 +378,14 @@ def timed_batches(ffi, data_dir, queries, batch, steps, warmup, opts, profile_ma
             "kernel_ms": {k: round(v[1], 3) for k, v in stats.items()}}
 
 
-def plain_path(ffi, data_dir, queries, batch):
+def plain_path(ffi, data_dir, queries, batch, opts=None):
     """The standing queries matched the reference's way, one query per session (per process in the
     reference: its descriptors are __constant__ globals, match_gpu.cu:20-24), against the batched
     form: one session holding every query.  Both apply warm-up batch 0 and then batches 1.. of the
     stream; a batch's plain latency is the sum of the single-query sessions' times for it.  Every
-    query's per-batch counts and hashes must be identical in the two forms."""
-    s, nb = open_session(ffi, data_dir, queries, batch)
+    query's per-batch counts and hashes must be identical in the two forms.  The batched session
+    takes the measured run's options (`opts`); the one-query sessions the library defaults."""
+    s, nb = open_session(ffi, data_dir, queries, batch, **(opts or {}))
     s.apply_batch(0)
     s.synchronize()
     multi_ms, multi = [], []
@@ -421,7 +422,7 @@ def config_block(config, meta, args, opts, scale):
     return {"workload": WORKLOADS[config], "config": config, "seed": args.seed, "scale": scale,
             "batch_edges": int(meta["batch_size"]), "queries": int(meta["queries"]),
             "technique": "GCSM-BU defaults (order 2, semi-BFS, local index, CP, heavy-vertex avoidance, relation switch)",
-            "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "16"))),
+            "lanes": opts.get("lanes", int(os.environ.get("CSM_LANES", "32"))),
             "semi_bfs_threshold": opts.get("semi_bfs_threshold") or header_default_threshold()}
 
 
@@ -440,6 +441,10 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="with --config c3: skip the c2 measurement beside it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="step lanes of the measured session (0: the library default, 32; with --config c4, 64: "
+                         "64 standing queries have about 500 steps per phase, and 64 lanes in teams of 16 take "
+                         "75 against 82 ms per batch)")
     ap.add_argument("--profile-mask", type=int, default=0,
                     help="kernel families to time with HIP events during the timed batches (off by default: "
                          "the roofline comes from a separate one-lane pass)")
@@ -490,12 +495,16 @@ def main():
         return 0
 
     opts, tuned, lib_tune = {}, None, None
+    lanes = args.lanes or (64 if args.config == "c4" else 0)
+    if lanes:
+        opts = {"lanes": lanes}
     if args.autotune:
         at = load_module("autotune")
         t = time.time()
         thr = header_default_threshold()
+        nb = -(-int(meta["stream"]) // batch)  # batches in the update stream
         best, table = at.tune(lambda o: open_session(ffi, data_dir, queries, batch, **o),
-                              [at.Candidate(lanes, thr) for lanes in (16, 24, 32)], warmup=1, batches=2)
+                              [at.Candidate(lanes, thr) for lanes in (16, 32, 64)], warmup=1, batches=max(1, min(2, nb - 1)))
         opts = {"lanes": best.lanes}
         lib_tune = {}
         tuned = {"lanes": {"chosen": best.lanes, "candidates": table, "batches": "1-2 after warm-up batch 0"},
@@ -516,7 +525,7 @@ def main():
     if args.config == "c4":  # the same batches with one lane: every (query, edge) step in turn
         m1 = timed_batches(ffi, data_dir, queries, batch, args.steps, args.warmup, {"lanes": 1}, label="one lane: ")
         one_lane = {"ms_per_step": m1["ms_per_step"], "p99_batch_ms": m1["p99_batch_ms"], "lanes": 1}
-        plain = plain_path(ffi, data_dir, queries, batch)
+        plain = plain_path(ffi, data_dir, queries, batch, opts)
     roofline = roofline_pass(ffi, data_dir, queries, batch, args.config)
     cpu = None if args.no_cpu_baseline else cpu_baseline(data_dir, queries, batch, args.cpu_seconds)
     out = {

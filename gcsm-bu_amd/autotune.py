@@ -17,12 +17,12 @@ import dataclasses
 import os
 import time
 
-MAX_LANES = 32  # kMaxLanes of the library
+MAX_LANES = 64  # kMaxLanes of the library
 
 
 @dataclasses.dataclass(frozen=True)
 class Candidate:
-    lanes: int               # concurrent step lanes (each with its own scratch), 1..32
+    lanes: int               # concurrent step lanes (each with its own scratch), 1..64
     semi_bfs_threshold: int  # records at a Cartesian-product depth at which the levels stop expanding
 
     def options(self):

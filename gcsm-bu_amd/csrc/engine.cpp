@@ -430,7 +430,8 @@ std::string Engine::build() {
 
     // ---- runtime buffers: one set per lane ----
     lanes_.assign(n_lanes, Lane{});
-    team_ = std::max<uint32_t>(1, std::min<uint32_t>(opt_.team, n_lanes));
+    // default: four teams, one per hardware queue of the runtime's default four
+    team_ = std::max<uint32_t>(1, std::min<uint32_t>(opt_.team ? opt_.team : (n_lanes + 3) / 4, n_lanes));
     lens_all_ = pool_.take<uint32_t>(static_cast<uint64_t>(kLaneWords) * n_lanes);
     chain_ctr_all_ = pool_.take<uint32_t>(2ull * kMaxQV * n_lanes);
     d_tajobs_ = pool_.take<AnchorJob>(n_lanes);

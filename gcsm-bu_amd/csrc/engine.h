@@ -60,13 +60,14 @@ struct EngineOptions {
     // C4 85-91 against 81-82 ms, C2 19.7-20.8 against 18.8-18.9 ms).
     bool graph_chains = false;
     uint32_t lanes = 32;              // concurrent matching steps, each with its own scratch
-    uint32_t team = 8;                // lanes per team: a team's steps share a stream, host waits and multi-step launches
+    uint32_t team = 0;                // lanes per team (0: a quarter of the lanes, four teams): a team's steps share a
+                                      // stream, host waits and multi-step launches
     uint64_t arena_slack_words = 1u << 20;  // adjacency arena room beyond the growth rule
 };
 
 constexpr uint64_t kMaxSlotDirectoryBytes = 4ull << 30;
 constexpr uint32_t kSplitMaxRecords = 1u << 14;  // root splitting only below this many records
-constexpr uint32_t kMaxLanes = 32;
+constexpr uint32_t kMaxLanes = 64;
 // Per-entry view runs a lane's level scratch holds, in records' worth of cp_max_records: levels
 // whose depth has more backward entries than this take fewer records per call.
 constexpr uint64_t kLevelRunsPerRecord = 3;

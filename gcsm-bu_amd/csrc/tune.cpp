@@ -32,7 +32,7 @@ std::string Engine::set_tunables(const Tunables& t) {
     if (t.stack_depth > kMaxQV) return tagged(Code::kArgument, "stack_depth outside 1..16");
     if (t.dfs_blocks && (t.dfs_blocks < 64 || t.dfs_blocks > 16384)) return tagged(Code::kArgument, "dfs_blocks outside 64..16384");
     if (t.local_blocks && (t.local_blocks < 16 || t.local_blocks > 2048)) return tagged(Code::kArgument, "local_blocks outside 16..2048");
-    if (t.team > kMaxLanes) return tagged(Code::kArgument, "team outside 1..32");
+    if (t.team > kMaxLanes) return tagged(Code::kArgument, "team outside 1..64");
     if (t.level_blocks && (t.level_blocks < 64 || t.level_blocks > 4096)) return tagged(Code::kArgument, "level_blocks outside 64..4096");
     if (t.semi_bfs_threshold) opt_.semi_bfs_threshold = t.semi_bfs_threshold;
     if (t.waves_per_anchor) opt_.waves_per_anchor = t.waves_per_anchor;
@@ -59,7 +59,6 @@ constexpr uint32_t kLocalBlocks[] = {64, 128, 256, 512};
 constexpr uint32_t kWavesPerAnchor[] = {1, 16, 256, 1024};
 constexpr uint32_t kStackDepth[] = {2, 4, kMaxQV};
 constexpr uint32_t kDfsBlocks[] = {1024, 2048, 4096};
-constexpr uint32_t kTeam[] = {1, 4, 8};
 constexpr uint32_t kLevelBlocks[] = {256, 384, 512, 1024};
 // A candidate replaces the best so far only when it is faster by this fraction (timing noise).
 constexpr double kMinGain = 0.01;
@@ -142,7 +141,11 @@ std::string Engine::tune(uint64_t first, uint64_t batch, uint32_t warmup, uint32
         if (err.empty()) coord(&Tunables::waves_per_anchor, kWavesPerAnchor, std::size(kWavesPerAnchor));
         if (err.empty()) coord(&Tunables::stack_depth, kStackDepth, std::size(kStackDepth));
         if (err.empty()) coord(&Tunables::dfs_blocks, kDfsBlocks, std::size(kDfsBlocks));
-        if (err.empty() && lanes_.size() > 1) coord(&Tunables::team, kTeam, std::size(kTeam));
+        if (err.empty() && lanes_.size() > 1) {  // an eighth, a quarter and half of the lanes: 8, 4 or 2 teams
+            const uint32_t nl = static_cast<uint32_t>(lanes_.size());
+            const uint32_t teams[] = {std::max(1u, nl / 8), std::max(1u, nl / 4), std::max(1u, nl / 2)};
+            coord(&Tunables::team, teams, std::size(teams));
+        }
         if (err.empty() && trail[best].t.team > 1) coord(&Tunables::level_blocks, kLevelBlocks, std::size(kLevelBlocks));
     }
     // leave the graph as it was, with the best settings (the starting ones after a failure)

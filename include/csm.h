@@ -48,7 +48,7 @@ typedef struct {
     uint32_t max_batch_edges;/* largest batch the scratch is sized for, 0 = the stream's batch */
     uint32_t semi_bfs_threshold; /* config.h:47 MIN_NUM_RESULTS_TO_GPU, 0 = CSM_DEFAULT_SEMI_BFS_THRESHOLD */
     uint32_t lanes;          /* extension: concurrent step lanes (each with its own ring and scratch; grouped
-                                in teams, csm_tunables.team), 0 = 16, at most 32 */
+                                in teams, csm_tunables.team), 0 = 32, at most 64 */
 } csm_options;
 
 /* TechniqueOption (utils/technique_option.h:37-56); values as the csmg flags. */
@@ -187,7 +187,8 @@ typedef struct {
     uint32_t dfs_blocks;         /* blocks of the persistent depth-first grid (64..16384, default 2048) */
     uint32_t local_blocks;       /* blocks of the local-index kernels (16..2048; default 512 for batches of
                                     2^18 edges or more, else 128) */
-    uint32_t team;               /* lanes per team (1..32, default 4): a team runs that many steps at a time on
+    uint32_t team;               /* lanes per team (1..64, default a quarter of the lanes, so four teams, one per
+                                    hardware queue of the runtime's default four): a team runs that many steps at a time on
                                     one stream from one host thread, each stage for all of them before one
                                     wait, and their semi-BFS level chains as one chain of multi-step launches */
     uint32_t level_blocks;       /* blocks per step of a team's level kernels (64..4096, default 384) */

@@ -88,7 +88,7 @@ def test_short_stream_and_bad_candidates_rejected():
     with pytest.raises(ValueError, match="2 batches"):
         at.tune(open_session, [a], warmup=1, batches=2, clock=clock)
     assert log == [("close",)]
-    for bad in ([], [at.Candidate(0, 1)], [at.Candidate(33, 1)], [at.Candidate(8, 0)], [at.Candidate(8, 1 << 32)]):
+    for bad in ([], [at.Candidate(0, 1)], [at.Candidate(65, 1)], [at.Candidate(8, 0)], [at.Candidate(8, 1 << 32)]):
         with pytest.raises(ValueError):
             at.tune(open_session, bad, clock=clock)
     with pytest.raises(ValueError):

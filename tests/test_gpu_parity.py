@@ -343,7 +343,7 @@ def test_scheduling_and_layout_switches(lib, tmp_path, monkeypatch, split, sdir,
     assert run_library(lib, str(data), str(upd), 2000, queries, lib.technique()) == want
 
 
-@pytest.mark.parametrize("team,lanes", [(2, 8), (3, 8), (8, 8), (4, 16)])
+@pytest.mark.parametrize("team,lanes", [(2, 8), (3, 8), (8, 8), (4, 16), (16, 64)])
 def test_teams_change_no_result(lib, tmp_path, monkeypatch, team, lanes):
     """Teams of lanes (steps run a stage at a time, their level chains as multi-job launches,
     captured once per launch shape and replayed as HIP graphs, or launched directly; the last team

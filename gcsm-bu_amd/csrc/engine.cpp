@@ -195,10 +195,10 @@ std::string Engine::build() {
     max_batch_ = std::max<uint32_t>(max_batch_, 1);
     // The local-index kernels run beside the other teams' kernels on fixed grids (their frontier
     // sizes are device-side; each launch splits the frontier over its waves itself).  Measured with
-    // 16 lanes in teams of 4: 128 four-wave blocks with 65,536-edge batches on a 400K-vertex graph
-    // (256 within 1%), 512 with 1,048,576-edge batches on a 4M-vertex graph (45.7 ms per batch;
-    // 256: 47.4, 384: 46.2, 1024: +1.5%).
-    local_grid_ = max_batch_ >= (1u << 18) ? 512 : 128;
+    // 32 lanes in teams of 8: 256 four-wave blocks with 65,536-edge batches on a 400K-vertex graph
+    // (18.6 ms per batch; 128: 19.0; also 64 standing queries with 64 lanes: 74.6 against 75.7 ms),
+    // 512 with 1,048,576-edge batches on a 4M-vertex graph (34.1 ms; 256 and 1024: 35.3, 128: 38.2).
+    local_grid_ = max_batch_ >= (1u << 18) ? 512 : 256;
     if (const char* t = std::getenv("CSM_LOCAL_GRID")) local_grid_ = std::max(1, std::atoi(t));
 
     // ---- host CSR of the initial graph ----

@@ -186,7 +186,7 @@ typedef struct {
                                     (1..16, default 16) */
     uint32_t dfs_blocks;         /* blocks of the persistent depth-first grid (64..16384, default 2048) */
     uint32_t local_blocks;       /* blocks of the local-index kernels (16..2048; default 512 for batches of
-                                    2^18 edges or more, else 128) */
+                                    2^18 edges or more, else 256) */
     uint32_t team;               /* lanes per team (1..64, default a quarter of the lanes, so four teams, one per
                                     hardware queue of the runtime's default four): a team runs that many steps at a time on
                                     one stream from one host thread, each stage for all of them before one

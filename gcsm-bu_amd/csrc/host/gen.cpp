@@ -696,7 +696,7 @@ int gen_main(int argc, char** argv) {
         if (!ok) { std::fprintf(stderr, "cannot write %s\n", out.c_str()); return 1; }
     }
     progress("graph and stream written");
-    std::vector<uint64_t> qcounts;
+    std::vector<uint64_t> qcounts, qtried;
     uint32_t qk = 0;
     if (!queries_from.empty()) {
         // queries selected earlier for this configuration, seed and scale
@@ -788,7 +788,7 @@ int gen_main(int argc, char** argv) {
                     }
                     if (cands.empty()) break;
                     tried += cands.size();
-                    if (sp.emb_cap <= 0) { q = cands[0]; found = true; break; }
+                    if (sp.emb_cap <= 0) { q = cands[0]; found = true; tried -= cands.size() - 1; break; }
                     std::vector<char> ok(cands.size(), 0);
                     std::vector<std::thread> th;
                     const unsigned nth = std::max(1u, std::min<unsigned>(kGroup, std::thread::hardware_concurrency()));
@@ -803,7 +803,12 @@ int gen_main(int argc, char** argv) {
                         });
                     for (auto& x : th) x.join();
                     for (size_t i = 0; i < cands.size(); i++)
-                        if (ok[i]) { q = cands[i]; found = true; break; }
+                        if (ok[i]) {  // the group's later candidates were never needed
+                            q = cands[i];
+                            found = true;
+                            tried -= cands.size() - 1 - i;
+                            break;
+                        }
                 }
                 if (!found && sp.directed && qs.first != kTree) {  // no cycle of this size: a tree instead
                     for (int t = 0; t < 64 && !found; t++) found = extract(g, vlab, qs.second, kTree, fin_keys, rng, q);
@@ -815,6 +820,7 @@ int gen_main(int argc, char** argv) {
             }
             write_query(out + "/q" + std::to_string(qk) + ".graph", q, vlab);
             qcounts.push_back(qcount);
+            qtried.push_back(tried);
             if (sp.exact)
                 std::fprintf(stderr, "gen %s: query %u: %zu vertices, %zu edges, %llu embeddings in the union graph, %llu candidates tried (%.1fs)\n",
                              sp.name, qk, q.verts.size(), q.edges.size(), (unsigned long long)qcount, (unsigned long long)tried, now_s() - t_start);
@@ -838,7 +844,11 @@ int gen_main(int argc, char** argv) {
     if (sp.exact && queries_from.empty()) {
         std::fprintf(f, "embedding_cap %.0f\n", sp.emb_cap);
         for (size_t k = 0; k < qcounts.size(); k++) std::fprintf(f, "union_embeddings_q%zu %llu\n", k, (unsigned long long)qcounts[k]);
+    } else if (!sp.exact && sp.emb_cap > 0) {
+        std::fprintf(f, "estimated_embedding_cap %.0f\n", sp.emb_cap);
     }
+    // the selection's bias: candidates drawn per kept query (every one before the kept one was rejected)
+    for (size_t k = 0; k < qtried.size(); k++) std::fprintf(f, "candidates_tried_q%zu %llu\n", k, (unsigned long long)qtried[k]);
     std::fclose(f);
     progress("done");
     return 0;

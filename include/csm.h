@@ -112,8 +112,8 @@ csm_status csm_apply_range(csm_session* s, uint64_t first, uint64_t n, csm_query
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc): csm_apply_range with every
  * lane's partial-result ring taken from the library's static device ring (256 MiB per GPU, split
  * equally over the session's lanes) instead of the session's pool; the call allocates nothing.
- * Same results as csm_apply_range.  Limits: at most 2^23 / lanes edges per call (524,288 with
- * the default 16 lanes), refused with CSM_ERR_LIMIT before any device work; a step whose first two
+ * Same results as csm_apply_range.  Limits: at most 2^23 / lanes edges per call (262,144 with
+ * the default 32 lanes), refused with CSM_ERR_LIMIT before any device work; a step whose first two
  * matching-order vertices are not its anchor edge needs its initial partial results to fit one
  * slice, else CSM_ERR_MEMORY.  All sessions on one GPU share the static ring: calls that overlap
  * on one GPU race. */

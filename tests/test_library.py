@@ -59,3 +59,27 @@ def test_csmg_prints_the_semi_bfs_threshold_in_use(tmp_path):
         r = subprocess.run([util.CSMG, "--query", str(tmp_path / "missing"), "--data", "x", "--update", "y"] + extra,
                            capture_output=True, text=True)
         assert f"MIN_NUM_RESULTS_TO_GPU: {want}\n" in r.stdout
+
+
+def header_struct_fields(name):
+    """The field names of `typedef struct { ... } name;` in include/csm.h, in order."""
+    text = re.sub(r"/\*.*?\*/", "", open(util.HEADER).read(), flags=re.S)
+    m = re.search(r"typedef struct \{([^}]*)\}\s*" + name + r"\s*;", text)
+    assert m, name
+    fields = []
+    for decl in m.group(1).split(";"):
+        decl = decl.strip()
+        if decl:  # "<type> a" or "<type> a, b, c"
+            fields += [x.strip() for x in decl.split(None, 1)[1].split(",")]
+    return fields
+
+
+def test_ctypes_structs_match_the_header():
+    """The ctypes mirrors of the ABI's structs have the header's fields in the header's order (a
+    missing or reordered field would shift every later one)."""
+    ffi = util.ffi()
+    for name, cls in [("csm_options", ffi.Options), ("csm_technique", ffi.Technique), ("csm_query_result", ffi.QueryResult),
+                      ("csm_batch_info", ffi.BatchInfo), ("csm_tunables", ffi.Tunables)]:
+        assert header_struct_fields(name) == [f[0] for f in cls._fields_], name
+    assert header_struct_fields("csm_tunables") == list(ffi.TUNABLES)
+    assert ctypes.sizeof(ffi.Tunables) == 4 * len(ffi.TUNABLES)

@@ -914,8 +914,11 @@ __global__ void __launch_bounds__(kLenThreads) k_level_scan_chain_multi(const Ch
     level_scan_tiles(J.ls.off, n, J.ls.tile_sum);
 }
 
+// At least six waves per SIMD (80 VGPRs, a few spilled outside the candidate loop, for the writing
+// form, which otherwise takes 86 and five waves): the kernel is latency-bound and shares the GPU
+// with the other teams' launches, so the extra resident wave measured faster.
 template <int kMode>
-__global__ void __launch_bounds__(256) k_level_flat_chain_multi(DevGraph g, const ChainJob* jobs, uint32_t k) {
+__global__ void __launch_bounds__(256, 6) k_level_flat_chain_multi(DevGraph g, const ChainJob* jobs, uint32_t k) {
     const ChainJob& J = jobs[blockIdx.y];
     if (k > J.L || (k == J.L) != (kMode == kCountAtEnd)) return;
     uint64_t o;

@@ -915,9 +915,9 @@ __global__ void __launch_bounds__(kLenThreads) k_level_scan_chain_multi(const Ch
 }
 
 // At least six waves per SIMD.  The writing form otherwise takes 86 VGPRs and five waves; bounded it
-// takes 80 and spills one 8-byte value, stored before its candidate loop and reloaded once per
-// 256-candidate step, outside the searches.  The kernel is latency-bound and shares the GPU with the
-// other teams' launches, so the extra resident wave measured faster.
+// takes 80 and spills one 8-byte value per lane, stored once before its candidate loop and reloaded
+// only when staged records are flushed to the ring.  The kernel is latency-bound and shares the GPU
+// with the other teams' launches, so the extra resident wave measured faster.
 template <int kMode>
 __global__ void __launch_bounds__(256, 6) k_level_flat_chain_multi(DevGraph g, const ChainJob* jobs, uint32_t k) {
     const ChainJob& J = jobs[blockIdx.y];
